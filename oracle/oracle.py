@@ -1,0 +1,135 @@
+"""ctypes front end of the CPU FP64 oracle (``pdlp_oracle.cpp``).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker.  The product package never
+imports this module.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "build", "liboracle.so")
+_lib = None
+
+# Slots shared with pdlp_oracle.cpp.
+P_ABS_PRIMAL, P_REL_PRIMAL, P_ABS_DUAL, P_REL_DUAL, P_ABS_GAP, P_REL_GAP = range(6)
+P_ITER_LIMIT, P_TIME_LIMIT, P_THREADS, P_MODE, P_NPARAMS = 6, 7, 8, 9, 10
+INFO_NAMES = [
+    "status", "iterations", "l2_primal_residual", "l2_relative_primal_residual",
+    "l2_dual_residual", "l2_relative_dual_residual", "primal_objective", "dual_objective",
+    "gap", "relative_gap", "solve_time", "eta0", "omega0", "beta", "gamma", "omega",
+    "restarts", "power_iterations", "sigma_max_sq", "b_norm", "c_norm",
+]
+TRACE_NAMES = ["k", "l2_primal", "l2_dual", "pobj", "dobj", "gap", "restarted", "omega", "fpe", "k_r"]
+
+
+def build() -> str:
+    subprocess.run(["make", "-s", "-C", _HERE, "build/liboracle.so"], check=True)
+    return _SO
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        _lib = ctypes.CDLL(_SO)
+        assert _lib.oracle_num_params() == P_NPARAMS
+        assert _lib.oracle_num_info() == len(INFO_NAMES)
+    return _lib
+
+
+def _p(a, dtype):
+    if a is None:
+        return None
+    a = np.ascontiguousarray(a, dtype=dtype)
+    return a
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def spmv(ptr, idx, val, x, ncols):
+    ptr, idx, val, x = _p(ptr, np.int32), _p(idx, np.int32), _p(val, np.float64), _p(x, np.float64)
+    y = np.zeros(len(ptr) - 1)
+    lib().oracle_spmv(ctypes.c_int(len(ptr) - 1), ctypes.c_int(ncols), _ptr(ptr), _ptr(idx), _ptr(val), _ptr(x), _ptr(y))
+    return y
+
+
+def transpose(ptr, idx, val, ncols):
+    ptr, idx, val = _p(ptr, np.int32), _p(idx, np.int32), _p(val, np.float64)
+    nnz = int(ptr[-1])
+    tp = np.zeros(ncols + 1, np.int32)
+    ti = np.zeros(nnz, np.int32)
+    tv = np.zeros(nnz)
+    lib().oracle_transpose(ctypes.c_int(len(ptr) - 1), ctypes.c_int(ncols), _ptr(ptr), _ptr(idx), _ptr(val),
+                           _ptr(tp), _ptr(ti), _ptr(tv))
+    return tp, ti, tv
+
+
+def setup(lp, mode: int = 0) -> dict:
+    """Scaling vectors and initial step/weight (mode 0 = Stable3, 1 = Methodical1)."""
+    a = _arrays(lp)
+    Dr, Dc, out = np.zeros(lp.m), np.zeros(lp.n), np.zeros(8)
+    sv = np.zeros(lp.nnz)
+    rc = lib().oracle_setup(ctypes.c_int(lp.m), ctypes.c_int(lp.n), *[_ptr(v) for v in a],
+                            ctypes.c_int(int(lp.maximize)), ctypes.c_int(mode), _ptr(Dr), _ptr(Dc), _ptr(out), _ptr(sv))
+    if rc != 0:
+        raise ValueError("crossing bounds")
+    keys = ["beta", "gamma", "eta", "omega", "sigma_max_sq", "power_iterations", "b_norm", "c_norm"]
+    d = dict(zip(keys, out.tolist()))
+    d.update(Dr=Dr, Dc=Dc, scaled_values=sv)
+    return d
+
+
+def kkt(lp, x, y, slack) -> dict:
+    ptr, idx, val = _p(lp.row_offsets, np.int32), _p(lp.col_indices, np.int32), _p(lp.values, np.float64)
+    c, lc, uc = _p(lp.c, np.float64), _p(lp.con_lb, np.float64), _p(lp.con_ub, np.float64)
+    x, y, slack = _p(x, np.float64), _p(y, np.float64), _p(slack, np.float64)
+    out = np.zeros(8)
+    lib().oracle_kkt(ctypes.c_int(lp.m), ctypes.c_int(lp.n), _ptr(ptr), _ptr(idx), _ptr(val), _ptr(c), _ptr(lc),
+                     _ptr(uc), ctypes.c_int(int(lp.maximize)), ctypes.c_double(lp.offset), _ptr(x), _ptr(y),
+                     _ptr(slack), _ptr(out))
+    return dict(zip(["l2_primal", "l2_dual", "pobj", "dobj", "gap", "absobj", "b_norm", "c_norm"], out.tolist()))
+
+
+def _arrays(lp):
+    return (
+        _p(lp.row_offsets, np.int32), _p(lp.col_indices, np.int32), _p(lp.values, np.float64),
+        _p(lp.c, np.float64), _p(lp.con_lb, np.float64), _p(lp.con_ub, np.float64),
+        _p(lp.var_lb, np.float64), _p(lp.var_ub, np.float64),
+    )
+
+
+def solve(lp, tol: float = 1e-4, iteration_limit: int = 2**31 - 1, time_limit: float = float("inf"),
+          threads: int = 1, x_init=None, y_init=None, trace_cap: int = 0) -> dict:
+    """Stable3 PDLP solve; returns dict with x, y, reduced_cost, info fields and optional trace."""
+    a = _arrays(lp)
+    params = np.zeros(P_NPARAMS)
+    params[:6] = tol
+    params[P_ITER_LIMIT] = iteration_limit
+    params[P_TIME_LIMIT] = time_limit
+    params[P_THREADS] = threads
+    x, y, rc = np.zeros(lp.n), np.zeros(lp.m), np.zeros(lp.n)
+    info = np.zeros(len(INFO_NAMES))
+    xi, yi = _p(x_init, np.float64), _p(y_init, np.float64)
+    trace = np.zeros((max(trace_cap, 1), len(TRACE_NAMES)))
+    tlen = ctypes.c_int(0)
+    st = lib().oracle_solve(ctypes.c_int(lp.m), ctypes.c_int(lp.n), *[_ptr(v) for v in a],
+                            ctypes.c_int(int(lp.maximize)), ctypes.c_double(lp.offset), _ptr(params),
+                            _ptr(xi), _ptr(yi), _ptr(x), _ptr(y), _ptr(rc), _ptr(info),
+                            _ptr(trace) if trace_cap else None, ctypes.c_int(trace_cap), ctypes.byref(tlen))
+    d = dict(zip(INFO_NAMES, info.tolist()))
+    d["status"] = int(st)
+    d["iterations"] = int(d["iterations"])
+    d.update(x=x, y=y, reduced_cost=rc)
+    if trace_cap:
+        d["trace"] = trace[: tlen.value].copy()
+    return d
