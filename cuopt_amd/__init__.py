@@ -1,0 +1,8 @@
+"""MI355X-native PDLP LP engine behind the cuOpt C API.
+
+The compute path is libcuopt_amd.so (hand-written HIP kernels for gfx950 plus a
+C++ host driver).  This package only loads it (``c_api``) and provides LP
+generators and an MPS reader; there is no Python or CPU fallback solver.
+"""
+
+__all__ = ["c_api", "generators", "mps"]

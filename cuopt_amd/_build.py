@@ -1,0 +1,50 @@
+"""Compiles libcuopt_amd.so (HIP for gfx950 + C++ host code) in-tree with hipcc."""
+
+import os
+import subprocess
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libcuopt_amd.so")
+OBJ = os.path.join(HERE, "_build")
+
+HIP_SOURCES = ["pdlp_solver.hip"]
+CXX_SOURCES = ["tile_plan.cpp", "settings.cpp", "cuopt_c.cpp"]
+
+COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-result"]
+# gfx950 only; packed FP32 instructions disabled (they are never emitted by hand here).
+HIP_FLAGS = ["--offload-arch=gfx950", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+
+def _hipcc():
+    for c in ("/opt/rocm/bin/hipcc", "hipcc"):
+        if os.path.exists(c) or c == "hipcc":
+            return c
+
+
+def build_library(verbose: bool = False) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    hipcc = _hipcc()
+    objs = []
+    jobs = []
+    for src in HIP_SOURCES:
+        o = os.path.join(OBJ, src + ".o")
+        jobs.append([hipcc, *COMMON, *HIP_FLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", o])
+        objs.append(o)
+    for src in CXX_SOURCES:
+        o = os.path.join(OBJ, src + ".o")
+        jobs.append([hipcc, *COMMON, "-D__HIP_PLATFORM_AMD__", "-c", os.path.join(CSRC, src), "-o", o])
+        objs.append(o)
+    procs = [(j, subprocess.Popen(j, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)) for j in jobs]
+    for j, p in procs:
+        out, _ = p.communicate()
+        text = out.decode(errors="replace")
+        if verbose and text.strip():
+            print(text)
+        if p.returncode != 0:
+            raise RuntimeError(f"compile failed: {' '.join(j)}\n{text}")
+    link = [hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", LIB, *objs]
+    r = subprocess.run(link, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(link)}\n{r.stdout.decode(errors='replace')}")
+    return LIB

@@ -1,0 +1,275 @@
+"""ctypes binding of libcuopt_amd.so: the cuOpt C API (LP subset) plus extensions.
+
+Mirrors the reference C API one to one (``cuOptCreateProblem``,
+``cuOptSetParameter``, ``cuOptSolve``, getters).  Loading fails loudly when the
+HIP library has not been built; nothing here computes a solution itself.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcuopt_amd.so")
+
+# constants (include/cuopt_c.h)
+CUOPT_SUCCESS, CUOPT_INVALID_ARGUMENT, CUOPT_MPS_FILE_ERROR, CUOPT_MPS_PARSE_ERROR = 0, 1, 2, 3
+CUOPT_VALIDATION_ERROR, CUOPT_OUT_OF_MEMORY, CUOPT_RUNTIME_ERROR = 4, 5, 6
+CUOPT_MINIMIZE, CUOPT_MAXIMIZE = 1, -1
+CUOPT_METHOD_CONCURRENT, CUOPT_METHOD_PDLP, CUOPT_METHOD_DUAL_SIMPLEX, CUOPT_METHOD_BARRIER = 0, 1, 2, 3
+STATUS = {0: "NoTermination", 1: "Optimal", 2: "PrimalInfeasible", 3: "DualInfeasible", 4: "IterationLimit",
+          5: "TimeLimit", 6: "NumericalError", 7: "PrimalFeasible", 9: "ConcurrentLimit"}
+INFO_NAMES = [
+    "steps", "l2_primal_residual", "l2_relative_primal_residual", "l2_dual_residual",
+    "l2_relative_dual_residual", "primal_objective", "dual_objective", "gap", "relative_gap",
+    "solve_time", "setup_time", "loop_time", "eta0", "omega0", "omega", "restarts", "power_iterations",
+    "sigma_max_sq", "beta", "gamma", "precision", "checks",
+]
+TRACE_WIDTH = 10
+PRECISION = {"fp64": 0, "fp32_mixed": 1, "mixed": 1, "fp32": 1}
+
+_lib = None
+
+
+class LibraryNotBuilt(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LibraryNotBuilt(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        i32, f64, vp = ctypes.c_int32, ctypes.c_double, ctypes.c_void_p
+        pi32, pf64, pch = ctypes.POINTER(i32), ctypes.POINTER(f64), ctypes.c_char_p
+        L.cuOptCreateProblem.argtypes = [i32, i32, i32, f64, vp, vp, vp, vp, vp, vp, vp, vp, vp, ctypes.POINTER(vp)]
+        L.cuOptCreateRangedProblem.argtypes = [i32, i32, i32, f64, vp, vp, vp, vp, vp, vp, vp, vp, vp,
+                                               ctypes.POINTER(vp)]
+        for f in ("cuOptDestroyProblem", "cuOptDestroySolverSettings", "cuOptDestroySolution"):
+            getattr(L, f).argtypes = [ctypes.POINTER(vp)]
+            getattr(L, f).restype = None
+        L.cuOptCreateSolverSettings.argtypes = [ctypes.POINTER(vp)]
+        L.cuOptSetParameter.argtypes = [vp, pch, pch]
+        L.cuOptGetParameter.argtypes = [vp, pch, i32, ctypes.c_char_p]
+        L.cuOptSetIntegerParameter.argtypes = [vp, pch, i32]
+        L.cuOptGetIntegerParameter.argtypes = [vp, pch, pi32]
+        L.cuOptSetFloatParameter.argtypes = [vp, pch, f64]
+        L.cuOptGetFloatParameter.argtypes = [vp, pch, pf64]
+        L.cuOptSetInitialPrimalSolution.argtypes = [vp, vp, i32]
+        L.cuOptSetInitialDualSolution.argtypes = [vp, vp, i32]
+        L.cuOptSolve.argtypes = [vp, vp, ctypes.POINTER(vp)]
+        L.cuOptIsMIP.argtypes = [vp, pi32]
+        for f in ("cuOptGetTerminationStatus", "cuOptGetErrorStatus"):
+            getattr(L, f).argtypes = [vp, pi32]
+        L.cuOptGetErrorString.argtypes = [vp, ctypes.c_char_p, i32]
+        for f in ("cuOptGetPrimalSolution", "cuOptGetDualSolution", "cuOptGetReducedCosts"):
+            getattr(L, f).argtypes = [vp, vp]
+        for f in ("cuOptGetObjectiveValue", "cuOptGetSolveTime", "cuOptGetDualObjectiveValue"):
+            getattr(L, f).argtypes = [vp, pf64]
+        for f in ("cuOptGetNumConstraints", "cuOptGetNumVariables", "cuOptGetObjectiveSense",
+                  "cuOptGetNumNonZeros"):
+            getattr(L, f).argtypes = [vp, pi32]
+        L.cuOptGetObjectiveOffset.argtypes = [vp, pf64]
+        for f in ("cuOptGetObjectiveCoefficients", "cuOptGetConstraintRightHandSide",
+                  "cuOptGetConstraintLowerBounds", "cuOptGetConstraintUpperBounds",
+                  "cuOptGetVariableLowerBounds", "cuOptGetVariableUpperBounds", "cuOptGetConstraintSense",
+                  "cuOptGetVariableTypes"):
+            getattr(L, f).argtypes = [vp, vp]
+        L.cuOptGetConstraintMatrix.argtypes = [vp, vp, vp, vp]
+        L.cuOptGetFloatSize.restype = ctypes.c_int8
+        L.cuOptGetIntSize.restype = ctypes.c_int8
+        L.cuOptGetVersion.argtypes = [pi32, pi32, pi32]
+        L.cuOptAmdGetTerminationInfo.argtypes = [vp, vp, i32]
+        L.cuOptAmdGetTrace.argtypes = [vp, vp, i32, pi32]
+        L.cuOptAmdSpmv.argtypes = [i32, i32, vp, vp, vp, vp, vp, i32]
+        L.cuOptAmdDeviceCount.restype = i32
+        _lib = L
+    return _lib
+
+
+def _a(x, dtype):
+    return np.ascontiguousarray(x, dtype=dtype)
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class CuOptError(RuntimeError):
+    def __init__(self, code, msg=""):
+        super().__init__(f"cuOpt error {code}: {msg}")
+        self.code = code
+
+
+def _check(rc, what=""):
+    if rc != CUOPT_SUCCESS:
+        raise CuOptError(rc, what)
+
+
+class Problem:
+    """Owns a cuOptOptimizationProblem handle."""
+
+    def __init__(self, lp, ranged: bool | None = None):
+        L = lib()
+        self.lp = lp
+        self.handle = ctypes.c_void_p()
+        self._keep = []
+        ptr, idx, val = _a(lp.row_offsets, np.int32), _a(lp.col_indices, np.int32), _a(lp.values, np.float64)
+        c, lb, ub = _a(lp.c, np.float64), _a(lp.var_lb, np.float64), _a(lp.var_ub, np.float64)
+        vt = np.frombuffer(b"C" * lp.n, dtype=np.uint8).copy() if lp.n else np.zeros(1, np.uint8)
+        sense = CUOPT_MAXIMIZE if lp.maximize else CUOPT_MINIMIZE
+        if ranged is None:
+            try:
+                lp.row_sense()
+                ranged = False
+            except ValueError:
+                ranged = True
+        if ranged:
+            lc, uc = _a(lp.con_lb, np.float64), _a(lp.con_ub, np.float64)
+            rc = L.cuOptCreateRangedProblem(lp.m, lp.n, sense, lp.offset, _p(c), _p(ptr), _p(idx), _p(val), _p(lc),
+                                            _p(uc), _p(lb), _p(ub), _p(vt), ctypes.byref(self.handle))
+        else:
+            s, rhs = lp.row_sense()
+            s = np.frombuffer(s.tobytes(), dtype=np.uint8).copy() if lp.m else np.zeros(1, np.uint8)
+            rhs = _a(rhs, np.float64) if lp.m else np.zeros(1)
+            rc = L.cuOptCreateProblem(lp.m, lp.n, sense, lp.offset, _p(c), _p(ptr), _p(idx), _p(val), _p(s), _p(rhs),
+                                      _p(lb), _p(ub), _p(vt), ctypes.byref(self.handle))
+        _check(rc, "cuOptCreateProblem")
+
+    def __del__(self):
+        if getattr(self, "handle", None) and self.handle.value:
+            lib().cuOptDestroyProblem(ctypes.byref(self.handle))
+
+
+class SolverSettings:
+    """Owns a cuOptSolverSettings handle."""
+
+    def __init__(self):
+        self.handle = ctypes.c_void_p()
+        _check(lib().cuOptCreateSolverSettings(ctypes.byref(self.handle)))
+        self._keep = []
+
+    def set(self, name: str, value) -> int:
+        L = lib()
+        if isinstance(value, bool):
+            return L.cuOptSetIntegerParameter(self.handle, name.encode(), int(value))
+        if isinstance(value, (int, np.integer)):
+            return L.cuOptSetIntegerParameter(self.handle, name.encode(), int(value))
+        if isinstance(value, float):
+            return L.cuOptSetFloatParameter(self.handle, name.encode(), value)
+        return L.cuOptSetParameter(self.handle, name.encode(), str(value).encode())
+
+    def get_string(self, name: str) -> str:
+        buf = ctypes.create_string_buffer(256)
+        _check(lib().cuOptGetParameter(self.handle, name.encode(), 256, buf))
+        return buf.value.decode()
+
+    def set_initial_primal(self, x):
+        x = _a(x, np.float64)
+        self._keep.append(x)
+        _check(lib().cuOptSetInitialPrimalSolution(self.handle, _p(x), len(x)))
+
+    def set_initial_dual(self, y):
+        y = _a(y, np.float64)
+        self._keep.append(y)
+        _check(lib().cuOptSetInitialDualSolution(self.handle, _p(y), len(y)))
+
+    def __del__(self):
+        if getattr(self, "handle", None) and self.handle.value:
+            lib().cuOptDestroySolverSettings(ctypes.byref(self.handle))
+
+
+@dataclasses.dataclass
+class Result:
+    status: int
+    error: int
+    error_string: str
+    x: np.ndarray
+    y: np.ndarray
+    reduced_cost: np.ndarray
+    primal_objective: float
+    dual_objective: float
+    solve_time: float
+    info: dict
+    trace: np.ndarray
+
+    @property
+    def iterations(self) -> int:
+        return int(self.info["steps"]) if self.info["steps"] == self.info["steps"] else -1
+
+
+def solve_problem(problem: Problem, settings: SolverSettings) -> Result:
+    L = lib()
+    sol = ctypes.c_void_p()
+    err = L.cuOptSolve(problem.handle, settings.handle, ctypes.byref(sol))
+    if not sol.value:
+        raise CuOptError(err, "cuOptSolve returned no solution")
+    try:
+        lp = problem.lp
+        st, es = ctypes.c_int32(), ctypes.c_int32()
+        L.cuOptGetTerminationStatus(sol, ctypes.byref(st))
+        L.cuOptGetErrorStatus(sol, ctypes.byref(es))
+        buf = ctypes.create_string_buffer(1024)
+        L.cuOptGetErrorString(sol, buf, 1024)
+        x, y, z = np.zeros(max(lp.n, 1)), np.zeros(max(lp.m, 1)), np.zeros(max(lp.n, 1))
+        L.cuOptGetPrimalSolution(sol, _p(x))
+        L.cuOptGetDualSolution(sol, _p(y))
+        L.cuOptGetReducedCosts(sol, _p(z))
+        po, do, tm = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
+        L.cuOptGetObjectiveValue(sol, ctypes.byref(po))
+        L.cuOptGetDualObjectiveValue(sol, ctypes.byref(do))
+        L.cuOptGetSolveTime(sol, ctypes.byref(tm))
+        info = np.zeros(len(INFO_NAMES))
+        L.cuOptAmdGetTerminationInfo(sol, _p(info), len(INFO_NAMES))
+        n = ctypes.c_int32()
+        L.cuOptAmdGetTrace(sol, None, 0, ctypes.byref(n))
+        tr = np.zeros((max(n.value, 1), TRACE_WIDTH))
+        L.cuOptAmdGetTrace(sol, _p(tr), n.value, ctypes.byref(n))
+        return Result(st.value, es.value, buf.value.decode(), x[: lp.n], y[: lp.m], z[: lp.n], po.value, do.value,
+                      tm.value, dict(zip(INFO_NAMES, info.tolist())), tr[: n.value])
+    finally:
+        L.cuOptDestroySolution(ctypes.byref(sol))
+
+
+def solve(lp, tol: float = 1e-4, precision: str = "fp64", iteration_limit: int | None = None,
+          time_limit: float | None = None, x_init=None, y_init=None, method: int = CUOPT_METHOD_PDLP,
+          log: bool = False) -> Result:
+    """Solve ``lp`` (a generators.LP) through the C ABI with all six tolerances set to ``tol``."""
+    prob = Problem(lp)
+    s = SolverSettings()
+    for name in ("absolute_primal_tolerance", "relative_primal_tolerance", "absolute_dual_tolerance",
+                 "relative_dual_tolerance", "absolute_gap_tolerance", "relative_gap_tolerance"):
+        _check(s.set(name, float(tol)), name)
+    _check(s.set("method", method))
+    _check(s.set("presolve", 0))
+    _check(s.set("pdlp_precision", PRECISION[precision]))
+    _check(s.set("log_to_console", bool(log)))
+    if iteration_limit is not None:
+        _check(s.set("iteration_limit", int(iteration_limit)))
+    if time_limit is not None:
+        _check(s.set("time_limit", float(time_limit)))
+    if x_init is not None:
+        s.set_initial_primal(x_init)
+    if y_init is not None:
+        s.set_initial_dual(y_init)
+    return solve_problem(prob, s)
+
+
+def spmv(row_offsets, col_indices, values, x, ncols: int, precision: str = "fp64") -> np.ndarray:
+    """y = A x on the GPU with the solver's tiled SpMV kernel."""
+    ptr, idx, val = _a(row_offsets, np.int32), _a(col_indices, np.int32), _a(values, np.float64)
+    x = _a(x, np.float64)
+    m = len(ptr) - 1
+    y = np.zeros(max(m, 1))
+    _check(lib().cuOptAmdSpmv(m, ncols, _p(ptr), _p(idx), _p(val), _p(x), _p(y), PRECISION[precision]),
+           "cuOptAmdSpmv")
+    return y[:m]
+
+
+def device_count() -> int:
+    return int(lib().cuOptAmdDeviceCount())

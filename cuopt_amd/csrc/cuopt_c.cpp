@@ -1,0 +1,596 @@
+// cuopt_c.cpp -- the cuOpt C ABI (LP subset) over the MI355X PDLP engine.
+//
+// Handle and error semantics follow cpp/src/pdlp/cuopt_c.cpp of the reference:
+// null arguments give CUOPT_INVALID_ARGUMENT, unknown parameters give
+// CUOPT_INVALID_ARGUMENT, cuOptSolve returns the solution's error status and
+// solver exceptions never cross the ABI.  Create copies the caller's host
+// arrays; getters fill caller-sized host buffers; destroy nulls the handle.
+//
+// Documented differences (see the solve entry): MIP problems, and the
+// dual-simplex / barrier methods, are rejected with CUOPT_INVALID_ARGUMENT;
+// every PDLP solver mode runs the Stable3 method; presolve is not applied.
+#include <cstdio>
+#include <cstring>
+#include <exception>
+#include <new>
+#include <string>
+
+#include "../../include/cuopt_amd_ext.h"
+#include "common.hpp"
+#include "lp_types.hpp"
+#include "settings.hpp"
+
+using namespace cuopt_amd;
+
+namespace {
+
+struct ProblemHandle {
+  LpHost lp;
+};
+
+struct SettingsHandle {
+  SolverSettingsRegistry reg;
+};
+
+struct SolutionHandle {
+  SolveResult res;
+  int m = 0, n = 0;
+};
+
+ProblemHandle* as_problem(cuOptOptimizationProblem p) { return static_cast<ProblemHandle*>(p); }
+SettingsHandle* as_settings(cuOptSolverSettings s) { return static_cast<SettingsHandle*>(s); }
+SolutionHandle* as_solution(cuOptSolution s) { return static_cast<SolutionHandle*>(s); }
+
+// Copies and validates the shared part of both create entry points.
+cuopt_int_t fill_common(LpHost& lp,
+                        cuopt_int_t m,
+                        cuopt_int_t n,
+                        cuopt_int_t sense,
+                        cuopt_float_t offset,
+                        const cuopt_float_t* c,
+                        const cuopt_int_t* ptr,
+                        const cuopt_int_t* idx,
+                        const cuopt_float_t* val,
+                        const cuopt_float_t* lb,
+                        const cuopt_float_t* ub,
+                        const char* vtypes)
+{
+  if (m < 0 || n < 0) return CUOPT_INVALID_ARGUMENT;
+  if (ptr[0] != 0) return CUOPT_INVALID_ARGUMENT;
+  for (int i = 0; i < m; ++i)
+    if (ptr[i + 1] < ptr[i]) return CUOPT_INVALID_ARGUMENT;
+  const int nnz = ptr[m];
+  for (int k = 0; k < nnz; ++k)
+    if (idx[k] < 0 || idx[k] >= n) return CUOPT_INVALID_ARGUMENT;
+  lp.m        = m;
+  lp.n        = n;
+  lp.maximize = (sense == CUOPT_MAXIMIZE);
+  lp.offset   = offset;
+  lp.c.assign(c, c + n);
+  lp.rowptr.assign(ptr, ptr + m + 1);
+  lp.colidx.assign(idx, idx + nnz);
+  lp.vals.assign(val, val + nnz);
+  lp.lb.assign(lb, lb + n);
+  lp.ub.assign(ub, ub + n);
+  lp.var_types.resize(n);
+  for (int j = 0; j < n; ++j) lp.var_types[j] = vtypes[j] == CUOPT_INTEGER ? CUOPT_INTEGER : CUOPT_CONTINUOUS;
+  return CUOPT_SUCCESS;
+}
+
+bool is_mip(const LpHost& lp)
+{
+  for (char t : lp.var_types)
+    if (t == CUOPT_INTEGER) return true;
+  return false;
+}
+
+const char* status_string(int s)
+{
+  switch (s) {
+    case CUOPT_TERIMINATION_STATUS_OPTIMAL: return "Optimal";
+    case CUOPT_TERIMINATION_STATUS_INFEASIBLE: return "Primal Infeasible";
+    case CUOPT_TERIMINATION_STATUS_UNBOUNDED: return "Dual Infeasible";
+    case CUOPT_TERIMINATION_STATUS_ITERATION_LIMIT: return "Iteration Limit";
+    case CUOPT_TERIMINATION_STATUS_TIME_LIMIT: return "Time Limit";
+    case CUOPT_TERIMINATION_STATUS_NUMERICAL_ERROR: return "A numerical error was encountered.";
+    case CUOPT_TERIMINATION_STATUS_PRIMAL_FEASIBLE: return "Primal Feasible";
+    case CUOPT_TERIMINATION_STATUS_CONCURRENT_LIMIT: return "Concurrent Limit";
+    default: return "No Termination";
+  }
+}
+
+template <typename T>
+cuopt_int_t copy_out(const std::vector<T>& v, T* out)
+{
+  if (out == nullptr) return CUOPT_INVALID_ARGUMENT;
+  if (!v.empty()) std::memcpy(out, v.data(), v.size() * sizeof(T));
+  return CUOPT_SUCCESS;
+}
+
+}  // namespace
+
+extern "C" {
+
+int8_t cuOptGetFloatSize(void) { return sizeof(cuopt_float_t); }
+int8_t cuOptGetIntSize(void) { return sizeof(cuopt_int_t); }
+
+cuopt_int_t cuOptGetVersion(cuopt_int_t* major, cuopt_int_t* minor, cuopt_int_t* patch)
+{
+  if (!major || !minor || !patch) return CUOPT_INVALID_ARGUMENT;
+  *major = 0;
+  *minor = 1;
+  *patch = 0;
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptCreateProblem(cuopt_int_t num_constraints,
+                               cuopt_int_t num_variables,
+                               cuopt_int_t objective_sense,
+                               cuopt_float_t objective_offset,
+                               const cuopt_float_t* objective_coefficients,
+                               const cuopt_int_t* row_offsets,
+                               const cuopt_int_t* column_indices,
+                               const cuopt_float_t* values,
+                               const char* constraint_sense,
+                               const cuopt_float_t* rhs,
+                               const cuopt_float_t* lower_bounds,
+                               const cuopt_float_t* upper_bounds,
+                               const char* variable_types,
+                               cuOptOptimizationProblem* problem_ptr)
+{
+  if (!problem_ptr || !objective_coefficients || !row_offsets || !column_indices || !values || !constraint_sense ||
+      !rhs || !lower_bounds || !upper_bounds || !variable_types)
+    return CUOPT_INVALID_ARGUMENT;
+  try {
+    auto* h = new ProblemHandle();
+    cuopt_int_t rc = fill_common(h->lp, num_constraints, num_variables, objective_sense, objective_offset,
+                                 objective_coefficients, row_offsets, column_indices, values, lower_bounds,
+                                 upper_bounds, variable_types);
+    if (rc != CUOPT_SUCCESS) {
+      delete h;
+      return rc;
+    }
+    for (int i = 0; i < num_constraints; ++i) {
+      const char s = constraint_sense[i];
+      if (s != CUOPT_EQUAL && s != CUOPT_LESS_THAN && s != CUOPT_GREATER_THAN) {
+        delete h;
+        return CUOPT_INVALID_ARGUMENT;
+      }
+    }
+    h->lp.ranged = false;
+    h->lp.sense.assign(constraint_sense, constraint_sense + num_constraints);
+    h->lp.rhs.assign(rhs, rhs + num_constraints);
+    *problem_ptr = h;
+  } catch (const std::bad_alloc&) {
+    return CUOPT_OUT_OF_MEMORY;
+  } catch (...) {
+    return CUOPT_INVALID_ARGUMENT;
+  }
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptCreateRangedProblem(cuopt_int_t num_constraints,
+                                     cuopt_int_t num_variables,
+                                     cuopt_int_t objective_sense,
+                                     cuopt_float_t objective_offset,
+                                     const cuopt_float_t* objective_coefficients,
+                                     const cuopt_int_t* row_offsets,
+                                     const cuopt_int_t* column_indices,
+                                     const cuopt_float_t* values,
+                                     const cuopt_float_t* constraint_lower_bounds,
+                                     const cuopt_float_t* constraint_upper_bounds,
+                                     const cuopt_float_t* variable_lower_bounds,
+                                     const cuopt_float_t* variable_upper_bounds,
+                                     const char* variable_types,
+                                     cuOptOptimizationProblem* problem_ptr)
+{
+  if (!problem_ptr || !objective_coefficients || !row_offsets || !column_indices || !values ||
+      !constraint_lower_bounds || !constraint_upper_bounds || !variable_lower_bounds || !variable_upper_bounds ||
+      !variable_types)
+    return CUOPT_INVALID_ARGUMENT;
+  try {
+    auto* h = new ProblemHandle();
+    cuopt_int_t rc = fill_common(h->lp, num_constraints, num_variables, objective_sense, objective_offset,
+                                 objective_coefficients, row_offsets, column_indices, values, variable_lower_bounds,
+                                 variable_upper_bounds, variable_types);
+    if (rc != CUOPT_SUCCESS) {
+      delete h;
+      return rc;
+    }
+    h->lp.ranged = true;
+    h->lp.lc.assign(constraint_lower_bounds, constraint_lower_bounds + num_constraints);
+    h->lp.uc.assign(constraint_upper_bounds, constraint_upper_bounds + num_constraints);
+    *problem_ptr = h;
+  } catch (const std::bad_alloc&) {
+    return CUOPT_OUT_OF_MEMORY;
+  } catch (...) {
+    return CUOPT_INVALID_ARGUMENT;
+  }
+  return CUOPT_SUCCESS;
+}
+
+void cuOptDestroyProblem(cuOptOptimizationProblem* problem_ptr)
+{
+  if (!problem_ptr || !*problem_ptr) return;
+  delete as_problem(*problem_ptr);
+  *problem_ptr = nullptr;
+}
+
+#define CUOPT_PROBLEM_GETTER_PRELUDE(out) \
+  if (!problem || !(out)) return CUOPT_INVALID_ARGUMENT; \
+  const LpHost& lp = as_problem(problem)->lp;
+
+cuopt_int_t cuOptGetNumConstraints(cuOptOptimizationProblem problem, cuopt_int_t* out)
+{
+  CUOPT_PROBLEM_GETTER_PRELUDE(out)
+  *out = lp.m;
+  return CUOPT_SUCCESS;
+}
+cuopt_int_t cuOptGetNumVariables(cuOptOptimizationProblem problem, cuopt_int_t* out)
+{
+  CUOPT_PROBLEM_GETTER_PRELUDE(out)
+  *out = lp.n;
+  return CUOPT_SUCCESS;
+}
+cuopt_int_t cuOptGetObjectiveSense(cuOptOptimizationProblem problem, cuopt_int_t* out)
+{
+  CUOPT_PROBLEM_GETTER_PRELUDE(out)
+  *out = lp.maximize ? CUOPT_MAXIMIZE : CUOPT_MINIMIZE;
+  return CUOPT_SUCCESS;
+}
+cuopt_int_t cuOptGetObjectiveOffset(cuOptOptimizationProblem problem, cuopt_float_t* out)
+{
+  CUOPT_PROBLEM_GETTER_PRELUDE(out)
+  *out = lp.offset;
+  return CUOPT_SUCCESS;
+}
+cuopt_int_t cuOptGetObjectiveCoefficients(cuOptOptimizationProblem problem, cuopt_float_t* out)
+{
+  CUOPT_PROBLEM_GETTER_PRELUDE(out)
+  return copy_out(lp.c, out);
+}
+cuopt_int_t cuOptGetNumNonZeros(cuOptOptimizationProblem problem, cuopt_int_t* out)
+{
+  CUOPT_PROBLEM_GETTER_PRELUDE(out)
+  *out = lp.rowptr[lp.m];
+  return CUOPT_SUCCESS;
+}
+cuopt_int_t cuOptGetConstraintMatrix(cuOptOptimizationProblem problem,
+                                     cuopt_int_t* offsets,
+                                     cuopt_int_t* indices,
+                                     cuopt_float_t* values)
+{
+  if (!problem || !offsets || !indices || !values) return CUOPT_INVALID_ARGUMENT;
+  const LpHost& lp = as_problem(problem)->lp;
+  copy_out(lp.rowptr, offsets);
+  copy_out(lp.colidx, indices);
+  copy_out(lp.vals, values);
+  return CUOPT_SUCCESS;
+}
+cuopt_int_t cuOptGetConstraintSense(cuOptOptimizationProblem problem, char* out)
+{
+  CUOPT_PROBLEM_GETTER_PRELUDE(out)
+  if (lp.ranged) return CUOPT_INVALID_ARGUMENT;
+  return copy_out(lp.sense, out);
+}
+cuopt_int_t cuOptGetConstraintRightHandSide(cuOptOptimizationProblem problem, cuopt_float_t* out)
+{
+  CUOPT_PROBLEM_GETTER_PRELUDE(out)
+  if (lp.ranged) return CUOPT_INVALID_ARGUMENT;
+  return copy_out(lp.rhs, out);
+}
+cuopt_int_t cuOptGetConstraintLowerBounds(cuOptOptimizationProblem problem, cuopt_float_t* out)
+{
+  CUOPT_PROBLEM_GETTER_PRELUDE(out)
+  if (!lp.ranged) return CUOPT_INVALID_ARGUMENT;
+  return copy_out(lp.lc, out);
+}
+cuopt_int_t cuOptGetConstraintUpperBounds(cuOptOptimizationProblem problem, cuopt_float_t* out)
+{
+  CUOPT_PROBLEM_GETTER_PRELUDE(out)
+  if (!lp.ranged) return CUOPT_INVALID_ARGUMENT;
+  return copy_out(lp.uc, out);
+}
+cuopt_int_t cuOptGetVariableLowerBounds(cuOptOptimizationProblem problem, cuopt_float_t* out)
+{
+  CUOPT_PROBLEM_GETTER_PRELUDE(out)
+  return copy_out(lp.lb, out);
+}
+cuopt_int_t cuOptGetVariableUpperBounds(cuOptOptimizationProblem problem, cuopt_float_t* out)
+{
+  CUOPT_PROBLEM_GETTER_PRELUDE(out)
+  return copy_out(lp.ub, out);
+}
+cuopt_int_t cuOptGetVariableTypes(cuOptOptimizationProblem problem, char* out)
+{
+  CUOPT_PROBLEM_GETTER_PRELUDE(out)
+  return copy_out(lp.var_types, out);
+}
+
+cuopt_int_t cuOptCreateSolverSettings(cuOptSolverSettings* settings_ptr)
+{
+  if (!settings_ptr) return CUOPT_INVALID_ARGUMENT;
+  try {
+    *settings_ptr = new SettingsHandle();
+  } catch (...) {
+    return CUOPT_OUT_OF_MEMORY;
+  }
+  return CUOPT_SUCCESS;
+}
+
+void cuOptDestroySolverSettings(cuOptSolverSettings* settings_ptr)
+{
+  if (!settings_ptr || !*settings_ptr) return;
+  delete as_settings(*settings_ptr);
+  *settings_ptr = nullptr;
+}
+
+cuopt_int_t cuOptSetParameter(cuOptSolverSettings settings, const char* name, const char* value)
+{
+  if (!settings || !name || !value) return CUOPT_INVALID_ARGUMENT;
+  try {
+    as_settings(settings)->reg.set_from_string(name, value);
+  } catch (...) {
+    return CUOPT_INVALID_ARGUMENT;
+  }
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptGetParameter(cuOptSolverSettings settings, const char* name, cuopt_int_t size, char* value)
+{
+  if (!settings || !name || !value || size <= 0) return CUOPT_INVALID_ARGUMENT;
+  try {
+    const std::string s = as_settings(settings)->reg.get_as_string(name);
+    std::snprintf(value, size, "%s", s.c_str());
+  } catch (...) {
+    return CUOPT_INVALID_ARGUMENT;
+  }
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptSetIntegerParameter(cuOptSolverSettings settings, const char* name, cuopt_int_t value)
+{
+  if (!settings || !name) return CUOPT_INVALID_ARGUMENT;
+  auto& reg = as_settings(settings)->reg;
+  try {
+    reg.set_int(name, value);
+  } catch (...) {
+    try {  // an integer call may target a boolean parameter (cuopt_c.cpp:682-690)
+      reg.set_bool(name, value != 0);
+    } catch (...) {
+      return CUOPT_INVALID_ARGUMENT;
+    }
+  }
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptGetIntegerParameter(cuOptSolverSettings settings, const char* name, cuopt_int_t* value)
+{
+  if (!settings || !name || !value) return CUOPT_INVALID_ARGUMENT;
+  auto& reg = as_settings(settings)->reg;
+  try {
+    *value = reg.get_int(name);
+  } catch (...) {
+    try {
+      *value = reg.get_bool(name) ? 1 : 0;
+    } catch (...) {
+      return CUOPT_INVALID_ARGUMENT;
+    }
+  }
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptSetFloatParameter(cuOptSolverSettings settings, const char* name, cuopt_float_t value)
+{
+  if (!settings || !name) return CUOPT_INVALID_ARGUMENT;
+  try {
+    as_settings(settings)->reg.set_float(name, value);
+  } catch (...) {
+    return CUOPT_INVALID_ARGUMENT;
+  }
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptGetFloatParameter(cuOptSolverSettings settings, const char* name, cuopt_float_t* value)
+{
+  if (!settings || !name || !value) return CUOPT_INVALID_ARGUMENT;
+  try {
+    *value = as_settings(settings)->reg.get_float(name);
+  } catch (...) {
+    return CUOPT_INVALID_ARGUMENT;
+  }
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptSetInitialPrimalSolution(cuOptSolverSettings settings, const cuopt_float_t* x, cuopt_int_t n)
+{
+  if (!settings || !x || n <= 0) return CUOPT_INVALID_ARGUMENT;
+  as_settings(settings)->reg.pdlp.initial_primal.assign(x, x + n);
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptSetInitialDualSolution(cuOptSolverSettings settings, const cuopt_float_t* y, cuopt_int_t m)
+{
+  if (!settings || !y || m <= 0) return CUOPT_INVALID_ARGUMENT;
+  as_settings(settings)->reg.pdlp.initial_dual.assign(y, y + m);
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptIsMIP(cuOptOptimizationProblem problem, cuopt_int_t* is_mip_ptr)
+{
+  if (!problem || !is_mip_ptr) return CUOPT_INVALID_ARGUMENT;
+  *is_mip_ptr = is_mip(as_problem(problem)->lp) ? 1 : 0;
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptSolve(cuOptOptimizationProblem problem, cuOptSolverSettings settings, cuOptSolution* solution_ptr)
+{
+  if (!problem || !settings || !solution_ptr) return CUOPT_INVALID_ARGUMENT;
+  const LpHost& lp         = as_problem(problem)->lp;
+  const PdlpSettings& conf = as_settings(settings)->reg.pdlp;
+  SolutionHandle* sol      = nullptr;
+  try {
+    sol = new SolutionHandle();
+  } catch (...) {
+    return CUOPT_OUT_OF_MEMORY;
+  }
+  sol->m = lp.m;
+  sol->n = lp.n;
+  SolveResult& R = sol->res;
+  auto fail = [&](int code, const std::string& msg) {
+    R.error         = code;
+    R.error_message = msg;
+    R.status        = CUOPT_TERIMINATION_STATUS_NO_TERMINATION;
+  };
+  if (is_mip(lp)) {
+    fail(CUOPT_INVALID_ARGUMENT, "MIP problems are not handled by this LP engine");
+  } else if (conf.method == CUOPT_METHOD_DUAL_SIMPLEX || conf.method == CUOPT_METHOD_BARRIER) {
+    fail(CUOPT_INVALID_ARGUMENT, "only the PDLP method (or concurrent, which runs PDLP) is available");
+  } else if (!conf.initial_primal.empty() && (int)conf.initial_primal.size() != lp.n) {
+    fail(CUOPT_VALIDATION_ERROR, "initial primal solution size does not match the number of variables");
+  } else if (!conf.initial_dual.empty() && (int)conf.initial_dual.size() != lp.m) {
+    fail(CUOPT_VALIDATION_ERROR, "initial dual solution size does not match the number of constraints");
+  } else {
+    try {
+      R = solve_lp(lp, conf);
+      if (conf.log_to_console) {
+        std::printf("Status: %s   Objective: %.8e  Iterations: %d  Time: %.3fs\n", status_string(R.status),
+                    R.primal_objective, (int)R.info[CUOPT_AMD_INFO_STEPS], R.solve_time);
+        std::fflush(stdout);
+      }
+    } catch (const validation_error& e) {
+      fail(CUOPT_VALIDATION_ERROR, e.what());
+    } catch (const std::bad_alloc& e) {
+      fail(CUOPT_OUT_OF_MEMORY, e.what());
+    } catch (const hip_error& e) {
+      const std::string msg = e.what();
+      fail(msg.find("out of memory") != std::string::npos ? CUOPT_OUT_OF_MEMORY : CUOPT_RUNTIME_ERROR, msg);
+    } catch (const std::exception& e) {
+      fail(CUOPT_RUNTIME_ERROR, e.what());
+    }
+  }
+  if (R.x.size() != (size_t)lp.n) R.x.assign(lp.n, 0.0);
+  if (R.y.size() != (size_t)lp.m) R.y.assign(lp.m, 0.0);
+  if (R.rc.size() != (size_t)lp.n) R.rc.assign(lp.n, 0.0);
+  *solution_ptr = sol;
+  return R.error;
+}
+
+void cuOptDestroySolution(cuOptSolution* solution_ptr)
+{
+  if (!solution_ptr || !*solution_ptr) return;
+  delete as_solution(*solution_ptr);
+  *solution_ptr = nullptr;
+}
+
+cuopt_int_t cuOptGetTerminationStatus(cuOptSolution solution, cuopt_int_t* out)
+{
+  if (!solution || !out) return CUOPT_INVALID_ARGUMENT;
+  *out = as_solution(solution)->res.status;
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptGetErrorStatus(cuOptSolution solution, cuopt_int_t* out)
+{
+  if (!solution || !out) return CUOPT_INVALID_ARGUMENT;
+  *out = as_solution(solution)->res.error;
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptGetErrorString(cuOptSolution solution, char* out, cuopt_int_t size)
+{
+  if (!solution || !out) return CUOPT_INVALID_ARGUMENT;
+  std::snprintf(out, size, "%s", as_solution(solution)->res.error_message.c_str());
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptGetPrimalSolution(cuOptSolution solution, cuopt_float_t* out)
+{
+  if (!solution || !out) return CUOPT_INVALID_ARGUMENT;
+  return copy_out(as_solution(solution)->res.x, out);
+}
+
+cuopt_int_t cuOptGetObjectiveValue(cuOptSolution solution, cuopt_float_t* out)
+{
+  if (!solution || !out) return CUOPT_INVALID_ARGUMENT;
+  *out = as_solution(solution)->res.primal_objective;
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptGetSolveTime(cuOptSolution solution, cuopt_float_t* out)
+{
+  if (!solution || !out) return CUOPT_INVALID_ARGUMENT;
+  *out = as_solution(solution)->res.solve_time;
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptGetDualSolution(cuOptSolution solution, cuopt_float_t* out)
+{
+  if (!solution || !out) return CUOPT_INVALID_ARGUMENT;
+  return copy_out(as_solution(solution)->res.y, out);
+}
+
+cuopt_int_t cuOptGetDualObjectiveValue(cuOptSolution solution, cuopt_float_t* out)
+{
+  if (!solution || !out) return CUOPT_INVALID_ARGUMENT;
+  *out = as_solution(solution)->res.dual_objective;
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptGetReducedCosts(cuOptSolution solution, cuopt_float_t* out)
+{
+  if (!solution || !out) return CUOPT_INVALID_ARGUMENT;
+  return copy_out(as_solution(solution)->res.rc, out);
+}
+
+// ---- extensions -------------------------------------------------------------
+
+cuopt_int_t cuOptAmdGetTerminationInfo(cuOptSolution solution, cuopt_float_t* info, cuopt_int_t n)
+{
+  if (!solution || !info || n < 0) return CUOPT_INVALID_ARGUMENT;
+  const SolveResult& R = as_solution(solution)->res;
+  for (int i = 0; i < n && i < CUOPT_AMD_INFO_COUNT; ++i) info[i] = R.info[i];
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptAmdGetTrace(cuOptSolution solution, cuopt_float_t* rows, cuopt_int_t cap, cuopt_int_t* len)
+{
+  if (!solution || !len || cap < 0 || (cap > 0 && !rows)) return CUOPT_INVALID_ARGUMENT;
+  const SolveResult& R = as_solution(solution)->res;
+  const int nrec       = (int)(R.trace.size() / CUOPT_AMD_TRACE_WIDTH);
+  *len                 = nrec;
+  const int ncopy      = nrec < cap ? nrec : cap;
+  if (ncopy > 0) std::memcpy(rows, R.trace.data(), (size_t)ncopy * CUOPT_AMD_TRACE_WIDTH * sizeof(double));
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptAmdSpmv(cuopt_int_t m,
+                         cuopt_int_t n,
+                         const cuopt_int_t* ptr,
+                         const cuopt_int_t* idx,
+                         const cuopt_float_t* val,
+                         const cuopt_float_t* x,
+                         cuopt_float_t* y,
+                         cuopt_int_t precision)
+{
+  if (m < 0 || n < 0 || !ptr || !idx || !val || !x || !y) return CUOPT_INVALID_ARGUMENT;
+  if (precision != CUOPT_AMD_PRECISION_FP64 && precision != CUOPT_AMD_PRECISION_FP32_MIXED)
+    return CUOPT_INVALID_ARGUMENT;
+  if (ptr[0] != 0) return CUOPT_INVALID_ARGUMENT;
+  for (int i = 0; i < m; ++i)
+    if (ptr[i + 1] < ptr[i]) return CUOPT_INVALID_ARGUMENT;
+  for (int k = 0; k < ptr[m]; ++k)
+    if (idx[k] < 0 || idx[k] >= n) return CUOPT_INVALID_ARGUMENT;
+  try {
+    tiled_spmv_host(m, n, ptr, idx, val, x, y, precision);
+  } catch (const std::bad_alloc&) {
+    return CUOPT_OUT_OF_MEMORY;
+  } catch (...) {
+    return CUOPT_RUNTIME_ERROR;
+  }
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptAmdDeviceCount(void) { return device_count(); }
+
+}  // extern "C"

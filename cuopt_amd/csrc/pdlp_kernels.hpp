@@ -1,0 +1,330 @@
+// pdlp_kernels.hpp -- fused epilogues and elementwise kernels of the PDLP hot loop.
+//
+// V is the iterate precision (double, or float for the FP32-mixed path).  The
+// step epilogues restate, per row, the reference's projection functors
+// (cpp/src/pdlp/pdhg.cu:431-545) fused with the Halpern update
+// (cpp/src/pdlp/pdlp.cu:2505-2563) and, on the steps that need them, the
+// fixed-point-error terms (cpp/src/pdlp/pdlp.cu:1748-1855).  Comparisons are
+// written as the std::min / std::max of the FP64 oracle so both pick the same
+// operand.
+#pragma once
+
+#include "spmv_tiles.hpp"
+
+namespace cuopt_amd {
+
+template <typename V>
+struct Vec2;
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+
+// std::min / std::max as the oracle evaluates them.
+template <typename T>
+__device__ inline T mn(T a, T b)
+{
+  return (b < a) ? b : a;
+}
+template <typename T>
+__device__ inline T mx(T a, T b)
+{
+  return (a < b) ? b : a;
+}
+__device__ inline double fin0(double v) { return isfinite(v) ? v : 0.0; }
+
+template <typename V>
+struct AccFor;
+template <>
+struct AccFor<double> {
+  using type = AccSum64;
+};
+template <>
+struct AccFor<float> {
+  using type = AccComp32;
+};
+
+// Scalars read by the step kernels from device memory (set between blocks).
+struct StepScalars {
+  double tau;        // primal step size eta / omega
+  double sigma;      // dual step size eta * omega
+  long long kr_base; // iterations since the last restart at the block start
+  long long pad;
+};
+
+template <typename V>
+__device__ inline V halpern_weight(const StepScalars* sc, int j)
+{
+  const long long kr = sc->kr_base + j;
+  return (V)((double)(kr + 1) / (double)(kr + 2));
+}
+
+// ---- primal step over rows of A~^T (columns j): AtY, projection, reflection, Halpern.
+template <typename V, bool MAJOR, bool FPE>
+struct PrimalStepEpi {
+  using Mat = V;
+  using Acc = typename AccFor<V>::type;
+  using V2  = typename Vec2<V>::type;
+  static constexpr int NRED = 0;
+  const V* y;  // gathered
+  const V* x;
+  const V* c;
+  const V2* bnd;
+  const V* x0;
+  V* xbar;
+  V* xnext;
+  V* xhat;
+  V* slack;
+  V* aty_out;
+  const StepScalars* sc;
+  int j;
+  __device__ bool skip() const { return false; }
+  __device__ typename Acc::Elem product(V v, int col, int) const { return Acc::prod(v, y[col]); }
+  __device__ void row(int r, V aty, double (&)[kMaxRed]) const
+  {
+    const V tau     = (V)sc->tau;
+    const V w       = halpern_weight<V>(sc, j);
+    const V xr      = x[r];
+    const V next    = xr - tau * (c[r] - aty);
+    const V2 b      = bnd[r];
+    const V clamped = mx(mn(next, b.y), b.x);
+    const V xb      = V(2) * clamped - xr;
+    xbar[r]         = xb;
+    if constexpr (MAJOR) {
+      xhat[r]  = clamped;
+      slack[r] = (clamped - next) / tau;
+    }
+    if constexpr (FPE) aty_out[r] = aty;
+    xnext[r] = w * xb + (V(1) - w) * x0[r];
+  }
+};
+
+// ---- dual step over rows of A~ (rows i): A x-bar, projection, reflection, Halpern.
+template <typename V, bool MAJOR, bool FPE>
+struct DualStepEpi {
+  using Mat = V;
+  using Acc = typename AccFor<V>::type;
+  using V2  = typename Vec2<V>::type;
+  static constexpr int NRED = FPE ? 1 : 0;
+  const V* xbar;  // gathered
+  const V* y;
+  const V2* cb;  // (lower, upper) of the scaled constraints
+  const V* y0;
+  V* ynext;
+  V* yhat;
+  V* ybar;
+  const StepScalars* sc;
+  int j;
+  __device__ bool skip() const { return false; }
+  __device__ typename Acc::Elem product(V v, int col, int) const { return Acc::prod(v, xbar[col]); }
+  __device__ void row(int r, V ax, double (&red)[kMaxRed]) const
+  {
+    const V sigma = (V)sc->sigma;
+    const V w     = halpern_weight<V>(sc, j);
+    const V yi    = y[r];
+    const V t     = yi / sigma - ax;
+    const V2 b    = cb[r];
+    const V proj  = mx(-b.y, mn(t, -b.x));
+    const V nd    = (t - proj) * sigma;
+    const V yb    = V(2) * nd - yi;
+    if constexpr (MAJOR) yhat[r] = nd;
+    if constexpr (FPE) {
+      ybar[r]        = yb;
+      const double d = (double)yb - (double)yi;
+      red[0] += d * d;
+    }
+    ynext[r] = w * yb + (V(1) - w) * y0[r];
+  }
+};
+
+// ---- fixed-point error: A~^T y-bar against the stored A~^T y, over columns j.
+// red[0] = (A~^T ybar - A~^T y) . (xbar - x), red[1] = ||xbar - x||^2.
+template <typename V>
+struct FpeEpi {
+  using Mat = V;
+  using Acc = typename AccFor<V>::type;
+  static constexpr int NRED = 2;
+  const V* ybar;  // gathered
+  const V* aty;
+  const V* xbar;
+  const V* x;
+  __device__ bool skip() const { return false; }
+  __device__ typename Acc::Elem product(V v, int col, int) const { return Acc::prod(v, ybar[col]); }
+  __device__ void row(int r, V atyb, double (&red)[kMaxRed]) const
+  {
+    const double dx = (double)xbar[r] - (double)x[r];
+    const double t  = (double)atyb - (double)aty[r];
+    red[0] += t * dx;
+    red[1] += dx * dx;
+  }
+};
+
+// ---- termination evaluation on the unscaled problem (FP64 matrix).
+// Rows i of A: gathers unscaled x-hat on the fly, owns y-hat.
+// red: 0 ||Ax - clamp(Ax)||^2, 1 sum of primal slack terms, 2 ||y0 - yhat||^2 (rescaled)
+template <typename V>
+struct KktRowsEpi {
+  using Mat = double;
+  using Acc = AccSum64;
+  static constexpr int NRED = 3;
+  const V* xhat;     // gathered (scaled)
+  const double* Dc;  // gathered scaling
+  double inv_beta;
+  V* yhat;
+  const double* Dr;
+  double inv_gamma, gamma;
+  const double2* cb_orig;  // unscaled constraint bounds
+  const V* y0;
+  double* yu_out;
+  __device__ bool skip() const { return false; }
+  __device__ double product(double v, int col, int) const
+  {
+    return v * (((double)xhat[col] * Dc[col]) * inv_beta);
+  }
+  __device__ void row(int r, double ax, double (&red)[kMaxRed]) const
+  {
+    const double dr = Dr[r];
+    const double yu = ((double)yhat[r] * dr) * inv_gamma;
+    const double2 b = cb_orig[r];
+    const double cl = mx(b.x, mn(ax, b.y));
+    const double rp = ax - cl;
+    red[0] += rp * rp;
+    red[1] += mx(yu, 0.0) * fin0(b.x) + mn(yu, 0.0) * fin0(b.y);
+    yu_out[r]     = yu;
+    double ys     = dr != 0.0 ? yu / dr : yu;
+    ys            = ys * gamma;
+    const V ynew  = (V)ys;
+    yhat[r]       = ynew;
+    const double d = (double)y0[r] - 1.0 * (double)ynew;
+    red[2] += d * d;
+  }
+};
+
+// Rows j of A^T: gathers unscaled y (written by KktRowsEpi), owns x-hat and slack.
+// red: 0 ||c - A^T y - s||^2, 1 x.c, 2 s.x, 3 ||x0 - xhat||^2 (rescaled)
+template <typename V>
+struct KktColsEpi {
+  using Mat = double;
+  using Acc = AccSum64;
+  static constexpr int NRED = 4;
+  const double* yu;  // gathered
+  V* xhat;
+  V* slack;
+  const double* Dc;
+  double inv_beta, beta, inv_gamma, gamma;
+  const double* c_orig;
+  const V* x0;
+  double* xu_out;
+  double* su_out;
+  __device__ bool skip() const { return false; }
+  __device__ double product(double v, int col, int) const { return v * yu[col]; }
+  __device__ void row(int r, double aty, double (&red)[kMaxRed]) const
+  {
+    const double dc = Dc[r];
+    const double xu = ((double)xhat[r] * dc) * inv_beta;
+    double su       = dc != 0.0 ? (double)slack[r] / dc : (double)slack[r];
+    su              = su * inv_gamma;
+    const double cj = c_orig[r];
+    const double g  = cj - aty;
+    const double rd = g - su;
+    red[0] += rd * rd;
+    red[1] += xu * cj;
+    red[2] += su * xu;
+    xu_out[r] = xu;
+    su_out[r] = su;
+    double xs = dc != 0.0 ? xu / dc : xu;
+    xs        = xs * beta;
+    double ss = su * dc;
+    ss        = ss * gamma;
+    const V xnew = (V)xs;
+    xhat[r]      = xnew;
+    slack[r]     = (V)ss;
+    const double d = (double)x0[r] - 1.0 * (double)xnew;
+    red[3] += d * d;
+  }
+};
+
+// ---- power iteration on A~ A~^T (FP64).  State in PowerState (device).
+struct PowerState {
+  double nrm;   // ||z|| of the current z
+  double s2;    // sigma_max^2 estimate (q . A A^T q)
+  double res;   // ||A A^T q - s2 q||
+  double done;  // 1 once res < tol
+  double iters;
+  double pad[3];
+};
+
+struct PowerAtEpi {  // atq = A^T (z / ||z||)
+  using Mat = double;
+  using Acc = AccSum64;
+  static constexpr int NRED = 0;
+  const double* z;
+  const PowerState* st;
+  double* atq;
+  __device__ bool skip() const { return st->done != 0.0; }
+  __device__ double product(double v, int col, int) const { return v * (z[col] / st->nrm); }
+  __device__ void row(int r, double s, double (&)[kMaxRed]) const { atq[r] = s; }
+};
+
+struct PowerAEpi {  // znew = A atq, red[0] = q . znew
+  using Mat = double;
+  using Acc = AccSum64;
+  static constexpr int NRED = 1;
+  const double* atq;
+  const double* zold;
+  const PowerState* st;
+  double* znew;
+  __device__ bool skip() const { return st->done != 0.0; }
+  __device__ double product(double v, int col, int) const { return v * atq[col]; }
+  __device__ void row(int r, double s, double (&red)[kMaxRed]) const
+  {
+    znew[r]        = s;
+    const double q = zold[r] / st->nrm;
+    red[0] += q * s;
+  }
+};
+
+// ---- Ruiz / Pock-Chambolle row statistics of |(a * D_r) * D_c|.
+// For A the gathered index is the column and rowid[k] the row; for A^T the
+// roles swap (rows of A^T are columns of A), selected by TRANSPOSED.
+template <bool TRANSPOSED, bool MAXNORM>
+struct ScaleStatEpi {
+  using Mat = double;
+  using Acc = typename std::conditional<MAXNORM, AccMax64, AccSum64>::type;
+  static constexpr int NRED = 0;
+  const int* rowid;
+  const double* Dr;
+  const double* Dc;
+  double alpha_pow;  // exponent applied to |a~| for sums (1 skips pow)
+  double* out;
+  __device__ bool skip() const { return false; }
+  __device__ double product(double v, int col, int k) const
+  {
+    const int rr = rowid[k];
+    const double dr = TRANSPOSED ? Dr[col] : Dr[rr];
+    const double dc = TRANSPOSED ? Dc[rr] : Dc[col];
+    const double a  = fabs((v * dr) * dc);
+    if constexpr (MAXNORM) return a;
+    return alpha_pow == 1.0 ? a : pow(a, alpha_pow);
+  }
+  __device__ void row(int r, double s, double (&)[kMaxRed]) const { out[r] = s; }
+};
+
+// ---- plain y = A x (FP64 or compensated FP32) for the test entry point.
+template <typename V>
+struct PlainEpi {
+  using Mat = V;
+  using Acc = typename AccFor<V>::type;
+  static constexpr int NRED = 0;
+  const V* x;
+  double* y;
+  __device__ bool skip() const { return false; }
+  __device__ typename Acc::Elem product(V v, int col, int) const { return Acc::prod(v, x[col]); }
+  __device__ void row(int r, V s, double (&)[kMaxRed]) const { y[r] = (double)s; }
+};
+
+}  // namespace cuopt_amd

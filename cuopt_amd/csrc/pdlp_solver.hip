@@ -1,0 +1,1204 @@
+// pdlp_solver.hip -- MI355X PDLP (Stable3) LP solver: setup, PDHG loop, termination.
+//
+// Host orchestration follows pdlp_solver_t::run_solver (cpp/src/pdlp/pdlp.cu:2013-2458):
+//   problem_t construction   (mip_heuristics/problem/problem.cu:47-100) ....... host CSR/CSC
+//   Ruiz + Pock-Chambolle     (initial_scaling.cu:151-364) ................... tiled max / sum passes
+//   scale_problem + bound/objective rescaling (initial_scaling.cu:416-568) .. elementwise kernels
+//   power iteration step size (pdlp.cu:2587-2739) ............................ device loop, early exit
+//   PDHG reflected step + Halpern (pdhg.cu:785-1030, pdlp.cu:2505-2563) ...... 2 fused tiled kernels / iteration
+//   fixed-point error         (pdlp.cu:1748-1855) ............................. 1 tiled kernel on FPE steps
+//   termination on the unscaled problem (convergence_information.cu:326-810) 2 fused tiled kernels / check
+//   cuPDLPx restart + PID primal weight (pdlp_restart_strategy.cu:688-985) .. host scalar logic, copy kernels
+//
+// Iterations between two termination checks are issued back to back on one
+// stream with no host synchronisation; the host reads one small record per check.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <type_traits>
+
+#include "common.hpp"
+#include "lp_types.hpp"
+#include "pdlp_kernels.hpp"
+#include "tile_plan.hpp"
+
+namespace cuopt_amd {
+
+// ---------------------------------------------------------------------------
+// Generic kernels
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(1024) void reduce_partials_kernel(const double* __restrict__ part,
+                                                               int nblocks,
+                                                               int nred,
+                                                               double* __restrict__ out,
+                                                               const double* skip)
+{
+  __shared__ double lds[1024];
+  if (skip && *skip != 0.0) return;
+  for (int k = 0; k < nred; ++k) {
+    double s = 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) s += part[(size_t)b * nred + k];
+    lds[threadIdx.x] = s;
+    __syncthreads();
+    for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+      if ((int)threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) out[k] = lds[0];
+    __syncthreads();
+  }
+}
+
+template <class F>
+__global__ __launch_bounds__(kBlock) void elementwise_kernel(int n, double* __restrict__ red_part, F f)
+{
+  __shared__ double lds[kBlock / kWave][kMaxRed];
+  if (f.skip()) return;
+  double red[kMaxRed];
+#pragma unroll
+  for (int k = 0; k < kMaxRed; ++k) red[k] = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) f(i, red);
+  if constexpr (F::NRED > 0) block_sum_store<F::NRED>(red, red_part + (size_t)blockIdx.x * F::NRED, lds);
+}
+
+// Power iteration finalisers (one workgroup).
+__global__ __launch_bounds__(1024) void power_finalize_s2(const double* __restrict__ part, int nb, PowerState* st)
+{
+  __shared__ double lds[1024];
+  if (st->done != 0.0) return;
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) s += part[b];
+  lds[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) st->s2 = lds[0];
+}
+
+__global__ __launch_bounds__(1024) void power_finalize_res(const double* __restrict__ part,
+                                                           int nb,
+                                                           PowerState* st,
+                                                           double tol)
+{
+  __shared__ double lds[2][1024];
+  if (st->done != 0.0) return;
+  double s0 = 0.0, s1 = 0.0;
+  for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+    s0 += part[2 * (size_t)b];
+    s1 += part[2 * (size_t)b + 1];
+  }
+  lds[0][threadIdx.x] = s0;
+  lds[1][threadIdx.x] = s1;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) {
+      lds[0][threadIdx.x] += lds[0][threadIdx.x + off];
+      lds[1][threadIdx.x] += lds[1][threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const double res = sqrt(lds[0][0]);
+    st->res          = res;
+    st->iters += 1.0;
+    if (res < tol)
+      st->done = 1.0;
+    else
+      st->nrm = sqrt(lds[1][0]);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Elementwise functors
+// ---------------------------------------------------------------------------
+
+struct SumSqF {  // red0 = sum v^2
+  static constexpr int NRED = 1;
+  const double* v;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int i, double (&red)[kMaxRed]) const { red[0] += v[i] * v[i]; }
+};
+
+struct PowerResF {  // residual of the power step and ||znew||^2
+  static constexpr int NRED = 2;
+  const double* zold;
+  const double* znew;
+  const PowerState* st;
+  __device__ bool skip() const { return st->done != 0.0; }
+  __device__ void operator()(int i, double (&red)[kMaxRed]) const
+  {
+    const double q = zold[i] / st->nrm;
+    const double r = q * -st->s2 + znew[i];
+    red[0] += r * r;
+    red[1] += znew[i] * znew[i];
+  }
+};
+
+struct FillF {
+  static constexpr int NRED = 0;
+  double* v;
+  double val;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int i, double (&)[kMaxRed]) const { v[i] = val; }
+};
+
+struct DivSqrtF {  // a_divides_sqrt_b_bounded (utils.cuh:154-161)
+  static constexpr int NRED = 0;
+  double* d;
+  const double* v;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int i, double (&)[kMaxRed]) const
+  {
+    d[i] = v[i] > 0.0 ? d[i] / sqrt(v[i]) : d[i];
+  }
+};
+
+struct ScaleValsF {  // a~ = (a * first) * second, in the reference operand order
+  static constexpr int NRED = 0;
+  const double* vals;
+  const int* rowid;
+  const int* colidx;
+  const double* first;   // indexed by row of the stored matrix
+  const double* second;  // indexed by column of the stored matrix
+  double* out;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int k, double (&)[kMaxRed]) const
+  {
+    out[k] = vals[k] * first[rowid[k]] * second[colidx[k]];
+  }
+};
+
+// Column vectors: c~ = c * Dc, bounds / Dc (divide_check_zero); red0 = ||c~||^2;
+// red1 = ||c||^2 of the unscaled objective.
+struct ScaleColsF {
+  static constexpr int NRED = 2;
+  const double* c;
+  const double* lb;
+  const double* ub;
+  const double* Dc;
+  double* cs;
+  double* lbs;
+  double* ubs;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int j, double (&red)[kMaxRed]) const
+  {
+    const double d = Dc[j];
+    const double v = c[j] * d;
+    cs[j]          = v;
+    if (d == 0.0) {
+      lbs[j] = 0.0;
+      ubs[j] = 0.0;
+    } else {
+      lbs[j] = lb[j] / d;
+      ubs[j] = ub[j] / d;
+    }
+    red[0] += v * v * 1.0;
+    red[1] += c[j] * c[j];
+  }
+};
+
+__device__ inline double bound_sq_term(double l, double u)
+{
+  double t = 0.0;
+  if (isfinite(l) && l != u) t += l * l;
+  if (isfinite(u)) t += u * u;
+  return t;
+}
+
+// Row vectors: scaled constraint bounds; red0 = scaled bound norm^2, red1 = unscaled.
+struct ScaleRowsF {
+  static constexpr int NRED = 2;
+  const double* lc;
+  const double* uc;
+  const double* Dr;
+  double* lcs;
+  double* ucs;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int i, double (&red)[kMaxRed]) const
+  {
+    const double l = lc[i] * Dr[i], u = uc[i] * Dr[i];
+    lcs[i]         = l;
+    ucs[i]         = u;
+    red[0] += bound_sq_term(l, u);
+    red[1] += bound_sq_term(lc[i], uc[i]);
+  }
+};
+
+// Multiply bounds by beta and c by gamma, then emit the working-precision copies.
+template <typename V>
+struct FinishColsF {
+  using V2 = typename Vec2<V>::type;
+  static constexpr int NRED = 0;
+  double* cs;
+  double* lbs;
+  double* ubs;
+  double beta, gamma;
+  bool rescale;
+  V* c_w;
+  V2* bnd_w;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int j, double (&)[kMaxRed]) const
+  {
+    double l = lbs[j], u = ubs[j], c = cs[j];
+    if (rescale) {
+      l = l * beta;
+      u = u * beta;
+      c = c * gamma;
+    }
+    lbs[j] = l;
+    ubs[j] = u;
+    cs[j]  = c;
+    c_w[j] = (V)c;
+    V2 b;
+    b.x      = (V)l;
+    b.y      = (V)u;
+    bnd_w[j] = b;
+  }
+};
+
+template <typename V>
+struct FinishRowsF {
+  using V2 = typename Vec2<V>::type;
+  static constexpr int NRED = 0;
+  double* lcs;
+  double* ucs;
+  double beta;
+  bool rescale;
+  V2* cb_w;
+  double2* cb_orig;
+  const double* lc;
+  const double* uc;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int i, double (&)[kMaxRed]) const
+  {
+    double l = lcs[i], u = ucs[i];
+    if (rescale) {
+      l = l * beta;
+      u = u * beta;
+    }
+    lcs[i] = l;
+    ucs[i] = u;
+    V2 b;
+    b.x     = (V)l;
+    b.y     = (V)u;
+    cb_w[i] = b;
+    cb_orig[i] = make_double2(lc[i], uc[i]);
+  }
+};
+
+template <typename V>
+struct ConvertF {
+  static constexpr int NRED = 0;
+  const double* in;
+  V* out;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int i, double (&)[kMaxRed]) const { out[i] = (V)in[i]; }
+};
+
+// Initial iterate: scale (scale_solutions, initial_scaling.cu:570-650) then
+// seed current, x-hat and anchor (update_primal_dual_solutions, pdlp.cu:1236-1393).
+template <typename V, bool PRIMAL>
+struct InitIterateF {
+  using V2 = typename Vec2<V>::type;
+  static constexpr int NRED = 0;
+  const double* init;  // may be null (zeros)
+  const double* D;
+  double factor;
+  bool rescale;
+  const V2* bnd;  // primal only: project the current iterate
+  V* cur;
+  V* hat;
+  V* anchor;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int i, double (&)[kMaxRed]) const
+  {
+    double v = 0.0;
+    if (init) {
+      v = init[i];
+      v = D[i] != 0.0 ? v / D[i] : v;
+      if (rescale) v = v * factor;
+    }
+    const V vw = (V)v;
+    hat[i]     = vw;
+    anchor[i]  = vw;
+    if constexpr (PRIMAL) {
+      const V2 b = bnd[i];
+      cur[i]     = mn(mx(vw, b.x), b.y);  // clamp functor: min(max(v, l), u)
+    } else {
+      cur[i] = vw;
+    }
+  }
+};
+
+template <typename V>
+struct ZeroF {
+  static constexpr int NRED = 0;
+  V* v;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int i, double (&)[kMaxRed]) const { v[i] = V(0); }
+};
+
+template <typename V>
+struct RestartCopyF {  // anchor = current = hat (cupdlpx_restart, pdlp_restart_strategy.cu:955-970)
+  static constexpr int NRED = 0;
+  const V* hat;
+  V* cur;
+  V* anchor;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int i, double (&)[kMaxRed]) const
+  {
+    const V v = hat[i];
+    anchor[i] = v;
+    cur[i]    = v;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Host helpers
+// ---------------------------------------------------------------------------
+
+namespace {
+
+inline int ew_grid(int n)
+{
+  const int b = (n + kBlock - 1) / kBlock;
+  return std::max(1, std::min(b, 2048));
+}
+
+template <class F>
+void launch_ew(int n, double* red, F f, hipStream_t s)
+{
+  if (n <= 0) return;
+  hipLaunchKernelGGL(elementwise_kernel<F>, dim3(ew_grid(n)), dim3(kBlock), 0, s, n, red, f);
+  CUOPT_HIP_CHECK(hipGetLastError());
+}
+
+struct PlanBufs {
+  DevBuf<Tile> tiles;
+  DevBuf<int> meta;
+  DevBuf<LongRow> longrows;
+  DevBuf<uint64_t> long_part;
+  DevBuf<unsigned> long_cnt;
+  int ntiles = 0;
+  PlanDev dev() const { return PlanDev{tiles.get(), meta.get(), longrows.get(), ntiles}; }
+};
+
+struct MatStruct {  // CSR structure shared by the FP64 and working-precision values
+  int nrows = 0, ncols = 0, nnz = 0;
+  std::vector<int> h_ptr;
+  DevBuf<int> ptr, idx, rowid;
+  PlanBufs plan;
+};
+
+void build_plan(MatStruct& M, hipStream_t s)
+{
+  TilePlan p = plan_tiles(M.h_ptr.data(), M.nrows, kTileNnz, kMaxRowsPerTile, kBlock);
+  M.plan.ntiles = (int)p.tiles.size();
+  M.plan.tiles.resize(p.tiles.size());
+  M.plan.tiles.upload(p.tiles, s);
+  M.plan.meta.resize(p.meta.size());
+  M.plan.meta.upload(p.meta, s);
+  M.plan.longrows.resize(std::max<size_t>(1, p.longrows.size()));
+  if (!p.longrows.empty()) M.plan.longrows.upload(p.longrows, s);
+  M.plan.long_part.resize(std::max<size_t>(1, p.tiles.size()));
+  M.plan.long_cnt.resize(std::max<size_t>(1, p.longrows.size()));
+  M.plan.long_cnt.zero(s);
+}
+
+template <class Epi>
+void launch_tiled(const MatStruct& M, const typename Epi::Mat* vals, double* red, Epi epi, hipStream_t s)
+{
+  if (M.plan.ntiles == 0) return;
+  hipLaunchKernelGGL(tiled_spmv_kernel<Epi>, dim3(M.plan.ntiles), dim3(kBlock), 0, s, M.ptr.get(),
+                     M.idx.get(), vals, M.plan.dev(), M.plan.long_part.get(), M.plan.long_cnt.get(), red, epi);
+  CUOPT_HIP_CHECK(hipGetLastError());
+}
+
+void launch_reduce(const double* part, int nblocks, int nred, double* out, hipStream_t s,
+                   const double* skip = nullptr)
+{
+  hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(1024), 0, s, part, nblocks, nred, out, skip);
+  CUOPT_HIP_CHECK(hipGetLastError());
+}
+
+// conditional_major (cpp/src/pdlp/utils.cuh:62-81)
+long long conditional_major(long long k)
+{
+  long long step = 10, threshold = 1000;
+  while (k >= threshold) {
+    step *= 10;
+    threshold *= 10;
+  }
+  return step;
+}
+
+constexpr int kMajor              = 200;    // major_iteration (Stable3)
+constexpr double kArtificial      = 0.36;   // default_artificial_restart_threshold
+constexpr double kSufficient      = 0.2;    // sufficient_reduction_for_restart
+constexpr double kNecessary       = 0.8;    // necessary_reduction_for_restart
+constexpr double kKp = 0.99, kKi = 0.01, kKd = 0.0, kISmooth = 0.3;  // restart PID (pdlp_hyper_params.cuh:51-54)
+constexpr int kRuizIters          = 10;
+constexpr double kPcAlpha         = 1.0;
+
+inline bool is_check(long long k)
+{
+  const bool major = ((k % kMajor) == 0 && k > 0) || k <= 0;
+  return major || (k % conditional_major(k)) == 0;
+}
+
+// Slots of the device scalar record.
+enum Slot {
+  S_KKT_RP2 = 0, S_KKT_PSUM, S_KKT_DD,           // from KktRowsEpi
+  S_KKT_RD2, S_KKT_XC, S_KKT_SX, S_KKT_DP,       // from KktColsEpi
+  S_FPE_NY, S_FPE_I, S_FPE_NX,                   // periodic fixed-point error terms
+  S_FPE0_NY, S_FPE0_I, S_FPE0_NX,                // post-restart fixed-point error terms
+  S_SETUP0, S_SETUP1,                            // setup reductions
+  S_COUNT
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// The solver
+// ---------------------------------------------------------------------------
+
+template <typename V>
+class PdlpSolver {
+  using V2 = typename Vec2<V>::type;
+
+ public:
+  PdlpSolver(const LpHost& lp,
+             const std::vector<double>& lc,
+             const std::vector<double>& uc,
+             const std::vector<double>& lb,
+             const std::vector<double>& ub,
+             const PdlpSettings& st)
+    : lp_(lp), lc_(lc), uc_(uc), lb_(lb), ub_(ub), st_(st), m_(lp.m), n_(lp.n)
+  {
+  }
+
+  ~PdlpSolver()
+  {
+    if (stream_) (void)hipStreamDestroy(stream_);
+    if (h_scal_) (void)hipHostFree(h_scal_);
+    if (h_step_) (void)hipHostFree(h_step_);
+  }
+
+  SolveResult run();
+
+ private:
+  using clock = std::chrono::steady_clock;
+  double elapsed() const { return std::chrono::duration<double>(clock::now() - t0_).count(); }
+
+  void build_structures();
+  void setup_scaling();
+  void power_iteration();
+  void init_iterates();
+  void read_scalars()
+  {
+    CUOPT_HIP_CHECK(hipMemcpyAsync(h_scal_, scal_.get(), S_COUNT * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
+  }
+  void upload_step_scalars()
+  {
+    // Pinned staging record: it is rewritten only after the next check's
+    // synchronising read, by which time this copy has executed.
+    *h_step_ = StepScalars{tau_, sigma_, kr_, 0};
+    CUOPT_HIP_CHECK(hipMemcpyAsync(step_sc_.get(), h_step_, sizeof(StepScalars), hipMemcpyHostToDevice, stream_));
+  }
+  void evaluate_kkt();
+  void launch_step(int j, bool major, bool fpe, double* ny_slot, double* fpe_slot);
+  void finish(SolveResult& R, int status);
+
+  const LpHost& lp_;
+  const std::vector<double>&lc_, &uc_, &lb_, &ub_;
+  const PdlpSettings& st_;
+  const int m_, n_;
+  clock::time_point t0_;
+  hipStream_t stream_ = nullptr;
+  double* h_scal_     = nullptr;
+  StepScalars* h_step_ = nullptr;
+
+  // host CSR (row-sorted) and its transpose
+  std::vector<int> a_idx_, at_idx_;
+  std::vector<double> a_val_, at_val_;
+  MatStruct A_, AT_;
+  DevBuf<double> a_orig_, at_orig_, a_s_, at_s_;
+  DevBuf<V> a_w_, at_w_;
+
+  DevBuf<double> Dr_, Dc_, rowv_, colv_;
+  DevBuf<double> c_orig_, lb_orig_, ub_orig_, lc_orig_, uc_orig_;
+  DevBuf<double> cs_, lbs_, ubs_, lcs_, ucs_;
+  DevBuf<double2> cb_orig_;
+  DevBuf<V> c_w_;
+  DevBuf<V2> bnd_w_, cb_w_;
+
+  DevBuf<V> x_[2], y_[2], x0_, y0_, xbar_, ybar_, xhat_, yhat_, slack_, aty_;
+  DevBuf<double> xu_, yu_, su_;
+  DevBuf<double> red_, scal_;
+  DevBuf<StepScalars> step_sc_;
+  int cur_ = 0;
+
+  double beta_ = 1.0, gamma_ = 1.0, b_norm_ = 0.0, c_norm_ = 0.0;
+  double eta_ = 1.0, omega_ = 1.0, best_omega_ = 1.0, tau_ = 1.0, sigma_ = 1.0;
+  double sigma_max_sq_ = 0.0;
+  int power_iters_     = 0;
+  long long kr_        = 0;
+  double obj_sf_ = 1.0, obj_off_ = 0.0;
+
+  // last KKT evaluation
+  double l2p_ = 0, l2d_ = 0, pobj_ = 0, dobj_ = 0, gap_ = 0, absobj_ = 0, dp_ = 0, dd_ = 0;
+};
+
+template <typename V>
+void PdlpSolver<V>::build_structures()
+{
+  const int nnz = lp_.rowptr[m_];
+  // Row-sorted CSR (problem_t keeps sorted column indices).
+  a_idx_.assign(lp_.colidx.begin(), lp_.colidx.begin() + nnz);
+  a_val_.assign(lp_.vals.begin(), lp_.vals.begin() + nnz);
+  for (int i = 0; i < m_; ++i) {
+    const int b = lp_.rowptr[i], e = lp_.rowptr[i + 1];
+    bool sorted = true;
+    for (int k = b + 1; k < e && sorted; ++k) sorted = a_idx_[k - 1] <= a_idx_[k];
+    if (!sorted) {
+      std::vector<std::pair<int, double>> row;
+      for (int k = b; k < e; ++k) row.emplace_back(a_idx_[k], a_val_[k]);
+      std::stable_sort(row.begin(), row.end(), [](const auto& p, const auto& q) { return p.first < q.first; });
+      for (int k = b; k < e; ++k) {
+        a_idx_[k] = row[k - b].first;
+        a_val_[k] = row[k - b].second;
+      }
+    }
+  }
+  // Stable counting-sort transpose (compute_transpose_of_problem, problem.cu:472-489).
+  std::vector<int> tptr(n_ + 1, 0);
+  for (int k = 0; k < nnz; ++k) tptr[a_idx_[k] + 1]++;
+  for (int j = 0; j < n_; ++j) tptr[j + 1] += tptr[j];
+  at_idx_.resize(nnz);
+  at_val_.resize(nnz);
+  std::vector<int> a_rowid(nnz), at_rowid(nnz);
+  {
+    std::vector<int> cur(tptr.begin(), tptr.end() - 1);
+    for (int i = 0; i < m_; ++i)
+      for (int k = lp_.rowptr[i]; k < lp_.rowptr[i + 1]; ++k) {
+        a_rowid[k]   = i;
+        const int d  = cur[a_idx_[k]]++;
+        at_idx_[d]   = i;
+        at_val_[d]   = a_val_[k];
+        at_rowid[d]  = a_idx_[k];
+      }
+  }
+  A_.nrows = m_;
+  A_.ncols = n_;
+  A_.nnz   = nnz;
+  A_.h_ptr.assign(lp_.rowptr.begin(), lp_.rowptr.begin() + m_ + 1);
+  AT_.nrows = n_;
+  AT_.ncols = m_;
+  AT_.nnz   = nnz;
+  AT_.h_ptr = tptr;
+  for (MatStruct* M : {&A_, &AT_}) {
+    M->ptr.resize(M->nrows + 1);
+    M->ptr.upload(M->h_ptr, stream_);
+    M->idx.resize(std::max(nnz, 1));
+    M->rowid.resize(std::max(nnz, 1));
+    build_plan(*M, stream_);
+  }
+  A_.idx.upload(a_idx_.data(), nnz, stream_);
+  AT_.idx.upload(at_idx_.data(), nnz, stream_);
+  A_.rowid.upload(a_rowid.data(), nnz, stream_);
+  AT_.rowid.upload(at_rowid.data(), nnz, stream_);
+  a_orig_.resize(std::max(nnz, 1));
+  at_orig_.resize(std::max(nnz, 1));
+  a_orig_.upload(a_val_.data(), nnz, stream_);
+  at_orig_.upload(at_val_.data(), nnz, stream_);
+  CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));  // host vectors above are temporaries
+}
+
+template <typename V>
+void PdlpSolver<V>::setup_scaling()
+{
+  const int nnz = A_.nnz;
+  const int red_blocks = std::max({A_.plan.ntiles, AT_.plan.ntiles, ew_grid(std::max(m_, n_)),
+                                   ew_grid(std::max(nnz, 1))});
+  red_.resize((size_t)red_blocks * kMaxRed);
+  scal_.resize(S_COUNT);
+  scal_.zero(stream_);
+  Dr_.resize(m_);
+  Dc_.resize(n_);
+  rowv_.resize(m_);
+  colv_.resize(n_);
+  launch_ew(m_, nullptr, FillF{Dr_.get(), 1.0}, stream_);
+  launch_ew(n_, nullptr, FillF{Dc_.get(), 1.0}, stream_);
+
+  // Ruiz infinity-norm equilibration (initial_scaling.cu:181-220).
+  for (int it = 0; it < kRuizIters; ++it) {
+    launch_tiled(A_, a_orig_.get(), nullptr,
+                 ScaleStatEpi<false, true>{A_.rowid.get(), Dr_.get(), Dc_.get(), 1.0, rowv_.get()}, stream_);
+    launch_tiled(AT_, at_orig_.get(), nullptr,
+                 ScaleStatEpi<true, true>{AT_.rowid.get(), Dr_.get(), Dc_.get(), 1.0, colv_.get()}, stream_);
+    launch_ew(m_, nullptr, DivSqrtF{Dr_.get(), rowv_.get()}, stream_);
+    launch_ew(n_, nullptr, DivSqrtF{Dc_.get(), colv_.get()}, stream_);
+  }
+  // Pock-Chambolle, alpha = 1 (initial_scaling.cu:233-364).
+  launch_tiled(A_, a_orig_.get(), nullptr,
+               ScaleStatEpi<false, false>{A_.rowid.get(), Dr_.get(), Dc_.get(), kPcAlpha, rowv_.get()}, stream_);
+  launch_tiled(AT_, at_orig_.get(), nullptr,
+               ScaleStatEpi<true, false>{AT_.rowid.get(), Dr_.get(), Dc_.get(), 2.0 - kPcAlpha, colv_.get()},
+               stream_);
+  launch_ew(m_, nullptr, DivSqrtF{Dr_.get(), rowv_.get()}, stream_);
+  launch_ew(n_, nullptr, DivSqrtF{Dc_.get(), colv_.get()}, stream_);
+
+  // Scaled matrix values, FP64 (scale_problem_kernel / scale_transposed_problem_kernel).
+  a_s_.resize(std::max(nnz, 1));
+  at_s_.resize(std::max(nnz, 1));
+  launch_ew(nnz, nullptr, ScaleValsF{a_orig_.get(), A_.rowid.get(), A_.idx.get(), Dr_.get(), Dc_.get(), a_s_.get()},
+            stream_);
+  launch_ew(nnz, nullptr,
+            ScaleValsF{at_orig_.get(), AT_.rowid.get(), AT_.idx.get(), Dc_.get(), Dr_.get(), at_s_.get()}, stream_);
+
+  // Vectors.
+  c_orig_.resize(n_);
+  lb_orig_.resize(n_);
+  ub_orig_.resize(n_);
+  lc_orig_.resize(m_);
+  uc_orig_.resize(m_);
+  std::vector<double> cmin(lp_.c.begin(), lp_.c.end());
+  if (lp_.maximize)
+    for (double& v : cmin) v = -v;  // convert_to_maximization_problem (problem_helpers.cuh:132-151)
+  c_orig_.upload(cmin, stream_);
+  lb_orig_.upload(lb_, stream_);
+  ub_orig_.upload(ub_, stream_);
+  lc_orig_.upload(lc_, stream_);
+  uc_orig_.upload(uc_, stream_);
+  cs_.resize(n_);
+  lbs_.resize(n_);
+  ubs_.resize(n_);
+  lcs_.resize(m_);
+  ucs_.resize(m_);
+  launch_ew(n_, red_.get(),
+            ScaleColsF{c_orig_.get(), lb_orig_.get(), ub_orig_.get(), Dc_.get(), cs_.get(), lbs_.get(), ubs_.get()},
+            stream_);
+  launch_reduce(red_.get(), ew_grid(n_), 2, scal_.get() + S_SETUP0, stream_);
+  CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
+  read_scalars();
+  const double cs2 = h_scal_[S_SETUP0], c2 = h_scal_[S_SETUP1];
+  launch_ew(m_, red_.get(), ScaleRowsF{lc_orig_.get(), uc_orig_.get(), Dr_.get(), lcs_.get(), ucs_.get()}, stream_);
+  launch_reduce(red_.get(), ew_grid(m_), 2, scal_.get() + S_SETUP0, stream_);
+  read_scalars();
+  const double bs2 = h_scal_[S_SETUP0], b2 = h_scal_[S_SETUP1];
+  // bound_objective_rescaling (initial_scaling.cu:94-149)
+  beta_   = 1.0 / (std::sqrt(bs2) + 1.0);
+  gamma_  = 1.0 / (std::sqrt(cs2) + 1.0);
+  b_norm_ = std::sqrt(b2);  // compute_sum_bounds on the unscaled problem
+  c_norm_ = std::sqrt(c2);
+
+  c_w_.resize(n_);
+  bnd_w_.resize(n_);
+  cb_w_.resize(m_);
+  cb_orig_.resize(m_);
+  launch_ew(n_, nullptr,
+            FinishColsF<V>{cs_.get(), lbs_.get(), ubs_.get(), beta_, gamma_, true, c_w_.get(), bnd_w_.get()}, stream_);
+  launch_ew(m_, nullptr,
+            FinishRowsF<V>{lcs_.get(), ucs_.get(), beta_, true, cb_w_.get(), cb_orig_.get(), lc_orig_.get(),
+                           uc_orig_.get()},
+            stream_);
+}
+
+template <typename V>
+void PdlpSolver<V>::power_iteration()
+{
+  // compute_initial_step_size, power-iteration branch (pdlp.cu:2622-2738).
+  std::vector<double> z(m_);
+  std::mt19937 gen(1);
+  std::normal_distribution<double> dist(0.0, 1.0);
+  for (int i = 0; i < m_; ++i) z[i] = dist(gen);
+  DevBuf<double> za(m_), zb(m_), atq(n_);
+  DevBuf<PowerState> st(1);
+  za.upload(z, stream_);
+  launch_ew(m_, red_.get(), SumSqF{za.get()}, stream_);
+  launch_reduce(red_.get(), ew_grid(m_), 1, scal_.get() + S_SETUP0, stream_);
+  read_scalars();
+  PowerState h{};
+  h.nrm = std::sqrt(h_scal_[S_SETUP0]);
+  CUOPT_HIP_CHECK(hipMemcpyAsync(st.get(), &h, sizeof(h), hipMemcpyHostToDevice, stream_));
+  double* zo = za.get();
+  double* zn = zb.get();
+  int launched = 0;
+  while (launched < 5000) {
+    const int batch = std::min(16, 5000 - launched);
+    for (int b = 0; b < batch; ++b) {
+      launch_tiled(AT_, at_s_.get(), nullptr, PowerAtEpi{zo, st.get(), atq.get()}, stream_);
+      launch_tiled(A_, a_s_.get(), red_.get(), PowerAEpi{atq.get(), zo, st.get(), zn}, stream_);
+      hipLaunchKernelGGL(power_finalize_s2, dim3(1), dim3(1024), 0, stream_, red_.get(), A_.plan.ntiles, st.get());
+      launch_ew(m_, red_.get(), PowerResF{zo, zn, st.get()}, stream_);
+      hipLaunchKernelGGL(power_finalize_res, dim3(1), dim3(1024), 0, stream_, red_.get(), ew_grid(m_), st.get(),
+                         1e-4);
+      CUOPT_HIP_CHECK(hipGetLastError());
+      std::swap(zo, zn);
+    }
+    launched += batch;
+    CUOPT_HIP_CHECK(hipMemcpyAsync(&h, st.get(), sizeof(h), hipMemcpyDeviceToHost, stream_));
+    CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
+    if (h.done != 0.0) break;
+  }
+  sigma_max_sq_ = h.s2;
+  power_iters_  = (int)h.iters;
+  eta_          = 0.998 / std::sqrt(sigma_max_sq_);
+}
+
+template <typename V>
+void PdlpSolver<V>::init_iterates()
+{
+  for (int b = 0; b < 2; ++b) {
+    x_[b].resize(n_);
+    y_[b].resize(m_);
+  }
+  x0_.resize(n_);
+  y0_.resize(m_);
+  xbar_.resize(n_);
+  ybar_.resize(m_);
+  xhat_.resize(n_);
+  yhat_.resize(m_);
+  slack_.resize(n_);
+  aty_.resize(n_);
+  xu_.resize(n_);
+  yu_.resize(m_);
+  su_.resize(n_);
+  step_sc_.resize(1);
+  DevBuf<double> xi, yi;
+  if (!st_.initial_primal.empty()) {
+    xi.resize(n_);
+    xi.upload(st_.initial_primal.data(), n_, stream_);
+  }
+  if (!st_.initial_dual.empty()) {
+    yi.resize(m_);
+    yi.upload(st_.initial_dual.data(), m_, stream_);
+  }
+  launch_ew(n_, nullptr,
+            InitIterateF<V, true>{xi.get(), Dc_.get(), beta_, true, bnd_w_.get(), x_[0].get(), xhat_.get(), x0_.get()},
+            stream_);
+  launch_ew(m_, nullptr,
+            InitIterateF<V, false>{yi.get(), Dr_.get(), gamma_, true, nullptr, y_[0].get(), yhat_.get(), y0_.get()},
+            stream_);
+  launch_ew(n_, nullptr, ZeroF<V>{slack_.get()}, stream_);
+  CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));  // xi / yi are freed on return
+  cur_ = 0;
+}
+
+template <typename V>
+void PdlpSolver<V>::evaluate_kkt()
+{
+  const double ib = 1.0 / beta_, ig = 1.0 / gamma_;
+  launch_tiled(A_, a_orig_.get(), red_.get(),
+               KktRowsEpi<V>{xhat_.get(), Dc_.get(), ib, yhat_.get(), Dr_.get(), ig, gamma_, cb_orig_.get(),
+                             y0_.get(), yu_.get()},
+               stream_);
+  launch_reduce(red_.get(), A_.plan.ntiles, 3, scal_.get() + S_KKT_RP2, stream_);
+  launch_tiled(AT_, at_orig_.get(), red_.get(),
+               KktColsEpi<V>{yu_.get(), xhat_.get(), slack_.get(), Dc_.get(), ib, beta_, ig, gamma_, c_orig_.get(),
+                             x0_.get(), xu_.get(), su_.get()},
+               stream_);
+  launch_reduce(red_.get(), AT_.plan.ntiles, 4, scal_.get() + S_KKT_RD2, stream_);
+  read_scalars();
+  const double* s = h_scal_;
+  l2p_            = std::sqrt(s[S_KKT_RP2]);
+  l2d_            = std::sqrt(s[S_KKT_RD2]);
+  double pobj     = s[S_KKT_XC];
+  double dobj     = s[S_KKT_SX] + s[S_KKT_PSUM];
+  if (obj_sf_ != 1.0 || obj_off_ != 0.0) {
+    pobj = obj_sf_ * (pobj + obj_off_);
+    dobj = obj_sf_ * (dobj + obj_off_);
+  }
+  pobj_   = pobj;
+  dobj_   = dobj;
+  gap_    = std::fabs(pobj - dobj);
+  absobj_ = std::fabs(pobj) + std::fabs(dobj);
+  dp_     = s[S_KKT_DP];
+  dd_     = s[S_KKT_DD];
+}
+
+template <typename V>
+void PdlpSolver<V>::launch_step(int j, bool major, bool fpe, double* ny_slot, double* fpe_slot)
+{
+  const int c = cur_, nx = cur_ ^ 1;
+#define CUOPT_PRIMAL(MAJ, FP)                                                                               \
+  launch_tiled(AT_, at_w_.get(), nullptr,                                                                    \
+               PrimalStepEpi<V, MAJ, FP>{y_[c].get(), x_[c].get(), c_w_.get(), bnd_w_.get(), x0_.get(),     \
+                                         xbar_.get(), x_[nx].get(), xhat_.get(), slack_.get(), aty_.get(),  \
+                                         step_sc_.get(), j},                                                \
+               stream_)
+#define CUOPT_DUAL(MAJ, FP)                                                                                  \
+  launch_tiled(A_, a_w_.get(), red_.get(),                                                                    \
+               DualStepEpi<V, MAJ, FP>{xbar_.get(), y_[c].get(), cb_w_.get(), y0_.get(), y_[nx].get(),        \
+                                       yhat_.get(), ybar_.get(), step_sc_.get(), j},                          \
+               stream_)
+  if (major) {
+    if (fpe) {
+      CUOPT_PRIMAL(true, true);
+      CUOPT_DUAL(true, true);
+    } else {
+      CUOPT_PRIMAL(true, false);
+      CUOPT_DUAL(true, false);
+    }
+  } else {
+    if (fpe) {
+      CUOPT_PRIMAL(false, true);
+      CUOPT_DUAL(false, true);
+    } else {
+      CUOPT_PRIMAL(false, false);
+      CUOPT_DUAL(false, false);
+    }
+  }
+#undef CUOPT_PRIMAL
+#undef CUOPT_DUAL
+  if (fpe) {
+    launch_reduce(red_.get(), A_.plan.ntiles, 1, ny_slot, stream_);
+    launch_tiled(AT_, at_w_.get(), red_.get(), FpeEpi<V>{ybar_.get(), aty_.get(), xbar_.get(), x_[c].get()},
+                 stream_);
+    launch_reduce(red_.get(), AT_.plan.ntiles, 2, fpe_slot, stream_);
+  }
+  cur_ = nx;
+}
+
+template <typename V>
+void PdlpSolver<V>::finish(SolveResult& R, int status)
+{
+  R.status = status;
+  R.x.resize(n_);
+  R.y.resize(m_);
+  R.rc.resize(n_);
+  xu_.download(R.x.data(), n_, stream_);
+  yu_.download(R.y.data(), m_, stream_);
+  su_.download(R.rc.data(), n_, stream_);
+  CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
+  if (lp_.maximize) {  // solve.cu:583-588
+    for (double& v : R.y) v = -v;
+    for (double& v : R.rc) v = -v;
+  }
+  R.primal_objective              = pobj_;
+  R.dual_objective                = dobj_;
+  R.info[CUOPT_AMD_INFO_L2_PRIMAL]  = l2p_;
+  R.info[CUOPT_AMD_INFO_REL_PRIMAL] = l2p_ / (1.0 + b_norm_);
+  R.info[CUOPT_AMD_INFO_L2_DUAL]    = l2d_;
+  R.info[CUOPT_AMD_INFO_REL_DUAL]   = l2d_ / (1.0 + c_norm_);
+  R.info[CUOPT_AMD_INFO_PRIMAL_OBJ] = pobj_;
+  R.info[CUOPT_AMD_INFO_DUAL_OBJ]   = dobj_;
+  R.info[CUOPT_AMD_INFO_GAP]        = gap_;
+  R.info[CUOPT_AMD_INFO_REL_GAP]    = gap_ / (1.0 + absobj_);
+  R.info[CUOPT_AMD_INFO_OMEGA]      = omega_;
+  R.solve_time                      = elapsed();
+  R.info[CUOPT_AMD_INFO_SOLVE_TIME] = R.solve_time;
+}
+
+template <typename V>
+SolveResult PdlpSolver<V>::run()
+{
+  SolveResult R;
+  t0_ = clock::now();
+  CUOPT_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  CUOPT_HIP_CHECK(hipHostMalloc((void**)&h_scal_, S_COUNT * sizeof(double), hipHostMallocDefault));
+  CUOPT_HIP_CHECK(hipHostMalloc((void**)&h_step_, sizeof(StepScalars), hipHostMallocDefault));
+  obj_sf_  = lp_.maximize ? -1.0 : 1.0;
+  obj_off_ = lp_.maximize ? -lp_.offset : lp_.offset;
+
+  build_structures();
+  setup_scaling();
+  power_iteration();
+  // working-precision matrices
+  if constexpr (std::is_same<V, double>::value) {
+    a_w_ = std::move(a_s_);
+    at_w_ = std::move(at_s_);
+  } else {
+    a_w_.resize(std::max(A_.nnz, 1));
+    at_w_.resize(std::max(A_.nnz, 1));
+    launch_ew(A_.nnz, nullptr, ConvertF<V>{a_s_.get(), a_w_.get()}, stream_);
+    launch_ew(A_.nnz, nullptr, ConvertF<V>{at_s_.get(), at_w_.get()}, stream_);
+  }
+  // Stable3: initial primal weight 1 (bound_objective_rescaling branch, pdlp.cu:2799-2807).
+  omega_ = best_omega_ = 1.0;
+  tau_   = eta_ / omega_;
+  sigma_ = eta_ * omega_;
+  init_iterates();
+  if constexpr (!std::is_same<V, double>::value) {
+    CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
+    a_s_.release();
+    at_s_.release();
+  }
+  R.info[CUOPT_AMD_INFO_ETA0]         = eta_;
+  R.info[CUOPT_AMD_INFO_OMEGA0]       = omega_;
+  R.info[CUOPT_AMD_INFO_POWER_ITERS]  = power_iters_;
+  R.info[CUOPT_AMD_INFO_SIGMA_MAX_SQ] = sigma_max_sq_;
+  R.info[CUOPT_AMD_INFO_BETA]         = beta_;
+  R.info[CUOPT_AMD_INFO_GAMMA]        = gamma_;
+  R.info[CUOPT_AMD_INFO_PRECISION]    = std::is_same<V, double>::value ? 0 : 1;
+  CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
+  const double setup_time           = elapsed();
+  R.info[CUOPT_AMD_INFO_SETUP_TIME] = setup_time;
+
+  const double tol_ap = st_.abs_primal, tol_rp = st_.rel_primal, tol_ad = st_.abs_dual, tol_rd = st_.rel_dual;
+  const double tol_ag = st_.abs_gap, tol_rg = st_.rel_gap;
+  const double nan    = std::numeric_limits<double>::quiet_NaN();
+  double fpe = nan, fpe0 = nan, last_trial = std::numeric_limits<double>::infinity();
+  double err_sum = 0.0, last_err = 0.0, best_gap = std::numeric_limits<double>::infinity();
+  long long k        = 0;
+  int restarts       = 0, checks = 0;
+  bool restarted_prev = false;   // a restart happened at the previous check
+  bool periodic_prev  = false;   // the previous block ran a periodic FPE step
+  kr_ = 0;
+
+  auto finish_with = [&](int status) {
+    finish(R, status);
+    R.info[CUOPT_AMD_INFO_STEPS]     = (double)k;
+    R.info[CUOPT_AMD_INFO_RESTARTS]  = restarts;
+    R.info[CUOPT_AMD_INFO_CHECKS]    = checks;
+    R.info[CUOPT_AMD_INFO_LOOP_TIME] = R.solve_time - setup_time;
+    return R;
+  };
+
+  while (true) {
+    // ---- termination check (k is a check iteration) ----
+    const bool is_major = ((k % kMajor) == 0 && k > 0) || k <= 0;
+    evaluate_kkt();
+    ++checks;
+    if (restarted_prev) {
+      fpe  = std::sqrt((h_scal_[S_FPE0_NX] * omega_ + h_scal_[S_FPE0_NY] / omega_) + 2.0 * h_scal_[S_FPE0_I] * eta_);
+      fpe0 = fpe;
+    }
+    if (periodic_prev)
+      fpe = std::sqrt((h_scal_[S_FPE_NX] * omega_ + h_scal_[S_FPE_NY] / omega_) + 2.0 * h_scal_[S_FPE_I] * eta_);
+    const bool optimal_gap = gap_ <= tol_ag + tol_rg * absobj_;
+    const bool pfeas       = l2p_ <= tol_ap + tol_rp * b_norm_;
+    const bool optimal     = l2d_ <= tol_ad + tol_rd * c_norm_ && pfeas && optimal_gap;
+    const size_t rec       = R.trace.size();
+    R.trace.insert(R.trace.end(), {(double)k, l2p_, l2d_, pobj_, dobj_, gap_, 0.0, omega_, fpe, (double)kr_});
+    if (k > 1 && optimal) return finish_with(CUOPT_TERIMINATION_STATUS_OPTIMAL);
+    // Divergence guard: a NaN residual can never satisfy the test again, so stop
+    // with a numerical error instead of running into the limits.
+    if (std::isnan(l2p_) || std::isnan(l2d_) || std::isnan(gap_))
+      return finish_with(CUOPT_TERIMINATION_STATUS_NUMERICAL_ERROR);
+    if (elapsed() >= st_.time_limit) return finish_with(CUOPT_TERIMINATION_STATUS_TIME_LIMIT);
+    if (k >= st_.iteration_limit) return finish_with(CUOPT_TERIMINATION_STATUS_ITERATION_LIMIT);
+
+    bool restarted = false;
+    if (is_major) {
+      bool do_restart = false;
+      if (k == kMajor) {
+        do_restart = true;
+      } else if (k > kMajor) {
+        if (fpe <= kSufficient * fpe0)
+          do_restart = true;
+        else if (fpe <= kNecessary * fpe0 && fpe > last_trial)
+          do_restart = true;
+        else if ((double)kr_ >= kArtificial * (double)k)
+          do_restart = true;
+      }
+      last_trial = fpe;
+      if (do_restart) {
+        // cupdlpx_new_primal_weight_computation (pdlp_restart_strategy.cu:757-835)
+        const double l2pd  = std::sqrt(dp_), l2dd = std::sqrt(dd_);
+        const double rel_d = l2d_ / (1.0 + c_norm_), rel_p = l2p_ / (1.0 + b_norm_);
+        const double ratio = (rel_p == 0.0) ? std::numeric_limits<double>::infinity() : rel_d / rel_p;
+        if (l2pd > 1e-16 && l2dd > 1e-16 && l2pd < 1e12 && l2dd < 1e12 && ratio > 1e-8 && ratio < 1e8) {
+          const double cur = omega_;
+          const double err = std::log(l2dd) - std::log(l2pd) - std::log(cur);
+          const double s   = err_sum * kISmooth + err;
+          err_sum          = s;
+          const double de  = err - last_err;
+          const double w   = std::exp(kKp * err + kKi * s + kKd * de) * cur;
+          omega_           = w;
+          last_err         = err;
+          tau_             = eta_ / w;
+          sigma_           = eta_ * w;
+        } else {
+          omega_   = best_omega_;
+          err_sum  = 0.0;
+          last_err = 0.0;
+          tau_     = eta_ / best_omega_;
+          sigma_   = eta_ * best_omega_;
+        }
+        const double g = std::fabs(std::log10(ratio));
+        if (g < best_gap) {
+          best_gap    = g;
+          best_omega_ = omega_;
+        }
+        launch_ew(n_, nullptr, RestartCopyF<V>{xhat_.get(), x_[cur_].get(), x0_.get()}, stream_);
+        launch_ew(m_, nullptr, RestartCopyF<V>{yhat_.get(), y_[cur_].get(), y0_.get()}, stream_);
+        kr_        = 0;
+        last_trial = std::numeric_limits<double>::infinity();
+        restarted  = true;
+        ++restarts;
+      }
+    }
+    R.trace[rec + 6] = restarted ? 1.0 : 0.0;
+    R.trace[rec + 7] = omega_;
+    upload_step_scalars();
+
+    // ---- PDHG iterations up to the next check ----
+    long long next = k + 1;
+    while (!is_check(next)) ++next;
+    periodic_prev = false;
+    for (long long s = k; s < next; ++s) {
+      const long long s2      = s + 2;
+      const bool should_major = ((s + 1) % kMajor == 0) || (s2 % conditional_major(s2)) == 0;
+      const bool periodic     = ((s + 1) % kMajor) == 0;
+      const bool post_restart = restarted && s == k;
+      const bool fpe_step     = periodic || post_restart;
+      double* ny_slot  = scal_.get() + (post_restart ? S_FPE0_NY : S_FPE_NY);
+      double* fpe_slot = scal_.get() + (post_restart ? S_FPE0_I : S_FPE_I);
+      launch_step((int)(s - k), should_major, fpe_step, ny_slot, fpe_slot);
+      if (periodic) periodic_prev = true;
+    }
+    kr_ += next - k;
+    k              = next;
+    restarted_prev = restarted;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// solve_lp: problem checks and dispatch (solve.cu:1123-1342, PDLP path)
+// ---------------------------------------------------------------------------
+
+SolveResult solve_lp(const LpHost& lp, const PdlpSettings& st)
+{
+  const auto t0 = std::chrono::steady_clock::now();
+  SolveResult R;
+  const int m = lp.m, n = lp.n;
+  // constraint bounds from row types (transform_bounds_functor, problem_helpers.cuh:26-51)
+  std::vector<double> lc(m), uc(m);
+  const double inf = std::numeric_limits<double>::infinity();
+  for (int i = 0; i < m; ++i) {
+    if (lp.ranged) {
+      lc[i] = lp.lc[i];
+      uc[i] = lp.uc[i];
+    } else {
+      const char t = lp.sense[i];
+      const double b = lp.rhs[i];
+      if (t == 'E') {
+        lc[i] = b;
+        uc[i] = b;
+      } else if (t == 'G') {
+        lc[i] = b;
+        uc[i] = inf;
+      } else if (t == 'L') {
+        lc[i] = -inf;
+        uc[i] = b;
+      } else {
+        throw validation_error(std::string("invalid constraint sense '") + t + "'");
+      }
+    }
+  }
+  std::vector<double> lb(lp.lb), ub(lp.ub);
+  // has_crossing_bounds (problem_checking.cu:316-339)
+  bool crossing = false;
+  for (int j = 0; j < n && !crossing; ++j) crossing = !(ub[j] >= lb[j]);
+  if (lp.ranged)
+    for (int i = 0; i < m && !crossing; ++i) crossing = !(uc[i] >= lc[i]);
+  auto seconds = [&]() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  };
+  if (crossing) {
+    R.status     = CUOPT_TERIMINATION_STATUS_INFEASIBLE;
+    R.solve_time = seconds();
+    R.x.assign(n, 0.0);
+    R.y.assign(m, 0.0);
+    R.rc.assign(n, 0.0);
+    return R;
+  }
+  if (m == 0) {
+    R.solve_time = seconds();
+    R.x.assign(n, 0.0);
+    R.rc.assign(n, 0.0);
+    if (st.method == CUOPT_METHOD_PDLP) {  // run_pdlp_solver (solve.cu:555-572)
+      R.status = CUOPT_TERIMINATION_STATUS_NUMERICAL_ERROR;
+      return R;
+    }
+    // No constraints: each variable sits at the bound its cost prefers, so the
+    // LP is solved in closed form (the reference's concurrent method hands this
+    // case to dual simplex, solve.cu:936-1103).
+    double obj = lp.offset;
+    for (int j = 0; j < n; ++j) {
+      const double cj = lp.maximize ? -lp.c[j] : lp.c[j];
+      double v        = 0.0;
+      if (cj > 0.0) v = lb[j];
+      else if (cj < 0.0) v = ub[j];
+      else v = std::min(std::max(0.0, lb[j]), ub[j]);
+      if (!std::isfinite(v)) {
+        R.status = CUOPT_TERIMINATION_STATUS_UNBOUNDED;
+        return R;
+      }
+      R.x[j] = v;
+      R.rc[j] = lp.c[j];
+      obj += lp.c[j] * v;
+    }
+    R.status           = CUOPT_TERIMINATION_STATUS_OPTIMAL;
+    R.primal_objective = obj;
+    R.dual_objective   = obj;
+    return R;
+  }
+  if (st.precision == CUOPT_AMD_PRECISION_FP32_MIXED) {
+    PdlpSolver<float> S(lp, lc, uc, lb, ub, st);
+    return S.run();
+  }
+  PdlpSolver<double> S(lp, lc, uc, lb, ub, st);
+  return S.run();
+}
+
+// ---------------------------------------------------------------------------
+// Test entry point: y = A x
+// ---------------------------------------------------------------------------
+
+template <typename V>
+static void spmv_impl(int m, int n, const int* rowptr, const int* colidx, const double* vals, const double* x,
+                      double* y)
+{
+  hipStream_t s;
+  CUOPT_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  try {
+    MatStruct M;
+    M.nrows = m;
+    M.ncols = n;
+    M.nnz   = rowptr[m];
+    M.h_ptr.assign(rowptr, rowptr + m + 1);
+    M.ptr.resize(m + 1);
+    M.ptr.upload(M.h_ptr, s);
+    M.idx.resize(std::max(M.nnz, 1));
+    M.idx.upload(colidx, M.nnz, s);
+    build_plan(M, s);
+    std::vector<V> hv(vals, vals + M.nnz), hx(x, x + n);
+    DevBuf<V> dv(std::max(M.nnz, 1)), dx(std::max(n, 1));
+    DevBuf<double> dy(std::max(m, 1));
+    dv.upload(hv.data(), M.nnz, s);
+    dx.upload(hx.data(), n, s);
+    launch_tiled(M, dv.get(), nullptr, PlainEpi<V>{dx.get(), dy.get()}, s);
+    dy.download(y, m, s);
+    CUOPT_HIP_CHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    (void)hipStreamDestroy(s);
+    throw;
+  }
+  CUOPT_HIP_CHECK(hipStreamDestroy(s));
+}
+
+void tiled_spmv_host(int m, int n, const int* rowptr, const int* colidx, const double* vals, const double* x,
+                     double* y, int precision)
+{
+  if (precision == CUOPT_AMD_PRECISION_FP32_MIXED)
+    spmv_impl<float>(m, n, rowptr, colidx, vals, x, y);
+  else
+    spmv_impl<double>(m, n, rowptr, colidx, vals, x, y);
+}
+
+int device_count()
+{
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess) return 0;
+  return c;
+}
+
+}  // namespace cuopt_amd

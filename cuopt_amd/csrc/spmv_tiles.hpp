@@ -1,0 +1,263 @@
+// spmv_tiles.hpp -- tiled row-block CSR SpMV with fused per-row epilogues (gfx950).
+//
+// One workgroup (kBlock = 256 lanes, four 64-wide wavefronts) owns one tile
+// (tile_plan.hpp).  It streams the tile's value/column-index arrays with
+// coalesced non-temporal loads, gathers the dense operand, and parks the
+// products in LDS; then groups of G lanes (G chosen per tile) reduce each row
+// from LDS and hand the row sum to an epilogue functor, which does the
+// projection / Halpern / residual arithmetic of that row in the same kernel.
+// Rows longer than a tile are split into chunks; every chunk writes a partial
+// sum and the last chunk to arrive (device-scope ticket counter) adds the
+// partials in chunk order and runs the epilogue.  All summation orders are
+// fixed by the tile plan, so results are bitwise reproducible run to run.
+//
+// Epilogues may also accumulate up to kMaxRed FP64 sums (norms, dots); each
+// workgroup writes its sums to a partials array, reduced afterwards in a fixed
+// order by reduce_partials_kernel.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "tile_plan.hpp"
+
+namespace cuopt_amd {
+
+constexpr int kBlock   = 256;
+constexpr int kWave    = 64;
+constexpr int kTileNnz = 2048;
+constexpr int kMaxRowsPerTile = 1024;
+constexpr int kMaxRed  = 8;
+
+struct PlanDev {
+  const Tile* tiles;
+  const int* meta;
+  const LongRow* longrows;
+  int ntiles;
+};
+
+// ---- wavefront helpers ------------------------------------------------------
+
+__device__ inline double shfl_xor_d(double v, int m) { return __shfl_xor(v, m, kWave); }
+__device__ inline float shfl_xor_f(float v, int m) { return __shfl_xor(v, m, kWave); }
+
+// Knuth's TwoSum: s + e == a + b exactly.
+__device__ inline void two_sum(float a, float b, float& s, float& e)
+{
+  s        = a + b;
+  float bb = s - a;
+  e        = (a - (s - bb)) + (b - bb);
+}
+
+// ---- accumulators -----------------------------------------------------------
+
+// FP64 sum.
+struct AccSum64 {
+  using Elem = double;
+  using Val  = double;
+  double s;
+  __device__ static AccSum64 zero() { return AccSum64{0.0}; }
+  __device__ static Elem prod(double a, double b) { return a * b; }
+  __device__ void add(Elem e) { s += e; }
+  __device__ void merge(const AccSum64& o) { s += o.s; }
+  __device__ AccSum64 shfl(int m) const { return AccSum64{shfl_xor_d(s, m)}; }
+  __device__ Val value() const { return s; }
+  __device__ uint64_t pack() const { return __double_as_longlong(s); }
+  __device__ static AccSum64 unpack(uint64_t u) { return AccSum64{__longlong_as_double(u)}; }
+};
+
+// FP64 max of non-negative terms (row / column infinity norms).
+struct AccMax64 {
+  using Elem = double;
+  using Val  = double;
+  double s;
+  __device__ static AccMax64 zero() { return AccMax64{0.0}; }
+  __device__ void add(Elem e) { s = (s < e) ? e : s; }
+  __device__ void merge(const AccMax64& o) { s = (s < o.s) ? o.s : s; }
+  __device__ AccMax64 shfl(int m) const { return AccMax64{shfl_xor_d(s, m)}; }
+  __device__ Val value() const { return s; }
+  __device__ uint64_t pack() const { return __double_as_longlong(s); }
+  __device__ static AccMax64 unpack(uint64_t u) { return AccMax64{__longlong_as_double(u)}; }
+};
+
+// Compensated FP32 sum: each product carries its exact rounding error
+// (fma(a, b, -p)), and the running sum carries a Neumaier compensation term.
+struct AccComp32 {
+  using Elem = float2;  // (product, product rounding error)
+  using Val  = float;
+  float s, c;
+  __device__ static AccComp32 zero() { return AccComp32{0.f, 0.f}; }
+  __device__ static Elem prod(float a, float b)
+  {
+    const float p = a * b;
+    return make_float2(p, __builtin_fmaf(a, b, -p));
+  }
+  __device__ void add(Elem e)
+  {
+    float t, err;
+    two_sum(s, e.x, t, err);
+    s = t;
+    c += err + e.y;
+  }
+  __device__ void merge(const AccComp32& o)
+  {
+    float t, err;
+    two_sum(s, o.s, t, err);
+    s = t;
+    c += o.c + err;
+  }
+  __device__ AccComp32 shfl(int m) const { return AccComp32{shfl_xor_f(s, m), shfl_xor_f(c, m)}; }
+  __device__ Val value() const { return s + c; }
+  __device__ uint64_t pack() const
+  {
+    return (uint64_t)__float_as_uint(s) | ((uint64_t)__float_as_uint(c) << 32);
+  }
+  __device__ static AccComp32 unpack(uint64_t u)
+  {
+    return AccComp32{__uint_as_float((uint32_t)u), __uint_as_float((uint32_t)(u >> 32))};
+  }
+};
+
+// ---- block reductions -------------------------------------------------------
+
+// Sum NR per-lane doubles over the workgroup; lane 0 writes them to out[0..NR).
+template <int NR>
+__device__ inline void block_sum_store(double (&v)[kMaxRed], double* out, double (*lds)[kMaxRed])
+{
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NR; ++k)
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v[k] += shfl_xor_d(v[k], off);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NR; ++k) lds[wid][k] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < NR; ++k) {
+      double s = lds[0][k];
+      for (int w = 1; w < kBlock / kWave; ++w) s += lds[w][k];
+      out[k] = s;
+    }
+}
+
+template <class Acc>
+__device__ inline Acc block_merge(Acc a, uint64_t* lds)
+{
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) a.merge(a.shfl(off));
+  if (lane == 0) lds[wid] = a.pack();
+  __syncthreads();
+  Acc r = Acc::unpack(lds[0]);
+  for (int w = 1; w < kBlock / kWave; ++w) r.merge(Acc::unpack(lds[w]));
+  return r;  // valid in every lane
+}
+
+__device__ inline uint64_t coherent_load_u64(const uint64_t* p)
+{
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// ---- the kernel -------------------------------------------------------------
+
+// Epi must provide:
+//   using Mat, Acc;  static constexpr int NRED (0..kMaxRed);
+//   __device__ bool skip() const;                       uniform early exit
+//   __device__ typename Acc::Elem product(Mat v, int col, int k) const;
+//   __device__ void row(int r, typename Acc::Val sum, double (&red)[kMaxRed]) const;
+template <class Epi>
+__global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restrict__ rowptr,
+                                                            const int* __restrict__ colidx,
+                                                            const typename Epi::Mat* __restrict__ vals,
+                                                            PlanDev plan,
+                                                            uint64_t* __restrict__ long_part,
+                                                            unsigned* __restrict__ long_cnt,
+                                                            double* __restrict__ red_part,
+                                                            Epi epi)
+{
+  using Acc  = typename Epi::Acc;
+  using Elem = typename Acc::Elem;
+  using Mat  = typename Epi::Mat;
+  __shared__ Elem prod[kTileNnz];
+  __shared__ double red_lds[kBlock / kWave][kMaxRed];
+  __shared__ uint64_t acc_lds[kBlock / kWave];
+
+  if (epi.skip()) return;
+  const int t    = blockIdx.x;
+  const Tile tl  = plan.tiles[t];
+  const int mt   = plan.meta[t];
+  const int tid  = threadIdx.x;
+  const int nnz  = tl.z1 - tl.z0;
+
+  // Stage the tile's products in LDS: all loads of a lane issued before use.
+  {
+    constexpr int kPer = kTileNnz / kBlock;
+    Mat v[kPer];
+    int c[kPer];
+#pragma unroll
+    for (int it = 0; it < kPer; ++it) {
+      const int e = it * kBlock + tid;
+      if (e < nnz) {
+        v[it] = __builtin_nontemporal_load(vals + tl.z0 + e);
+        c[it] = __builtin_nontemporal_load(colidx + tl.z0 + e);
+      }
+    }
+#pragma unroll
+    for (int it = 0; it < kPer; ++it) {
+      const int e = it * kBlock + tid;
+      if (e < nnz) prod[e] = epi.product(v[it], c[it], tl.z0 + e);
+    }
+  }
+  __syncthreads();
+
+  double red[kMaxRed];
+#pragma unroll
+  for (int k = 0; k < kMaxRed; ++k) red[k] = 0.0;
+
+  if (mt >= 0) {
+    const int G    = 1 << mt;
+    const int sub  = tid & (G - 1);
+    const int grp  = tid >> mt;
+    const int ngrp = kBlock >> mt;
+    for (int r = tl.r0 + grp; r < tl.r1; r += ngrp) {
+      const int b = rowptr[r] - tl.z0, e = rowptr[r + 1] - tl.z0;
+      Acc a = Acc::zero();
+      for (int k = b + sub; k < e; k += G) a.add(prod[k]);
+      for (int off = G >> 1; off > 0; off >>= 1) a.merge(a.shfl(off));
+      if (sub == 0) epi.row(r, a.value(), red);
+    }
+  } else {
+    const int lid = -mt - 1;
+    Acc a         = Acc::zero();
+    for (int k = tid; k < nnz; k += kBlock) a.add(prod[k]);
+    a = block_merge(a, acc_lds);
+    if (tid == 0) {
+      const LongRow lr = plan.longrows[lid];
+      long_part[t]     = a.pack();
+      __threadfence();
+      const unsigned ticket = atomicAdd(long_cnt + lid, 1u);
+      if (ticket == (unsigned)lr.nchunks - 1) {
+        __threadfence();
+        Acc tot = Acc::unpack(coherent_load_u64(long_part + lr.first));
+        for (int ch = 1; ch < lr.nchunks; ++ch)
+          tot.merge(Acc::unpack(coherent_load_u64(long_part + lr.first + ch)));
+        atomicExch(long_cnt + lid, 0u);
+        epi.row(lr.row, tot.value(), red);
+      }
+    }
+  }
+  if constexpr (Epi::NRED > 0) block_sum_store<Epi::NRED>(red, red_part + (size_t)t * Epi::NRED, red_lds);
+}
+
+// out[k] = sum over blocks b (in order of a fixed tree) of part[b * nred + k].
+// One workgroup; `skip` (optional device flag) makes it a no-op when nonzero.
+__global__ __launch_bounds__(1024) void reduce_partials_kernel(const double* __restrict__ part,
+                                                               int nblocks,
+                                                               int nred,
+                                                               double* __restrict__ out,
+                                                               const double* skip);
+
+}  // namespace cuopt_amd

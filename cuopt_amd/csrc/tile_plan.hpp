@@ -1,0 +1,43 @@
+// tile_plan.hpp -- host-side partition of a CSR matrix into SpMV work tiles.
+//
+// A tile is one workgroup's share of a sparse matrix-vector product: a run of
+// consecutive rows whose nonzeros fit in the workgroup's LDS product buffer
+// (kTileNnz entries), or one fixed-size chunk of a row too long for that.
+// Tiles are balanced by nonzeros, so the extreme row skew of Zipf-distributed
+// constraint matrices (a single row holding ~10 % of all nonzeros next to
+// hundreds of thousands of empty rows) becomes an even grid of workgroups.
+//
+// Per tile the planner also picks how many lanes cooperate on one row
+// (1..64, a power of two) from the tile's row count and longest row, so that
+// short rows use one lane each and long rows use whole wavefronts.
+#pragma once
+
+#include <cstdint>
+#include <vector>
+
+namespace cuopt_amd {
+
+struct alignas(16) Tile {
+  int r0, r1;  // rows [r0, r1)
+  int z0, z1;  // nonzeros [z0, z1)
+};
+
+struct alignas(16) LongRow {
+  int row;      // the long row
+  int first;    // index of its first chunk tile
+  int nchunks;  // number of chunk tiles
+  int pad;
+};
+
+struct TilePlan {
+  std::vector<Tile> tiles;
+  // meta[t] >= 0: log2(lanes per row) for a row-run tile;
+  // meta[t] <  0: chunk of long row number -(meta[t] + 1).
+  std::vector<int> meta;
+  std::vector<LongRow> longrows;
+};
+
+// Partition rows [0, nrows) of a CSR matrix with offsets rowptr.
+TilePlan plan_tiles(const int* rowptr, int nrows, int tile_nnz, int max_rows, int block);
+
+}  // namespace cuopt_amd

@@ -1,0 +1,82 @@
+/*
+ * cuopt_amd_ext.h -- additions to the cuOpt C API exported by libcuopt_amd.so.
+ *
+ * None of these exists in the reference C API; each says what it mirrors.
+ * A caller that only uses cuopt_c.h never needs this header.
+ */
+#ifndef CUOPT_AMD_EXT_H
+#define CUOPT_AMD_EXT_H
+
+#include "cuopt_c.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Integer parameter selecting the PDHG iterate arithmetic (no reference
+ * counterpart; the reference LP path is FP64 only, constants.h:17-18):
+ *   0 = FP64 iterates and matrices (default, the reference arithmetic);
+ *   1 = FP32 iterates and scaled matrices with compensated FP32 row sums,
+ *       FP64 scaling factors, FP64 termination / restart arithmetic. */
+#define CUOPT_AMD_PDLP_PRECISION "pdlp_precision"
+#define CUOPT_AMD_PRECISION_FP64       0
+#define CUOPT_AMD_PRECISION_FP32_MIXED 1
+
+/* Slots of cuOptAmdGetTerminationInfo.  Slots 0-9 mirror
+ * additional_termination_information_t (pdlp/solver_solution.hpp:53-93). */
+#define CUOPT_AMD_INFO_STEPS               0  /* number_of_steps_taken */
+#define CUOPT_AMD_INFO_L2_PRIMAL           1  /* l2_primal_residual */
+#define CUOPT_AMD_INFO_REL_PRIMAL          2  /* l2_relative_primal_residual */
+#define CUOPT_AMD_INFO_L2_DUAL             3  /* l2_dual_residual */
+#define CUOPT_AMD_INFO_REL_DUAL            4  /* l2_relative_dual_residual */
+#define CUOPT_AMD_INFO_PRIMAL_OBJ          5
+#define CUOPT_AMD_INFO_DUAL_OBJ            6
+#define CUOPT_AMD_INFO_GAP                 7
+#define CUOPT_AMD_INFO_REL_GAP             8
+#define CUOPT_AMD_INFO_SOLVE_TIME          9  /* seconds, from solve entry */
+#define CUOPT_AMD_INFO_SETUP_TIME          10 /* seconds spent before the first PDHG step */
+#define CUOPT_AMD_INFO_LOOP_TIME           11 /* seconds in the PDHG loop (incl. checks) */
+#define CUOPT_AMD_INFO_ETA0                12 /* initial step size */
+#define CUOPT_AMD_INFO_OMEGA0              13 /* initial primal weight */
+#define CUOPT_AMD_INFO_OMEGA               14 /* final primal weight */
+#define CUOPT_AMD_INFO_RESTARTS            15
+#define CUOPT_AMD_INFO_POWER_ITERS         16
+#define CUOPT_AMD_INFO_SIGMA_MAX_SQ        17
+#define CUOPT_AMD_INFO_BETA                18 /* bound rescaling */
+#define CUOPT_AMD_INFO_GAMMA               19 /* objective rescaling */
+#define CUOPT_AMD_INFO_PRECISION           20
+#define CUOPT_AMD_INFO_CHECKS              21 /* termination evaluations */
+#define CUOPT_AMD_INFO_COUNT               22
+
+/* Copies up to n slots of the last solve's statistics into info. */
+cuopt_int_t cuOptAmdGetTerminationInfo(cuOptSolution solution, cuopt_float_t* info, cuopt_int_t n);
+
+/* Per-check trace of the solve: rows of CUOPT_AMD_TRACE_WIDTH doubles
+ * (k, l2 primal residual, l2 dual residual, primal obj, dual obj, gap,
+ * restarted, primal weight after the check, fixed-point error, iterations since
+ * restart).  Writes min(cap, recorded) rows and the recorded count to *len. */
+#define CUOPT_AMD_TRACE_WIDTH 10
+cuopt_int_t cuOptAmdGetTrace(cuOptSolution solution, cuopt_float_t* rows, cuopt_int_t cap, cuopt_int_t* len);
+
+/* y = A x on the GPU with the solver's tiled row-block SpMV (the kernel that
+ * replaces the cusparseSpMV calls of pdhg.cu:285-347).  Host arrays in and
+ * out.  precision: CUOPT_AMD_PRECISION_FP64 (FP64 products and sums) or
+ * CUOPT_AMD_PRECISION_FP32_MIXED (values and x rounded to FP32, compensated
+ * FP32 row sums).  Returns CUOPT_SUCCESS or an error code. */
+cuopt_int_t cuOptAmdSpmv(cuopt_int_t num_rows,
+                         cuopt_int_t num_cols,
+                         const cuopt_int_t* row_offsets,
+                         const cuopt_int_t* column_indices,
+                         const cuopt_float_t* values,
+                         const cuopt_float_t* x,
+                         cuopt_float_t* y,
+                         cuopt_int_t precision);
+
+/* Number of GPUs visible to the library (0 when no HIP device is usable). */
+cuopt_int_t cuOptAmdDeviceCount(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CUOPT_AMD_EXT_H */

@@ -1,0 +1,155 @@
+"""C-ABI boundary tests that need no GPU: exports, parameter registry, problem
+handles, argument validation and the host-side early exits of cuOptSolve.
+
+Expectations follow the reference C API tests
+(cpp/tests/linear_programming/c_api_tests/c_api_test.c, c_api_tests.cpp:21-261)
+and the registry ranges of cpp/src/math_optimization/solver_settings.cu:61-132.
+"""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from cuopt_amd import c_api
+from cuopt_amd import generators as G
+from cuopt_amd.mps import read_mps
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def declared_functions():
+    names = []
+    for h in ("cuopt_c.h", "cuopt_amd_ext.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        names += re.findall(r"^\s*(?:cuopt_int_t|void|int8_t)\s+(cuOpt\w+)\s*\(", text, flags=re.M)
+    return sorted(set(names))
+
+
+def test_library_exports_every_declared_symbol():
+    L = c_api.lib()
+    names = declared_functions()
+    assert len(names) >= 45
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def test_type_sizes_and_version():
+    L = c_api.lib()
+    assert L.cuOptGetFloatSize() == 8  # c_api_tests.cpp:23
+    assert L.cuOptGetIntSize() == 4
+    a, b, c = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    assert L.cuOptGetVersion(ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)) == 0
+    assert L.cuOptGetVersion(None, ctypes.byref(b), ctypes.byref(c)) == c_api.CUOPT_INVALID_ARGUMENT
+
+
+def test_parameter_defaults_and_ranges():
+    s = c_api.SolverSettings()
+    L = c_api.lib()
+    v = ctypes.c_double()
+    assert L.cuOptGetFloatParameter(s.handle, b"absolute_primal_tolerance", ctypes.byref(v)) == 0
+    assert v.value == 1e-4
+    assert s.set("relative_gap_tolerance", 1e-8) == 0
+    assert L.cuOptGetFloatParameter(s.handle, b"relative_gap_tolerance", ctypes.byref(v)) == 0 and v.value == 1e-8
+    assert s.set("relative_gap_tolerance", 0.5) == c_api.CUOPT_INVALID_ARGUMENT  # range [0, 1e-1]
+    assert s.set("no_such_parameter", 3) == c_api.CUOPT_INVALID_ARGUMENT
+    i = ctypes.c_int32()
+    assert L.cuOptGetIntegerParameter(s.handle, b"method", ctypes.byref(i)) == 0 and i.value == 0
+    assert L.cuOptGetIntegerParameter(s.handle, b"pdlp_solver_mode", ctypes.byref(i)) == 0 and i.value == 4
+    assert s.set("pdlp_solver_mode", 7) == c_api.CUOPT_INVALID_ARGUMENT
+    assert s.set("iteration_limit", -1) == c_api.CUOPT_INVALID_ARGUMENT
+    # a boolean through the integer setter (cuopt_c.cpp:682-690)
+    assert s.set("infeasibility_detection", 1) == 0
+    assert L.cuOptGetIntegerParameter(s.handle, b"infeasibility_detection", ctypes.byref(i)) == 0 and i.value == 1
+    # string forms
+    assert L.cuOptSetParameter(s.handle, b"time_limit", b"12.5") == 0
+    assert s.get_string("time_limit").startswith("12.5")
+    assert L.cuOptSetParameter(s.handle, b"log_to_console", b"false") == 0
+    assert s.get_string("log_to_console") == "false"
+    assert L.cuOptSetParameter(s.handle, b"log_to_console", b"maybe") == c_api.CUOPT_INVALID_ARGUMENT
+    assert L.cuOptSetParameter(s.handle, b"iteration_limit", b"12x") == c_api.CUOPT_INVALID_ARGUMENT
+    # MIP-only names are accepted so that existing callers keep working
+    assert s.set("mip_relative_gap", 1e-3) == 0
+    assert s.set("pdlp_precision", 1) == 0 and s.set("pdlp_precision", 2) == c_api.CUOPT_INVALID_ARGUMENT
+
+
+def test_null_arguments_are_rejected():
+    L = c_api.lib()
+    assert L.cuOptCreateSolverSettings(None) == c_api.CUOPT_INVALID_ARGUMENT
+    h = ctypes.c_void_p()
+    assert L.cuOptCreateProblem(1, 1, 1, 0.0, None, None, None, None, None, None, None, None, None,
+                                ctypes.byref(h)) == c_api.CUOPT_INVALID_ARGUMENT
+    assert L.cuOptSolve(None, None, None) == c_api.CUOPT_INVALID_ARGUMENT
+    assert L.cuOptSetParameter(None, b"method", b"1") == c_api.CUOPT_INVALID_ARGUMENT
+
+
+def test_problem_round_trip_getters():
+    lp = G.random_lp(m=30, n=70, seed=1)
+    p = c_api.Problem(lp)
+    L = c_api.lib()
+    i = ctypes.c_int32()
+    assert L.cuOptGetNumConstraints(p.handle, ctypes.byref(i)) == 0 and i.value == 30
+    assert L.cuOptGetNumVariables(p.handle, ctypes.byref(i)) == 0 and i.value == 70
+    assert L.cuOptGetNumNonZeros(p.handle, ctypes.byref(i)) == 0 and i.value == lp.nnz
+    ptr, idx, val = np.zeros(31, np.int32), np.zeros(lp.nnz, np.int32), np.zeros(lp.nnz)
+    assert L.cuOptGetConstraintMatrix(p.handle, ptr.ctypes.data, idx.ctypes.data, val.ctypes.data) == 0
+    assert np.array_equal(ptr, lp.row_offsets) and np.array_equal(idx, lp.col_indices)
+    assert np.array_equal(val, lp.values)
+    c = np.zeros(70)
+    assert L.cuOptGetObjectiveCoefficients(p.handle, c.ctypes.data) == 0 and np.array_equal(c, lp.c)
+    sense = np.zeros(30, np.uint8)
+    assert L.cuOptGetConstraintSense(p.handle, sense.ctypes.data) == 0
+    assert bytes(sense) == b"E" * 30
+    ub = np.zeros(70)
+    assert L.cuOptGetVariableUpperBounds(p.handle, ub.ctypes.data) == 0 and np.all(ub == 10.0)
+    mip = ctypes.c_int32()
+    assert L.cuOptIsMIP(p.handle, ctypes.byref(mip)) == 0 and mip.value == 0
+
+
+def test_malformed_csr_is_rejected():
+    lp = G.random_lp(m=10, n=20, seed=2)
+    bad = lp.col_indices.copy()
+    bad[3] = 20  # column out of range
+    lp2 = G.LP(**{**lp.__dict__, "col_indices": bad})
+    with pytest.raises(c_api.CuOptError) as e:
+        c_api.Problem(lp2)
+    assert e.value.code == c_api.CUOPT_INVALID_ARGUMENT
+    ptr = lp.row_offsets.copy()
+    ptr[5] = ptr[6] + 1  # decreasing offsets
+    lp3 = G.LP(**{**lp.__dict__, "row_offsets": ptr})
+    with pytest.raises(c_api.CuOptError):
+        c_api.Problem(lp3)
+
+
+def test_crossing_bounds_report_infeasible_without_gpu_work():
+    lp = G.random_lp(m=10, n=20, seed=3)
+    lp.var_lb = lp.var_lb.copy()
+    lp.var_lb[4] = 11.0  # above the upper bound 10 (has_crossing_bounds, solve.cu:1176-1179)
+    r = c_api.solve(lp)
+    assert r.error == 0 and r.status == 2
+
+
+def test_no_constraints():
+    lp = read_mps(os.path.join(GOLDEN, "lp-model-no-constraints.mps"))
+    r = c_api.solve(lp, method=c_api.CUOPT_METHOD_PDLP)
+    assert r.status == 6  # PDLP cannot run without constraints (solve.cu:563-568)
+    r = c_api.solve(lp, method=c_api.CUOPT_METHOD_CONCURRENT)
+    assert r.status == 1 and abs(r.primal_objective - 1.0) < 1e-12  # pdlp_test.cu:1028-1043
+
+
+def test_mip_and_other_methods_are_rejected():
+    lp = G.random_lp(m=10, n=20, seed=4)
+    r = c_api.solve(lp, method=c_api.CUOPT_METHOD_DUAL_SIMPLEX)
+    assert r.error == c_api.CUOPT_INVALID_ARGUMENT and "PDLP" in r.error_string
+
+
+def test_solve_without_gpu_fails_cleanly():
+    if c_api.device_count() > 0:
+        pytest.skip("a GPU is present")
+    lp = G.random_lp(m=20, n=50, seed=5)
+    r = c_api.solve(lp, iteration_limit=10)
+    assert r.error == c_api.CUOPT_RUNTIME_ERROR
+    assert "HIP" in r.error_string

@@ -1,0 +1,205 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU FP64 oracle,
+closed-form optima, committed HiGHS optima and reference fixtures.
+
+Tolerances, stated per test:
+* FP64 SpMV: |y_gpu - y_ref| <= 64 eps64 * sum_j |a_ij x_j|  (summation order differs)
+* FP32 compensated SpMV vs FP64 products of the same FP32-rounded inputs:
+  |dy| <= 4 eps32 |y| + 64 eps32^2 sum_j |a_ij x_j|
+* FP64 solve vs oracle: same status and iteration count, objective within
+  1e-9 relative, iterates within 1e-7 (the same arithmetic, different sum order)
+* FP32-mixed solve at tol: objective within 10 tol of the closed-form optimum
+"""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from cuopt_amd import c_api
+from cuopt_amd import generators as G
+from cuopt_amd.mps import read_mps
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+TL = 120.0  # seconds: every solve carries a time limit
+EPS64 = np.finfo(np.float64).eps
+EPS32 = float(np.finfo(np.float32).eps)
+
+
+def rel(a, b):
+    return abs(a - b) / (1.0 + abs(b))
+
+
+def abs_products(lp, x):
+    import scipy.sparse as sp
+
+    A = sp.csr_matrix((np.abs(lp.values), lp.col_indices, lp.row_offsets), shape=(lp.m, lp.n))
+    return A @ np.abs(x)
+
+
+@pytest.fixture(scope="module")
+def highs():
+    with open(os.path.join(GOLDEN, "highs_optima.json")) as f:
+        return json.load(f)["cases"]
+
+
+def _spmv_cases():
+    rng = np.random.default_rng(0)
+    cases = [G.random_lp(m=500, n=1200, seed=1), G.skewed_lp(m=20000, n=40000, seed=2)]
+    # edge shapes: empty rows, a row of exactly one tile, one just over, a very long row
+    lens = np.array([0, 0, 2048, 2049, 1, 0, 10000, 3, 0, 4096, 0])
+    ptr = np.concatenate([[0], np.cumsum(lens)]).astype(np.int32)
+    n = 12000
+    idx = np.concatenate([np.sort(rng.choice(n, size=l, replace=False)) for l in lens if l] + []).astype(np.int32)
+    val = rng.standard_normal(idx.size)
+    edge = G.LP(m=len(lens), n=n, row_offsets=ptr, col_indices=idx, values=val, c=np.zeros(n),
+                con_lb=np.zeros(len(lens)), con_ub=np.zeros(len(lens)), var_lb=np.zeros(n), var_ub=np.ones(n))
+    cases.append(edge)
+    return cases
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_spmv_fp64_matches_oracle(case):
+    lp = _spmv_cases()[case]
+    x = np.random.default_rng(case).standard_normal(lp.n)
+    y = c_api.spmv(lp.row_offsets, lp.col_indices, lp.values, x, lp.n, "fp64")
+    ref = O.spmv(lp.row_offsets, lp.col_indices, lp.values, x, lp.n)
+    bound = 64 * EPS64 * abs_products(lp, x) + 1e-300
+    assert np.all(np.abs(y - ref) <= bound)
+    # transpose too (the A^T operand of the primal step)
+    tp, ti, tv = O.transpose(lp.row_offsets, lp.col_indices, lp.values, lp.n)
+    yv = np.random.default_rng(9).standard_normal(lp.m)
+    yt = c_api.spmv(tp, ti, tv, yv, lp.m, "fp64")
+    reft = O.spmv(tp, ti, tv, yv, lp.m)
+    assert np.allclose(yt, reft, rtol=0, atol=np.max(np.abs(reft)) * 1e-12 + 1e-300)
+
+
+@pytest.mark.parametrize("case", range(3))
+def test_spmv_fp32_compensated(case):
+    lp = _spmv_cases()[case]
+    x = np.random.default_rng(case + 5).standard_normal(lp.n)
+    y = c_api.spmv(lp.row_offsets, lp.col_indices, lp.values, x, lp.n, "fp32_mixed")
+    v32 = lp.values.astype(np.float32).astype(np.float64)
+    x32 = x.astype(np.float32).astype(np.float64)
+    ref = O.spmv(lp.row_offsets, lp.col_indices, v32, x32, lp.n)  # exact-ish sums of the rounded inputs
+    absp = abs_products(lp, x32)
+    bound = 4 * EPS32 * np.abs(ref) + 64 * EPS32 * EPS32 * absp + 1e-30
+    assert np.all(np.abs(y - ref) <= bound)
+    # and far better than a plain FP32 sum would be on the long rows
+    assert np.max(np.abs(y - ref) / (absp + 1e-30)) < 1e-9
+
+
+def test_setup_matches_oracle():
+    lp = G.random_lp(m=400, n=1000, seed=3)
+    r = c_api.solve(lp, tol=1e-4, iteration_limit=0, time_limit=TL)
+    s = O.setup(lp)
+    assert r.status == 4
+    assert rel(r.info["beta"], s["beta"]) < 1e-14
+    assert rel(r.info["gamma"], s["gamma"]) < 1e-14
+    assert r.info["power_iterations"] == s["power_iterations"]
+    assert abs(r.info["sigma_max_sq"] - s["sigma_max_sq"]) < 1e-10 * s["sigma_max_sq"]
+    assert abs(r.info["eta0"] - s["eta"]) < 1e-10 * s["eta"]
+
+
+@pytest.mark.parametrize("tol", [1e-4, 1e-8])
+def test_solve_fp64_matches_oracle(tol):
+    lp = G.random_lp(m=500, n=1250, seed=7)
+    r = c_api.solve(lp, tol=tol, precision="fp64", time_limit=TL, iteration_limit=100000)
+    o = O.solve(lp, tol=tol, trace_cap=100000)
+    assert r.error == 0 and r.status == 1 and o["status"] == 1
+    assert r.iterations == o["iterations"]
+    assert rel(r.primal_objective, o["primal_objective"]) < 1e-9
+    assert np.max(np.abs(r.x - o["x"])) < 1e-7
+    assert np.max(np.abs(r.y - o["y"])) < 1e-7
+    assert np.max(np.abs(r.reduced_cost - o["reduced_cost"])) < 1e-7
+    assert rel(r.primal_objective, lp.obj_opt) < 50 * tol
+    # same restart decisions at the same checks
+    tr, ot = r.trace, o["trace"]
+    assert tr.shape == ot.shape
+    assert np.array_equal(tr[:, 0], ot[:, 0]) and np.array_equal(tr[:, 6], ot[:, 6])
+    assert np.allclose(tr[:, 1:3], ot[:, 1:3], rtol=1e-6, atol=1e-12)
+
+
+def test_skewed_fp64_matches_oracle_and_closed_form():
+    lp = G.skewed_lp(m=20000, n=40000, seed=5)
+    r = c_api.solve(lp, tol=1e-6, precision="fp64", time_limit=TL, iteration_limit=200000)
+    o = O.solve(lp, tol=1e-6, threads=8)
+    assert r.status == 1 and o["status"] == 1
+    assert abs(r.iterations - o["iterations"]) <= 200
+    assert rel(r.primal_objective, lp.obj_opt) < 1e-4
+    assert rel(r.primal_objective, o["primal_objective"]) < 1e-6
+
+
+@pytest.mark.parametrize("tol", [1e-4])
+def test_fp32_mixed_reaches_tolerance(tol):
+    lp = G.random_lp(m=500, n=1250, seed=7)
+    r = c_api.solve(lp, tol=tol, precision="fp32_mixed", time_limit=TL, iteration_limit=200000)
+    assert r.error == 0 and r.status == 1
+    assert r.info["precision"] == 1
+    assert r.info["l2_primal_residual"] <= tol + tol * O.setup(lp)["b_norm"]
+    assert rel(r.primal_objective, lp.obj_opt) < 10 * tol
+
+
+def test_fp32_mixed_skewed():
+    lp = G.skewed_lp(m=20000, n=40000, seed=5)
+    r = c_api.solve(lp, tol=1e-4, precision="fp32_mixed", time_limit=TL, iteration_limit=200000)
+    assert r.status == 1
+    assert rel(r.primal_objective, lp.obj_opt) < 1e-3
+
+
+@pytest.mark.parametrize("fname", ["afiro_original.mps", "good-max.mps", "max_offset.mps",
+                                   "lp_model_with_var_bounds.mps"])
+def test_fixtures_fp64(highs, fname):
+    lp = read_mps(os.path.join(GOLDEN, fname))
+    r = c_api.solve(lp, tol=1e-8, time_limit=TL, iteration_limit=1000000)
+    o = O.solve(lp, tol=1e-8)
+    assert r.status == 1
+    assert r.iterations == o["iterations"]
+    assert rel(r.primal_objective, highs[fname]["objective"]) < 1e-6
+    assert rel(r.dual_objective, o["dual_objective"]) < 1e-8
+    assert np.allclose(r.y, o["y"], atol=1e-7)
+
+
+def test_transport_matches_highs(highs):
+    case = highs["transport_small"]
+    lp = G.transportation_lp(**case["kwargs"])
+    r = c_api.solve(lp, tol=1e-8, time_limit=TL, iteration_limit=1000000)
+    assert r.status == 1
+    assert rel(r.primal_objective, case["objective"]) < 1e-6
+
+
+def test_iteration_limit_semantics():
+    lp = read_mps(os.path.join(GOLDEN, "afiro_original.mps"))
+    r = c_api.solve(lp, tol=0.0, iteration_limit=10, time_limit=TL)
+    o = O.solve(lp, tol=0.0, iteration_limit=10)
+    assert r.status == 4 and r.iterations == 10
+    assert np.any(r.x != 0.0)
+    assert np.allclose(r.x, o["x"], atol=1e-12)
+
+
+def test_time_limit_semantics():
+    lp = G.random_lp(m=2000, n=5000, seed=0)
+    r = c_api.solve(lp, tol=0.0, time_limit=0.5)
+    assert r.status == 5
+    assert r.solve_time < 5 * 0.5 + 2.0  # pdlp_test.cu:159-175 allows 5x
+
+
+def test_initial_point_is_a_fixed_point():
+    lp = G.random_lp(m=300, n=800, seed=8)
+    r = c_api.solve(lp, tol=1e-8, x_init=lp.x_opt, y_init=lp.y_opt, time_limit=TL, iteration_limit=10000)
+    assert r.status == 1 and r.iterations == 10
+    assert np.allclose(r.x, lp.x_opt, atol=1e-8)
+
+
+def test_deterministic_bitwise():
+    lp = G.skewed_lp(m=20000, n=40000, seed=6)
+    a = c_api.solve(lp, tol=1e-4, precision="fp64", time_limit=TL, iteration_limit=3000)
+    b = c_api.solve(lp, tol=1e-4, precision="fp64", time_limit=TL, iteration_limit=3000)
+    assert a.iterations == b.iterations
+    assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+    c = c_api.solve(lp, tol=1e-4, precision="fp32_mixed", time_limit=TL, iteration_limit=3000)
+    d = c_api.solve(lp, tol=1e-4, precision="fp32_mixed", time_limit=TL, iteration_limit=3000)
+    assert np.array_equal(c.x, d.x) and np.array_equal(c.y, d.y)
