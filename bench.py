@@ -1,0 +1,151 @@
+#!/usr/bin/env python3
+"""Benchmark of the PDLP LP path on one MI355X.
+
+Workload: the skewed LP with m = 500 000 rows and
+n = 1 000 000 columns, 1 + Poisson(4) nonzeros per column at Zipf(1.1) rows
+(4.75 M nonzeros, a 474 k-nonzero row, 207 k empty rows), half equality and
+half <= rows, built-in optimum (x0, y0).  Synthetic, seeded (seed 0).
+
+A "step" is one PDHG iteration of the Stable3 loop (primal step kernel over
+A^T, dual step kernel over A, termination checks at the reference cadence).
+`value` is FP32-mixed PDHG iterations per second measured over exactly
+--steps iterations after --warmup untimed ones, inputs resident in HBM.
+Also reported: time to 1e-4 relative KKT for FP64 and FP32-mixed full solves,
+their objective error against the closed-form optimum, the roofline of the
+dominant kernel (HIP events over the timed window) and the CPU oracle baseline.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+METRIC = "time-to-1e-4 rel-KKT & PDHG iters/s, FP32-mixed vs FP64, 1 MI355X; % HBM peak"
+
+
+def kernel_bytes(m: int, n: int, nnz: int, sv: int) -> dict:
+    """Algorithmic bytes per launch of the two PDHG kernels (each operand touched once).
+
+    primal step (rows of A^T): values+indices nnz*(sv+4), row offsets 4(n+1), gather y: m*sv,
+      read x, c, l, u, x0 (5 n sv), write x-bar and the next x (2 n sv)
+    dual step (rows of A): nnz*(sv+4), 4(m+1), gather x-bar: n*sv, read y, l_c, u_c, y0 (4 m sv),
+      write the next y (m sv)
+    Together: 2 nnz (sv+4) + 4 (m+n+2) + sv (8n + 6m).
+    """
+    primal = nnz * (sv + 4) + 4 * (n + 1) + m * sv + 7 * n * sv
+    dual = nnz * (sv + 4) + 4 * (m + 1) + n * sv + 5 * m * sv
+    return {"primal": primal, "dual": dual, "iteration": primal + dual}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--warmup", type=int, default=200)
+    ap.add_argument("--tol", type=float, default=1e-4)
+    ap.add_argument("--no-solves", action="store_true", help="skip the time-to-tolerance solves")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-iters", type=int, default=60)
+    ap.add_argument("--precisions", default="fp32_mixed,fp64")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (run with --gpus 1)", file=sys.stderr)
+        return 2
+
+    import numpy as np
+
+    from cuopt_amd import c_api
+    from cuopt_amd import generators as G
+
+    if c_api.device_count() < 1:
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        return 1
+    t = time.time()
+    lp = G.skewed_lp(m=500_000, n=1_000_000, seed=0)
+    gen_s = time.time() - t
+    print(f"[bench] generated skewed LP m={lp.m} n={lp.n} nnz={lp.nnz} in {gen_s:.1f}s", file=sys.stderr, flush=True)
+
+    precs = args.precisions.split(",")
+    windows = {}
+    for p in precs:
+        b = c_api.benchmark_steps(lp, args.warmup, args.steps, precision=p, tol=0.0)
+        windows[p] = b
+        print(f"[bench] {p}: {args.steps} steps in {b['window_ms']:.1f} ms "
+              f"(primal {b['primal_kernel_ms'] * 1e3:.1f} us, dual {b['dual_kernel_ms'] * 1e3:.1f} us, "
+              f"{int(b['checks'])} checks, setup {b['setup_s']:.2f}s)", file=sys.stderr, flush=True)
+
+    solves = {}
+    if not args.no_solves:
+        for p in precs:
+            r = c_api.solve(lp, tol=args.tol, precision=p, time_limit=300.0)
+            err = abs(r.primal_objective - lp.obj_opt) / (1.0 + abs(lp.obj_opt))
+            solves[p] = {
+                "status": c_api.STATUS.get(r.status, r.status), "iterations": r.iterations,
+                "time_to_tol_s": round(r.solve_time, 4), "loop_s": round(r.info["loop_time"], 4),
+                "setup_s": round(r.info["setup_time"], 4),
+                "rel_primal": r.info["l2_relative_primal_residual"], "rel_dual": r.info["l2_relative_dual_residual"],
+                "rel_gap": r.info["relative_gap"], "objective": r.primal_objective,
+                "obj_rel_err_vs_closed_form": err,
+            }
+            print(f"[bench] solve {p} tol {args.tol}: {solves[p]}", file=sys.stderr, flush=True)
+
+    head = "fp32_mixed" if "fp32_mixed" in windows else precs[0]
+    w = windows[head]
+    sv = 4 if head == "fp32_mixed" else 8
+    iters = w["iterations"]
+    ms_per_step = w["window_ms"] / iters
+    value = 1000.0 / ms_per_step
+    kb = kernel_bytes(lp.m, lp.n, lp.nnz, sv)
+    dom = "primal" if w["primal_kernel_ms"] >= w["dual_kernel_ms"] else "dual"
+    dom_ms = w[f"{dom}_kernel_ms"]
+    achieved = kb[dom] / (dom_ms * 1e-3) / 1e9
+    roofline = {
+        "bound": "hbm", "kernel": f"{dom}_step ({head})", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "algorithmic_bytes_per_launch": kb[dom], "mean_launch_us": round(dom_ms * 1e3, 2),
+        "iteration_GBps": round(kb["iteration"] / ((w["primal_kernel_ms"] + w["dual_kernel_ms"]) * 1e-3) / 1e9, 1),
+    }
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import oracle as O
+
+        threads = max(1, min(16, os.cpu_count() or 1))
+        r = O.solve(lp, tol=0.0, iteration_limit=args.cpu_iters, threads=threads)
+        loop = r["solve_time"] - r["setup_time"]
+        cpu = {"value": round(r["iterations"] / loop, 3), "unit": "PDHG iters/s", "cores": threads, "kind": "port",
+               "sample": f"FP64 oracle Stable3 loop, {r['iterations']} iterations of the same LP "
+                         f"(setup {r['setup_time']:.1f}s untimed), {threads} OpenMP threads"}
+        print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
+
+    other = {p: {"iters_per_s": round(1000.0 * b["iterations"] / b["window_ms"], 1),
+                 "primal_kernel_us": round(b["primal_kernel_ms"] * 1e3, 2),
+                 "dual_kernel_us": round(b["dual_kernel_ms"] * 1e3, 2)} for p, b in windows.items()}
+    speedup = None
+    if "fp32_mixed" in solves and "fp64" in solves:
+        speedup = round(solves["fp64"]["time_to_tol_s"] / solves["fp32_mixed"]["time_to_tol_s"], 3)
+    out = {
+        "metric": METRIC, "value": round(value, 2), "unit": "PDHG iters/s", "n_gpus": 1, "steps": iters,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 5), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32" if head == "fp32_mixed" else "f64", "data": "synthetic",
+        "config": {"workload": "skewed LP m=500000 n=1000000 nnz=%d, Zipf(1.1) rows, 1+Poisson(4) per column, "
+                               "half = / half <=, seed 0" % lp.nnz, "precision": head, "parallelism": "1 GPU"},
+        "windows": other, "time_to_tolerance": solves, "fp64_over_fp32_time_ratio": speedup,
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -88,6 +88,7 @@ def lib():
         L.cuOptAmdGetTrace.argtypes = [vp, vp, i32, pi32]
         L.cuOptAmdSpmv.argtypes = [i32, i32, vp, vp, vp, vp, vp, i32]
         L.cuOptAmdDeviceCount.restype = i32
+        L.cuOptAmdBenchmarkSteps.argtypes = [vp, vp, i32, i32, vp, i32]
         _lib = L
     return _lib
 
@@ -269,6 +270,25 @@ def spmv(row_offsets, col_indices, values, x, ncols: int, precision: str = "fp64
     _check(lib().cuOptAmdSpmv(m, ncols, _p(ptr), _p(idx), _p(val), _p(x), _p(y), PRECISION[precision]),
            "cuOptAmdSpmv")
     return y[:m]
+
+
+BENCH_NAMES = ["window_ms", "primal_kernel_ms", "dual_kernel_ms", "iterations", "setup_s", "checks", "status",
+               "precision"]
+
+
+def benchmark_steps(lp, warmup: int, steps: int, precision: str = "fp32_mixed", tol: float = 0.0) -> dict:
+    """Times exactly ``steps`` PDHG iterations after ``warmup`` (cuOptAmdBenchmarkSteps)."""
+    prob = Problem(lp)
+    s = SolverSettings()
+    for name in ("absolute_primal_tolerance", "relative_primal_tolerance", "absolute_dual_tolerance",
+                 "relative_dual_tolerance", "absolute_gap_tolerance", "relative_gap_tolerance"):
+        _check(s.set(name, float(tol)), name)
+    _check(s.set("pdlp_precision", PRECISION[precision]))
+    _check(s.set("log_to_console", False))
+    out = np.zeros(len(BENCH_NAMES))
+    _check(lib().cuOptAmdBenchmarkSteps(prob.handle, s.handle, int(warmup), int(steps), _p(out), len(out)),
+           "cuOptAmdBenchmarkSteps")
+    return dict(zip(BENCH_NAMES, out.tolist()))
 
 
 def device_count() -> int:

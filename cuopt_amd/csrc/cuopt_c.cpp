@@ -12,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <exception>
+#include <limits>
 #include <new>
 #include <string>
 
@@ -586,6 +587,32 @@ cuopt_int_t cuOptAmdSpmv(cuopt_int_t m,
   } catch (const std::bad_alloc&) {
     return CUOPT_OUT_OF_MEMORY;
   } catch (...) {
+    return CUOPT_RUNTIME_ERROR;
+  }
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptAmdBenchmarkSteps(cuOptOptimizationProblem problem,
+                                   cuOptSolverSettings settings,
+                                   cuopt_int_t warmup,
+                                   cuopt_int_t steps,
+                                   cuopt_float_t* out,
+                                   cuopt_int_t n)
+{
+  if (!problem || !settings || !out || warmup < 0 || steps <= 0 || n < 0) return CUOPT_INVALID_ARGUMENT;
+  const LpHost& lp = as_problem(problem)->lp;
+  if (is_mip(lp) || lp.m == 0) return CUOPT_INVALID_ARGUMENT;
+  PdlpSettings conf  = as_settings(settings)->reg.pdlp;
+  conf.bench_warmup  = warmup;
+  conf.bench_steps   = steps;
+  conf.iteration_limit = std::numeric_limits<int>::max();
+  try {
+    SolveResult R = solve_lp(lp, conf);
+    for (int i = 0; i < n && i < CUOPT_AMD_BENCH_COUNT; ++i) out[i] = R.bench[i];
+    if (n > 6 && R.bench[3] < steps) out[6] = R.status;  // terminated before the window closed
+  } catch (const std::bad_alloc&) {
+    return CUOPT_OUT_OF_MEMORY;
+  } catch (const std::exception&) {
     return CUOPT_RUNTIME_ERROR;
   }
   return CUOPT_SUCCESS;

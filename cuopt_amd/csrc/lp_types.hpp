@@ -54,6 +54,9 @@ struct PdlpSettings {
   int random_seed              = -1;
   std::string log_file, solution_file, user_problem_file;
   std::vector<double> initial_primal, initial_dual;
+  // Benchmark window (cuOptAmdBenchmarkSteps): time iterations [bench_warmup,
+  // bench_warmup + bench_steps) and stop there.  Disabled when bench_steps <= 0.
+  long long bench_warmup = 0, bench_steps = 0;
 };
 
 struct SolveResult {
@@ -65,6 +68,7 @@ struct SolveResult {
   double solve_time       = 0.0;
   double info[CUOPT_AMD_INFO_COUNT];
   std::vector<double> trace;  // rows of CUOPT_AMD_TRACE_WIDTH
+  double bench[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // see cuOptAmdBenchmarkSteps
   SolveResult()
   {
     for (double& v : info) v = std::nan("");

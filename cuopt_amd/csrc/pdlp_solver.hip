@@ -543,6 +543,7 @@ class PdlpSolver {
   DevBuf<double> red_, scal_;
   DevBuf<StepScalars> step_sc_;
   int cur_ = 0;
+  hipEvent_t* ev_ = nullptr;  // events of the current timed iteration (benchmark window)
 
   double beta_ = 1.0, gamma_ = 1.0, b_norm_ = 0.0, c_norm_ = 0.0;
   double eta_ = 1.0, omega_ = 1.0, best_omega_ = 1.0, tau_ = 1.0, sigma_ = 1.0;
@@ -838,25 +839,25 @@ void PdlpSolver<V>::launch_step(int j, bool major, bool fpe, double* ny_slot, do
                DualStepEpi<V, MAJ, FP>{xbar_.get(), y_[c].get(), cb_w_.get(), y0_.get(), y_[nx].get(),        \
                                        yhat_.get(), ybar_.get(), step_sc_.get(), j},                          \
                stream_)
+  if (ev_[0]) CUOPT_HIP_CHECK(hipEventRecord(ev_[0], stream_));
   if (major) {
-    if (fpe) {
-      CUOPT_PRIMAL(true, true);
-      CUOPT_DUAL(true, true);
-    } else {
-      CUOPT_PRIMAL(true, false);
-      CUOPT_DUAL(true, false);
-    }
+    if (fpe) CUOPT_PRIMAL(true, true);
+    else CUOPT_PRIMAL(true, false);
   } else {
-    if (fpe) {
-      CUOPT_PRIMAL(false, true);
-      CUOPT_DUAL(false, true);
-    } else {
-      CUOPT_PRIMAL(false, false);
-      CUOPT_DUAL(false, false);
-    }
+    if (fpe) CUOPT_PRIMAL(false, true);
+    else CUOPT_PRIMAL(false, false);
+  }
+  if (ev_[1]) CUOPT_HIP_CHECK(hipEventRecord(ev_[1], stream_));
+  if (major) {
+    if (fpe) CUOPT_DUAL(true, true);
+    else CUOPT_DUAL(true, false);
+  } else {
+    if (fpe) CUOPT_DUAL(false, true);
+    else CUOPT_DUAL(false, false);
   }
 #undef CUOPT_PRIMAL
 #undef CUOPT_DUAL
+  if (ev_[2]) CUOPT_HIP_CHECK(hipEventRecord(ev_[2], stream_));
   if (fpe) {
     launch_reduce(red_.get(), A_.plan.ntiles, 1, ny_slot, stream_);
     launch_tiled(AT_, at_w_.get(), red_.get(), FpeEpi<V>{ybar_.get(), aty_.get(), xbar_.get(), x_[c].get()},
@@ -961,7 +962,39 @@ SolveResult PdlpSolver<V>::run()
     return R;
   };
 
+  const bool bench         = st_.bench_steps > 0;
+  const long long bench_w  = st_.bench_warmup, bench_end = st_.bench_warmup + st_.bench_steps;
+  clock::time_point tb0;
+  int bench_checks = 0;
+  std::vector<hipEvent_t> events;  // 3 per timed iteration
+  auto bench_stop  = [&]() {
+    CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
+    const double ms = std::chrono::duration<double, std::milli>(clock::now() - tb0).count();
+    double tp = 0.0, td = 0.0;
+    const size_t nit = events.size() / 3;
+    for (size_t i = 0; i < nit; ++i) {
+      float a = 0.f, b = 0.f;
+      CUOPT_HIP_CHECK(hipEventElapsedTime(&a, events[3 * i], events[3 * i + 1]));
+      CUOPT_HIP_CHECK(hipEventElapsedTime(&b, events[3 * i + 1], events[3 * i + 2]));
+      tp += a;
+      td += b;
+    }
+    for (hipEvent_t e : events) (void)hipEventDestroy(e);
+    events.clear();
+    R.bench[0] = ms;
+    R.bench[1] = nit ? tp / nit : 0.0;
+    R.bench[2] = nit ? td / nit : 0.0;
+    R.bench[3] = (double)nit;
+    R.bench[4] = setup_time;
+    R.bench[5] = bench_checks;
+    R.bench[6] = CUOPT_TERIMINATION_STATUS_ITERATION_LIMIT;
+    R.bench[7] = std::is_same<V, double>::value ? 0 : 1;
+    return finish_with(CUOPT_TERIMINATION_STATUS_ITERATION_LIMIT);
+  };
+
   while (true) {
+    if (bench && k == bench_end) return bench_stop();
+    if (bench && k > bench_w && k < bench_end) ++bench_checks;
     // ---- termination check (k is a check iteration) ----
     const bool is_major = ((k % kMajor) == 0 && k > 0) || k <= 0;
     evaluate_kkt();
@@ -1044,6 +1077,23 @@ SolveResult PdlpSolver<V>::run()
     while (!is_check(next)) ++next;
     periodic_prev = false;
     for (long long s = k; s < next; ++s) {
+      if (bench && s == bench_w) {
+        CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
+        tb0 = clock::now();
+      }
+      if (bench && s == bench_end) {
+        k = s;
+        return bench_stop();
+      }
+      const bool timed = bench && s >= bench_w;
+      hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+      if (timed) {
+        for (hipEvent_t& e : ev) {
+          CUOPT_HIP_CHECK(hipEventCreate(&e));
+          events.push_back(e);
+        }
+      }
+      ev_ = ev;
       const long long s2      = s + 2;
       const bool should_major = ((s + 1) % kMajor == 0) || (s2 % conditional_major(s2)) == 0;
       const bool periodic     = ((s + 1) % kMajor) == 0;
@@ -1052,6 +1102,7 @@ SolveResult PdlpSolver<V>::run()
       double* ny_slot  = scal_.get() + (post_restart ? S_FPE0_NY : S_FPE_NY);
       double* fpe_slot = scal_.get() + (post_restart ? S_FPE0_I : S_FPE_I);
       launch_step((int)(s - k), should_major, fpe_step, ny_slot, fpe_slot);
+      ev_ = nullptr;
       if (periodic) periodic_prev = true;
     }
     kr_ += next - k;

@@ -72,6 +72,23 @@ cuopt_int_t cuOptAmdSpmv(cuopt_int_t num_rows,
                          cuopt_float_t* y,
                          cuopt_int_t precision);
 
+/* Benchmark of the PDHG loop (no reference counterpart).  Runs the full solve
+ * setup, `warmup` PDHG iterations, then exactly `steps` iterations between two
+ * device synchronisations, termination checks at the reference cadence
+ * included, and stops.  HIP events on the solver's stream bracket every
+ * primal-step and dual-step kernel of the timed window.
+ * out (n slots): [0] timed window in ms, [1] mean primal-step kernel ms,
+ * [2] mean dual-step kernel ms, [3] iterations timed, [4] setup seconds,
+ * [5] termination checks inside the window, [6] termination status,
+ * [7] precision used. */
+#define CUOPT_AMD_BENCH_COUNT 8
+cuopt_int_t cuOptAmdBenchmarkSteps(cuOptOptimizationProblem problem,
+                                   cuOptSolverSettings settings,
+                                   cuopt_int_t warmup,
+                                   cuopt_int_t steps,
+                                   cuopt_float_t* out,
+                                   cuopt_int_t n);
+
 /* Number of GPUs visible to the library (0 when no HIP device is usable). */
 cuopt_int_t cuOptAmdDeviceCount(void);
 

@@ -24,7 +24,7 @@ INFO_NAMES = [
     "status", "iterations", "l2_primal_residual", "l2_relative_primal_residual",
     "l2_dual_residual", "l2_relative_dual_residual", "primal_objective", "dual_objective",
     "gap", "relative_gap", "solve_time", "eta0", "omega0", "beta", "gamma", "omega",
-    "restarts", "power_iterations", "sigma_max_sq", "b_norm", "c_norm",
+    "restarts", "power_iterations", "sigma_max_sq", "b_norm", "c_norm", "setup_time",
 ]
 TRACE_NAMES = ["k", "l2_primal", "l2_dual", "pobj", "dobj", "gap", "restarted", "omega", "fpe", "k_r"]
 

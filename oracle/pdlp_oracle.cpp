@@ -468,7 +468,7 @@ enum ParamSlot {
 enum InfoSlot {
   I_STATUS = 0, I_ITERS, I_L2_PRIMAL, I_REL_PRIMAL, I_L2_DUAL, I_REL_DUAL, I_POBJ, I_DOBJ,
   I_GAP, I_REL_GAP, I_SOLVE_TIME, I_ETA0, I_OMEGA0, I_BETA, I_GAMMA, I_OMEGA, I_RESTARTS,
-  I_POWER_ITERS, I_SIGMA_SQ, I_B_NORM, I_C_NORM, I_NINFO
+  I_POWER_ITERS, I_SIGMA_SQ, I_B_NORM, I_C_NORM, I_SETUP_TIME, I_NINFO
 };
 constexpr int kTraceWidth = 10;  // k, l2p, l2d, pobj, dobj, gap, restarted, omega, fpe, k_r
 
@@ -717,6 +717,7 @@ int oracle_solve(int m, int n, const int* rowptr, const int* colidx, const doubl
   info[I_SIGMA_SQ]    = U.sigma_sq;
   info[I_B_NORM]      = b_norm;
   info[I_C_NORM]      = c_norm;
+  info[I_SETUP_TIME]  = elapsed();
 
   std::vector<double> x(n, 0.0), y(m, 0.0), xhat(n, 0.0), yhat(m, 0.0), slack(n, 0.0);
   std::vector<double> x0(n, 0.0), y0(m, 0.0), xbar(n), ybar(m), aty(n), ax(m);

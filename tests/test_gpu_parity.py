@@ -4,7 +4,7 @@ closed-form optima, committed HiGHS optima and reference fixtures.
 Tolerances, stated per test:
 * FP64 SpMV: |y_gpu - y_ref| <= 64 eps64 * sum_j |a_ij x_j|  (summation order differs)
 * FP32 compensated SpMV vs FP64 products of the same FP32-rounded inputs:
-  |dy| <= 4 eps32 |y| + 64 eps32^2 sum_j |a_ij x_j|
+  |dy| <= eps32 |y| + 64 eps32^2 sum_j |a_ij x_j|
 * FP64 solve vs oracle: same status and iteration count, objective within
   1e-9 relative, iterates within 1e-7 (the same arithmetic, different sum order)
 * FP32-mixed solve at tol: objective within 10 tol of the closed-form optimum
@@ -86,10 +86,19 @@ def test_spmv_fp32_compensated(case):
     x32 = x.astype(np.float32).astype(np.float64)
     ref = O.spmv(lp.row_offsets, lp.col_indices, v32, x32, lp.n)  # exact-ish sums of the rounded inputs
     absp = abs_products(lp, x32)
-    bound = 4 * EPS32 * np.abs(ref) + 64 * EPS32 * EPS32 * absp + 1e-30
+    # compensated sum: one final FP32 rounding of the row value plus O(n eps^2) terms
+    bound = EPS32 * np.abs(ref) + 64 * EPS32 * EPS32 * absp + 1e-30
     assert np.all(np.abs(y - ref) <= bound)
-    # and far better than a plain FP32 sum would be on the long rows
-    assert np.max(np.abs(y - ref) / (absp + 1e-30)) < 1e-9
+    # a plain left-to-right FP32 sum of the same products is far worse on long rows
+    ptr = lp.row_offsets
+    longest = int(np.argmax(np.diff(ptr)))
+    if ptr[longest + 1] - ptr[longest] > 2048:
+        seg = slice(ptr[longest], ptr[longest + 1])
+        prods = (lp.values[seg].astype(np.float32) * x.astype(np.float32)[lp.col_indices[seg]]).astype(np.float32)
+        naive = np.float32(0)
+        for p in prods:
+            naive = np.float32(naive + p)
+        assert abs(y[longest] - ref[longest]) * 8 < abs(float(naive) - ref[longest]) + EPS32 * abs(ref[longest])
 
 
 def test_setup_matches_oracle():
