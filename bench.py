@@ -56,6 +56,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-iters", type=int, default=60)
     ap.add_argument("--precisions", default="fp32_mixed,fp64")
+    ap.add_argument("--solve-time-limit", type=float, default=300.0)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -87,7 +88,7 @@ def main() -> int:
     solves = {}
     if not args.no_solves:
         for p in precs:
-            r = c_api.solve(lp, tol=args.tol, precision=p, time_limit=300.0)
+            r = c_api.solve(lp, tol=args.tol, precision=p, time_limit=args.solve_time_limit)
             err = abs(r.primal_objective - lp.obj_opt) / (1.0 + abs(lp.obj_opt))
             solves[p] = {
                 "status": c_api.STATUS.get(r.status, r.status), "iterations": r.iterations,

@@ -7,9 +7,10 @@
 // from LDS and hand the row sum to an epilogue functor, which does the
 // projection / Halpern / residual arithmetic of that row in the same kernel.
 // Rows longer than a tile are split into chunks; every chunk writes a partial
-// sum and the last chunk to arrive (device-scope ticket counter) adds the
-// partials in chunk order and runs the epilogue.  All summation orders are
-// fixed by the tile plan, so results are bitwise reproducible run to run.
+// sum and the last chunk to arrive (device-scope ticket counter) loads all
+// partials of the row in parallel, merges them in a fixed tree and runs the
+// epilogue.  All summation orders are fixed by the tile plan, so results are
+// bitwise reproducible run to run.
 //
 // Epilogues may also accumulate up to kMaxRed FP64 sums (norms, dots); each
 // workgroup writes its sums to a partials array, reduced afterwards in a fixed
@@ -143,6 +144,8 @@ __device__ inline void block_sum_store(double (&v)[kMaxRed], double* out, double
     }
 }
 
+// Workgroup merge in a fixed order (wave butterflies, then waves 0..3).
+// Callers separate two uses of the same lds with a __syncthreads().
 template <class Acc>
 __device__ inline Acc block_merge(Acc a, uint64_t* lds)
 {
@@ -230,20 +233,28 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
       if (sub == 0) epi.row(r, a.value(), red);
     }
   } else {
-    const int lid = -mt - 1;
-    Acc a         = Acc::zero();
+    // Chunk of a long row: workgroup partial, then the last chunk to finish
+    // merges all partials of the row with every lane loading in parallel.
+    __shared__ int last_chunk;
+    const int lid    = -mt - 1;
+    const LongRow lr = plan.longrows[lid];
+    Acc a            = Acc::zero();
     for (int k = tid; k < nnz; k += kBlock) a.add(prod[k]);
     a = block_merge(a, acc_lds);
     if (tid == 0) {
-      const LongRow lr = plan.longrows[lid];
-      long_part[t]     = a.pack();
+      long_part[t] = a.pack();
       __threadfence();
       const unsigned ticket = atomicAdd(long_cnt + lid, 1u);
-      if (ticket == (unsigned)lr.nchunks - 1) {
-        __threadfence();
-        Acc tot = Acc::unpack(coherent_load_u64(long_part + lr.first));
-        for (int ch = 1; ch < lr.nchunks; ++ch)
-          tot.merge(Acc::unpack(coherent_load_u64(long_part + lr.first + ch)));
+      last_chunk            = ticket == (unsigned)lr.nchunks - 1;
+    }
+    __syncthreads();
+    if (last_chunk) {
+      __threadfence();
+      Acc tot = Acc::zero();
+      for (int ch = tid; ch < lr.nchunks; ch += kBlock)
+        tot.merge(Acc::unpack(coherent_load_u64(long_part + lr.first + ch)));
+      tot = block_merge(tot, acc_lds);
+      if (tid == 0) {
         atomicExch(long_cnt + lid, 0u);
         epi.row(lr.row, tot.value(), red);
       }
