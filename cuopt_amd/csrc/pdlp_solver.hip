@@ -383,8 +383,9 @@ struct PlanBufs {
   DevBuf<int> meta;
   DevBuf<LongRow> longrows;
   DevBuf<uint64_t> long_part;
-  DevBuf<unsigned> long_cnt;
-  int ntiles = 0;
+  int ntiles = 0, nlong = 0;
+  // workgroups that write reduction partials: the tiles, then one per long row
+  int red_blocks() const { return ntiles + nlong; }
   PlanDev dev() const { return PlanDev{tiles.get(), meta.get(), longrows.get(), ntiles}; }
 };
 
@@ -406,8 +407,7 @@ void build_plan(MatStruct& M, hipStream_t s)
   M.plan.longrows.resize(std::max<size_t>(1, p.longrows.size()));
   if (!p.longrows.empty()) M.plan.longrows.upload(p.longrows, s);
   M.plan.long_part.resize(std::max<size_t>(1, p.tiles.size()));
-  M.plan.long_cnt.resize(std::max<size_t>(1, p.longrows.size()));
-  M.plan.long_cnt.zero(s);
+  M.plan.nlong = (int)p.longrows.size();
 }
 
 template <class Epi>
@@ -415,8 +415,13 @@ void launch_tiled(const MatStruct& M, const typename Epi::Mat* vals, double* red
 {
   if (M.plan.ntiles == 0) return;
   hipLaunchKernelGGL(tiled_spmv_kernel<Epi>, dim3(M.plan.ntiles), dim3(kBlock), 0, s, M.ptr.get(),
-                     M.idx.get(), vals, M.plan.dev(), M.plan.long_part.get(), M.plan.long_cnt.get(), red, epi);
+                     M.idx.get(), vals, M.plan.dev(), M.plan.long_part.get(), red, epi);
   CUOPT_HIP_CHECK(hipGetLastError());
+  if (M.plan.nlong > 0) {
+    hipLaunchKernelGGL(long_rows_kernel<Epi>, dim3(M.plan.nlong), dim3(kBlock), 0, s, M.plan.dev(),
+                       M.plan.long_part.get(), red, epi);
+    CUOPT_HIP_CHECK(hipGetLastError());
+  }
 }
 
 void launch_reduce(const double* part, int nblocks, int nred, double* out, hipStream_t s,
@@ -625,7 +630,7 @@ template <typename V>
 void PdlpSolver<V>::setup_scaling()
 {
   const int nnz = A_.nnz;
-  const int red_blocks = std::max({A_.plan.ntiles, AT_.plan.ntiles, ew_grid(std::max(m_, n_)),
+  const int red_blocks = std::max({A_.plan.red_blocks(), AT_.plan.red_blocks(), ew_grid(std::max(m_, n_)),
                                    ew_grid(std::max(nnz, 1))});
   red_.resize((size_t)red_blocks * kMaxRed);
   scal_.resize(S_COUNT);
@@ -736,7 +741,7 @@ void PdlpSolver<V>::power_iteration()
     for (int b = 0; b < batch; ++b) {
       launch_tiled(AT_, at_s_.get(), nullptr, PowerAtEpi{zo, st.get(), atq.get()}, stream_);
       launch_tiled(A_, a_s_.get(), red_.get(), PowerAEpi{atq.get(), zo, st.get(), zn}, stream_);
-      hipLaunchKernelGGL(power_finalize_s2, dim3(1), dim3(1024), 0, stream_, red_.get(), A_.plan.ntiles, st.get());
+      hipLaunchKernelGGL(power_finalize_s2, dim3(1), dim3(1024), 0, stream_, red_.get(), A_.plan.red_blocks(), st.get());
       launch_ew(m_, red_.get(), PowerResF{zo, zn, st.get()}, stream_);
       hipLaunchKernelGGL(power_finalize_res, dim3(1), dim3(1024), 0, stream_, red_.get(), ew_grid(m_), st.get(),
                          1e-4);
@@ -800,12 +805,12 @@ void PdlpSolver<V>::evaluate_kkt()
                KktRowsEpi<V>{xhat_.get(), Dc_.get(), ib, yhat_.get(), Dr_.get(), ig, gamma_, cb_orig_.get(),
                              y0_.get(), yu_.get()},
                stream_);
-  launch_reduce(red_.get(), A_.plan.ntiles, 3, scal_.get() + S_KKT_RP2, stream_);
+  launch_reduce(red_.get(), A_.plan.red_blocks(), 3, scal_.get() + S_KKT_RP2, stream_);
   launch_tiled(AT_, at_orig_.get(), red_.get(),
                KktColsEpi<V>{yu_.get(), xhat_.get(), slack_.get(), Dc_.get(), ib, beta_, ig, gamma_, c_orig_.get(),
                              x0_.get(), xu_.get(), su_.get()},
                stream_);
-  launch_reduce(red_.get(), AT_.plan.ntiles, 4, scal_.get() + S_KKT_RD2, stream_);
+  launch_reduce(red_.get(), AT_.plan.red_blocks(), 4, scal_.get() + S_KKT_RD2, stream_);
   read_scalars();
   const double* s = h_scal_;
   l2p_            = std::sqrt(s[S_KKT_RP2]);
@@ -859,10 +864,10 @@ void PdlpSolver<V>::launch_step(int j, bool major, bool fpe, double* ny_slot, do
 #undef CUOPT_DUAL
   if (ev_[2]) CUOPT_HIP_CHECK(hipEventRecord(ev_[2], stream_));
   if (fpe) {
-    launch_reduce(red_.get(), A_.plan.ntiles, 1, ny_slot, stream_);
+    launch_reduce(red_.get(), A_.plan.red_blocks(), 1, ny_slot, stream_);
     launch_tiled(AT_, at_w_.get(), red_.get(), FpeEpi<V>{ybar_.get(), aty_.get(), xbar_.get(), x_[c].get()},
                  stream_);
-    launch_reduce(red_.get(), AT_.plan.ntiles, 2, fpe_slot, stream_);
+    launch_reduce(red_.get(), AT_.plan.red_blocks(), 2, fpe_slot, stream_);
   }
   cur_ = nx;
 }

@@ -7,10 +7,11 @@
 // from LDS and hand the row sum to an epilogue functor, which does the
 // projection / Halpern / residual arithmetic of that row in the same kernel.
 // Rows longer than a tile are split into chunks; every chunk writes a partial
-// sum and the last chunk to arrive (device-scope ticket counter) loads all
-// partials of the row in parallel, merges them in a fixed tree and runs the
-// epilogue.  All summation orders are fixed by the tile plan, so results are
-// bitwise reproducible run to run.
+// sum and a second small launch (long_rows_kernel, one workgroup per long row)
+// merges them in a fixed tree and runs the epilogue.  No workgroup waits on or
+// signals another inside a launch: on a multi-XCD part every device-scope
+// fence would write back and invalidate the XCD's L2.  All summation orders
+// are fixed by the tile plan, so results are bitwise reproducible run to run.
 //
 // Epilogues may also accumulate up to kMaxRed FP64 sums (norms, dots); each
 // workgroup writes its sums to a partials array, reduced afterwards in a fixed
@@ -159,11 +160,6 @@ __device__ inline Acc block_merge(Acc a, uint64_t* lds)
   return r;  // valid in every lane
 }
 
-__device__ inline uint64_t coherent_load_u64(const uint64_t* p)
-{
-  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // ---- the kernel -------------------------------------------------------------
 
 // Epi must provide:
@@ -177,7 +173,6 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
                                                             const typename Epi::Mat* __restrict__ vals,
                                                             PlanDev plan,
                                                             uint64_t* __restrict__ long_part,
-                                                            unsigned* __restrict__ long_cnt,
                                                             double* __restrict__ red_part,
                                                             Epi epi)
 {
@@ -233,34 +228,41 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
       if (sub == 0) epi.row(r, a.value(), red);
     }
   } else {
-    // Chunk of a long row: workgroup partial, then the last chunk to finish
-    // merges all partials of the row with every lane loading in parallel.
-    __shared__ int last_chunk;
-    const int lid    = -mt - 1;
-    const LongRow lr = plan.longrows[lid];
-    Acc a            = Acc::zero();
+    // Chunk of a long row: only the workgroup partial is stored here; the
+    // row's epilogue runs in long_rows_kernel after this launch.
+    Acc a = Acc::zero();
     for (int k = tid; k < nnz; k += kBlock) a.add(prod[k]);
     a = block_merge(a, acc_lds);
-    if (tid == 0) {
-      long_part[t] = a.pack();
-      __threadfence();
-      const unsigned ticket = atomicAdd(long_cnt + lid, 1u);
-      last_chunk            = ticket == (unsigned)lr.nchunks - 1;
-    }
-    __syncthreads();
-    if (last_chunk) {
-      __threadfence();
-      Acc tot = Acc::zero();
-      for (int ch = tid; ch < lr.nchunks; ch += kBlock)
-        tot.merge(Acc::unpack(coherent_load_u64(long_part + lr.first + ch)));
-      tot = block_merge(tot, acc_lds);
-      if (tid == 0) {
-        atomicExch(long_cnt + lid, 0u);
-        epi.row(lr.row, tot.value(), red);
-      }
-    }
+    if (tid == 0) long_part[t] = a.pack();
   }
   if constexpr (Epi::NRED > 0) block_sum_store<Epi::NRED>(red, red_part + (size_t)t * Epi::NRED, red_lds);
+}
+
+// Epilogue of the rows longer than a tile: workgroup b merges the chunk
+// partials of long row b (all lanes load, fixed-tree merge) and runs the row
+// epilogue.  Runs after tiled_spmv_kernel on the same stream, so the kernel
+// boundary makes the partials visible across XCDs without in-kernel fences.
+// Reduction partials go to red_part[(ntiles + b) * NRED ...].
+template <class Epi>
+__global__ __launch_bounds__(kBlock) void long_rows_kernel(PlanDev plan,
+                                                           const uint64_t* __restrict__ long_part,
+                                                           double* __restrict__ red_part,
+                                                           Epi epi)
+{
+  using Acc = typename Epi::Acc;
+  __shared__ double red_lds[kBlock / kWave][kMaxRed];
+  __shared__ uint64_t acc_lds[kBlock / kWave];
+  if (epi.skip()) return;
+  const LongRow lr = plan.longrows[blockIdx.x];
+  Acc tot          = Acc::zero();
+  for (int ch = threadIdx.x; ch < lr.nchunks; ch += kBlock) tot.merge(Acc::unpack(long_part[lr.first + ch]));
+  tot = block_merge(tot, acc_lds);
+  double red[kMaxRed];
+#pragma unroll
+  for (int k = 0; k < kMaxRed; ++k) red[k] = 0.0;
+  if (threadIdx.x == 0) epi.row(lr.row, tot.value(), red);
+  if constexpr (Epi::NRED > 0)
+    block_sum_store<Epi::NRED>(red, red_part + (size_t)(plan.ntiles + blockIdx.x) * Epi::NRED, red_lds);
 }
 
 // out[k] = sum over blocks b (in order of a fixed tree) of part[b * nred + k].
