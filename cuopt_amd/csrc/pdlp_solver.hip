@@ -15,6 +15,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 #include <type_traits>
@@ -391,14 +392,24 @@ struct PlanBufs {
 
 struct MatStruct {  // CSR structure shared by the FP64 and working-precision values
   int nrows = 0, ncols = 0, nnz = 0;
+  int tile = 2048;  // LDS capacity of the tiles (512, 1024 or 2048 nonzeros)
   std::vector<int> h_ptr;
   DevBuf<int> ptr, idx, rowid;
   PlanBufs plan;
 };
 
+// Tile capacity override for experiments (512, 1024 or 2048; anything else keeps the default).
+int tile_from_env(const char* name, int def)
+{
+  const char* v = std::getenv(name);
+  if (!v) return def;
+  const int t = std::atoi(v);
+  return (t == 512 || t == 1024 || t == 2048) ? t : def;
+}
+
 void build_plan(MatStruct& M, hipStream_t s)
 {
-  TilePlan p = plan_tiles(M.h_ptr.data(), M.nrows, kTileNnz, kMaxRowsPerTile, kBlock);
+  TilePlan p = plan_tiles(M.h_ptr.data(), M.nrows, M.tile - kVecPad, kMaxRowsPerTile, kBlock);
   M.plan.ntiles = (int)p.tiles.size();
   M.plan.tiles.resize(p.tiles.size());
   M.plan.tiles.upload(p.tiles, s);
@@ -414,8 +425,15 @@ template <class Epi>
 void launch_tiled(const MatStruct& M, const typename Epi::Mat* vals, double* red, Epi epi, hipStream_t s)
 {
   if (M.plan.ntiles == 0) return;
-  hipLaunchKernelGGL(tiled_spmv_kernel<Epi>, dim3(M.plan.ntiles), dim3(kBlock), 0, s, M.ptr.get(),
-                     M.idx.get(), vals, M.plan.dev(), M.plan.long_part.get(), red, epi);
+  if (M.tile == 512)
+    hipLaunchKernelGGL((tiled_spmv_kernel<Epi, 512>), dim3(M.plan.ntiles), dim3(kBlock), 0, s, M.ptr.get(),
+                       M.idx.get(), vals, M.plan.dev(), M.plan.long_part.get(), red, epi);
+  else if (M.tile == 1024)
+    hipLaunchKernelGGL((tiled_spmv_kernel<Epi, 1024>), dim3(M.plan.ntiles), dim3(kBlock), 0, s, M.ptr.get(),
+                       M.idx.get(), vals, M.plan.dev(), M.plan.long_part.get(), red, epi);
+  else
+    hipLaunchKernelGGL((tiled_spmv_kernel<Epi, 2048>), dim3(M.plan.ntiles), dim3(kBlock), 0, s, M.ptr.get(),
+                       M.idx.get(), vals, M.plan.dev(), M.plan.long_part.get(), red, epi);
   CUOPT_HIP_CHECK(hipGetLastError());
   if (M.plan.nlong > 0) {
     hipLaunchKernelGGL(long_rows_kernel<Epi>, dim3(M.plan.nlong), dim3(kBlock), 0, s, M.plan.dev(),
@@ -608,10 +626,13 @@ void PdlpSolver<V>::build_structures()
   AT_.ncols = m_;
   AT_.nnz   = nnz;
   AT_.h_ptr = tptr;
+  A_.tile  = tile_from_env("CUOPT_AMD_SPMV_TILE_A", 2048);
+  AT_.tile = tile_from_env("CUOPT_AMD_SPMV_TILE_AT", 2048);
   for (MatStruct* M : {&A_, &AT_}) {
     M->ptr.resize(M->nrows + 1);
     M->ptr.upload(M->h_ptr, stream_);
-    M->idx.resize(std::max(nnz, 1));
+    M->idx.resize(nnz + kArrayPad);
+    M->idx.zero(stream_);
     M->rowid.resize(std::max(nnz, 1));
     build_plan(*M, stream_);
   }
@@ -619,8 +640,10 @@ void PdlpSolver<V>::build_structures()
   AT_.idx.upload(at_idx_.data(), nnz, stream_);
   A_.rowid.upload(a_rowid.data(), nnz, stream_);
   AT_.rowid.upload(at_rowid.data(), nnz, stream_);
-  a_orig_.resize(std::max(nnz, 1));
-  at_orig_.resize(std::max(nnz, 1));
+  a_orig_.resize(nnz + kArrayPad);
+  at_orig_.resize(nnz + kArrayPad);
+  a_orig_.zero(stream_);
+  at_orig_.zero(stream_);
   a_orig_.upload(a_val_.data(), nnz, stream_);
   at_orig_.upload(at_val_.data(), nnz, stream_);
   CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));  // host vectors above are temporaries
@@ -661,8 +684,10 @@ void PdlpSolver<V>::setup_scaling()
   launch_ew(n_, nullptr, DivSqrtF{Dc_.get(), colv_.get()}, stream_);
 
   // Scaled matrix values, FP64 (scale_problem_kernel / scale_transposed_problem_kernel).
-  a_s_.resize(std::max(nnz, 1));
-  at_s_.resize(std::max(nnz, 1));
+  a_s_.resize(nnz + kArrayPad);
+  at_s_.resize(nnz + kArrayPad);
+  a_s_.zero(stream_);
+  at_s_.zero(stream_);
   launch_ew(nnz, nullptr, ScaleValsF{a_orig_.get(), A_.rowid.get(), A_.idx.get(), Dr_.get(), Dc_.get(), a_s_.get()},
             stream_);
   launch_ew(nnz, nullptr,
@@ -921,8 +946,10 @@ SolveResult PdlpSolver<V>::run()
     a_w_ = std::move(a_s_);
     at_w_ = std::move(at_s_);
   } else {
-    a_w_.resize(std::max(A_.nnz, 1));
-    at_w_.resize(std::max(A_.nnz, 1));
+    a_w_.resize(A_.nnz + kArrayPad);
+    at_w_.resize(A_.nnz + kArrayPad);
+    a_w_.zero(stream_);
+    at_w_.zero(stream_);
     launch_ew(A_.nnz, nullptr, ConvertF<V>{a_s_.get(), a_w_.get()}, stream_);
     launch_ew(A_.nnz, nullptr, ConvertF<V>{at_s_.get(), at_w_.get()}, stream_);
   }
@@ -1223,11 +1250,13 @@ static void spmv_impl(int m, int n, const int* rowptr, const int* colidx, const 
     M.h_ptr.assign(rowptr, rowptr + m + 1);
     M.ptr.resize(m + 1);
     M.ptr.upload(M.h_ptr, s);
-    M.idx.resize(std::max(M.nnz, 1));
+    M.idx.resize(M.nnz + kArrayPad);
+    M.idx.zero(s);
     M.idx.upload(colidx, M.nnz, s);
     build_plan(M, s);
     std::vector<V> hv(vals, vals + M.nnz), hx(x, x + n);
-    DevBuf<V> dv(std::max(M.nnz, 1)), dx(std::max(n, 1));
+    DevBuf<V> dv(M.nnz + kArrayPad), dx(std::max(n, 1));
+    dv.zero(s);
     DevBuf<double> dy(std::max(m, 1));
     dv.upload(hv.data(), M.nnz, s);
     dx.upload(hx.data(), n, s);

@@ -28,9 +28,36 @@ namespace cuopt_amd {
 
 constexpr int kBlock   = 256;
 constexpr int kWave    = 64;
-constexpr int kTileNnz = 2048;
 constexpr int kMaxRowsPerTile = 1024;
 constexpr int kMaxRed  = 8;
+// LDS product capacity of a tile (template parameter TILE of the kernel).  The
+// planner caps a tile at TILE - kVecPad nonzeros so that the aligned vector
+// loads covering [z0, z1) never need more than TILE slots.
+constexpr int kVecPad = 4;
+// Value / index arrays read by the tiled kernel carry this many trailing
+// elements so the last aligned vector load stays inside the allocation.
+constexpr int kArrayPad = 16;
+
+// 16-byte vector types for the value and index streams.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x2 __attribute__((ext_vector_type(2)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef int i32x2 __attribute__((ext_vector_type(2)));
+
+template <typename Mat>
+struct VecOf;
+template <>
+struct VecOf<float> {
+  static constexpr int N = 4;
+  using V = f32x4;
+  using I = i32x4;
+};
+template <>
+struct VecOf<double> {
+  static constexpr int N = 2;
+  using V = f64x2;
+  using I = i32x2;
+};
 
 struct PlanDev {
   const Tile* tiles;
@@ -167,7 +194,7 @@ __device__ inline Acc block_merge(Acc a, uint64_t* lds)
 //   __device__ bool skip() const;                       uniform early exit
 //   __device__ typename Acc::Elem product(Mat v, int col, int k) const;
 //   __device__ void row(int r, typename Acc::Val sum, double (&red)[kMaxRed]) const;
-template <class Epi>
+template <class Epi, int TILE>
 __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restrict__ rowptr,
                                                             const int* __restrict__ colidx,
                                                             const typename Epi::Mat* __restrict__ vals,
@@ -179,7 +206,7 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
   using Acc  = typename Epi::Acc;
   using Elem = typename Acc::Elem;
   using Mat  = typename Epi::Mat;
-  __shared__ Elem prod[kTileNnz];
+  __shared__ Elem prod[TILE];
   __shared__ double red_lds[kBlock / kWave][kMaxRed];
   __shared__ uint64_t acc_lds[kBlock / kWave];
 
@@ -190,23 +217,34 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
   const int tid  = threadIdx.x;
   const int nnz  = tl.z1 - tl.z0;
 
-  // Stage the tile's products in LDS: all loads of a lane issued before use.
+  // Stage the tile's products in LDS.  Lanes read 16-byte vectors of values
+  // and indices from the aligned base below z0 (elements outside [z0, z1) are
+  // loaded but dropped); all loads of a lane are issued before the gathers.
   {
-    constexpr int kPer = kTileNnz / kBlock;
-    Mat v[kPer];
-    int c[kPer];
+    using VT          = VecOf<Mat>;
+    constexpr int VN  = VT::N;
+    constexpr int kIt = (TILE + kBlock * VN - 1) / (kBlock * VN);
+    const int base    = tl.z0 & ~(VN - 1);
+    typename VT::V v[kIt];
+    typename VT::I c[kIt];
 #pragma unroll
-    for (int it = 0; it < kPer; ++it) {
-      const int e = it * kBlock + tid;
-      if (e < nnz) {
-        v[it] = __builtin_nontemporal_load(vals + tl.z0 + e);
-        c[it] = __builtin_nontemporal_load(colidx + tl.z0 + e);
+    for (int it = 0; it < kIt; ++it) {
+      const int e0 = base + (it * kBlock + tid) * VN;
+      if (e0 < tl.z1) {
+        v[it] = __builtin_nontemporal_load(reinterpret_cast<const typename VT::V*>(vals + e0));
+        c[it] = __builtin_nontemporal_load(reinterpret_cast<const typename VT::I*>(colidx + e0));
       }
     }
 #pragma unroll
-    for (int it = 0; it < kPer; ++it) {
-      const int e = it * kBlock + tid;
-      if (e < nnz) prod[e] = epi.product(v[it], c[it], tl.z0 + e);
+    for (int it = 0; it < kIt; ++it) {
+      const int e0 = base + (it * kBlock + tid) * VN;
+      if (e0 < tl.z1) {
+#pragma unroll
+        for (int i = 0; i < VN; ++i) {
+          const int e = e0 + i;
+          if (e >= tl.z0 && e < tl.z1) prod[e - tl.z0] = epi.product(v[it][i], c[it][i], e);
+        }
+      }
     }
   }
   __syncthreads();
