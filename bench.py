@@ -57,6 +57,8 @@ def main() -> int:
     ap.add_argument("--cpu-iters", type=int, default=60)
     ap.add_argument("--precisions", default="fp32_mixed,fp64")
     ap.add_argument("--solve-time-limit", type=float, default=300.0)
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="also run one plain device-to-device copy of this many MiB (PMC calibration)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -71,6 +73,18 @@ def main() -> int:
     if c_api.device_count() < 1:
         print("bench.py: no HIP device visible", file=sys.stderr)
         return 1
+    if args.copy_calibration_mb:
+        import torch
+
+        nel = args.copy_calibration_mb * (1 << 20) // 4
+        src = torch.ones(nel, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] copy calibration: {args.copy_calibration_mb} MiB read + {args.copy_calibration_mb} MiB "
+              f"written by one copy kernel", file=sys.stderr, flush=True)
+        del src, dst
     t = time.time()
     lp = G.skewed_lp(m=500_000, n=1_000_000, seed=0)
     gen_s = time.time() - t
