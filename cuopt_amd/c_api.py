@@ -273,7 +273,7 @@ def spmv(row_offsets, col_indices, values, x, ncols: int, precision: str = "fp64
 
 
 BENCH_NAMES = ["window_ms", "primal_kernel_ms", "dual_kernel_ms", "iterations", "setup_s", "checks", "status",
-               "precision"]
+               "precision", "event_iterations"]
 
 
 def benchmark_steps(lp, warmup: int, steps: int, precision: str = "fp32_mixed", tol: float = 0.0) -> dict:

@@ -609,7 +609,7 @@ cuopt_int_t cuOptAmdBenchmarkSteps(cuOptOptimizationProblem problem,
   try {
     SolveResult R = solve_lp(lp, conf);
     for (int i = 0; i < n && i < CUOPT_AMD_BENCH_COUNT; ++i) out[i] = R.bench[i];
-    if (n > 6 && R.bench[3] < steps) out[6] = R.status;  // terminated before the window closed
+    if (n > 6 && R.bench[3] == 0) out[6] = R.status;  // terminated before the window closed
   } catch (const std::bad_alloc&) {
     return CUOPT_OUT_OF_MEMORY;
   } catch (const std::exception&) {

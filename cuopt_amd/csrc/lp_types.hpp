@@ -68,7 +68,7 @@ struct SolveResult {
   double solve_time       = 0.0;
   double info[CUOPT_AMD_INFO_COUNT];
   std::vector<double> trace;  // rows of CUOPT_AMD_TRACE_WIDTH
-  double bench[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // see cuOptAmdBenchmarkSteps
+  double bench[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};  // see cuOptAmdBenchmarkSteps
   SolveResult()
   {
     for (double& v : info) v = std::nan("");

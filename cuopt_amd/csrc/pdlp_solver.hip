@@ -17,7 +17,9 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <random>
+#include <string>
 #include <type_traits>
 
 #include "common.hpp"
@@ -393,6 +395,7 @@ struct PlanBufs {
 struct MatStruct {  // CSR structure shared by the FP64 and working-precision values
   int nrows = 0, ncols = 0, nnz = 0;
   int tile = 2048;  // LDS capacity of the tiles (512, 1024 or 2048 nonzeros)
+  int direct_maxlen = 0;  // longest row of a tile still processed one lane per row without LDS
   std::vector<int> h_ptr;
   DevBuf<int> ptr, idx, rowid;
   PlanBufs plan;
@@ -409,7 +412,7 @@ int tile_from_env(const char* name, int def)
 
 void build_plan(MatStruct& M, hipStream_t s)
 {
-  TilePlan p = plan_tiles(M.h_ptr.data(), M.nrows, M.tile - kVecPad, kMaxRowsPerTile, kBlock);
+  TilePlan p = plan_tiles(M.h_ptr.data(), M.nrows, M.tile - kVecPad, kMaxRowsPerTile, kBlock, M.direct_maxlen);
   M.plan.ntiles = (int)p.tiles.size();
   M.plan.tiles.resize(p.tiles.size());
   M.plan.tiles.upload(p.tiles, s);
@@ -507,6 +510,7 @@ class PdlpSolver {
 
   ~PdlpSolver()
   {
+    for (auto& g : graphs_) (void)hipGraphExecDestroy(g.second);
     if (stream_) (void)hipStreamDestroy(stream_);
     if (h_scal_) (void)hipHostFree(h_scal_);
     if (h_step_) (void)hipHostFree(h_step_);
@@ -567,6 +571,8 @@ class PdlpSolver {
   DevBuf<StepScalars> step_sc_;
   int cur_ = 0;
   hipEvent_t* ev_ = nullptr;  // events of the current timed iteration (benchmark window)
+  bool use_graphs_ = true;
+  std::map<std::string, hipGraphExec_t> graphs_;  // captured iteration blocks by pattern
 
   double beta_ = 1.0, gamma_ = 1.0, b_norm_ = 0.0, c_norm_ = 0.0;
   double eta_ = 1.0, omega_ = 1.0, best_omega_ = 1.0, tau_ = 1.0, sigma_ = 1.0;
@@ -628,6 +634,7 @@ void PdlpSolver<V>::build_structures()
   AT_.h_ptr = tptr;
   A_.tile  = tile_from_env("CUOPT_AMD_SPMV_TILE_A", 2048);
   AT_.tile = tile_from_env("CUOPT_AMD_SPMV_TILE_AT", 2048);
+  if (const char* d = std::getenv("CUOPT_AMD_DIRECT_MAXLEN")) A_.direct_maxlen = AT_.direct_maxlen = std::atoi(d);
   for (MatStruct* M : {&A_, &AT_}) {
     M->ptr.resize(M->nrows + 1);
     M->ptr.upload(M->h_ptr, stream_);
@@ -935,6 +942,7 @@ SolveResult PdlpSolver<V>::run()
   CUOPT_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   CUOPT_HIP_CHECK(hipHostMalloc((void**)&h_scal_, S_COUNT * sizeof(double), hipHostMallocDefault));
   CUOPT_HIP_CHECK(hipHostMalloc((void**)&h_step_, sizeof(StepScalars), hipHostMallocDefault));
+  if (const char* g = std::getenv("CUOPT_AMD_GRAPHS")) use_graphs_ = std::atoi(g) != 0;
   obj_sf_  = lp_.maximize ? -1.0 : 1.0;
   obj_off_ = lp_.maximize ? -lp_.offset : lp_.offset;
 
@@ -994,14 +1002,24 @@ SolveResult PdlpSolver<V>::run()
     return R;
   };
 
-  const bool bench         = st_.bench_steps > 0;
-  const long long bench_w  = st_.bench_warmup, bench_end = st_.bench_warmup + st_.bench_steps;
+  // Benchmark window (cuOptAmdBenchmarkSteps): [bench_w, bench_end) is timed on
+  // the production path; the next `ev_iters` iterations run with direct
+  // launches bracketed by HIP events for per-kernel times.  Window edges are
+  // moved to check iterations, where the host synchronises anyway.
+  const bool bench = st_.bench_steps > 0;
+  auto round_up_check = [&](long long v) {
+    while (!is_check(v)) ++v;
+    return v;
+  };
+  const long long bench_w   = bench ? round_up_check(st_.bench_warmup) : -1;
+  const long long bench_end = bench ? round_up_check(bench_w + st_.bench_steps) : -1;
+  const long long ev_end    = bench ? round_up_check(bench_end + std::min<long long>(st_.bench_steps, 400)) : -1;
   clock::time_point tb0;
+  double window_ms = 0.0;
   int bench_checks = 0;
-  std::vector<hipEvent_t> events;  // 3 per timed iteration
+  std::vector<hipEvent_t> events;  // 3 per evented iteration
   auto bench_stop  = [&]() {
     CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
-    const double ms = std::chrono::duration<double, std::milli>(clock::now() - tb0).count();
     double tp = 0.0, td = 0.0;
     const size_t nit = events.size() / 3;
     for (size_t i = 0; i < nit; ++i) {
@@ -1013,20 +1031,29 @@ SolveResult PdlpSolver<V>::run()
     }
     for (hipEvent_t e : events) (void)hipEventDestroy(e);
     events.clear();
-    R.bench[0] = ms;
+    R.bench[0] = window_ms;
     R.bench[1] = nit ? tp / nit : 0.0;
     R.bench[2] = nit ? td / nit : 0.0;
-    R.bench[3] = (double)nit;
+    R.bench[3] = (double)(bench_end - bench_w);
     R.bench[4] = setup_time;
     R.bench[5] = bench_checks;
     R.bench[6] = CUOPT_TERIMINATION_STATUS_ITERATION_LIMIT;
     R.bench[7] = std::is_same<V, double>::value ? 0 : 1;
+    R.bench[8] = (double)nit;
     return finish_with(CUOPT_TERIMINATION_STATUS_ITERATION_LIMIT);
   };
 
   while (true) {
-    if (bench && k == bench_end) return bench_stop();
-    if (bench && k > bench_w && k < bench_end) ++bench_checks;
+    if (bench && k == bench_w) {
+      CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
+      tb0 = clock::now();
+    }
+    if (bench && k == bench_end) {
+      CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
+      window_ms = std::chrono::duration<double, std::milli>(clock::now() - tb0).count();
+    }
+    if (bench && k >= ev_end) return bench_stop();
+    if (bench && k >= bench_w && k < bench_end) ++bench_checks;
     // ---- termination check (k is a check iteration) ----
     const bool is_major = ((k % kMajor) == 0 && k > 0) || k <= 0;
     evaluate_kkt();
@@ -1108,34 +1135,58 @@ SolveResult PdlpSolver<V>::run()
     long long next = k + 1;
     while (!is_check(next)) ++next;
     periodic_prev = false;
-    for (long long s = k; s < next; ++s) {
-      if (bench && s == bench_w) {
-        CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
-        tb0 = clock::now();
+    const bool evented = bench && k >= bench_end;
+    std::string key;
+    if (use_graphs_ && !evented) {
+      key = std::to_string(next - k) + (restarted ? "R:" : ":");
+      for (long long s = k; s < next; ++s) {
+        const long long s2 = s + 2;
+        if (((s + 1) % kMajor == 0) || (s2 % conditional_major(s2)) == 0) key += std::to_string(s - k) + ",";
+        if (((s + 1) % kMajor) == 0) key += "p" + std::to_string(s - k) + ",";
       }
-      if (bench && s == bench_end) {
-        k = s;
-        return bench_stop();
-      }
-      const bool timed = bench && s >= bench_w;
-      hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
-      if (timed) {
-        for (hipEvent_t& e : ev) {
-          CUOPT_HIP_CHECK(hipEventCreate(&e));
-          events.push_back(e);
+    }
+    auto launch_block = [&]() {
+      for (long long s = k; s < next; ++s) {
+        hipEvent_t ev[3] = {nullptr, nullptr, nullptr};
+        if (evented) {
+          for (hipEvent_t& e : ev) {
+            CUOPT_HIP_CHECK(hipEventCreate(&e));
+            events.push_back(e);
+          }
         }
+        ev_ = ev;
+        const long long s2      = s + 2;
+        const bool should_major = ((s + 1) % kMajor == 0) || (s2 % conditional_major(s2)) == 0;
+        const bool periodic     = ((s + 1) % kMajor) == 0;
+        const bool post_restart = restarted && s == k;
+        const bool fpe_step     = periodic || post_restart;
+        double* ny_slot  = scal_.get() + (post_restart ? S_FPE0_NY : S_FPE_NY);
+        double* fpe_slot = scal_.get() + (post_restart ? S_FPE0_I : S_FPE_I);
+        launch_step((int)(s - k), should_major, fpe_step, ny_slot, fpe_slot);
+        ev_ = nullptr;
       }
-      ev_ = ev;
-      const long long s2      = s + 2;
-      const bool should_major = ((s + 1) % kMajor == 0) || (s2 % conditional_major(s2)) == 0;
-      const bool periodic     = ((s + 1) % kMajor) == 0;
-      const bool post_restart = restarted && s == k;
-      const bool fpe_step     = periodic || post_restart;
-      double* ny_slot  = scal_.get() + (post_restart ? S_FPE0_NY : S_FPE_NY);
-      double* fpe_slot = scal_.get() + (post_restart ? S_FPE0_I : S_FPE_I);
-      launch_step((int)(s - k), should_major, fpe_step, ny_slot, fpe_slot);
-      ev_ = nullptr;
-      if (periodic) periodic_prev = true;
+    };
+    for (long long s = k; s < next; ++s)
+      if (((s + 1) % kMajor) == 0) periodic_prev = true;
+    if (!key.empty()) {
+      // Blocks have even length, so the ping-pong buffers are in the same state
+      // at every block start and a captured block is valid for every replay.
+      auto it = graphs_.find(key);
+      if (it == graphs_.end()) {
+        const int cur0 = cur_;
+        hipGraph_t g;
+        CUOPT_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        launch_block();
+        CUOPT_HIP_CHECK(hipStreamEndCapture(stream_, &g));
+        hipGraphExec_t ge;
+        CUOPT_HIP_CHECK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+        CUOPT_HIP_CHECK(hipGraphDestroy(g));
+        it = graphs_.emplace(key, ge).first;
+        cur_ = cur0;
+      }
+      CUOPT_HIP_CHECK(hipGraphLaunch(it->second, stream_));
+    } else {
+      launch_block();
     }
     kr_ += next - k;
     k              = next;

@@ -217,6 +217,23 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
   const int tid  = threadIdx.x;
   const int nnz  = tl.z1 - tl.z0;
 
+  if (mt == kMetaDirect) {
+    // Short rows: one lane per row straight from global memory, summed in
+    // storage order; no LDS staging and no barrier before the epilogues.
+    double red[kMaxRed];
+#pragma unroll
+    for (int k = 0; k < kMaxRed; ++k) red[k] = 0.0;
+    for (int r = tl.r0 + tid; r < tl.r1; r += kBlock) {
+      const int b = rowptr[r], e = rowptr[r + 1];
+      Acc a       = Acc::zero();
+      for (int k = b; k < e; ++k)
+        a.add(epi.product(__builtin_nontemporal_load(vals + k), __builtin_nontemporal_load(colidx + k), k));
+      epi.row(r, a.value(), red);
+    }
+    if constexpr (Epi::NRED > 0) block_sum_store<Epi::NRED>(red, red_part + (size_t)t * Epi::NRED, red_lds);
+    return;
+  }
+
   // Stage the tile's products in LDS.  Lanes read 16-byte vectors of values
   // and indices from the aligned base below z0 (elements outside [z0, z1) are
   // loaded but dropped); all loads of a lane are issued before the gathers.

@@ -73,15 +73,17 @@ cuopt_int_t cuOptAmdSpmv(cuopt_int_t num_rows,
                          cuopt_int_t precision);
 
 /* Benchmark of the PDHG loop (no reference counterpart).  Runs the full solve
- * setup, `warmup` PDHG iterations, then exactly `steps` iterations between two
- * device synchronisations, termination checks at the reference cadence
- * included, and stops.  HIP events on the solver's stream bracket every
- * primal-step and dual-step kernel of the timed window.
+ * setup and `warmup` PDHG iterations, then times `steps` iterations between two
+ * device synchronisations on the production path (graph-captured iteration
+ * blocks, termination checks at the reference cadence included), then runs up
+ * to 400 more iterations with direct launches bracketed by HIP events on the
+ * solver's stream to time the primal-step and dual-step kernels, and stops.
+ * Window edges are moved up to the next termination-check iteration.
  * out (n slots): [0] timed window in ms, [1] mean primal-step kernel ms,
- * [2] mean dual-step kernel ms, [3] iterations timed, [4] setup seconds,
- * [5] termination checks inside the window, [6] termination status,
- * [7] precision used. */
-#define CUOPT_AMD_BENCH_COUNT 8
+ * [2] mean dual-step kernel ms (long-row launch included), [3] iterations in
+ * the timed window, [4] setup seconds, [5] termination checks inside the window,
+ * [6] termination status, [7] precision used, [8] iterations with events. */
+#define CUOPT_AMD_BENCH_COUNT 9
 cuopt_int_t cuOptAmdBenchmarkSteps(cuOptOptimizationProblem problem,
                                    cuOptSolverSettings settings,
                                    cuopt_int_t warmup,

@@ -212,3 +212,20 @@ def test_deterministic_bitwise():
     c = c_api.solve(lp, tol=1e-4, precision="fp32_mixed", time_limit=TL, iteration_limit=3000)
     d = c_api.solve(lp, tol=1e-4, precision="fp32_mixed", time_limit=TL, iteration_limit=3000)
     assert np.array_equal(c.x, d.x) and np.array_equal(c.y, d.y)
+
+
+def test_graph_replays_match_direct_launches(monkeypatch):
+    # Iteration blocks between checks are replayed from captured graphs; every
+    # replay must give bitwise the result of launching the same kernels directly.
+    lp = G.skewed_lp(m=20000, n=40000, seed=7)
+    out = {}
+    for flag in ("1", "0"):
+        monkeypatch.setenv("CUOPT_AMD_GRAPHS", flag)
+        for prec in ("fp64", "fp32_mixed"):
+            out[flag, prec] = c_api.solve(lp, tol=1e-5, precision=prec, time_limit=TL, iteration_limit=50000)
+    for prec in ("fp64", "fp32_mixed"):
+        a, b = out["1", prec], out["0", prec]
+        assert a.status == b.status == 1
+        assert a.iterations == b.iterations
+        assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
+        assert np.array_equal(a.trace, b.trace)
