@@ -119,6 +119,8 @@ def main() -> int:
                 "status": c_api.STATUS.get(r.status, r.status), "iterations": r.iterations,
                 "time_to_tol_s": round(r.solve_time, 4), "loop_s": round(r.info["loop_time"], 4),
                 "setup_s": round(r.info["setup_time"], 4),
+                "setup_phases_s": {k: round(r.info[k + "_time"], 4) for k in ("structure", "scaling", "power")},
+                "power_iterations": int(r.info["power_iterations"]),
                 "rel_primal": r.info["l2_relative_primal_residual"], "rel_dual": r.info["l2_relative_dual_residual"],
                 "rel_gap": r.info["relative_gap"], "objective": r.primal_objective,
                 "obj_rel_err_vs_closed_form": err,

@@ -27,7 +27,8 @@ INFO_NAMES = [
     "steps", "l2_primal_residual", "l2_relative_primal_residual", "l2_dual_residual",
     "l2_relative_dual_residual", "primal_objective", "dual_objective", "gap", "relative_gap",
     "solve_time", "setup_time", "loop_time", "eta0", "omega0", "omega", "restarts", "power_iterations",
-    "sigma_max_sq", "beta", "gamma", "precision", "checks",
+    "sigma_max_sq", "beta", "gamma", "precision", "checks", "structure_time", "scaling_time", "power_time",
+    "refine_rounds",
 ]
 TRACE_WIDTH = 10
 PRECISION = {"fp64": 0, "fp32_mixed": 1, "mixed": 1, "fp32": 1}
@@ -50,6 +51,8 @@ def lib():
         L.cuOptCreateProblem.argtypes = [i32, i32, i32, f64, vp, vp, vp, vp, vp, vp, vp, vp, vp, ctypes.POINTER(vp)]
         L.cuOptCreateRangedProblem.argtypes = [i32, i32, i32, f64, vp, vp, vp, vp, vp, vp, vp, vp, vp,
                                                ctypes.POINTER(vp)]
+        L.cuOptReadProblem.argtypes = [pch, ctypes.POINTER(vp)]
+        L.cuOptWriteProblem.argtypes = [vp, pch, i32]
         for f in ("cuOptDestroyProblem", "cuOptDestroySolverSettings", "cuOptDestroySolution"):
             getattr(L, f).argtypes = [ctypes.POINTER(vp)]
             getattr(L, f).restype = None
@@ -142,9 +145,58 @@ class Problem:
                                       _p(lb), _p(ub), _p(vt), ctypes.byref(self.handle))
         _check(rc, "cuOptCreateProblem")
 
+    @classmethod
+    def read_mps(cls, path: str) -> "Problem":
+        """cuOptReadProblem; the returned problem's ``lp`` is rebuilt from the C getters."""
+        self = cls.__new__(cls)
+        self.handle = ctypes.c_void_p()
+        self._keep = []
+        _check(lib().cuOptReadProblem(str(path).encode(), ctypes.byref(self.handle)), f"cuOptReadProblem {path}")
+        self.lp = problem_to_lp(self.handle)
+        return self
+
+    def write_mps(self, path: str, fmt: int = 0) -> int:
+        """cuOptWriteProblem; returns the C status code."""
+        return lib().cuOptWriteProblem(self.handle, str(path).encode(), fmt)
+
     def __del__(self):
         if getattr(self, "handle", None) and self.handle.value:
             lib().cuOptDestroyProblem(ctypes.byref(self.handle))
+
+
+def problem_to_lp(handle):
+    """An LP from the problem getters (constraint bounds from whichever form the problem holds)."""
+    from .generators import LP
+
+    L = lib()
+    i32 = ctypes.c_int32
+    m, n, nnz, sense = i32(), i32(), i32(), i32()
+    L.cuOptGetNumConstraints(handle, ctypes.byref(m))
+    L.cuOptGetNumVariables(handle, ctypes.byref(n))
+    L.cuOptGetNumNonZeros(handle, ctypes.byref(nnz))
+    L.cuOptGetObjectiveSense(handle, ctypes.byref(sense))
+    m, n, nnz = m.value, n.value, nnz.value
+    off = ctypes.c_double()
+    L.cuOptGetObjectiveOffset(handle, ctypes.byref(off))
+    ptr, idx, val = np.zeros(m + 1, np.int32), np.zeros(max(nnz, 1), np.int32), np.zeros(max(nnz, 1))
+    _check(L.cuOptGetConstraintMatrix(handle, _p(ptr), _p(idx), _p(val)))
+    c, lb, ub = np.zeros(max(n, 1)), np.zeros(max(n, 1)), np.zeros(max(n, 1))
+    L.cuOptGetObjectiveCoefficients(handle, _p(c))
+    L.cuOptGetVariableLowerBounds(handle, _p(lb))
+    L.cuOptGetVariableUpperBounds(handle, _p(ub))
+    lo, hi = np.zeros(max(m, 1)), np.zeros(max(m, 1))
+    if L.cuOptGetConstraintLowerBounds(handle, _p(lo)) == CUOPT_SUCCESS:
+        L.cuOptGetConstraintUpperBounds(handle, _p(hi))
+    else:
+        s, rhs = np.zeros(max(m, 1), np.uint8), np.zeros(max(m, 1))
+        _check(L.cuOptGetConstraintSense(handle, _p(s)))
+        _check(L.cuOptGetConstraintRightHandSide(handle, _p(rhs)))
+        t = s.tobytes().decode()
+        lo = np.array([-np.inf if t[i] == "L" else rhs[i] for i in range(m)])
+        hi = np.array([np.inf if t[i] == "G" else rhs[i] for i in range(m)])
+    return LP(m=m, n=n, row_offsets=ptr, col_indices=idx[:nnz], values=val[:nnz], c=c[:n], con_lb=lo[:m],
+              con_ub=hi[:m], var_lb=lb[:n], var_ub=ub[:n], maximize=sense.value == CUOPT_MAXIMIZE,
+              offset=off.value, name="mps")
 
 
 class SolverSettings:
@@ -239,7 +291,7 @@ def solve_problem(problem: Problem, settings: SolverSettings) -> Result:
 
 def solve(lp, tol: float = 1e-4, precision: str = "fp64", iteration_limit: int | None = None,
           time_limit: float | None = None, x_init=None, y_init=None, method: int = CUOPT_METHOD_PDLP,
-          log: bool = False) -> Result:
+          log: bool = False, refinement: bool | None = None) -> Result:
     """Solve ``lp`` (a generators.LP) through the C ABI with all six tolerances set to ``tol``."""
     prob = Problem(lp)
     s = SolverSettings()
@@ -254,6 +306,8 @@ def solve(lp, tol: float = 1e-4, precision: str = "fp64", iteration_limit: int |
         _check(s.set("iteration_limit", int(iteration_limit)))
     if time_limit is not None:
         _check(s.set("time_limit", float(time_limit)))
+    if refinement is not None:
+        _check(s.set("pdlp_refinement", int(bool(refinement))))
     if x_init is not None:
         s.set_initial_primal(x_init)
     if y_init is not None:

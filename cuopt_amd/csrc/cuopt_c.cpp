@@ -19,6 +19,7 @@
 #include "../../include/cuopt_amd_ext.h"
 #include "common.hpp"
 #include "lp_types.hpp"
+#include "mps_io.hpp"
 #include "settings.hpp"
 
 using namespace cuopt_amd;
@@ -121,6 +122,46 @@ cuopt_int_t cuOptGetVersion(cuopt_int_t* major, cuopt_int_t* minor, cuopt_int_t*
   *major = 0;
   *minor = 1;
   *patch = 0;
+  return CUOPT_SUCCESS;
+}
+
+// cuOptReadProblem (cpp/src/pdlp/cuopt_c.cpp:99-123): open failures give
+// CUOPT_MPS_FILE_ERROR, malformed content CUOPT_MPS_PARSE_ERROR, and the handle
+// is nulled on error.
+cuopt_int_t cuOptReadProblem(const char* filename, cuOptOptimizationProblem* problem_ptr)
+{
+  if (!problem_ptr) return CUOPT_INVALID_ARGUMENT;
+  *problem_ptr = nullptr;
+  if (!filename) return CUOPT_INVALID_ARGUMENT;
+  try {
+    auto* h = new ProblemHandle();
+    try {
+      h->lp = read_mps(filename);
+    } catch (...) {
+      delete h;
+      throw;
+    }
+    *problem_ptr = h;
+  } catch (const mps_file_error&) {
+    return CUOPT_MPS_FILE_ERROR;
+  } catch (const std::bad_alloc&) {
+    return CUOPT_OUT_OF_MEMORY;
+  } catch (const std::exception&) {
+    return CUOPT_MPS_PARSE_ERROR;
+  }
+  return CUOPT_SUCCESS;
+}
+
+// cuOptWriteProblem (cpp/src/pdlp/cuopt_c.cpp:125-143).
+cuopt_int_t cuOptWriteProblem(cuOptOptimizationProblem problem, const char* filename, cuopt_int_t format)
+{
+  if (!problem || !filename || filename[0] == '\0' || format != CUOPT_FILE_FORMAT_MPS)
+    return CUOPT_INVALID_ARGUMENT;
+  try {
+    write_mps(as_problem(problem)->lp, filename);
+  } catch (...) {
+    return CUOPT_MPS_FILE_ERROR;
+  }
   return CUOPT_SUCCESS;
 }
 
@@ -268,28 +309,31 @@ cuopt_int_t cuOptGetConstraintMatrix(cuOptOptimizationProblem problem,
   copy_out(lp.vals, values);
   return CUOPT_SUCCESS;
 }
+// A problem holds the row-type form (created by cuOptCreateProblem), the ranged
+// form (cuOptCreateRangedProblem), or both (read from MPS); a getter of a form
+// the problem does not hold returns CUOPT_INVALID_ARGUMENT.
 cuopt_int_t cuOptGetConstraintSense(cuOptOptimizationProblem problem, char* out)
 {
   CUOPT_PROBLEM_GETTER_PRELUDE(out)
-  if (lp.ranged) return CUOPT_INVALID_ARGUMENT;
+  if (lp.m > 0 && lp.sense.empty()) return CUOPT_INVALID_ARGUMENT;
   return copy_out(lp.sense, out);
 }
 cuopt_int_t cuOptGetConstraintRightHandSide(cuOptOptimizationProblem problem, cuopt_float_t* out)
 {
   CUOPT_PROBLEM_GETTER_PRELUDE(out)
-  if (lp.ranged) return CUOPT_INVALID_ARGUMENT;
+  if (lp.m > 0 && lp.rhs.empty()) return CUOPT_INVALID_ARGUMENT;
   return copy_out(lp.rhs, out);
 }
 cuopt_int_t cuOptGetConstraintLowerBounds(cuOptOptimizationProblem problem, cuopt_float_t* out)
 {
   CUOPT_PROBLEM_GETTER_PRELUDE(out)
-  if (!lp.ranged) return CUOPT_INVALID_ARGUMENT;
+  if (lp.m > 0 && lp.lc.empty()) return CUOPT_INVALID_ARGUMENT;
   return copy_out(lp.lc, out);
 }
 cuopt_int_t cuOptGetConstraintUpperBounds(cuOptOptimizationProblem problem, cuopt_float_t* out)
 {
   CUOPT_PROBLEM_GETTER_PRELUDE(out)
-  if (!lp.ranged) return CUOPT_INVALID_ARGUMENT;
+  if (lp.m > 0 && lp.uc.empty()) return CUOPT_INVALID_ARGUMENT;
   return copy_out(lp.uc, out);
 }
 cuopt_int_t cuOptGetVariableLowerBounds(cuOptOptimizationProblem problem, cuopt_float_t* out)
@@ -606,6 +650,7 @@ cuopt_int_t cuOptAmdBenchmarkSteps(cuOptOptimizationProblem problem,
   conf.bench_warmup  = warmup;
   conf.bench_steps   = steps;
   conf.iteration_limit = std::numeric_limits<int>::max();
+  conf.refinement      = 0;  // the window times the plain PDHG loop
   try {
     SolveResult R = solve_lp(lp, conf);
     for (int i = 0; i < n && i < CUOPT_AMD_BENCH_COUNT; ++i) out[i] = R.bench[i];

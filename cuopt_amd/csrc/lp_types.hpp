@@ -41,6 +41,7 @@ struct PdlpSettings {
   int method           = CUOPT_METHOD_CONCURRENT;
   int presolve         = CUOPT_PRESOLVE_DEFAULT;
   int precision        = CUOPT_AMD_PRECISION_FP64;
+  int refinement       = 1;  // FP64 iterative refinement of FP32-mixed solves
   bool infeasibility_detection = false;
   bool strict_infeasibility    = false;
   bool per_constraint_residual = false;

@@ -1,0 +1,379 @@
+// mps_io.cpp -- see mps_io.hpp.
+#include "mps_io.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <fstream>
+#include <limits>
+#include <sstream>
+#include <unordered_map>
+#include <unordered_set>
+#include <vector>
+
+namespace cuopt_amd {
+
+namespace {
+
+constexpr double kInf = std::numeric_limits<double>::infinity();
+
+std::vector<std::string> tokens(const std::string& line)
+{
+  std::vector<std::string> t;
+  std::istringstream ss(line);
+  std::string w;
+  while (ss >> w) t.push_back(w);
+  return t;
+}
+
+std::string upper(std::string s)
+{
+  for (char& ch : s) ch = (char)std::toupper((unsigned char)ch);
+  return s;
+}
+
+double number(const std::string& s, const std::string& line)
+{
+  char* end = nullptr;
+  const double v = std::strtod(s.c_str(), &end);
+  if (end == s.c_str() || *end != '\0') throw mps_parse_error("bad number '" + s + "' in line: " + line);
+  return v;
+}
+
+}  // namespace
+
+LpHost read_mps(const std::string& path)
+{
+  std::ifstream in(path);
+  if (!in) throw mps_file_error("Error opening MPS file: " + path);
+
+  enum Section { NONE, NAME, OBJSENSE, ROWS, COLUMNS, RHS, RANGES, BOUNDS };
+  Section sec   = NONE;
+  bool maximize = false, integer_block = false;
+  std::string obj_row;
+  std::vector<std::string> row_names;
+  std::vector<char> row_type;
+  std::unordered_map<std::string, int> row_id, col_id;
+  std::vector<std::string> col_names;
+  std::vector<char> vtype;
+  struct Entry {
+    int row, col;
+    double val;
+  };
+  std::vector<Entry> entries;
+  std::vector<double> cost, lb, ub;
+  std::vector<char> has_bound;
+  std::unordered_map<int, double> rhs, range;
+  double offset = 0.0;
+
+  auto col_of = [&](const std::string& name) {
+    auto it = col_id.find(name);
+    if (it != col_id.end()) return it->second;
+    const int j = (int)col_names.size();
+    col_id.emplace(name, j);
+    col_names.push_back(name);
+    vtype.push_back(integer_block ? 'I' : 'C');
+    cost.push_back(0.0);
+    lb.push_back(0.0);
+    ub.push_back(kInf);
+    has_bound.push_back(0);
+    return j;
+  };
+  auto row_of = [&](const std::string& name, const std::string& line) {
+    auto it = row_id.find(name);
+    if (it == row_id.end()) throw mps_parse_error("unknown row '" + name + "' in line: " + line);
+    return it->second;
+  };
+
+  std::string line;
+  while (std::getline(in, line)) {
+    if (!line.empty() && line.back() == '\r') line.pop_back();
+    const auto t = tokens(line);
+    if (t.empty() || t[0][0] == '*') continue;
+    if (!std::isspace((unsigned char)line[0])) {  // section header
+      const std::string h = upper(t[0]);
+      if (h == "NAME") {
+        sec = NAME;
+      } else if (h == "OBJSENSE" || h == "OBJSENCE") {
+        sec = OBJSENSE;
+        if (t.size() > 1) maximize = upper(t[1]).rfind("MAX", 0) == 0;
+      } else if (h == "ROWS") {
+        sec = ROWS;
+      } else if (h == "COLUMNS") {
+        sec = COLUMNS;
+      } else if (h == "RHS") {
+        sec = RHS;
+      } else if (h == "RANGES") {
+        sec = RANGES;
+      } else if (h == "BOUNDS") {
+        sec = BOUNDS;
+      } else if (h == "ENDATA") {
+        break;
+      } else if (h == "QUADOBJ" || h == "QMATRIX" || h == "QSECTION" || h == "QCMATRIX") {
+        throw mps_parse_error("quadratic sections are not supported by the LP solver: " + h);
+      } else if (sec == OBJSENSE && (h.rfind("MAX", 0) == 0 || h.rfind("MIN", 0) == 0)) {
+        maximize = h.rfind("MAX", 0) == 0;
+      } else {
+        throw mps_parse_error("unknown section: " + line);
+      }
+      continue;
+    }
+    switch (sec) {
+      case OBJSENSE: maximize = upper(t[0]).rfind("MAX", 0) == 0; break;
+      case ROWS: {
+        if (t.size() < 2) throw mps_parse_error("bad ROWS line: " + line);
+        const std::string ty = upper(t[0]);
+        if (ty == "N") {
+          if (obj_row.empty()) obj_row = t[1];  // further N rows are free rows, dropped
+          break;
+        }
+        if (ty != "E" && ty != "L" && ty != "G") throw mps_parse_error("bad row type in line: " + line);
+        if (row_id.count(t[1])) throw mps_parse_error("duplicate row '" + t[1] + "'");
+        row_id.emplace(t[1], (int)row_names.size());
+        row_names.push_back(t[1]);
+        row_type.push_back(ty[0]);
+        break;
+      }
+      case COLUMNS: {
+        if (t.size() >= 3 && (t[1] == "'MARKER'" || upper(t[1]) == "MARKER")) {
+          const std::string mk = upper(t[2]);
+          if (mk.find("INTORG") != std::string::npos) integer_block = true;
+          if (mk.find("INTEND") != std::string::npos) integer_block = false;
+          break;
+        }
+        if (t.size() < 3 || t.size() % 2 == 0) throw mps_parse_error("bad COLUMNS line: " + line);
+        const int j = col_of(t[0]);
+        for (size_t p = 1; p + 1 < t.size(); p += 2) {
+          const double v = number(t[p + 1], line);
+          if (t[p] == obj_row) {
+            cost[j] += v;
+          } else {
+            auto it = row_id.find(t[p]);
+            if (it == row_id.end()) {
+              throw mps_parse_error("unknown row '" + t[p] + "' in line: " + line);
+            }
+            entries.push_back(Entry{it->second, j, v});
+          }
+        }
+        break;
+      }
+      case RHS:
+      case RANGES: {
+        // optional set name: an odd token count means the first token names the set
+        const size_t first = (t.size() % 2 == 1) ? 1 : 0;
+        if (t.size() - first < 2) throw mps_parse_error("bad RHS/RANGES line: " + line);
+        for (size_t p = first; p + 1 < t.size(); p += 2) {
+          const double v = number(t[p + 1], line);
+          if (sec == RHS && t[p] == obj_row) {
+            offset = -v;
+          } else if (sec == RHS) {
+            rhs[row_of(t[p], line)] = v;
+          } else {
+            range[row_of(t[p], line)] = v;
+          }
+        }
+        break;
+      }
+      case BOUNDS: {
+        const std::string bt = upper(t[0]);
+        const bool valueless = bt == "FR" || bt == "MI" || bt == "PL" || bt == "BV";
+        // records: type [set] column [value]
+        std::string name;
+        double v = 0.0;
+        if (valueless) {
+          if (t.size() < 2) throw mps_parse_error("bad BOUNDS line: " + line);
+          name = t.size() >= 3 ? t[2] : t[1];
+        } else {
+          if (t.size() < 3) throw mps_parse_error("bad BOUNDS line: " + line);
+          name = t.size() >= 4 ? t[2] : t[1];
+          v    = number(t.size() >= 4 ? t[3] : t[2], line);
+        }
+        const int j       = col_of(name);
+        const bool first  = !has_bound[j];
+        if (bt == "LO") {
+          lb[j] = v;
+        } else if (bt == "UP") {
+          ub[j] = v;
+          if (first && v < 0.0) lb[j] = -kInf;
+        } else if (bt == "FX") {
+          lb[j] = v;
+          ub[j] = v;
+        } else if (bt == "FR") {
+          lb[j] = -kInf;
+          ub[j] = kInf;
+        } else if (bt == "MI") {
+          lb[j] = -kInf;
+        } else if (bt == "PL") {
+          ub[j] = kInf;
+        } else if (bt == "BV") {
+          lb[j]    = 0.0;
+          ub[j]    = 1.0;
+          vtype[j] = 'I';
+        } else if (bt == "LI") {
+          if (first) ub[j] = kInf;
+          lb[j]    = v;
+          vtype[j] = 'I';
+        } else if (bt == "UI") {
+          ub[j] = v;
+          if (first && v < 0.0) lb[j] = -kInf;
+          vtype[j] = 'I';
+        } else if (bt == "SC") {
+          throw mps_parse_error("semi-continuous bounds are not supported: " + line);
+        } else {
+          throw mps_parse_error("invalid bound type in line: " + line);
+        }
+        has_bound[j] = 1;
+        break;
+      }
+      case NAME: break;
+      default: throw mps_parse_error("data line outside a section: " + line);
+    }
+  }
+
+  LpHost lp;
+  lp.m        = (int)row_names.size();
+  lp.n        = (int)col_names.size();
+  lp.maximize = maximize;
+  lp.offset   = offset;
+  lp.c        = cost;
+  lp.lb       = lb;
+  lp.ub       = ub;
+  lp.var_types = vtype;
+  // CSR, entries of a row in column order (duplicates kept as given)
+  std::stable_sort(entries.begin(), entries.end(),
+                   [](const Entry& a, const Entry& b) { return a.row != b.row ? a.row < b.row : a.col < b.col; });
+  lp.rowptr.assign(lp.m + 1, 0);
+  for (const Entry& e : entries) lp.rowptr[e.row + 1]++;
+  for (int i = 0; i < lp.m; ++i) lp.rowptr[i + 1] += lp.rowptr[i];
+  lp.colidx.reserve(entries.size());
+  lp.vals.reserve(entries.size());
+  for (const Entry& e : entries) {
+    lp.colidx.push_back(e.col);
+    lp.vals.push_back(e.val);
+  }
+  lp.sense = row_type;
+  lp.rhs.assign(lp.m, 0.0);
+  for (const auto& kv : rhs) lp.rhs[kv.first] = kv.second;
+  lp.ranged = !range.empty();
+  if (lp.ranged) {
+    lp.lc.resize(lp.m);
+    lp.uc.resize(lp.m);
+    for (int i = 0; i < lp.m; ++i) {
+      const double b = lp.rhs[i];
+      double lo = b, hi = b;
+      if (row_type[i] == 'L') lo = -kInf;
+      if (row_type[i] == 'G') hi = kInf;
+      auto it = range.find(i);
+      if (it != range.end()) {
+        const double r = it->second;
+        if (row_type[i] == 'E') {
+          if (r > 0) hi = b + std::fabs(r);
+          else lo = b - std::fabs(r);
+        } else if (row_type[i] == 'L') {
+          lo = b - std::fabs(r);
+        } else {
+          hi = b + std::fabs(r);
+        }
+      }
+      lp.lc[i] = lo;
+      lp.uc[i] = hi;
+    }
+  }
+  return lp;
+}
+
+void write_mps(const LpHost& lp, const std::string& path)
+{
+  FILE* f = std::fopen(path.c_str(), "w");
+  if (!f) throw mps_file_error("Error opening MPS file for writing: " + path);
+  auto num = [](double v) {
+    char b[64];
+    std::snprintf(b, sizeof(b), "%.17g", v);
+    return std::string(b);
+  };
+  // Row bounds in the ranged form, whatever form the problem was created in.
+  std::vector<double> lo(lp.m), hi(lp.m);
+  for (int i = 0; i < lp.m; ++i) {
+    if (lp.ranged) {
+      lo[i] = lp.lc[i];
+      hi[i] = lp.uc[i];
+    } else {
+      const double b = lp.rhs[i];
+      lo[i]          = lp.sense[i] == 'L' ? -kInf : b;
+      hi[i]          = lp.sense[i] == 'G' ? kInf : b;
+    }
+  }
+  std::fprintf(f, "NAME cuopt_amd\n");
+  if (lp.maximize) std::fprintf(f, "OBJSENSE\n    MAX\n");
+  std::fprintf(f, "ROWS\n N OBJ\n");
+  std::vector<char> ty(lp.m);
+  for (int i = 0; i < lp.m; ++i) {
+    if (lo[i] == hi[i]) ty[i] = 'E';
+    else if (std::isinf(lo[i]) && lo[i] < 0) ty[i] = 'L';
+    else ty[i] = 'G';  // finite lower bound; a finite upper bound becomes a range
+    std::fprintf(f, " %c R%d\n", ty[i], i);
+  }
+  // columns: transpose the CSR rows
+  std::vector<int> cptr(lp.n + 1, 0);
+  const int nnz = lp.rowptr[lp.m];
+  for (int k = 0; k < nnz; ++k) cptr[lp.colidx[k] + 1]++;
+  for (int j = 0; j < lp.n; ++j) cptr[j + 1] += cptr[j];
+  std::vector<int> crow(nnz);
+  std::vector<double> cval(nnz);
+  {
+    std::vector<int> cur(cptr.begin(), cptr.end() - 1);
+    for (int i = 0; i < lp.m; ++i)
+      for (int k = lp.rowptr[i]; k < lp.rowptr[i + 1]; ++k) {
+        const int d = cur[lp.colidx[k]]++;
+        crow[d]     = i;
+        cval[d]     = lp.vals[k];
+      }
+  }
+  std::fprintf(f, "COLUMNS\n");
+  bool in_int = false;
+  for (int j = 0; j < lp.n; ++j) {
+    const bool is_int = !lp.var_types.empty() && lp.var_types[j] == 'I';
+    if (is_int != in_int) {
+      std::fprintf(f, "    MARKER 'MARKER' '%s'\n", is_int ? "INTORG" : "INTEND");
+      in_int = is_int;
+    }
+    std::fprintf(f, "    C%d OBJ %s\n", j, num(lp.c[j]).c_str());
+    for (int k = cptr[j]; k < cptr[j + 1]; ++k) std::fprintf(f, "    C%d R%d %s\n", j, crow[k], num(cval[k]).c_str());
+  }
+  if (in_int) std::fprintf(f, "    MARKER 'MARKER' 'INTEND'\n");
+  std::fprintf(f, "RHS\n");
+  if (lp.offset != 0.0) std::fprintf(f, "    RHS OBJ %s\n", num(-lp.offset).c_str());
+  for (int i = 0; i < lp.m; ++i) {
+    const double b = ty[i] == 'L' ? hi[i] : lo[i];
+    if (b != 0.0) std::fprintf(f, "    RHS R%d %s\n", i, num(b).c_str());
+  }
+  bool any_range = false;
+  for (int i = 0; i < lp.m; ++i)
+    if (ty[i] == 'G' && std::isfinite(hi[i])) {
+      if (!any_range) std::fprintf(f, "RANGES\n");
+      any_range = true;
+      std::fprintf(f, "    RNG R%d %s\n", i, num(hi[i] - lo[i]).c_str());
+    }
+  std::fprintf(f, "BOUNDS\n");
+  for (int j = 0; j < lp.n; ++j) {
+    const double l = lp.lb[j], u = lp.ub[j];
+    if (l == u) {
+      std::fprintf(f, " FX BND C%d %s\n", j, num(l).c_str());
+      continue;
+    }
+    if (std::isinf(l) && std::isinf(u)) {
+      std::fprintf(f, " FR BND C%d\n", j);
+      continue;
+    }
+    if (std::isinf(l)) std::fprintf(f, " MI BND C%d\n", j);
+    else if (l != 0.0) std::fprintf(f, " LO BND C%d %s\n", j, num(l).c_str());
+    if (std::isfinite(u)) std::fprintf(f, " UP BND C%d %s\n", j, num(u).c_str());
+  }
+  std::fprintf(f, "ENDATA\n");
+  const bool ok = std::ferror(f) == 0;
+  std::fclose(f);
+  if (!ok) throw mps_file_error("Error writing MPS file: " + path);
+}
+
+}  // namespace cuopt_amd

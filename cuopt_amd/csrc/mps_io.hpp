@@ -1,0 +1,35 @@
+// mps_io.hpp -- MPS reader and writer behind cuOptReadProblem / cuOptWriteProblem.
+//
+// Reading follows the semantics of the reference parser
+// (cpp/libmps_parser/src/mps_parser.cpp): whitespace-separated (free) records,
+// which also covers fixed-format files whose names hold no blanks; OBJSENSE as a
+// one-line or two-line section; the right-hand side of the objective row is the
+// negated objective offset; integer markers in COLUMNS; RANGES by row type
+// (E: sign picks [b, b+|R|] or [b-|R|, b]; L: [b-|R|, b]; G: [b, b+|R|]);
+// BOUNDS types UP LO FX FR MI PL BV LI UI with the CPLEX rule that a negative
+// UP / UI on a variable without an earlier bound sets its lower bound to -inf
+// (mps_parser.cpp:1370-1440).  Default variable bounds are [0, +inf).
+#pragma once
+
+#include <stdexcept>
+#include <string>
+
+#include "lp_types.hpp"
+
+namespace cuopt_amd {
+
+struct mps_file_error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+struct mps_parse_error : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+// Problems without RANGES keep the row-type form (sense / rhs); with RANGES the
+// ranged form (lc / uc) is what the solver uses, and sense / rhs stay readable.
+LpHost read_mps(const std::string& path);
+
+// Free-format MPS with generated names R<i> / C<j>; values printed to round-trip.
+void write_mps(const LpHost& lp, const std::string& path);
+
+}  // namespace cuopt_amd

@@ -83,23 +83,26 @@ struct PrimalStepEpi {
   const StepScalars* sc;
   int j;
   __device__ bool skip() const { return false; }
+  struct Pre {
+    V x, c, x0;
+    V2 b;
+  };
   __device__ typename Acc::Elem product(V v, int col, int) const { return Acc::prod(v, y[col]); }
-  __device__ void row(int r, V aty, double (&)[kMaxRed]) const
+  __device__ Pre load(int r) const { return Pre{x[r], c[r], x0[r], bnd[r]}; }
+  __device__ void row(int r, V aty, const Pre& p, double (&)[kMaxRed]) const
   {
     const V tau     = (V)sc->tau;
     const V w       = halpern_weight<V>(sc, j);
-    const V xr      = x[r];
-    const V next    = xr - tau * (c[r] - aty);
-    const V2 b      = bnd[r];
-    const V clamped = mx(mn(next, b.y), b.x);
-    const V xb      = V(2) * clamped - xr;
+    const V next    = p.x - tau * (p.c - aty);
+    const V clamped = mx(mn(next, p.b.y), p.b.x);
+    const V xb      = V(2) * clamped - p.x;
     xbar[r]         = xb;
     if constexpr (MAJOR) {
       xhat[r]  = clamped;
       slack[r] = (clamped - next) / tau;
     }
     if constexpr (FPE) aty_out[r] = aty;
-    xnext[r] = w * xb + (V(1) - w) * x0[r];
+    xnext[r] = w * xb + (V(1) - w) * p.x0;
   }
 };
 
@@ -119,25 +122,39 @@ struct DualStepEpi {
   V* ybar;
   const StepScalars* sc;
   int j;
+  // Refinement rounds only (else null): per-row position of the kink of the
+  // dual objective term, which sits at y = 0 in the original problem and at
+  // -Dd * y* / Dr in the coordinates of a correction problem.
+  const V* kink;
   __device__ bool skip() const { return false; }
+  struct Pre {
+    V y, y0, k;
+    V2 b;
+  };
   __device__ typename Acc::Elem product(V v, int col, int) const { return Acc::prod(v, xbar[col]); }
-  __device__ void row(int r, V ax, double (&red)[kMaxRed]) const
+  __device__ Pre load(int r) const { return Pre{y[r], y0[r], kink ? kink[r] : V(0), cb[r]}; }
+  __device__ void row(int r, V ax, const Pre& p, double (&red)[kMaxRed]) const
   {
     const V sigma = (V)sc->sigma;
     const V w     = halpern_weight<V>(sc, j);
-    const V yi    = y[r];
-    const V t     = yi / sigma - ax;
-    const V2 b    = cb[r];
-    const V proj  = mx(-b.y, mn(t, -b.x));
-    const V nd    = (t - proj) * sigma;
-    const V yb    = V(2) * nd - yi;
+    V nd;
+    if (kink) {  // the projection of the original problem, moved to the kink
+      const V t    = (p.y - p.k) / sigma - ax;
+      const V proj = mx(-p.b.y, mn(t, -p.b.x));
+      nd           = (t - proj) * sigma + p.k;
+    } else {
+      const V t    = p.y / sigma - ax;
+      const V proj = mx(-p.b.y, mn(t, -p.b.x));
+      nd           = (t - proj) * sigma;
+    }
+    const V yb    = V(2) * nd - p.y;
     if constexpr (MAJOR) yhat[r] = nd;
     if constexpr (FPE) {
       ybar[r]        = yb;
-      const double d = (double)yb - (double)yi;
+      const double d = (double)yb - (double)p.y;
       red[0] += d * d;
     }
-    ynext[r] = w * yb + (V(1) - w) * y0[r];
+    ynext[r] = w * yb + (V(1) - w) * p.y0;
   }
 };
 
@@ -153,11 +170,15 @@ struct FpeEpi {
   const V* xbar;
   const V* x;
   __device__ bool skip() const { return false; }
+  struct Pre {
+    V aty, xbar, x;
+  };
   __device__ typename Acc::Elem product(V v, int col, int) const { return Acc::prod(v, ybar[col]); }
-  __device__ void row(int r, V atyb, double (&red)[kMaxRed]) const
+  __device__ Pre load(int r) const { return Pre{aty[r], xbar[r], x[r]}; }
+  __device__ void row(int, V atyb, const Pre& p, double (&red)[kMaxRed]) const
   {
-    const double dx = (double)xbar[r] - (double)x[r];
-    const double t  = (double)atyb - (double)aty[r];
+    const double dx = (double)p.xbar - (double)p.x;
+    const double t  = (double)atyb - (double)p.aty;
     red[0] += t * dx;
     red[1] += dx * dx;
   }
@@ -180,26 +201,46 @@ struct KktRowsEpi {
   const double2* cb_orig;  // unscaled constraint bounds
   const V* y0;
   double* yu_out;
+  // Refinement rounds (else null): the iterates are corrections around the
+  // FP64 point (xstar, ystar) in units 1/inv_dp, 1/inv_dd; the residuals are
+  // those of the original problem at the corrected point, and the iterates are
+  // not rewritten.
+  const double* xstar;
+  const double* ystar;
+  double inv_dp, inv_dd;
   __device__ bool skip() const { return false; }
   __device__ double product(double v, int col, int) const
   {
-    return v * (((double)xhat[col] * Dc[col]) * inv_beta);
+    const double xs = ((double)xhat[col] * Dc[col]) * inv_beta;
+    return v * (xstar ? xstar[col] + xs * inv_dp : xs);
   }
-  __device__ void row(int r, double ax, double (&red)[kMaxRed]) const
+  struct Pre {
+    double dr;
+    double2 b;
+    V yhat, y0;
+  };
+  __device__ Pre load(int r) const { return Pre{Dr[r], cb_orig[r], yhat[r], y0[r]}; }
+  __device__ void row(int r, double ax, const Pre& p, double (&red)[kMaxRed]) const
   {
-    const double dr = Dr[r];
-    const double yu = ((double)yhat[r] * dr) * inv_gamma;
-    const double2 b = cb_orig[r];
+    const double dr = p.dr;
+    double yu       = ((double)p.yhat * dr) * inv_gamma;
+    if (ystar) yu = ystar[r] + yu * inv_dd;
+    const double2 b = p.b;
     const double cl = mx(b.x, mn(ax, b.y));
     const double rp = ax - cl;
     red[0] += rp * rp;
     red[1] += mx(yu, 0.0) * fin0(b.x) + mn(yu, 0.0) * fin0(b.y);
     yu_out[r]     = yu;
+    if (ystar) {
+      const double d = (double)p.y0 - (double)p.yhat;
+      red[2] += d * d;
+      return;
+    }
     double ys     = dr != 0.0 ? yu / dr : yu;
     ys            = ys * gamma;
     const V ynew  = (V)ys;
     yhat[r]       = ynew;
-    const double d = (double)y0[r] - 1.0 * (double)ynew;
+    const double d = (double)p.y0 - 1.0 * (double)ynew;
     red[2] += d * d;
   }
 };
@@ -220,15 +261,26 @@ struct KktColsEpi {
   const V* x0;
   double* xu_out;
   double* su_out;
+  const double* xstar;  // refinement rounds, as in KktRowsEpi (else null)
+  double inv_dp, inv_dd;
   __device__ bool skip() const { return false; }
+  struct Pre {
+    double dc, c;
+    V xhat, slack, x0;
+  };
   __device__ double product(double v, int col, int) const { return v * yu[col]; }
-  __device__ void row(int r, double aty, double (&red)[kMaxRed]) const
+  __device__ Pre load(int r) const { return Pre{Dc[r], c_orig[r], xhat[r], slack[r], x0[r]}; }
+  __device__ void row(int r, double aty, const Pre& p, double (&red)[kMaxRed]) const
   {
-    const double dc = Dc[r];
-    const double xu = ((double)xhat[r] * dc) * inv_beta;
-    double su       = dc != 0.0 ? (double)slack[r] / dc : (double)slack[r];
+    const double dc = p.dc;
+    double xu       = ((double)p.xhat * dc) * inv_beta;
+    double su       = dc != 0.0 ? (double)p.slack / dc : (double)p.slack;
     su              = su * inv_gamma;
-    const double cj = c_orig[r];
+    if (xstar) {
+      xu = xstar[r] + xu * inv_dp;
+      su = su * inv_dd;
+    }
+    const double cj = p.c;
     const double g  = cj - aty;
     const double rd = g - su;
     red[0] += rd * rd;
@@ -236,6 +288,11 @@ struct KktColsEpi {
     red[2] += su * xu;
     xu_out[r] = xu;
     su_out[r] = su;
+    if (xstar) {
+      const double d = (double)p.x0 - (double)p.xhat;
+      red[3] += d * d;
+      return;
+    }
     double xs = dc != 0.0 ? xu / dc : xu;
     xs        = xs * beta;
     double ss = su * dc;
@@ -243,7 +300,7 @@ struct KktColsEpi {
     const V xnew = (V)xs;
     xhat[r]      = xnew;
     slack[r]     = (V)ss;
-    const double d = (double)x0[r] - 1.0 * (double)xnew;
+    const double d = (double)p.x0 - 1.0 * (double)xnew;
     red[3] += d * d;
   }
 };
@@ -266,8 +323,10 @@ struct PowerAtEpi {  // atq = A^T (z / ||z||)
   const PowerState* st;
   double* atq;
   __device__ bool skip() const { return st->done != 0.0; }
+  struct Pre {};
   __device__ double product(double v, int col, int) const { return v * (z[col] / st->nrm); }
-  __device__ void row(int r, double s, double (&)[kMaxRed]) const { atq[r] = s; }
+  __device__ Pre load(int) const { return Pre{}; }
+  __device__ void row(int r, double s, const Pre&, double (&)[kMaxRed]) const { atq[r] = s; }
 };
 
 struct PowerAEpi {  // znew = A atq, red[0] = q . znew
@@ -279,11 +338,15 @@ struct PowerAEpi {  // znew = A atq, red[0] = q . znew
   const PowerState* st;
   double* znew;
   __device__ bool skip() const { return st->done != 0.0; }
+  struct Pre {
+    double zold;
+  };
   __device__ double product(double v, int col, int) const { return v * atq[col]; }
-  __device__ void row(int r, double s, double (&red)[kMaxRed]) const
+  __device__ Pre load(int r) const { return Pre{zold[r]}; }
+  __device__ void row(int r, double s, const Pre& p, double (&red)[kMaxRed]) const
   {
     znew[r]        = s;
-    const double q = zold[r] / st->nrm;
+    const double q = p.zold / st->nrm;
     red[0] += q * s;
   }
 };
@@ -311,7 +374,9 @@ struct ScaleStatEpi {
     if constexpr (MAXNORM) return a;
     return alpha_pow == 1.0 ? a : pow(a, alpha_pow);
   }
-  __device__ void row(int r, double s, double (&)[kMaxRed]) const { out[r] = s; }
+  struct Pre {};
+  __device__ Pre load(int) const { return Pre{}; }
+  __device__ void row(int r, double s, const Pre&, double (&)[kMaxRed]) const { out[r] = s; }
 };
 
 // ---- plain y = A x (FP64 or compensated FP32) for the test entry point.
@@ -323,8 +388,10 @@ struct PlainEpi {
   const V* x;
   double* y;
   __device__ bool skip() const { return false; }
+  struct Pre {};
   __device__ typename Acc::Elem product(V v, int col, int) const { return Acc::prod(v, x[col]); }
-  __device__ void row(int r, V s, double (&)[kMaxRed]) const { y[r] = (double)s; }
+  __device__ Pre load(int) const { return Pre{}; }
+  __device__ void row(int r, V s, const Pre&, double (&)[kMaxRed]) const { y[r] = (double)s; }
 };
 
 }  // namespace cuopt_amd

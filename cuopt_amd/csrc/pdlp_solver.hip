@@ -361,6 +361,74 @@ struct RestartCopyF {  // anchor = current = hat (cupdlpx_restart, pdlp_restart_
   }
 };
 
+// ---- refinement rounds (FP32-mixed solves refined in FP64) -----------------
+
+// x* = the last FP64 primal point, clamped into the variable bounds so that the
+// zero correction is feasible for the shifted bounds.
+struct CaptureColsF {
+  static constexpr int NRED = 0;
+  const double* xu;
+  const double* lb;
+  const double* ub;
+  double* xstar;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int j, double (&)[kMaxRed]) const { xstar[j] = fmin(fmax(xu[j], lb[j]), ub[j]); }
+};
+
+// Correction problem, columns: objective Dd * (c - A^T y*), bounds Dp * ([l, u] - x*),
+// both in the scaled coordinates of the solver (x~ = x / Dc, c~ = c * Dc).
+template <typename V>
+struct CorrectionColsF {
+  using V2 = typename Vec2<V>::type;
+  static constexpr int NRED = 0;
+  const double* xstar;
+  const double* aty;
+  const double* c;
+  const double* lb;
+  const double* ub;
+  const double* Dc;
+  double dp, dd;
+  V* c_w;
+  V2* bnd_w;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int j, double (&)[kMaxRed]) const
+  {
+    const double d = Dc[j], x = xstar[j];
+    c_w[j]         = (V)((dd * (c[j] - aty[j])) * d);
+    V2 b;
+    b.x      = d != 0.0 ? (V)((dp * (lb[j] - x)) / d) : V(0);
+    b.y      = d != 0.0 ? (V)((dp * (ub[j] - x)) / d) : V(0);
+    bnd_w[j] = b;
+  }
+};
+
+// Correction problem, rows: constraint bounds Dp * ([l_c, u_c] - A x*) (scaled by
+// Dr) and the kink of the dual term at y = 0, i.e. at -Dd * y* / Dr.
+template <typename V>
+struct CorrectionRowsF {
+  using V2 = typename Vec2<V>::type;
+  static constexpr int NRED = 0;
+  const double* ystar;
+  const double* ax;
+  const double* lc;
+  const double* uc;
+  const double* Dr;
+  double dp, dd;
+  V2* cb_w;
+  V* kink;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int i, double (&)[kMaxRed]) const
+  {
+    const double r = Dr[i];
+    V2 b;
+    b.x     = (V)((dp * (lc[i] - ax[i])) * r);
+    b.y     = (V)((dp * (uc[i] - ax[i])) * r);
+    cb_w[i] = b;
+    const double k = -dd * ystar[i];
+    kink[i] = (V)(r != 0.0 ? k / r : k);
+  }
+};
+
 // ---------------------------------------------------------------------------
 // Host helpers
 // ---------------------------------------------------------------------------
@@ -395,7 +463,6 @@ struct PlanBufs {
 struct MatStruct {  // CSR structure shared by the FP64 and working-precision values
   int nrows = 0, ncols = 0, nnz = 0;
   int tile = 2048;  // LDS capacity of the tiles (512, 1024 or 2048 nonzeros)
-  int direct_maxlen = 0;  // longest row of a tile still processed one lane per row without LDS
   std::vector<int> h_ptr;
   DevBuf<int> ptr, idx, rowid;
   PlanBufs plan;
@@ -412,7 +479,7 @@ int tile_from_env(const char* name, int def)
 
 void build_plan(MatStruct& M, hipStream_t s)
 {
-  TilePlan p = plan_tiles(M.h_ptr.data(), M.nrows, M.tile - kVecPad, kMaxRowsPerTile, kBlock, M.direct_maxlen);
+  TilePlan p = plan_tiles(M.h_ptr.data(), M.nrows, M.tile - kVecPad, kMaxRowsPerTile, kBlock);
   M.plan.ntiles = (int)p.tiles.size();
   M.plan.tiles.resize(p.tiles.size());
   M.plan.tiles.upload(p.tiles, s);
@@ -469,6 +536,23 @@ constexpr double kSufficient      = 0.2;    // sufficient_reduction_for_restart
 constexpr double kNecessary       = 0.8;    // necessary_reduction_for_restart
 constexpr double kKp = 0.99, kKi = 0.01, kKd = 0.0, kISmooth = 0.3;  // restart PID (pdlp_hyper_params.cuh:51-54)
 constexpr int kRuizIters          = 10;
+// FP64 refinement of FP32-mixed solves: tolerance of the FP32 solve, and the
+// reduction of the relative KKT error that ends a correction round.
+constexpr double kRefineFrom      = 1e-4;
+constexpr double kRefineGain      = 1e-3;
+// pdhg_loop result that ends a correction round without an answer.
+constexpr int kRoundDone          = -1;
+
+struct Tolerances {
+  double ap, rp, ad, rd, ag, rg;  // absolute / relative primal, dual, gap
+};
+
+struct RoundSpec {
+  Tolerances tol;                 // optimality test on the original problem
+  bool refine      = false;       // iterates are a correction around (x*, y*)
+  double target    = 0.0;         // correction round: stop at this relative KKT error
+  long long cap    = 0;           // correction round: iteration budget
+};
 constexpr double kPcAlpha         = 1.0;
 
 inline bool is_check(long long k)
@@ -541,6 +625,17 @@ class PdlpSolver {
   void evaluate_kkt();
   void launch_step(int j, bool major, bool fpe, double* ny_slot, double* fpe_slot);
   void finish(SolveResult& R, int status);
+  int pdhg_loop(SolveResult& R, const RoundSpec& spec);
+  void begin_refinement_round();
+  // Termination test of the last evaluation (termination_strategy.cu:311-352).
+  bool is_optimal(const Tolerances& t) const
+  {
+    return l2d_ <= t.ad + t.rd * c_norm_ && l2p_ <= t.ap + t.rp * b_norm_ && gap_ <= t.ag + t.rg * absobj_;
+  }
+  double rel_kkt_error() const
+  {
+    return std::max({l2p_ / (1.0 + b_norm_), l2d_ / (1.0 + c_norm_), gap_ / (1.0 + absobj_)});
+  }
 
   const LpHost& lp_;
   const std::vector<double>&lc_, &uc_, &lb_, &ub_;
@@ -573,6 +668,17 @@ class PdlpSolver {
   hipEvent_t* ev_ = nullptr;  // events of the current timed iteration (benchmark window)
   bool use_graphs_ = true;
   std::map<std::string, hipGraphExec_t> graphs_;  // captured iteration blocks by pattern
+
+  // refinement rounds: the FP64 point (x*, y*), A x*, A^T y*, dual kinks
+  DevBuf<double> xstar_, ystar_, ax_, aty64_;
+  DevBuf<V> kink_;
+  struct {
+    double dp = 0.0, dd = 0.0;  // primal / dual scale of the current correction
+  } refine_;
+  bool refine_active_ = false;
+  long long k_done_   = 0;  // iterations of all finished loops
+  int restarts_ = 0, checks_ = 0;
+  double setup_time_ = 0.0;
 
   double beta_ = 1.0, gamma_ = 1.0, b_norm_ = 0.0, c_norm_ = 0.0;
   double eta_ = 1.0, omega_ = 1.0, best_omega_ = 1.0, tau_ = 1.0, sigma_ = 1.0;
@@ -634,7 +740,6 @@ void PdlpSolver<V>::build_structures()
   AT_.h_ptr = tptr;
   A_.tile  = tile_from_env("CUOPT_AMD_SPMV_TILE_A", 2048);
   AT_.tile = tile_from_env("CUOPT_AMD_SPMV_TILE_AT", 2048);
-  if (const char* d = std::getenv("CUOPT_AMD_DIRECT_MAXLEN")) A_.direct_maxlen = AT_.direct_maxlen = std::atoi(d);
   for (MatStruct* M : {&A_, &AT_}) {
     M->ptr.resize(M->nrows + 1);
     M->ptr.upload(M->h_ptr, stream_);
@@ -833,14 +938,18 @@ template <typename V>
 void PdlpSolver<V>::evaluate_kkt()
 {
   const double ib = 1.0 / beta_, ig = 1.0 / gamma_;
+  const bool rf   = refine_active_;
+  const double* xs = rf ? xstar_.get() : nullptr;
+  const double* ys = rf ? ystar_.get() : nullptr;
+  const double idp = rf ? 1.0 / refine_.dp : 1.0, idd = rf ? 1.0 / refine_.dd : 1.0;
   launch_tiled(A_, a_orig_.get(), red_.get(),
                KktRowsEpi<V>{xhat_.get(), Dc_.get(), ib, yhat_.get(), Dr_.get(), ig, gamma_, cb_orig_.get(),
-                             y0_.get(), yu_.get()},
+                             y0_.get(), yu_.get(), xs, ys, idp, idd},
                stream_);
   launch_reduce(red_.get(), A_.plan.red_blocks(), 3, scal_.get() + S_KKT_RP2, stream_);
   launch_tiled(AT_, at_orig_.get(), red_.get(),
                KktColsEpi<V>{yu_.get(), xhat_.get(), slack_.get(), Dc_.get(), ib, beta_, ig, gamma_, c_orig_.get(),
-                             x0_.get(), xu_.get(), su_.get()},
+                             x0_.get(), xu_.get(), su_.get(), xs, idp, idd},
                stream_);
   launch_reduce(red_.get(), AT_.plan.red_blocks(), 4, scal_.get() + S_KKT_RD2, stream_);
   read_scalars();
@@ -874,7 +983,8 @@ void PdlpSolver<V>::launch_step(int j, bool major, bool fpe, double* ny_slot, do
 #define CUOPT_DUAL(MAJ, FP)                                                                                  \
   launch_tiled(A_, a_w_.get(), red_.get(),                                                                    \
                DualStepEpi<V, MAJ, FP>{xbar_.get(), y_[c].get(), cb_w_.get(), y0_.get(), y_[nx].get(),        \
-                                       yhat_.get(), ybar_.get(), step_sc_.get(), j},                          \
+                                       yhat_.get(), ybar_.get(), step_sc_.get(), j,                           \
+                                       refine_active_ ? kink_.get() : nullptr},                               \
                stream_)
   if (ev_[0]) CUOPT_HIP_CHECK(hipEventRecord(ev_[0], stream_));
   if (major) {
@@ -946,9 +1056,19 @@ SolveResult PdlpSolver<V>::run()
   obj_sf_  = lp_.maximize ? -1.0 : 1.0;
   obj_off_ = lp_.maximize ? -lp_.offset : lp_.offset;
 
+  auto phase_end = [&](int slot, double since) {
+    CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
+    const double now = elapsed();
+    R.info[slot]     = now - since;
+    return now;
+  };
+  double tph = elapsed();
   build_structures();
+  tph = phase_end(CUOPT_AMD_INFO_STRUCTURE_TIME, tph);
   setup_scaling();
+  tph = phase_end(CUOPT_AMD_INFO_SCALING_TIME, tph);
   power_iteration();
+  phase_end(CUOPT_AMD_INFO_POWER_TIME, tph);
   // working-precision matrices
   if constexpr (std::is_same<V, double>::value) {
     a_w_ = std::move(a_s_);
@@ -979,34 +1099,121 @@ SolveResult PdlpSolver<V>::run()
   R.info[CUOPT_AMD_INFO_GAMMA]        = gamma_;
   R.info[CUOPT_AMD_INFO_PRECISION]    = std::is_same<V, double>::value ? 0 : 1;
   CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
-  const double setup_time           = elapsed();
-  R.info[CUOPT_AMD_INFO_SETUP_TIME] = setup_time;
+  setup_time_                       = elapsed();
+  R.info[CUOPT_AMD_INFO_SETUP_TIME] = setup_time_;
 
-  const double tol_ap = st_.abs_primal, tol_rp = st_.rel_primal, tol_ad = st_.abs_dual, tol_rd = st_.rel_dual;
-  const double tol_ag = st_.abs_gap, tol_rg = st_.rel_gap;
+  const Tolerances user{st_.abs_primal, st_.rel_primal, st_.abs_dual, st_.rel_dual, st_.abs_gap, st_.rel_gap};
+  const bool refine = !std::is_same<V, double>::value && st_.refinement != 0 && st_.bench_steps <= 0 &&
+                      std::min({user.ap, user.rp, user.ad, user.rd, user.ag, user.rg}) < kRefineFrom;
+  int rounds = 0;
+  auto finish_with = [&](int status) {
+    finish(R, status);
+    R.info[CUOPT_AMD_INFO_STEPS]         = (double)k_done_;
+    R.info[CUOPT_AMD_INFO_RESTARTS]      = restarts_;
+    R.info[CUOPT_AMD_INFO_CHECKS]        = checks_;
+    R.info[CUOPT_AMD_INFO_LOOP_TIME]     = R.solve_time - setup_time_;
+    R.info[CUOPT_AMD_INFO_REFINE_ROUNDS] = rounds;
+    return R;
+  };
+  if (!refine) return finish_with(pdhg_loop(R, RoundSpec{user}));
+
+  // FP32 solve to the looser tolerance, then FP64 refinement rounds.
+  Tolerances loose = user;
+  for (double* t : {&loose.ap, &loose.rp, &loose.ad, &loose.rd, &loose.ag, &loose.rg}) *t = std::max(*t, kRefineFrom);
+  int status = pdhg_loop(R, RoundSpec{loose});
+  const long long first_iters = k_done_;
+  while (status == CUOPT_TERIMINATION_STATUS_OPTIMAL || status == kRoundDone) {
+    if (is_optimal(user)) return finish_with(CUOPT_TERIMINATION_STATUS_OPTIMAL);
+    begin_refinement_round();
+    ++rounds;
+    RoundSpec spec{user};
+    spec.refine = true;
+    spec.target = kRefineGain * rel_kkt_error();
+    spec.cap    = std::max<long long>(2 * first_iters, 2000);
+    status      = pdhg_loop(R, spec);
+  }
+  return finish_with(status);
+}
+
+template <typename V>
+void PdlpSolver<V>::begin_refinement_round()
+{
+  // The last check left the FP64 point of the original problem in xu_ / yu_
+  // and its residual norms in l2p_, l2d_, gap_.  Scale factors put the
+  // residuals of the correction problem at about one.
+  double dp = 1.0 / std::max(l2p_, 1e-14 * (1.0 + b_norm_));
+  double dd = 1.0 / std::max(l2d_, 1e-14 * (1.0 + c_norm_));
+  dp        = std::max(dp, 1.0);
+  dd        = std::max(dd, 1.0);
+  if (dp * dd * gap_ > 1.0) {  // also keep the correction's duality gap at about one
+    const double f = std::sqrt(1.0 / (dp * dd * gap_));
+    dp = std::max(dp * f, 1.0);
+    dd = std::max(dd * f, 1.0);
+  }
+  // Primal weight carried over into the units of the new correction problem:
+  // scaled primal iterates were x * beta / Dc before the first round and
+  // Dp * (x - x*) / Dc after it (dual: y * gamma / Dr, Dd * (y - y*) / Dr).
+  const double w_prev = omega_;
+  const double p_prev = refine_.dp > 0.0 ? refine_.dp : beta_;
+  const double d_prev = refine_.dd > 0.0 ? refine_.dd : gamma_;
+  const double ratio  = (dd / d_prev) / (dp / p_prev);
+  if (refine_.dp == 0.0) {
+    xstar_.resize(n_);
+    ystar_.resize(m_);
+    ax_.resize(m_);
+    aty64_.resize(n_);
+    kink_.resize(m_);
+  }
+  refine_.dp = dp;
+  refine_.dd = dd;
+  launch_ew(n_, nullptr, CaptureColsF{xu_.get(), lb_orig_.get(), ub_orig_.get(), xstar_.get()}, stream_);
+  CUOPT_HIP_CHECK(hipMemcpyAsync(ystar_.get(), yu_.get(), (size_t)m_ * sizeof(double), hipMemcpyDeviceToDevice,
+                                 stream_));
+  launch_tiled(A_, a_orig_.get(), nullptr, PlainEpi<double>{xstar_.get(), ax_.get()}, stream_);
+  launch_tiled(AT_, at_orig_.get(), nullptr, PlainEpi<double>{ystar_.get(), aty64_.get()}, stream_);
+  launch_ew(n_, nullptr,
+            CorrectionColsF<V>{xstar_.get(), aty64_.get(), c_orig_.get(), lb_orig_.get(), ub_orig_.get(), Dc_.get(),
+                               dp, dd, c_w_.get(), bnd_w_.get()},
+            stream_);
+  launch_ew(m_, nullptr,
+            CorrectionRowsF<V>{ystar_.get(), ax_.get(), lc_orig_.get(), uc_orig_.get(), Dr_.get(), dp, dd,
+                               cb_w_.get(), kink_.get()},
+            stream_);
+  // Correction problems are solved without the bound / objective rescaling.
+  beta_  = 1.0;
+  gamma_ = 1.0;
+  launch_ew(n_, nullptr,
+            InitIterateF<V, true>{nullptr, Dc_.get(), 1.0, false, bnd_w_.get(), x_[0].get(), xhat_.get(), x0_.get()},
+            stream_);
+  launch_ew(m_, nullptr,
+            InitIterateF<V, false>{nullptr, Dr_.get(), 1.0, false, nullptr, y_[0].get(), yhat_.get(), y0_.get()},
+            stream_);
+  launch_ew(n_, nullptr, ZeroF<V>{slack_.get()}, stream_);
+  cur_   = 0;
+  omega_ = best_omega_ = w_prev * ratio;
+  tau_   = eta_ / omega_;
+  sigma_ = eta_ * omega_;
+}
+
+template <typename V>
+int PdlpSolver<V>::pdhg_loop(SolveResult& R, const RoundSpec& spec)
+{
+  const Tolerances& tol = spec.tol;
   const double nan    = std::numeric_limits<double>::quiet_NaN();
   double fpe = nan, fpe0 = nan, last_trial = std::numeric_limits<double>::infinity();
   double err_sum = 0.0, last_err = 0.0, best_gap = std::numeric_limits<double>::infinity();
   long long k        = 0;
-  int restarts       = 0, checks = 0;
   bool restarted_prev = false;   // a restart happened at the previous check
   bool periodic_prev  = false;   // the previous block ran a periodic FPE step
+  const long long k0  = k_done_;  // iterations of earlier rounds
   kr_ = 0;
-
-  auto finish_with = [&](int status) {
-    finish(R, status);
-    R.info[CUOPT_AMD_INFO_STEPS]     = (double)k;
-    R.info[CUOPT_AMD_INFO_RESTARTS]  = restarts;
-    R.info[CUOPT_AMD_INFO_CHECKS]    = checks;
-    R.info[CUOPT_AMD_INFO_LOOP_TIME] = R.solve_time - setup_time;
-    return R;
-  };
+  refine_active_ = spec.refine;
 
   // Benchmark window (cuOptAmdBenchmarkSteps): [bench_w, bench_end) is timed on
   // the production path; the next `ev_iters` iterations run with direct
   // launches bracketed by HIP events for per-kernel times.  Window edges are
   // moved to check iterations, where the host synchronises anyway.
-  const bool bench = st_.bench_steps > 0;
+  const bool bench = st_.bench_steps > 0 && !spec.refine;
   auto round_up_check = [&](long long v) {
     while (!is_check(v)) ++v;
     return v;
@@ -1018,6 +1225,10 @@ SolveResult PdlpSolver<V>::run()
   double window_ms = 0.0;
   int bench_checks = 0;
   std::vector<hipEvent_t> events;  // 3 per evented iteration
+  auto done = [&](int status) {
+    k_done_ = k0 + k;
+    return status;
+  };
   auto bench_stop  = [&]() {
     CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
     double tp = 0.0, td = 0.0;
@@ -1035,12 +1246,12 @@ SolveResult PdlpSolver<V>::run()
     R.bench[1] = nit ? tp / nit : 0.0;
     R.bench[2] = nit ? td / nit : 0.0;
     R.bench[3] = (double)(bench_end - bench_w);
-    R.bench[4] = setup_time;
+    R.bench[4] = setup_time_;
     R.bench[5] = bench_checks;
     R.bench[6] = CUOPT_TERIMINATION_STATUS_ITERATION_LIMIT;
     R.bench[7] = std::is_same<V, double>::value ? 0 : 1;
     R.bench[8] = (double)nit;
-    return finish_with(CUOPT_TERIMINATION_STATUS_ITERATION_LIMIT);
+    return done(CUOPT_TERIMINATION_STATUS_ITERATION_LIMIT);
   };
 
   while (true) {
@@ -1057,25 +1268,23 @@ SolveResult PdlpSolver<V>::run()
     // ---- termination check (k is a check iteration) ----
     const bool is_major = ((k % kMajor) == 0 && k > 0) || k <= 0;
     evaluate_kkt();
-    ++checks;
+    ++checks_;
     if (restarted_prev) {
       fpe  = std::sqrt((h_scal_[S_FPE0_NX] * omega_ + h_scal_[S_FPE0_NY] / omega_) + 2.0 * h_scal_[S_FPE0_I] * eta_);
       fpe0 = fpe;
     }
     if (periodic_prev)
       fpe = std::sqrt((h_scal_[S_FPE_NX] * omega_ + h_scal_[S_FPE_NY] / omega_) + 2.0 * h_scal_[S_FPE_I] * eta_);
-    const bool optimal_gap = gap_ <= tol_ag + tol_rg * absobj_;
-    const bool pfeas       = l2p_ <= tol_ap + tol_rp * b_norm_;
-    const bool optimal     = l2d_ <= tol_ad + tol_rd * c_norm_ && pfeas && optimal_gap;
-    const size_t rec       = R.trace.size();
-    R.trace.insert(R.trace.end(), {(double)k, l2p_, l2d_, pobj_, dobj_, gap_, 0.0, omega_, fpe, (double)kr_});
-    if (k > 1 && optimal) return finish_with(CUOPT_TERIMINATION_STATUS_OPTIMAL);
+    const size_t rec = R.trace.size();
+    R.trace.insert(R.trace.end(), {(double)(k0 + k), l2p_, l2d_, pobj_, dobj_, gap_, 0.0, omega_, fpe, (double)kr_});
+    if (k > 1 && is_optimal(tol)) return done(CUOPT_TERIMINATION_STATUS_OPTIMAL);
     // Divergence guard: a NaN residual can never satisfy the test again, so stop
     // with a numerical error instead of running into the limits.
     if (std::isnan(l2p_) || std::isnan(l2d_) || std::isnan(gap_))
-      return finish_with(CUOPT_TERIMINATION_STATUS_NUMERICAL_ERROR);
-    if (elapsed() >= st_.time_limit) return finish_with(CUOPT_TERIMINATION_STATUS_TIME_LIMIT);
-    if (k >= st_.iteration_limit) return finish_with(CUOPT_TERIMINATION_STATUS_ITERATION_LIMIT);
+      return done(CUOPT_TERIMINATION_STATUS_NUMERICAL_ERROR);
+    if (elapsed() >= st_.time_limit) return done(CUOPT_TERIMINATION_STATUS_TIME_LIMIT);
+    if (k0 + k >= st_.iteration_limit) return done(CUOPT_TERIMINATION_STATUS_ITERATION_LIMIT);
+    if (spec.refine && k > 1 && (rel_kkt_error() <= spec.target || k >= spec.cap)) return done(kRoundDone);
 
     bool restarted = false;
     if (is_major) {
@@ -1124,7 +1333,7 @@ SolveResult PdlpSolver<V>::run()
         kr_        = 0;
         last_trial = std::numeric_limits<double>::infinity();
         restarted  = true;
-        ++restarts;
+        ++restarts_;
       }
     }
     R.trace[rec + 6] = restarted ? 1.0 : 0.0;
@@ -1138,7 +1347,7 @@ SolveResult PdlpSolver<V>::run()
     const bool evented = bench && k >= bench_end;
     std::string key;
     if (use_graphs_ && !evented) {
-      key = std::to_string(next - k) + (restarted ? "R:" : ":");
+      key = std::to_string(next - k) + (restarted ? "R" : "") + (spec.refine ? "K:" : ":");
       for (long long s = k; s < next; ++s) {
         const long long s2 = s + 2;
         if (((s + 1) % kMajor == 0) || (s2 % conditional_major(s2)) == 0) key += std::to_string(s - k) + ",";

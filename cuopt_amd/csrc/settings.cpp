@@ -114,6 +114,7 @@ SolverSettingsRegistry::SolverSettingsRegistry()
     {"mip_reliability_branching", si(-1), -1, imax, -1},
     {CUOPT_AMD_PDLP_PRECISION, &p.precision, CUOPT_AMD_PRECISION_FP64, CUOPT_AMD_PRECISION_FP32_MIXED,
      CUOPT_AMD_PRECISION_FP64},
+    {CUOPT_AMD_PDLP_REFINEMENT, &p.refinement, 0, 1, 1},
   };
   // Bool parameters (solver_settings.cu:109-123).
   bools_ = {

@@ -30,6 +30,8 @@ constexpr int kBlock   = 256;
 constexpr int kWave    = 64;
 constexpr int kMaxRowsPerTile = 1024;
 constexpr int kMaxRed  = 8;
+// Rows per lane group whose operands are loaded before the matrix stream.
+constexpr int kPreRows = 4;
 // LDS product capacity of a tile (template parameter TILE of the kernel).  The
 // planner caps a tile at TILE - kVecPad nonzeros so that the aligned vector
 // loads covering [z0, z1) never need more than TILE slots.
@@ -191,9 +193,11 @@ __device__ inline Acc block_merge(Acc a, uint64_t* lds)
 
 // Epi must provide:
 //   using Mat, Acc;  static constexpr int NRED (0..kMaxRed);
+//   struct Pre;                                         per-row operands of the epilogue
 //   __device__ bool skip() const;                       uniform early exit
 //   __device__ typename Acc::Elem product(Mat v, int col, int k) const;
-//   __device__ void row(int r, typename Acc::Val sum, double (&red)[kMaxRed]) const;
+//   __device__ Pre load(int r) const;                   issued before the products
+//   __device__ void row(int r, typename Acc::Val sum, const Pre& p, double (&red)[kMaxRed]) const;
 template <class Epi, int TILE>
 __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restrict__ rowptr,
                                                             const int* __restrict__ colidx,
@@ -217,21 +221,25 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
   const int tid  = threadIdx.x;
   const int nnz  = tl.z1 - tl.z0;
 
-  if (mt == kMetaDirect) {
-    // Short rows: one lane per row straight from global memory, summed in
-    // storage order; no LDS staging and no barrier before the epilogues.
-    double red[kMaxRed];
+  // Row-run tiles: load each lane's first kPreRows row pointers and epilogue
+  // operands now, so their latency overlaps the matrix stream below instead of
+  // being paid once per row after the barrier.
+  const int G    = mt >= 0 ? 1 << mt : 1;
+  const int sub  = tid & (G - 1);
+  const int grp  = mt >= 0 ? tid >> mt : 0;
+  const int ngrp = mt >= 0 ? kBlock >> mt : 0;
+  int rb[kPreRows], re[kPreRows];
+  typename Epi::Pre pre[kPreRows];
+  if (mt >= 0) {
 #pragma unroll
-    for (int k = 0; k < kMaxRed; ++k) red[k] = 0.0;
-    for (int r = tl.r0 + tid; r < tl.r1; r += kBlock) {
-      const int b = rowptr[r], e = rowptr[r + 1];
-      Acc a       = Acc::zero();
-      for (int k = b; k < e; ++k)
-        a.add(epi.product(__builtin_nontemporal_load(vals + k), __builtin_nontemporal_load(colidx + k), k));
-      epi.row(r, a.value(), red);
+    for (int i = 0; i < kPreRows; ++i) {
+      const int r = tl.r0 + grp + i * ngrp;
+      if (r < tl.r1) {
+        rb[i]  = rowptr[r] - tl.z0;
+        re[i]  = rowptr[r + 1] - tl.z0;
+        pre[i] = epi.load(r);
+      }
     }
-    if constexpr (Epi::NRED > 0) block_sum_store<Epi::NRED>(red, red_part + (size_t)t * Epi::NRED, red_lds);
-    return;
   }
 
   // Stage the tile's products in LDS.  Lanes read 16-byte vectors of values
@@ -271,16 +279,23 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
   for (int k = 0; k < kMaxRed; ++k) red[k] = 0.0;
 
   if (mt >= 0) {
-    const int G    = 1 << mt;
-    const int sub  = tid & (G - 1);
-    const int grp  = tid >> mt;
-    const int ngrp = kBlock >> mt;
-    for (int r = tl.r0 + grp; r < tl.r1; r += ngrp) {
+#pragma unroll
+    for (int i = 0; i < kPreRows; ++i) {
+      const int r = tl.r0 + grp + i * ngrp;
+      if (r < tl.r1) {
+        Acc a = Acc::zero();
+        for (int k = rb[i] + sub; k < re[i]; k += G) a.add(prod[k]);
+        for (int off = G >> 1; off > 0; off >>= 1) a.merge(a.shfl(off));
+        if (sub == 0) epi.row(r, a.value(), pre[i], red);
+      }
+    }
+    // Tiles with more rows than kPreRows per lane group: the rest load as they go.
+    for (int r = tl.r0 + grp + kPreRows * ngrp; r < tl.r1; r += ngrp) {
       const int b = rowptr[r] - tl.z0, e = rowptr[r + 1] - tl.z0;
       Acc a = Acc::zero();
       for (int k = b + sub; k < e; k += G) a.add(prod[k]);
       for (int off = G >> 1; off > 0; off >>= 1) a.merge(a.shfl(off));
-      if (sub == 0) epi.row(r, a.value(), red);
+      if (sub == 0) epi.row(r, a.value(), epi.load(r), red);
     }
   } else {
     // Chunk of a long row: only the workgroup partial is stored here; the
@@ -315,7 +330,7 @@ __global__ __launch_bounds__(kBlock) void long_rows_kernel(PlanDev plan,
   double red[kMaxRed];
 #pragma unroll
   for (int k = 0; k < kMaxRed; ++k) red[k] = 0.0;
-  if (threadIdx.x == 0) epi.row(lr.row, tot.value(), red);
+  if (threadIdx.x == 0) epi.row(lr.row, tot.value(), epi.load(lr.row), red);
   if constexpr (Epi::NRED > 0)
     block_sum_store<Epi::NRED>(red, red_part + (size_t)(plan.ntiles + blockIdx.x) * Epi::NRED, red_lds);
 }

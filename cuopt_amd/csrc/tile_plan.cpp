@@ -28,7 +28,7 @@ int pick_lanes_log2(int rows, int maxlen, int block)
 
 }  // namespace
 
-TilePlan plan_tiles(const int* rowptr, int nrows, int tile_nnz, int max_rows, int block, int direct_maxlen)
+TilePlan plan_tiles(const int* rowptr, int nrows, int tile_nnz, int max_rows, int block)
 {
   TilePlan p;
   int r = 0;
@@ -56,8 +56,7 @@ TilePlan plan_tiles(const int* rowptr, int nrows, int tile_nnz, int max_rows, in
       ++r;
     }
     p.tiles.push_back(Tile{r0, r, z0, rowptr[r]});
-    const int lg = pick_lanes_log2(r - r0, maxlen, block);
-    p.meta.push_back(lg == 0 && maxlen <= direct_maxlen ? kMetaDirect : lg);
+    p.meta.push_back(pick_lanes_log2(r - r0, maxlen, block));
   }
   return p;
 }

@@ -29,22 +29,15 @@ struct alignas(16) LongRow {
   int pad;
 };
 
-// meta value of a row-run tile whose rows are all short: one lane per row,
-// reading the value / index streams directly (no LDS staging).
-constexpr int kMetaDirect = 16;
-
 struct TilePlan {
   std::vector<Tile> tiles;
-  // meta[t] in [0, 6]: log2(lanes per row) for a row-run tile staged in LDS;
-  // meta[t] == kMetaDirect: row-run tile processed one lane per row from global memory;
+  // meta[t] in [0, 6]: log2(lanes per row) for a row-run tile;
   // meta[t] <  0: chunk of long row number -(meta[t] + 1).
   std::vector<int> meta;
   std::vector<LongRow> longrows;
 };
 
 // Partition rows [0, nrows) of a CSR matrix with offsets rowptr.
-// Tiles whose longest row has at most direct_maxlen nonzeros (and that the cost
-// model would give one lane per row) use the direct mode; 0 disables it.
-TilePlan plan_tiles(const int* rowptr, int nrows, int tile_nnz, int max_rows, int block, int direct_maxlen);
+TilePlan plan_tiles(const int* rowptr, int nrows, int tile_nnz, int max_rows, int block);
 
 }  // namespace cuopt_amd

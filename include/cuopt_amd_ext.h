@@ -22,6 +22,15 @@ extern "C" {
 #define CUOPT_AMD_PRECISION_FP64       0
 #define CUOPT_AMD_PRECISION_FP32_MIXED 1
 
+/* Integer parameter (0/1, default 1) enabling FP64 iterative refinement of
+ * FP32-mixed solves (no reference counterpart).  When the requested tolerances
+ * are tighter than 1e-4, the FP32 solve stops at 1e-4 and FP32 PDHG then
+ * re-solves correction problems: the original LP written in the coordinates
+ * (x - x*) * Dp, (y - y*) * Dd around the current FP64 solution (x*, y*), with
+ * Dp, Dd the inverse primal / dual residual norms.  Optimality is always
+ * tested in FP64 on the original problem.  Ignored by FP64 solves. */
+#define CUOPT_AMD_PDLP_REFINEMENT "pdlp_refinement"
+
 /* Slots of cuOptAmdGetTerminationInfo.  Slots 0-9 mirror
  * additional_termination_information_t (pdlp/solver_solution.hpp:53-93). */
 #define CUOPT_AMD_INFO_STEPS               0  /* number_of_steps_taken */
@@ -46,7 +55,11 @@ extern "C" {
 #define CUOPT_AMD_INFO_GAMMA               19 /* objective rescaling */
 #define CUOPT_AMD_INFO_PRECISION           20
 #define CUOPT_AMD_INFO_CHECKS              21 /* termination evaluations */
-#define CUOPT_AMD_INFO_COUNT               22
+#define CUOPT_AMD_INFO_STRUCTURE_TIME      22 /* setup: CSR sort, transpose, tile plans, uploads (s) */
+#define CUOPT_AMD_INFO_SCALING_TIME        23 /* setup: Ruiz, Pock-Chambolle, bound/objective rescaling (s) */
+#define CUOPT_AMD_INFO_POWER_TIME          24 /* setup: power iteration for the step size (s) */
+#define CUOPT_AMD_INFO_REFINE_ROUNDS       25 /* FP64 refinement rounds run after the FP32 solve */
+#define CUOPT_AMD_INFO_COUNT               26
 
 /* Copies up to n slots of the last solve's statistics into info. */
 cuopt_int_t cuOptAmdGetTerminationInfo(cuOptSolution solution, cuopt_float_t* info, cuopt_int_t n);

@@ -125,6 +125,15 @@ cuopt_int_t cuOptGetVersion(cuopt_int_t* version_major,
                             cuopt_int_t* version_minor,
                             cuopt_int_t* version_patch);
 
+/* ---- MPS files (cuopt_c.h:112, :124) --------------------------------------- */
+/* Reads a free- or fixed-format MPS file into a new problem.  Returns
+ * CUOPT_MPS_FILE_ERROR when the file cannot be opened and CUOPT_MPS_PARSE_ERROR
+ * for malformed content (problem_ptr is then NULL).  cuopt_c.h:102-112 */
+cuopt_int_t cuOptReadProblem(const char* filename, cuOptOptimizationProblem* problem_ptr);
+/* Writes the problem as free-format MPS; format must be CUOPT_FILE_FORMAT_MPS.
+ * cuopt_c.h:114-126 */
+cuopt_int_t cuOptWriteProblem(cuOptOptimizationProblem problem, const char* filename, cuopt_int_t format);
+
 /* ---- problem (cuopt_c.h:171, :240, :414) ---------------------------------- */
 /* minimize/maximize c^T x + offset  s.t.  A x {<=,>=,=} rhs,  lb <= x <= ub.
  * A is CSR with num_constraints+1 row offsets.  cuopt_c.h:171-184 */

@@ -22,6 +22,8 @@ from cuopt_amd.mps import read_mps  # noqa: E402
 CASES = {
     "transport_small": ("transportation_lp", dict(sources=200, sinks=500, arcs=50, seed=3)),
     "transport_medium": ("transportation_lp", dict(sources=2000, sinks=5000, arcs=50, seed=4)),
+    # the full 20k x 50k transportation LP (n = 1M arcs, m = 70k rows)
+    "transport_full": ("transportation_lp", dict(sources=20000, sinks=50000, arcs=50, seed=0)),
     "skewed_small": ("skewed_lp", dict(m=2000, n=4000, seed=5)),
     "random_small": ("random_lp", dict(m=200, n=500, seed=6)),
 }

@@ -153,3 +153,52 @@ def test_solve_without_gpu_fails_cleanly():
     r = c_api.solve(lp, iteration_limit=10)
     assert r.error == c_api.CUOPT_RUNTIME_ERROR
     assert "HIP" in r.error_string
+
+
+# ---- MPS files through cuOptReadProblem / cuOptWriteProblem ---------------------
+GOLDEN_MPS = ["afiro_original.mps", "good-max.mps", "max_offset.mps", "lp-model-no-constraints.mps",
+              "good-mps-fixed-ranges.mps", "lp_model_with_var_bounds.mps"]
+
+
+def assert_same_lp(a, b):
+    assert (a.m, a.n, a.maximize) == (b.m, b.n, b.maximize)
+    assert a.offset == b.offset
+    assert np.array_equal(a.row_offsets, b.row_offsets) and np.array_equal(a.col_indices, b.col_indices)
+    for f in ("values", "c", "con_lb", "con_ub", "var_lb", "var_ub"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+
+
+@pytest.mark.parametrize("fname", GOLDEN_MPS)
+def test_read_mps_matches_python_reader(fname):
+    path = os.path.join(GOLDEN, fname)
+    assert_same_lp(c_api.Problem.read_mps(path).lp, read_mps(path))
+
+
+@pytest.mark.parametrize("fname", GOLDEN_MPS)
+def test_write_then_read_mps_round_trips(fname, tmp_path):
+    p = c_api.Problem.read_mps(os.path.join(GOLDEN, fname))
+    out = tmp_path / "w.mps"
+    assert p.write_mps(out) == c_api.CUOPT_SUCCESS
+    assert_same_lp(c_api.Problem.read_mps(out).lp, p.lp)
+
+
+def test_read_mps_known_values():
+    # afiro: 27 rows, 32 columns, 83 nonzeros; good-max maximises with OBJSENSE
+    afiro = c_api.Problem.read_mps(os.path.join(GOLDEN, "afiro_original.mps")).lp
+    assert (afiro.m, afiro.n, afiro.nnz) == (27, 32, 83)
+    assert c_api.Problem.read_mps(os.path.join(GOLDEN, "good-max.mps")).lp.maximize
+
+
+def test_read_mps_errors(tmp_path):
+    L = c_api.lib()
+    h = ctypes.c_void_p(1)
+    assert L.cuOptReadProblem(str(tmp_path / "missing.mps").encode(), ctypes.byref(h)) == c_api.CUOPT_MPS_FILE_ERROR
+    assert h.value is None
+    bad = tmp_path / "bad.mps"
+    bad.write_text("NAME x\nROWS\n N OBJ\n L R1\nCOLUMNS\n    X1 R9 1.0\nENDATA\n")
+    assert L.cuOptReadProblem(str(bad).encode(), ctypes.byref(h)) == c_api.CUOPT_MPS_PARSE_ERROR
+    bad.write_text("NAME x\nROWS\n N OBJ\nCOLUMNS\n    X1 OBJ abc\nENDATA\n")
+    assert L.cuOptReadProblem(str(bad).encode(), ctypes.byref(h)) == c_api.CUOPT_MPS_PARSE_ERROR
+    p = c_api.Problem.read_mps(os.path.join(GOLDEN, "afiro_original.mps"))
+    assert p.write_mps(tmp_path / "x.mps", fmt=7) == c_api.CUOPT_INVALID_ARGUMENT
+    assert L.cuOptWriteProblem(p.handle, b"", 0) == c_api.CUOPT_INVALID_ARGUMENT

@@ -180,6 +180,53 @@ def test_transport_matches_highs(highs):
     assert rel(r.primal_objective, case["objective"]) < 1e-6
 
 
+def primal_violation(lp, x):
+    """Independent numpy check: ||Ax - clamp(Ax, l_c, u_c)||_2 and the largest bound violation of x."""
+    ax = lp.to_scipy() @ x
+    rp = ax - np.clip(ax, lp.con_lb, lp.con_ub)
+    bv = np.max(np.maximum(lp.var_lb - x, 0.0) + np.maximum(x - lp.var_ub, 0.0), initial=0.0)
+    return float(np.linalg.norm(rp)), float(bv)
+
+
+def bound_norm(lp):
+    l, u = lp.con_lb, lp.con_ub
+    t = np.where(np.isfinite(l) & (l != u), l * l, 0.0) + np.where(np.isfinite(u), u * u, 0.0)
+    return float(np.sqrt(t.sum()))
+
+
+# FP32-mixed solves refined in FP64: the FP32 loop stops at 1e-4 and correction
+# rounds take the original problem's FP64 KKT test to 1e-8.  Pinned by the HiGHS
+# optimum (transportation, <= rows and = rows) and the closed form (skewed LP
+# with = and <= rows, whose duals on <= rows exercise the dual kink).
+def test_fp32_refinement_transport_1e8(highs):
+    case = highs["transport_medium"]
+    lp = G.transportation_lp(**case["kwargs"])
+    r = c_api.solve(lp, tol=1e-8, precision="fp32_mixed", time_limit=TL, iteration_limit=2_000_000)
+    assert r.status == 1, (r.status, r.info)
+    assert r.info["refine_rounds"] >= 1
+    assert rel(r.primal_objective, case["objective"]) < 1e-7
+    rp, bv = primal_violation(lp, r.x)
+    assert rp <= 1e-8 * (1.0 + bound_norm(lp)) * 1.01 and bv <= 1e-12
+
+
+def test_fp32_refinement_skewed_1e8():
+    lp = G.skewed_lp(m=20000, n=40000, seed=5)
+    r = c_api.solve(lp, tol=1e-8, precision="fp32_mixed", time_limit=TL, iteration_limit=2_000_000)
+    assert r.status == 1, (r.status, r.info)
+    assert r.info["refine_rounds"] >= 1
+    assert rel(r.primal_objective, lp.obj_opt) < 1e-7
+    rp, bv = primal_violation(lp, r.x)
+    assert rp <= 1e-8 * (1.0 + bound_norm(lp)) * 1.01 and bv <= 1e-12
+
+
+def test_fp32_refinement_off_is_plain_loop():
+    lp = G.random_lp(m=500, n=1250, seed=7)
+    a = c_api.solve(lp, tol=1e-4, precision="fp32_mixed", time_limit=TL, iteration_limit=200000)
+    b = c_api.solve(lp, tol=1e-4, precision="fp32_mixed", time_limit=TL, iteration_limit=200000, refinement=False)
+    assert a.info["refine_rounds"] == 0 and b.info["refine_rounds"] == 0
+    assert a.iterations == b.iterations and np.array_equal(a.x, b.x)
+
+
 def test_iteration_limit_semantics():
     lp = read_mps(os.path.join(GOLDEN, "afiro_original.mps"))
     r = c_api.solve(lp, tol=0.0, iteration_limit=10, time_limit=TL)
@@ -228,4 +275,4 @@ def test_graph_replays_match_direct_launches(monkeypatch):
         assert a.status == b.status == 1
         assert a.iterations == b.iterations
         assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
-        assert np.array_equal(a.trace, b.trace)
+        assert np.array_equal(a.trace, b.trace, equal_nan=True)  # fpe is NaN before the first one
