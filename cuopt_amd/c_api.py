@@ -92,6 +92,7 @@ def lib():
         L.cuOptAmdSpmv.argtypes = [i32, i32, vp, vp, vp, vp, vp, i32]
         L.cuOptAmdDeviceCount.restype = i32
         L.cuOptAmdBenchmarkSteps.argtypes = [vp, vp, i32, i32, vp, i32]
+        L.cuOptAmdBatchSolve.argtypes = [i32, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -262,8 +263,13 @@ def solve_problem(problem: Problem, settings: SolverSettings) -> Result:
     err = L.cuOptSolve(problem.handle, settings.handle, ctypes.byref(sol))
     if not sol.value:
         raise CuOptError(err, "cuOptSolve returned no solution")
+    return _take_result(sol, problem.lp)
+
+
+def _take_result(sol, lp) -> Result:
+    """Reads every getter of a solution handle, then destroys it."""
+    L = lib()
     try:
-        lp = problem.lp
         st, es = ctypes.c_int32(), ctypes.c_int32()
         L.cuOptGetTerminationStatus(sol, ctypes.byref(st))
         L.cuOptGetErrorStatus(sol, ctypes.byref(es))
@@ -289,11 +295,10 @@ def solve_problem(problem: Problem, settings: SolverSettings) -> Result:
         L.cuOptDestroySolution(ctypes.byref(sol))
 
 
-def solve(lp, tol: float = 1e-4, precision: str = "fp64", iteration_limit: int | None = None,
-          time_limit: float | None = None, x_init=None, y_init=None, method: int = CUOPT_METHOD_PDLP,
-          log: bool = False, refinement: bool | None = None) -> Result:
-    """Solve ``lp`` (a generators.LP) through the C ABI with all six tolerances set to ``tol``."""
-    prob = Problem(lp)
+def make_settings(tol: float = 1e-4, precision: str = "fp64", iteration_limit: int | None = None,
+                  time_limit: float | None = None, method: int = CUOPT_METHOD_PDLP, log: bool = False,
+                  refinement: bool | None = None) -> SolverSettings:
+    """Settings with all six tolerances at ``tol``, presolve off and the given limits."""
     s = SolverSettings()
     for name in ("absolute_primal_tolerance", "relative_primal_tolerance", "absolute_dual_tolerance",
                  "relative_dual_tolerance", "absolute_gap_tolerance", "relative_gap_tolerance"):
@@ -308,11 +313,36 @@ def solve(lp, tol: float = 1e-4, precision: str = "fp64", iteration_limit: int |
         _check(s.set("time_limit", float(time_limit)))
     if refinement is not None:
         _check(s.set("pdlp_refinement", int(bool(refinement))))
+    return s
+
+
+def solve(lp, tol: float = 1e-4, precision: str = "fp64", iteration_limit: int | None = None,
+          time_limit: float | None = None, x_init=None, y_init=None, method: int = CUOPT_METHOD_PDLP,
+          log: bool = False, refinement: bool | None = None) -> Result:
+    """Solve ``lp`` (a generators.LP) through the C ABI with all six tolerances set to ``tol``."""
+    prob = Problem(lp)
+    s = make_settings(tol, precision, iteration_limit, time_limit, method, log, refinement)
     if x_init is not None:
         s.set_initial_primal(x_init)
     if y_init is not None:
         s.set_initial_dual(y_init)
     return solve_problem(prob, s)
+
+
+def batch_solve(lps, tol: float = 1e-4, precision: str = "fp64", iteration_limit: int | None = None,
+                time_limit: float | None = None) -> list[Result]:
+    """cuOptAmdBatchSolve over ``lps``: one persistent launch for the whole list."""
+    probs = [Problem(lp) for lp in lps]
+    s = make_settings(tol, precision, iteration_limit, time_limit)
+    hs = (ctypes.c_void_p * max(len(probs), 1))(*[p.handle.value for p in probs])
+    sols = (ctypes.c_void_p * max(len(probs), 1))()
+    lib().cuOptAmdBatchSolve(len(probs), hs, s.handle, sols)
+    out = []
+    for i, p in enumerate(probs):
+        if not sols[i]:
+            raise CuOptError(CUOPT_RUNTIME_ERROR, "cuOptAmdBatchSolve returned no solution")
+        out.append(_take_result(ctypes.c_void_p(sols[i]), p.lp))
+    return out
 
 
 def spmv(row_offsets, col_indices, values, x, ncols: int, precision: str = "fp64") -> np.ndarray:

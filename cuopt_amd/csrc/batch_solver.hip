@@ -1,0 +1,904 @@
+// batch_solver.hip -- many independent LPs solved in one persistent launch (gfx950).
+//
+// The heterogeneous batch of the reference (call_batch_solve, cython_solve.cu:286-352,
+// which runs independent solve_lp calls on a few streams) becomes a single kernel
+// launch: workgroup g owns LP order[g] for its whole life -- CSR transpose,
+// Ruiz / Pock-Chambolle scaling, bound / objective rescaling, the power
+// iteration for the step size, and the Stable3 loop with FP64 termination
+// checks, cuPDLPx restarts and the PID primal weight -- so the batch needs no
+// host synchronisation and no per-iteration launch at all.  LPs are handed out
+// largest first, so the hardware dispatcher's in-order assignment of
+// workgroups to free CUs balances the uneven sizes.
+//
+// Per LP the arithmetic is the single-LP solver's (pdlp_solver.hip): the same
+// fused epilogues (pdlp_kernels.hpp), the same scaling order of operations and
+// restart logic; only the order of the row reductions differs (workgroup
+// lane groups instead of tiles), so FP64 batch results agree with single-LP
+// results to rounding.
+//
+// SpMV inside a workgroup: groups of G lanes (G = 2^lg chosen per LP and matrix
+// by the host from the mean row length) own a row, read its nonzeros with a
+// stride of G (consecutive groups own consecutive rows, so a wavefront reads a
+// contiguous run of the value / index arrays), merge by shuffles, and run the
+// row epilogue.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <numeric>
+#include <random>
+#include <thread>
+#include <vector>
+
+#include "batch_solver.hpp"
+#include "common.hpp"
+#include "pdlp_kernels.hpp"
+
+namespace cuopt_amd {
+
+namespace {
+
+constexpr int kBT = 256;  // lanes per LP workgroup
+
+struct BatchDesc {
+  int m, n, nnz;
+  int row0, col0;     // offsets in m- and n-sized arrays
+  int aptr0, tptr0;   // offsets of the row-pointer arrays of A (m+1) and A^T (n+1)
+  int lga, lgt;       // log2 lanes per row for A and A^T
+  long long nz0;      // offset in nonzero arrays
+  double obj_sf, obj_off;
+};
+
+struct BatchOut {
+  double l2p, l2d, pobj, dobj, gap, absobj, b_norm, c_norm;
+  double eta, omega, sigma_max_sq, beta, gamma;
+  int status, iters, restarts, checks, power_iters, pad[3];
+};
+
+template <typename V>
+struct BatchArgs {
+  using V2 = typename Vec2<V>::type;
+  const BatchDesc* desc;
+  const int* order;
+  // structure: A from the host; A^T built in the kernel
+  const int* a_ptr;
+  const int* a_idx;
+  const double* a_orig;
+  int* t_ptr;
+  int* t_idx;
+  int* t_pos;
+  double* t_orig;
+  int* t_fill;
+  V* a_w;
+  V* t_w;
+  // unscaled problem (minimisation form)
+  const double* c;
+  const double* lb;
+  const double* ub;
+  const double* lc;
+  const double* uc;
+  // scaling and working vectors
+  double *Dr, *Dc, *rowv, *colv;
+  V* c_w;
+  V2* bnd_w;
+  V2* cb_w;
+  double2* cb_orig;
+  V *x0, *x1, *y0v, *y1, *xa, *ya, *xbar, *ybar, *xhat, *yhat, *slack, *aty;
+  double *xu, *yu, *su;
+  double *pz, *pz2, *patq;
+  const double* z0;
+  BatchOut* out;
+  double tol_ap, tol_rp, tol_ad, tol_rd, tol_ag, tol_rg;
+  long long iter_limit;
+  unsigned long long tick_limit;  // wall-clock ticks from kernel start (0 = none)
+};
+
+constexpr int kMajorB = 200;
+constexpr double kArtificialB = 0.36, kSufficientB = 0.2, kNecessaryB = 0.8;
+constexpr double kKpB = 0.99, kKiB = 0.01, kKdB = 0.0, kISmoothB = 0.3;
+
+__device__ inline long long cond_major(long long k)
+{
+  long long step = 10, threshold = 1000;
+  while (k >= threshold) {
+    step *= 10;
+    threshold *= 10;
+  }
+  return step;
+}
+__device__ inline bool check_at(long long k)
+{
+  const bool major = ((k % kMajorB) == 0 && k > 0) || k <= 0;
+  return major || (k % cond_major(k)) == 0;
+}
+
+// ---- workgroup reductions (fixed order) -------------------------------------
+
+template <int NR>
+__device__ inline void wg_sum(double (&v)[kMaxRed], double (*lds)[kMaxRed])
+{
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int k = 0; k < NR; ++k)
+#pragma unroll
+    for (int off = kWave / 2; off > 0; off >>= 1) v[k] += __shfl_xor(v[k], off, kWave);
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < NR; ++k) lds[wid][k] = v[k];
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < NR; ++k) {
+    double s = lds[0][k];
+    for (int w = 1; w < kBT / kWave; ++w) s += lds[w][k];
+    v[k] = s;
+  }
+  __syncthreads();
+}
+
+__device__ inline void zero_red(double (&r)[kMaxRed])
+{
+#pragma unroll
+  for (int k = 0; k < kMaxRed; ++k) r[k] = 0.0;
+}
+
+// ---- row passes ---------------------------------------------------------------
+
+// Rows [0, nrows) of one LP's CSR; Epi as in spmv_tiles.hpp, with the row index
+// passed as the third argument of product().
+template <class Epi>
+__device__ inline void rows_pass(const int* __restrict__ ptr,
+                                 const int* __restrict__ idx,
+                                 const typename Epi::Mat* __restrict__ vals,
+                                 int nrows,
+                                 int lg,
+                                 const Epi& epi,
+                                 double (&red)[kMaxRed])
+{
+  using Acc   = typename Epi::Acc;
+  const int G = 1 << lg, sub = threadIdx.x & (G - 1);
+  for (int r = threadIdx.x >> lg; r < nrows; r += kBT >> lg) {
+    const int b = ptr[r], e = ptr[r + 1];
+    const auto pre = epi.load(r);
+    Acc a = Acc::zero();
+    for (int k = b + sub; k < e; k += G) a.add(epi.product(vals[k], idx[k], r));
+    for (int off = G >> 1; off > 0; off >>= 1) a.merge(a.shfl(off));
+    if (sub == 0) epi.row(r, a.value(), pre, red);
+  }
+}
+
+// Ruiz / Pock-Chambolle statistics of |(a * D_row-side) * D_col-side| in the
+// reference operand order; TRANSPOSED passes A^T (its rows are columns of A).
+template <bool TRANSPOSED, bool MAXNORM>
+struct BScaleEpi {
+  using Mat = double;
+  using Acc = typename std::conditional<MAXNORM, AccMax64, AccSum64>::type;
+  struct Pre {};
+  const double* Dr;
+  const double* Dc;
+  double* out;
+  __device__ double product(double v, int col, int r) const
+  {
+    const double dr = TRANSPOSED ? Dr[col] : Dr[r];
+    const double dc = TRANSPOSED ? Dc[r] : Dc[col];
+    return fabs((v * dr) * dc);
+  }
+  __device__ Pre load(int) const { return Pre{}; }
+  __device__ void row(int r, double s, const Pre&, double (&)[kMaxRed]) const { out[r] = s; }
+};
+
+// Power iteration on the FP64 scaled matrix, values formed on the fly as in
+// the scaled copy of the single-LP solver: A~^T q, then A~ (A~^T q).
+struct BPowerAtEpi {
+  using Mat = double;
+  using Acc = AccSum64;
+  struct Pre {};
+  const double* Dr;
+  const double* Dc;
+  const double* z;
+  double nrm;
+  double* atq;
+  __device__ double product(double v, int col, int r) const { return ((v * Dc[r]) * Dr[col]) * (z[col] / nrm); }
+  __device__ Pre load(int) const { return Pre{}; }
+  __device__ void row(int r, double s, const Pre&, double (&)[kMaxRed]) const { atq[r] = s; }
+};
+struct BPowerAEpi {
+  using Mat = double;
+  using Acc = AccSum64;
+  struct Pre {
+    double zold;
+  };
+  const double* Dr;
+  const double* Dc;
+  const double* atq;
+  const double* zold;
+  double nrm;
+  double* znew;
+  __device__ double product(double v, int col, int r) const { return ((v * Dr[r]) * Dc[col]) * atq[col]; }
+  __device__ Pre load(int r) const { return Pre{zold[r]}; }
+  __device__ void row(int r, double s, const Pre& p, double (&red)[kMaxRed]) const
+  {
+    znew[r] = s;
+    red[0] += (p.zold / nrm) * s;
+  }
+};
+
+__device__ inline double bound_sq(double l, double u)
+{
+  double t = 0.0;
+  if (isfinite(l) && l != u) t += l * l;
+  if (isfinite(u)) t += u * u;
+  return t;
+}
+
+// ---- the kernel ----------------------------------------------------------------
+
+template <typename V>
+__global__ __launch_bounds__(kBT) void batch_pdlp_kernel(BatchArgs<V> A)
+{
+  using V2 = typename Vec2<V>::type;
+  __shared__ double lds[kBT / kWave][kMaxRed];
+  __shared__ StepScalars sc;
+  __shared__ int scan_lds[kBT];
+  const unsigned long long t_start = wall_clock64();
+  const int tid = threadIdx.x;
+  const BatchDesc d = A.desc[A.order[blockIdx.x]];
+  const int m = d.m, n = d.n, nnz = d.nnz;
+  // this LP's slices
+  const int* aptr     = A.a_ptr + d.aptr0;
+  const int* aidx     = A.a_idx + d.nz0;
+  const double* aorig = A.a_orig + d.nz0;
+  int* tptr           = A.t_ptr + d.tptr0;
+  int* tidx           = A.t_idx + d.nz0;
+  int* tpos           = A.t_pos + d.nz0;
+  double* torig       = A.t_orig + d.nz0;
+  int* tfill          = A.t_fill + d.col0;
+  V* aw               = A.a_w + d.nz0;
+  V* tw               = A.t_w + d.nz0;
+  const double *c = A.c + d.col0, *lb = A.lb + d.col0, *ub = A.ub + d.col0;
+  const double *lc = A.lc + d.row0, *uc = A.uc + d.row0;
+  double *Dr = A.Dr + d.row0, *Dc = A.Dc + d.col0, *rowv = A.rowv + d.row0, *colv = A.colv + d.col0;
+  V* c_w = A.c_w + d.col0;
+  V2* bnd_w = A.bnd_w + d.col0;
+  V2* cb_w = A.cb_w + d.row0;
+  double2* cb_orig = A.cb_orig + d.row0;
+  V* xs[2] = {A.x0 + d.col0, A.x1 + d.col0};
+  V* ys[2] = {A.y0v + d.row0, A.y1 + d.row0};
+  V *xanc = A.xa + d.col0, *yanc = A.ya + d.row0, *xbar = A.xbar + d.col0, *ybar = A.ybar + d.row0;
+  V *xhat = A.xhat + d.col0, *yhat = A.yhat + d.row0, *slack = A.slack + d.col0, *aty = A.aty + d.col0;
+  double *xu = A.xu + d.col0, *yu = A.yu + d.row0, *su = A.su + d.col0;
+  double *pz = A.pz + d.row0, *pz2 = A.pz2 + d.row0, *patq = A.patq + d.col0;
+  double red[kMaxRed];
+
+  // ---- A^T: counting sort by column, then each column segment ordered by source position
+  for (int j = tid; j < n; j += kBT) tfill[j] = 0;
+  __syncthreads();
+  for (int k = tid; k < nnz; k += kBT) atomicAdd(&tfill[aidx[k]], 1);
+  __syncthreads();
+  {
+    const int per = (n + kBT - 1) / kBT, j0 = min(n, tid * per), j1 = min(n, j0 + per);
+    int s = 0;
+    for (int j = j0; j < j1; ++j) s += tfill[j];
+    scan_lds[tid] = s;
+    __syncthreads();
+    if (tid == 0) {
+      int acc = 0;
+      for (int t = 0; t < kBT; ++t) {
+        const int v = scan_lds[t];
+        scan_lds[t] = acc;
+        acc += v;
+      }
+    }
+    __syncthreads();
+    int acc = scan_lds[tid];
+    for (int j = j0; j < j1; ++j) {
+      const int v = tfill[j];
+      tptr[j]     = acc;
+      tfill[j]    = acc;
+      acc += v;
+    }
+    if (tid == kBT - 1) tptr[n] = acc;
+  }
+  __syncthreads();
+  for (int i = tid; i < m; i += kBT)
+    for (int k = aptr[i]; k < aptr[i + 1]; ++k) {
+      const int dst = atomicAdd(&tfill[aidx[k]], 1);
+      tpos[dst]     = k;
+      tidx[dst]     = i;
+    }
+  __syncthreads();
+  for (int j = tid; j < n; j += kBT) {  // insertion sort of the (short) column by source position
+    const int b = tptr[j], e = tptr[j + 1];
+    for (int p = b + 1; p < e; ++p) {
+      const int kp = tpos[p], ip = tidx[p];
+      int q = p - 1;
+      while (q >= b && tpos[q] > kp) {
+        tpos[q + 1] = tpos[q];
+        tidx[q + 1] = tidx[q];
+        --q;
+      }
+      tpos[q + 1] = kp;
+      tidx[q + 1] = ip;
+    }
+    for (int p = b; p < e; ++p) torig[p] = aorig[tpos[p]];
+  }
+  __syncthreads();
+
+  // ---- Ruiz (x10) and Pock-Chambolle (alpha = 1) equilibration
+  for (int i = tid; i < m; i += kBT) Dr[i] = 1.0;
+  for (int j = tid; j < n; j += kBT) Dc[j] = 1.0;
+  __syncthreads();
+  for (int it = 0; it <= 10; ++it) {
+    const bool pc = it == 10;
+    zero_red(red);
+    if (pc) {
+      rows_pass(aptr, aidx, aorig, m, d.lga, BScaleEpi<false, false>{Dr, Dc, rowv}, red);
+      rows_pass(tptr, tidx, torig, n, d.lgt, BScaleEpi<true, false>{Dr, Dc, colv}, red);
+    } else {
+      rows_pass(aptr, aidx, aorig, m, d.lga, BScaleEpi<false, true>{Dr, Dc, rowv}, red);
+      rows_pass(tptr, tidx, torig, n, d.lgt, BScaleEpi<true, true>{Dr, Dc, colv}, red);
+    }
+    __syncthreads();
+    for (int i = tid; i < m; i += kBT) Dr[i] = rowv[i] > 0.0 ? Dr[i] / sqrt(rowv[i]) : Dr[i];
+    for (int j = tid; j < n; j += kBT) Dc[j] = colv[j] > 0.0 ? Dc[j] / sqrt(colv[j]) : Dc[j];
+    __syncthreads();
+  }
+  // scaled working matrices, reference operand order (vals * first[row]) * second[col]
+  for (int i = tid; i < m; i += kBT)
+    for (int k = aptr[i]; k < aptr[i + 1]; ++k) aw[k] = (V)((aorig[k] * Dr[i]) * Dc[aidx[k]]);
+  for (int j = tid; j < n; j += kBT)
+    for (int k = tptr[j]; k < tptr[j + 1]; ++k) tw[k] = (V)((torig[k] * Dc[j]) * Dr[tidx[k]]);
+  // vectors and norms (bound_objective_rescaling)
+  zero_red(red);
+  for (int j = tid; j < n; j += kBT) {
+    const double cs = c[j] * Dc[j];
+    red[0] += cs * cs * 1.0;
+    red[1] += c[j] * c[j];
+  }
+  for (int i = tid; i < m; i += kBT) {
+    red[2] += bound_sq(lc[i] * Dr[i], uc[i] * Dr[i]);
+    red[3] += bound_sq(lc[i], uc[i]);
+  }
+  wg_sum<4>(red, lds);
+  const double beta = 1.0 / (sqrt(red[2]) + 1.0), gamma = 1.0 / (sqrt(red[0]) + 1.0);
+  const double b_norm = sqrt(red[3]), c_norm = sqrt(red[1]);
+  for (int j = tid; j < n; j += kBT) {
+    const double dc = Dc[j];
+    double l = dc == 0.0 ? 0.0 : lb[j] / dc, u = dc == 0.0 ? 0.0 : ub[j] / dc;
+    l = l * beta;
+    u = u * beta;
+    c_w[j] = (V)((c[j] * dc) * gamma);
+    V2 bb;
+    bb.x     = (V)l;
+    bb.y     = (V)u;
+    bnd_w[j] = bb;
+  }
+  for (int i = tid; i < m; i += kBT) {
+    V2 bb;
+    bb.x       = (V)((lc[i] * Dr[i]) * beta);
+    bb.y       = (V)((uc[i] * Dr[i]) * beta);
+    cb_w[i]    = bb;
+    cb_orig[i] = make_double2(lc[i], uc[i]);
+  }
+  // ---- power iteration (compute_initial_step_size, pdlp.cu:2622-2738)
+  for (int i = tid; i < m; i += kBT) pz[i] = A.z0[i];
+  zero_red(red);
+  for (int i = tid; i < m; i += kBT) red[0] += A.z0[i] * A.z0[i];
+  wg_sum<1>(red, lds);
+  double nrm = sqrt(red[0]), s2 = 0.0;
+  int piters = 0;
+  double* zo = pz;
+  double* zn = pz2;
+  while (piters < 5000) {
+    zero_red(red);
+    rows_pass(tptr, tidx, torig, n, d.lgt, BPowerAtEpi{Dr, Dc, zo, nrm, patq}, red);
+    __syncthreads();
+    rows_pass(aptr, aidx, aorig, m, d.lga, BPowerAEpi{Dr, Dc, patq, zo, nrm, zn}, red);
+    wg_sum<1>(red, lds);
+    s2 = red[0];
+    zero_red(red);
+    for (int i = tid; i < m; i += kBT) {
+      const double q = zo[i] / nrm;
+      const double r = q * -s2 + zn[i];
+      red[0] += r * r;
+      red[1] += zn[i] * zn[i];
+    }
+    wg_sum<2>(red, lds);
+    ++piters;
+    if (sqrt(red[0]) < 1e-4) break;
+    nrm = sqrt(red[1]);
+    double* t = zo;
+    zo        = zn;
+    zn        = t;
+  }
+  const double eta = 0.998 / sqrt(s2);
+
+  // ---- initial iterates (zeros, projected)
+  for (int j = tid; j < n; j += kBT) {
+    const V2 bb = bnd_w[j];
+    xhat[j]     = V(0);
+    xanc[j]     = V(0);
+    xs[0][j]    = mn(mx(V(0), bb.x), bb.y);
+    slack[j]    = V(0);
+  }
+  for (int i = tid; i < m; i += kBT) {
+    yhat[i]  = V(0);
+    yanc[i]  = V(0);
+    ys[0][i] = V(0);
+  }
+  __syncthreads();
+
+  // ---- Stable3 loop (pdlp_solver_t::run_solver, pdlp.cu:2013-2458)
+  const double nan = __longlong_as_double(0x7ff8000000000000LL), inf = __longlong_as_double(0x7ff0000000000000LL);
+  double omega = 1.0, best_omega = 1.0, tau = eta, sigma = eta;
+  double fpe = nan, fpe0 = nan, last_trial = inf, err_sum = 0.0, last_err = 0.0, best_gap = inf;
+  double fpe_t[3] = {0, 0, 0}, fpe0_t[3] = {0, 0, 0};  // (||dy||^2, interaction, ||dx||^2)
+  double l2p = 0, l2d = 0, pobj = 0, dobj = 0, gap = 0, absobj = 0;
+  long long k = 0, kr = 0;
+  int cur = 0, restarts = 0, checks = 0, status = 0;
+  bool restarted_prev = false, periodic_prev = false;
+  while (true) {
+    // termination check on the unscaled problem (FP64)
+    const bool is_major = ((k % kMajorB) == 0 && k > 0) || k <= 0;
+    {
+      zero_red(red);
+      rows_pass(aptr, aidx, aorig, m, d.lga,
+                KktRowsEpi<V>{xhat, Dc, 1.0 / beta, yhat, Dr, 1.0 / gamma, gamma, cb_orig, yanc, yu, nullptr,
+                              nullptr, 1.0, 1.0},
+                red);
+      wg_sum<3>(red, lds);
+      const double rp2 = red[0], psum = red[1], dd = red[2];  // dd: ||y0 - yhat||^2
+      zero_red(red);
+      rows_pass(tptr, tidx, torig, n, d.lgt,
+                KktColsEpi<V>{yu, xhat, slack, Dc, 1.0 / beta, beta, 1.0 / gamma, gamma, c, xanc, xu, su, nullptr,
+                              1.0, 1.0},
+                red);
+      wg_sum<4>(red, lds);
+      const double dp = red[3];  // ||x0 - xhat||^2
+      l2p        = sqrt(rp2);
+      l2d        = sqrt(red[0]);
+      double po  = red[1], du = red[2] + psum;
+      if (d.obj_sf != 1.0 || d.obj_off != 0.0) {
+        po = d.obj_sf * (po + d.obj_off);
+        du = d.obj_sf * (du + d.obj_off);
+      }
+      pobj   = po;
+      dobj   = du;
+      gap    = fabs(po - du);
+      absobj = fabs(po) + fabs(du);
+      ++checks;
+      if (restarted_prev) {
+        fpe  = sqrt((fpe0_t[2] * omega + fpe0_t[0] / omega) + 2.0 * fpe0_t[1] * eta);
+        fpe0 = fpe;
+      }
+      if (periodic_prev) fpe = sqrt((fpe_t[2] * omega + fpe_t[0] / omega) + 2.0 * fpe_t[1] * eta);
+      const bool optimal = l2d <= A.tol_ad + A.tol_rd * c_norm && l2p <= A.tol_ap + A.tol_rp * b_norm &&
+                           gap <= A.tol_ag + A.tol_rg * absobj;
+      if (k > 1 && optimal) {
+        status = CUOPT_TERIMINATION_STATUS_OPTIMAL;
+        break;
+      }
+      if (isnan(l2p) || isnan(l2d) || isnan(gap)) {
+        status = CUOPT_TERIMINATION_STATUS_NUMERICAL_ERROR;
+        break;
+      }
+      if (A.tick_limit && wall_clock64() - t_start >= A.tick_limit) {
+        status = CUOPT_TERIMINATION_STATUS_TIME_LIMIT;
+        break;
+      }
+      if (k >= A.iter_limit) {
+        status = CUOPT_TERIMINATION_STATUS_ITERATION_LIMIT;
+        break;
+      }
+      // cuPDLPx restart and PID primal weight (pdlp_restart_strategy.cu:688-985)
+      bool restarted = false;
+      if (is_major) {
+        bool do_restart = false;
+        if (k == kMajorB) {
+          do_restart = true;
+        } else if (k > kMajorB) {
+          if (fpe <= kSufficientB * fpe0)
+            do_restart = true;
+          else if (fpe <= kNecessaryB * fpe0 && fpe > last_trial)
+            do_restart = true;
+          else if ((double)kr >= kArtificialB * (double)k)
+            do_restart = true;
+        }
+        last_trial = fpe;
+        if (do_restart) {
+          const double l2pd = sqrt(dp), l2dd = sqrt(dd);
+          const double rel_d = l2d / (1.0 + c_norm), rel_p = l2p / (1.0 + b_norm);
+          const double ratio = (rel_p == 0.0) ? inf : rel_d / rel_p;
+          if (l2pd > 1e-16 && l2dd > 1e-16 && l2pd < 1e12 && l2dd < 1e12 && ratio > 1e-8 && ratio < 1e8) {
+            const double cw  = omega;
+            const double err = log(l2dd) - log(l2pd) - log(cw);
+            const double s   = err_sum * kISmoothB + err;
+            err_sum          = s;
+            const double de  = err - last_err;
+            const double w   = exp(kKpB * err + kKiB * s + kKdB * de) * cw;
+            omega            = w;
+            last_err         = err;
+            tau              = eta / w;
+            sigma            = eta * w;
+          } else {
+            omega    = best_omega;
+            err_sum  = 0.0;
+            last_err = 0.0;
+            tau      = eta / best_omega;
+            sigma    = eta * best_omega;
+          }
+          const double g = fabs(log10(ratio));
+          if (g < best_gap) {
+            best_gap   = g;
+            best_omega = omega;
+          }
+          for (int j = tid; j < n; j += kBT) {
+            const V v  = xhat[j];
+            xanc[j]    = v;
+            xs[cur][j] = v;
+          }
+          for (int i = tid; i < m; i += kBT) {
+            const V v  = yhat[i];
+            yanc[i]    = v;
+            ys[cur][i] = v;
+          }
+          kr         = 0;
+          last_trial = inf;
+          restarted  = true;
+          ++restarts;
+        }
+      }
+      if (tid == 0) sc = StepScalars{tau, sigma, 0, 0};
+      __syncthreads();
+
+      // PDHG iterations up to the next check
+      long long next = k + 1;
+      while (!check_at(next)) ++next;
+      periodic_prev = false;
+      for (long long s = k; s < next; ++s) {
+        const long long s2i     = s + 2;
+        const bool should_major = ((s + 1) % kMajorB == 0) || (s2i % cond_major(s2i)) == 0;
+        const bool periodic     = ((s + 1) % kMajorB) == 0;
+        const bool post_restart = restarted && s == k;
+        const bool fpe_step     = periodic || post_restart;
+        const int ci = cur, nx = cur ^ 1;
+        const int jj = (int)kr;
+        zero_red(red);
+#define CUOPT_BPRIMAL(MAJ, FP)                                                                              \
+  rows_pass(tptr, tidx, tw, n, d.lgt,                                                                        \
+            PrimalStepEpi<V, MAJ, FP>{ys[ci], xs[ci], c_w, bnd_w, xanc, xbar, xs[nx], xhat, slack, aty, &sc, jj}, \
+            red)
+#define CUOPT_BDUAL(MAJ, FP)                                                                                 \
+  rows_pass(aptr, aidx, aw, m, d.lga,                                                                         \
+            DualStepEpi<V, MAJ, FP>{xbar, ys[ci], cb_w, yanc, ys[nx], yhat, ybar, &sc, jj, nullptr}, red)
+        if (should_major) {
+          if (fpe_step) CUOPT_BPRIMAL(true, true);
+          else CUOPT_BPRIMAL(true, false);
+        } else {
+          if (fpe_step) CUOPT_BPRIMAL(false, true);
+          else CUOPT_BPRIMAL(false, false);
+        }
+        __syncthreads();
+        if (should_major) {
+          if (fpe_step) CUOPT_BDUAL(true, true);
+          else CUOPT_BDUAL(true, false);
+        } else {
+          if (fpe_step) CUOPT_BDUAL(false, true);
+          else CUOPT_BDUAL(false, false);
+        }
+#undef CUOPT_BPRIMAL
+#undef CUOPT_BDUAL
+        if (fpe_step) {
+          wg_sum<1>(red, lds);
+          const double ny = red[0];
+          zero_red(red);
+          rows_pass(tptr, tidx, tw, n, d.lgt, FpeEpi<V>{ybar, aty, xbar, xs[ci]}, red);
+          wg_sum<2>(red, lds);
+          if (post_restart) {
+            fpe0_t[0] = ny;
+            fpe0_t[1] = red[0];
+            fpe0_t[2] = red[1];
+          } else {
+            fpe_t[0] = ny;
+            fpe_t[1] = red[0];
+            fpe_t[2] = red[1];
+          }
+        } else {
+          __syncthreads();
+        }
+        if (periodic) periodic_prev = true;
+        cur = nx;
+        ++kr;
+      }
+      k              = next;
+      restarted_prev = restarted;
+    }
+  }
+  if (tid == 0) {
+    BatchOut o;
+    o.l2p = l2p;
+    o.l2d = l2d;
+    o.pobj = pobj;
+    o.dobj = dobj;
+    o.gap = gap;
+    o.absobj = absobj;
+    o.b_norm = b_norm;
+    o.c_norm = c_norm;
+    o.eta = eta;
+    o.omega = omega;
+    o.sigma_max_sq = s2;
+    o.beta = beta;
+    o.gamma = gamma;
+    o.status = status;
+    o.iters = (int)k;
+    o.restarts = restarts;
+    o.checks = checks;
+    o.power_iters = piters;
+    A.out[A.order[blockIdx.x]] = o;
+  }
+}
+
+int lanes_log2(int nrows, long long nnz)
+{
+  const double avg = nrows > 0 ? (double)nnz / nrows : 0.0;
+  int lg           = 0;
+  while (lg < 6 && (double)(4 << lg) <= avg) ++lg;  // about two nonzeros per lane
+  return lg;
+}
+
+template <typename V>
+std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
+                                   const std::vector<int>& active,
+                                   const std::vector<std::vector<double>>& lcs,
+                                   const std::vector<std::vector<double>>& ucs,
+                                   const PdlpSettings& st,
+                                   std::vector<SolveResult> R)
+{
+  using V2 = typename Vec2<V>::type;
+  const auto t0 = std::chrono::steady_clock::now();
+  const int nb  = (int)active.size();
+  std::vector<BatchDesc> desc(lps.size());
+  long long tot_nnz = 0;
+  int tot_m = 0, tot_n = 0, max_m = 0, tot_ap = 0, tot_tp = 0;
+  for (int a : active) {
+    const LpHost& lp = *lps[a];
+    BatchDesc& d     = desc[a];
+    d.m = lp.m;
+    d.n = lp.n;
+    d.nnz = lp.rowptr[lp.m];
+    d.row0 = tot_m;
+    d.col0 = tot_n;
+    d.aptr0 = tot_ap;
+    d.tptr0 = tot_tp;
+    d.nz0 = tot_nnz;
+    d.lga = lanes_log2(lp.m, d.nnz);
+    d.lgt = lanes_log2(lp.n, d.nnz);
+    d.obj_sf  = lp.maximize ? -1.0 : 1.0;
+    d.obj_off = lp.maximize ? -lp.offset : lp.offset;
+    tot_m += lp.m;
+    tot_n += lp.n;
+    tot_ap += lp.m + 1;
+    tot_tp += lp.n + 1;
+    tot_nnz += d.nnz;
+    max_m = std::max(max_m, lp.m);
+  }
+  // host arrays (rows sorted by column, as problem_t keeps them)
+  std::vector<int> h_ptr(tot_ap), h_idx(tot_nnz);
+  std::vector<double> h_val(tot_nnz), h_c(tot_n), h_lb(tot_n), h_ub(tot_n), h_lc(tot_m), h_uc(tot_m);
+  auto fill = [&](int t, int nt) {
+    for (int q = t; q < nb; q += nt) {
+      const int a      = active[q];
+      const LpHost& lp = *lps[a];
+      const BatchDesc& d = desc[a];
+      std::copy(lp.rowptr.begin(), lp.rowptr.begin() + lp.m + 1, h_ptr.begin() + d.aptr0);
+      std::copy(lp.colidx.begin(), lp.colidx.begin() + d.nnz, h_idx.begin() + d.nz0);
+      std::copy(lp.vals.begin(), lp.vals.begin() + d.nnz, h_val.begin() + d.nz0);
+      for (int i = 0; i < lp.m; ++i) {
+        const int b = lp.rowptr[i], e = lp.rowptr[i + 1];
+        bool sorted = true;
+        for (int k = b + 1; k < e && sorted; ++k) sorted = lp.colidx[k - 1] <= lp.colidx[k];
+        if (sorted) continue;
+        std::vector<std::pair<int, double>> row;
+        for (int k = b; k < e; ++k) row.emplace_back(lp.colidx[k], lp.vals[k]);
+        std::stable_sort(row.begin(), row.end(), [](const auto& p, const auto& q) { return p.first < q.first; });
+        for (int k = b; k < e; ++k) {
+          h_idx[d.nz0 + k] = row[k - b].first;
+          h_val[d.nz0 + k] = row[k - b].second;
+        }
+      }
+      for (int j = 0; j < lp.n; ++j) {
+        h_c[d.col0 + j]  = lp.maximize ? -lp.c[j] : lp.c[j];
+        h_lb[d.col0 + j] = lp.lb[j];
+        h_ub[d.col0 + j] = lp.ub[j];
+      }
+      std::copy(lcs[a].begin(), lcs[a].end(), h_lc.begin() + d.row0);
+      std::copy(ucs[a].begin(), ucs[a].end(), h_uc.begin() + d.row0);
+    }
+  };
+  {
+    const int nt = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t) th.emplace_back(fill, t, nt);
+    for (auto& t : th) t.join();
+  }
+  // processing order: largest first
+  std::vector<int> order(active);
+  std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return desc[x].nnz > desc[y].nnz; });
+  std::vector<double> z0(std::max(max_m, 1));
+  {
+    std::mt19937 gen(1);
+    std::normal_distribution<double> dist(0.0, 1.0);
+    for (double& v : z0) v = dist(gen);
+  }
+
+  hipStream_t s;
+  CUOPT_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  struct StreamGuard {
+    hipStream_t s;
+    ~StreamGuard() { (void)hipStreamDestroy(s); }
+  } guard{s};
+  const size_t M = std::max(tot_m, 1), N = std::max(tot_n, 1), Z = std::max<long long>(tot_nnz, 1);
+  DevBuf<BatchDesc> d_desc(std::max<size_t>(desc.size(), 1));
+  DevBuf<int> d_order(std::max(nb, 1)), a_ptr(std::max(tot_ap, 1)), a_idx(Z), t_ptr(std::max(tot_tp, 1)),
+    t_idx(Z), t_pos(Z), t_fill(N);
+  DevBuf<double> a_orig(Z), t_orig(Z), c(N), lb(N), ub(N), lc(M), uc(M), Dr(M), Dc(N), rowv(M), colv(N);
+  DevBuf<V> a_w(Z), t_w(Z), c_w(N), x0(N), x1(N), y0(M), y1(M), xa(N), ya(M), xbar(N), ybar(M), xhat(N), yhat(M),
+    slack(N), aty(N);
+  DevBuf<V2> bnd_w(N), cb_w(M);
+  DevBuf<double2> cb_orig(M);
+  DevBuf<double> xu(N), yu(M), su(N), pz(M), pz2(M), patq(N), dz0(z0.size());
+  DevBuf<BatchOut> out(std::max<size_t>(lps.size(), 1));
+  d_desc.upload(desc, s);
+  d_order.upload(order, s);
+  a_ptr.upload(h_ptr, s);
+  a_idx.upload(h_idx, s);
+  a_orig.upload(h_val, s);
+  c.upload(h_c, s);
+  lb.upload(h_lb, s);
+  ub.upload(h_ub, s);
+  lc.upload(h_lc, s);
+  uc.upload(h_uc, s);
+  dz0.upload(z0, s);
+
+  BatchArgs<V> args{};
+  args.desc = d_desc.get();
+  args.order = d_order.get();
+  args.a_ptr = a_ptr.get();
+  args.a_idx = a_idx.get();
+  args.a_orig = a_orig.get();
+  args.t_ptr = t_ptr.get();
+  args.t_idx = t_idx.get();
+  args.t_pos = t_pos.get();
+  args.t_orig = t_orig.get();
+  args.t_fill = t_fill.get();
+  args.a_w = a_w.get();
+  args.t_w = t_w.get();
+  args.c = c.get();
+  args.lb = lb.get();
+  args.ub = ub.get();
+  args.lc = lc.get();
+  args.uc = uc.get();
+  args.Dr = Dr.get();
+  args.Dc = Dc.get();
+  args.rowv = rowv.get();
+  args.colv = colv.get();
+  args.c_w = c_w.get();
+  args.bnd_w = bnd_w.get();
+  args.cb_w = cb_w.get();
+  args.cb_orig = cb_orig.get();
+  args.x0 = x0.get();
+  args.x1 = x1.get();
+  args.y0v = y0.get();
+  args.y1 = y1.get();
+  args.xa = xa.get();
+  args.ya = ya.get();
+  args.xbar = xbar.get();
+  args.ybar = ybar.get();
+  args.xhat = xhat.get();
+  args.yhat = yhat.get();
+  args.slack = slack.get();
+  args.aty = aty.get();
+  args.xu = xu.get();
+  args.yu = yu.get();
+  args.su = su.get();
+  args.pz = pz.get();
+  args.pz2 = pz2.get();
+  args.patq = patq.get();
+  args.z0 = dz0.get();
+  args.out = out.get();
+  args.tol_ap = st.abs_primal;
+  args.tol_rp = st.rel_primal;
+  args.tol_ad = st.abs_dual;
+  args.tol_rd = st.rel_dual;
+  args.tol_ag = st.abs_gap;
+  args.tol_rg = st.rel_gap;
+  args.iter_limit = st.iteration_limit;
+  args.tick_limit = 0;
+  if (std::isfinite(st.time_limit)) {
+    int dev = 0, khz = 0;
+    CUOPT_HIP_CHECK(hipGetDevice(&dev));
+    CUOPT_HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    const double ticks = std::max(1.0, st.time_limit * 1e3 * (double)khz);
+    args.tick_limit    = ticks >= 1.8e19 ? 0ULL : (unsigned long long)ticks;
+  }
+  if (nb > 0) {
+    hipLaunchKernelGGL(batch_pdlp_kernel<V>, dim3(nb), dim3(kBT), 0, s, args);
+    CUOPT_HIP_CHECK(hipGetLastError());
+  }
+  std::vector<BatchOut> h_out(std::max<size_t>(lps.size(), 1));
+  std::vector<double> h_xu(N), h_yu(M), h_su(N);
+  out.download(h_out.data(), h_out.size(), s);
+  xu.download(h_xu.data(), N, s);
+  yu.download(h_yu.data(), M, s);
+  su.download(h_su.data(), N, s);
+  CUOPT_HIP_CHECK(hipStreamSynchronize(s));
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  for (int a : active) {
+    const LpHost& lp   = *lps[a];
+    const BatchDesc& d = desc[a];
+    const BatchOut& o  = h_out[a];
+    SolveResult& r     = R[a];
+    r.status = o.status;
+    r.x.assign(h_xu.begin() + d.col0, h_xu.begin() + d.col0 + lp.n);
+    r.y.assign(h_yu.begin() + d.row0, h_yu.begin() + d.row0 + lp.m);
+    r.rc.assign(h_su.begin() + d.col0, h_su.begin() + d.col0 + lp.n);
+    if (lp.maximize) {  // solve.cu:583-588
+      for (double& v : r.y) v = -v;
+      for (double& v : r.rc) v = -v;
+    }
+    r.primal_objective                  = o.pobj;
+    r.dual_objective                    = o.dobj;
+    r.solve_time                        = secs;
+    r.info[CUOPT_AMD_INFO_STEPS]        = o.iters;
+    r.info[CUOPT_AMD_INFO_L2_PRIMAL]    = o.l2p;
+    r.info[CUOPT_AMD_INFO_REL_PRIMAL]   = o.l2p / (1.0 + o.b_norm);
+    r.info[CUOPT_AMD_INFO_L2_DUAL]      = o.l2d;
+    r.info[CUOPT_AMD_INFO_REL_DUAL]     = o.l2d / (1.0 + o.c_norm);
+    r.info[CUOPT_AMD_INFO_PRIMAL_OBJ]   = o.pobj;
+    r.info[CUOPT_AMD_INFO_DUAL_OBJ]     = o.dobj;
+    r.info[CUOPT_AMD_INFO_GAP]          = o.gap;
+    r.info[CUOPT_AMD_INFO_REL_GAP]      = o.gap / (1.0 + o.absobj);
+    r.info[CUOPT_AMD_INFO_SOLVE_TIME]   = secs;
+    r.info[CUOPT_AMD_INFO_ETA0]         = o.eta;
+    r.info[CUOPT_AMD_INFO_OMEGA0]       = 1.0;
+    r.info[CUOPT_AMD_INFO_OMEGA]        = o.omega;
+    r.info[CUOPT_AMD_INFO_RESTARTS]     = o.restarts;
+    r.info[CUOPT_AMD_INFO_POWER_ITERS]  = o.power_iters;
+    r.info[CUOPT_AMD_INFO_SIGMA_MAX_SQ] = o.sigma_max_sq;
+    r.info[CUOPT_AMD_INFO_BETA]         = o.beta;
+    r.info[CUOPT_AMD_INFO_GAMMA]        = o.gamma;
+    r.info[CUOPT_AMD_INFO_PRECISION]    = std::is_same<V, double>::value ? 0 : 1;
+    r.info[CUOPT_AMD_INFO_CHECKS]       = o.checks;
+    r.info[CUOPT_AMD_INFO_REFINE_ROUNDS] = 0;
+  }
+  return R;
+}
+
+}  // namespace
+
+std::vector<SolveResult> batch_solve(const std::vector<const LpHost*>& lps, const PdlpSettings& st)
+{
+  const auto t0 = std::chrono::steady_clock::now();
+  std::vector<SolveResult> R(lps.size());
+  std::vector<int> active;
+  std::vector<std::vector<double>> lcs(lps.size()), ucs(lps.size());
+  for (size_t a = 0; a < lps.size(); ++a) {
+    const LpHost& lp = *lps[a];
+    constraint_bounds(lp, lcs[a], ucs[a]);
+    bool crossing = false;
+    for (int j = 0; j < lp.n && !crossing; ++j) crossing = !(lp.ub[j] >= lp.lb[j]);
+    for (int i = 0; i < lp.m && !crossing; ++i) crossing = !(ucs[a][i] >= lcs[a][i]);
+    if (crossing || lp.m == 0) {  // as solve_lp: infeasible bounds, or PDLP without constraints
+      R[a].status = crossing ? CUOPT_TERIMINATION_STATUS_INFEASIBLE : CUOPT_TERIMINATION_STATUS_NUMERICAL_ERROR;
+      R[a].x.assign(lp.n, 0.0);
+      R[a].y.assign(lp.m, 0.0);
+      R[a].rc.assign(lp.n, 0.0);
+      R[a].solve_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+      continue;
+    }
+    active.push_back((int)a);
+  }
+  if (st.precision == CUOPT_AMD_PRECISION_FP32_MIXED) return run_batch<float>(lps, active, lcs, ucs, st, std::move(R));
+  return run_batch<double>(lps, active, lcs, ucs, st, std::move(R));
+}
+
+}  // namespace cuopt_amd

@@ -18,6 +18,7 @@
 
 #include "../../include/cuopt_amd_ext.h"
 #include "common.hpp"
+#include "batch_solver.hpp"
 #include "lp_types.hpp"
 #include "mps_io.hpp"
 #include "settings.hpp"
@@ -519,6 +520,67 @@ cuopt_int_t cuOptSolve(cuOptOptimizationProblem problem, cuOptSolverSettings set
   if (R.rc.size() != (size_t)lp.n) R.rc.assign(lp.n, 0.0);
   *solution_ptr = sol;
   return R.error;
+}
+
+cuopt_int_t cuOptAmdBatchSolve(cuopt_int_t num_problems,
+                               const cuOptOptimizationProblem* problems,
+                               cuOptSolverSettings settings,
+                               cuOptSolution* solutions)
+{
+  if (num_problems < 0 || (num_problems > 0 && (!problems || !solutions)) || !settings) return CUOPT_INVALID_ARGUMENT;
+  for (int i = 0; i < num_problems; ++i) {
+    solutions[i] = nullptr;
+    if (!problems[i]) return CUOPT_INVALID_ARGUMENT;
+  }
+  const PdlpSettings& conf = as_settings(settings)->reg.pdlp;
+  std::vector<SolutionHandle*> sols(num_problems, nullptr);
+  try {
+    for (auto& s : sols) s = new SolutionHandle();
+  } catch (...) {
+    for (auto* s : sols) delete s;
+    return CUOPT_OUT_OF_MEMORY;
+  }
+  std::vector<const LpHost*> lps;
+  std::vector<int> where;  // batch index -> problem index
+  int first_error = CUOPT_SUCCESS;
+  auto fail = [&](SolveResult& R, int code, const std::string& msg) {
+    R.error         = code;
+    R.error_message = msg;
+    R.status        = CUOPT_TERIMINATION_STATUS_NO_TERMINATION;
+    if (first_error == CUOPT_SUCCESS) first_error = code;
+  };
+  for (int i = 0; i < num_problems; ++i) {
+    const LpHost& lp = as_problem(problems[i])->lp;
+    sols[i]->m       = lp.m;
+    sols[i]->n       = lp.n;
+    if (is_mip(lp))
+      fail(sols[i]->res, CUOPT_INVALID_ARGUMENT, "MIP problems are not handled by this LP engine");
+    else if (conf.method == CUOPT_METHOD_DUAL_SIMPLEX || conf.method == CUOPT_METHOD_BARRIER)
+      fail(sols[i]->res, CUOPT_INVALID_ARGUMENT, "only the PDLP method (or concurrent, which runs PDLP) is available");
+    else {
+      lps.push_back(&lp);
+      where.push_back(i);
+    }
+  }
+  try {
+    std::vector<SolveResult> res = batch_solve(lps, conf);
+    for (size_t b = 0; b < res.size(); ++b) sols[where[b]]->res = std::move(res[b]);
+  } catch (const std::exception& e) {
+    const std::string msg = e.what();
+    int code              = CUOPT_RUNTIME_ERROR;
+    if (dynamic_cast<const validation_error*>(&e)) code = CUOPT_VALIDATION_ERROR;
+    if (dynamic_cast<const std::bad_alloc*>(&e) || msg.find("out of memory") != std::string::npos)
+      code = CUOPT_OUT_OF_MEMORY;
+    for (int i : where) fail(sols[i]->res, code, msg);
+  }
+  for (int i = 0; i < num_problems; ++i) {
+    SolveResult& R = sols[i]->res;
+    if (R.x.size() != (size_t)sols[i]->n) R.x.assign(sols[i]->n, 0.0);
+    if (R.y.size() != (size_t)sols[i]->m) R.y.assign(sols[i]->m, 0.0);
+    if (R.rc.size() != (size_t)sols[i]->n) R.rc.assign(sols[i]->n, 0.0);
+    solutions[i] = sols[i];
+  }
+  return first_error;
 }
 
 void cuOptDestroySolution(cuOptSolution* solution_ptr)

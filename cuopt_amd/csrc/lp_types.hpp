@@ -84,6 +84,10 @@ struct validation_error : std::runtime_error {
 // solve_lp for the PDLP path (solve.cu:1123-1342 with method PDLP).
 SolveResult solve_lp(const LpHost& lp, const PdlpSettings& settings);
 
+// [lc, uc] of every row: the ranged form as given, or the row types mapped as
+// E: [b, b], L: [-inf, b], G: [b, +inf] (problem_helpers.cuh:26-51).
+void constraint_bounds(const LpHost& lp, std::vector<double>& lc, std::vector<double>& uc);
+
 // y = A x with the solver's tiled SpMV (test entry point).
 void tiled_spmv_host(int m, int n, const int* rowptr, const int* colidx, const double* vals,
                      const double* x, double* y, int precision);

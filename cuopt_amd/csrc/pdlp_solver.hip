@@ -1407,13 +1407,12 @@ int PdlpSolver<V>::pdhg_loop(SolveResult& R, const RoundSpec& spec)
 // solve_lp: problem checks and dispatch (solve.cu:1123-1342, PDLP path)
 // ---------------------------------------------------------------------------
 
-SolveResult solve_lp(const LpHost& lp, const PdlpSettings& st)
+// Constraint bounds from row types (transform_bounds_functor, problem_helpers.cuh:26-51).
+void constraint_bounds(const LpHost& lp, std::vector<double>& lc, std::vector<double>& uc)
 {
-  const auto t0 = std::chrono::steady_clock::now();
-  SolveResult R;
-  const int m = lp.m, n = lp.n;
-  // constraint bounds from row types (transform_bounds_functor, problem_helpers.cuh:26-51)
-  std::vector<double> lc(m), uc(m);
+  const int m = lp.m;
+  lc.resize(m);
+  uc.resize(m);
   const double inf = std::numeric_limits<double>::infinity();
   for (int i = 0; i < m; ++i) {
     if (lp.ranged) {
@@ -1436,6 +1435,15 @@ SolveResult solve_lp(const LpHost& lp, const PdlpSettings& st)
       }
     }
   }
+}
+
+SolveResult solve_lp(const LpHost& lp, const PdlpSettings& st)
+{
+  const auto t0 = std::chrono::steady_clock::now();
+  SolveResult R;
+  const int m = lp.m, n = lp.n;
+  std::vector<double> lc, uc;
+  constraint_bounds(lp, lc, uc);
   std::vector<double> lb(lp.lb), ub(lp.ub);
   // has_crossing_bounds (problem_checking.cu:316-339)
   bool crossing = false;

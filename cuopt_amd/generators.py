@@ -213,6 +213,15 @@ def transportation_lp(sources: int = 20_000, sinks: int = 50_000, arcs: int = 50
     )
 
 
+def batch_lps(count: int = 4096, m_min: int = 50, m_max: int = 2000, seed: int = 0) -> list[LP]:
+    """The batched workload: ``count`` random equality LPs (random_lp), one seed
+    each, with m log-uniform in [m_min, m_max], n = floor(2.5 m), 8 nonzeros per column."""
+    rng = np.random.default_rng(seed)
+    ms = np.exp(rng.uniform(np.log(m_min), np.log(m_max), size=count)).astype(np.int64)
+    return [random_lp(m=int(m), n=int(2.5 * m), nnz_per_col=8, seed=int(seed) * 1_000_003 + i)
+            for i, m in enumerate(ms)]
+
+
 def kkt_residuals(lp: LP, x: np.ndarray, y: np.ndarray, s: np.ndarray | None = None) -> dict:
     """Independent numpy evaluation of primal/dual feasibility of (x, y) for ``lp``.
 

@@ -107,6 +107,19 @@ cuopt_int_t cuOptAmdBenchmarkSteps(cuOptOptimizationProblem problem,
 /* Number of GPUs visible to the library (0 when no HIP device is usable). */
 cuopt_int_t cuOptAmdDeviceCount(void);
 
+/* Solves num_problems independent LPs together (the heterogeneous batch of the
+ * reference's Python BatchSolve, cython_solve.cu:286-352) in one persistent
+ * launch: one workgroup per LP runs setup and the whole Stable3 loop on the
+ * device.  Uses the settings' tolerances, limits and pdlp_precision; initial
+ * solutions and refinement are not used.  solutions[i] receives a solution
+ * handle for problems[i] (destroy each with cuOptDestroySolution); every
+ * solution reports the wall time of the whole batch as its solve time.
+ * Returns the first non-success error status, or CUOPT_SUCCESS. */
+cuopt_int_t cuOptAmdBatchSolve(cuopt_int_t num_problems,
+                               const cuOptOptimizationProblem* problems,
+                               cuOptSolverSettings settings,
+                               cuOptSolution* solutions);
+
 #ifdef __cplusplus
 }
 #endif

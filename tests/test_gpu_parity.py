@@ -276,3 +276,51 @@ def test_graph_replays_match_direct_launches(monkeypatch):
         assert a.iterations == b.iterations
         assert np.array_equal(a.x, b.x) and np.array_equal(a.y, b.y)
         assert np.array_equal(a.trace, b.trace, equal_nan=True)  # fpe is NaN before the first one
+
+
+# ---- batched persistent solve (cuOptAmdBatchSolve) --------------------------------
+# Same arithmetic as the single-LP path except the order of row reductions: FP64
+# batch results match single-LP solves (status, objective to 1e-7 relative,
+# iteration counts within a few checks) and the closed-form optima.
+def test_batch_fp64_matches_single_solves(highs):
+    lps = [G.random_lp(m=m, n=int(2.5 * m), seed=100 + m) for m in (50, 73, 120, 200, 333, 400)]
+    lps.append(read_mps(os.path.join(GOLDEN, "afiro_original.mps")))
+    lps.append(read_mps(os.path.join(GOLDEN, "good-max.mps")))
+    lps.append(G.skewed_lp(m=2000, n=4000, seed=5))
+    rs = c_api.batch_solve(lps, tol=1e-8, precision="fp64", time_limit=TL, iteration_limit=500000)
+    for lp, r in zip(lps, rs):
+        s = c_api.solve(lp, tol=1e-8, precision="fp64", time_limit=TL, iteration_limit=500000)
+        assert r.status == s.status == 1, (lp.name, r.status, s.status)
+        assert rel(r.primal_objective, s.primal_objective) < 1e-7, lp.name
+        assert abs(r.iterations - s.iterations) <= max(200, s.iterations // 10), (lp.name, r.iterations, s.iterations)
+        assert r.info["power_iterations"] == s.info["power_iterations"], lp.name
+        assert abs(r.info["eta0"] - s.info["eta0"]) <= 1e-10 * s.info["eta0"]
+        if lp.obj_opt is not None:
+            assert rel(r.primal_objective, lp.obj_opt) < 1e-6
+    assert rel(rs[6].primal_objective, highs["afiro_original.mps"]["objective"]) < 1e-6
+    assert rel(rs[7].primal_objective, highs["good-max.mps"]["objective"]) < 1e-6
+
+
+def test_batch_fp32_mixed_config0_style():
+    lps = G.batch_lps(count=64, m_min=50, m_max=400, seed=1)
+    rs = c_api.batch_solve(lps, tol=1e-4, precision="fp32_mixed", time_limit=TL, iteration_limit=200000)
+    for lp, r in zip(lps, rs):
+        assert r.status == 1 and r.info["precision"] == 1
+        assert rel(r.primal_objective, lp.obj_opt) < 1e-3
+
+
+def test_batch_edge_cases_keep_order():
+    good = G.random_lp(m=60, n=150, seed=3)
+    crossing = G.random_lp(m=60, n=150, seed=4)
+    crossing.var_lb = crossing.var_lb.copy()
+    crossing.var_lb[0] = 11.0  # above its upper bound of 10
+    empty = read_mps(os.path.join(GOLDEN, "lp-model-no-constraints.mps"))
+    rs = c_api.batch_solve([good, crossing, empty, good], tol=1e-6, time_limit=TL, iteration_limit=100000)
+    assert [r.status for r in rs] == [1, 2, 6, 1]
+    assert np.array_equal(rs[0].x, rs[3].x)
+
+
+def test_batch_iteration_limit():
+    lps = [G.random_lp(m=100, n=250, seed=s) for s in range(4)]
+    rs = c_api.batch_solve(lps, tol=0.0, time_limit=TL, iteration_limit=50)
+    assert all(r.status == 4 and r.iterations == 50 for r in rs)
