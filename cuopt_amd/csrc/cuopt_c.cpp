@@ -22,6 +22,7 @@
 #include "lp_types.hpp"
 #include "mps_io.hpp"
 #include "settings.hpp"
+#include "unrolled.hpp"
 
 using namespace cuopt_amd;
 
@@ -728,3 +729,101 @@ cuopt_int_t cuOptAmdBenchmarkSteps(cuOptOptimizationProblem problem,
 cuopt_int_t cuOptAmdDeviceCount(void) { return device_count(); }
 
 }  // extern "C"
+
+// ---- differentiable unrolled PDHG (no reference counterpart) ----------------------
+
+namespace {
+template <class F>
+cuopt_int_t guarded(F f)
+{
+  try {
+    f();
+  } catch (const validation_error&) {
+    return CUOPT_VALIDATION_ERROR;
+  } catch (const std::bad_alloc&) {
+    return CUOPT_OUT_OF_MEMORY;
+  } catch (const hip_error& e) {
+    return std::string(e.what()).find("out of memory") != std::string::npos ? CUOPT_OUT_OF_MEMORY
+                                                                            : CUOPT_RUNTIME_ERROR;
+  } catch (...) {
+    return CUOPT_RUNTIME_ERROR;
+  }
+  return CUOPT_SUCCESS;
+}
+}  // namespace
+
+cuopt_int_t cuOptAmdUnrolledCreate(cuopt_int_t m,
+                                   cuopt_int_t n,
+                                   const cuopt_int_t* row_offsets,
+                                   const cuopt_int_t* column_indices,
+                                   const cuopt_float_t* values,
+                                   const cuopt_float_t* lower_bounds,
+                                   const cuopt_float_t* upper_bounds,
+                                   cuopt_float_t ridge,
+                                   cuopt_int_t precision,
+                                   cuOptAmdUnrolled* handle_ptr)
+{
+  if (!handle_ptr) return CUOPT_INVALID_ARGUMENT;
+  *handle_ptr = nullptr;
+  if (m < 0 || n < 0 || !row_offsets || !column_indices || !values || !lower_bounds || !upper_bounds || !(ridge >= 0))
+    return CUOPT_INVALID_ARGUMENT;
+  if (precision != CUOPT_AMD_PRECISION_FP64 && precision != CUOPT_AMD_PRECISION_FP32_MIXED)
+    return CUOPT_INVALID_ARGUMENT;
+  if (row_offsets[0] != 0) return CUOPT_INVALID_ARGUMENT;
+  for (int i = 0; i < m; ++i)
+    if (row_offsets[i + 1] < row_offsets[i]) return CUOPT_INVALID_ARGUMENT;
+  for (int k = 0; k < row_offsets[m]; ++k)
+    if (column_indices[k] < 0 || column_indices[k] >= n) return CUOPT_INVALID_ARGUMENT;
+  UnrolledHandle* h = nullptr;
+  const cuopt_int_t rc = guarded([&] {
+    h = unrolled_create(m, n, row_offsets, column_indices, values, lower_bounds, upper_bounds, ridge, precision);
+  });
+  *handle_ptr = h;
+  return rc;
+}
+
+void cuOptAmdUnrolledDestroy(cuOptAmdUnrolled* handle_ptr)
+{
+  if (!handle_ptr || !*handle_ptr) return;
+  unrolled_destroy(static_cast<UnrolledHandle*>(*handle_ptr));
+  *handle_ptr = nullptr;
+}
+
+cuopt_int_t cuOptAmdUnrolledStepSize(cuOptAmdUnrolled handle, cuopt_float_t* eta)
+{
+  if (!handle || !eta) return CUOPT_INVALID_ARGUMENT;
+  *eta = unrolled_step_size(static_cast<UnrolledHandle*>(handle));
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptAmdUnrolledForward(cuOptAmdUnrolled handle,
+                                    cuopt_int_t batch,
+                                    cuopt_int_t steps,
+                                    cuopt_int_t checkpoint_every,
+                                    const cuopt_float_t* c,
+                                    const cuopt_float_t* lc,
+                                    const cuopt_float_t* uc,
+                                    cuopt_int_t bounds_per_batch,
+                                    cuopt_float_t* x_out,
+                                    cuopt_float_t* y_out,
+                                    void* stream)
+{
+  if (!handle || !c || !lc || !uc || !x_out || batch <= 0 || steps < 0 || checkpoint_every <= 0)
+    return CUOPT_INVALID_ARGUMENT;
+  return guarded([&] {
+    unrolled_forward(static_cast<UnrolledHandle*>(handle), batch, steps, checkpoint_every, c, lc, uc,
+                     bounds_per_batch, x_out, y_out, stream);
+  });
+}
+
+cuopt_int_t cuOptAmdUnrolledBackward(cuOptAmdUnrolled handle,
+                                     const cuopt_float_t* gx,
+                                     const cuopt_float_t* gy,
+                                     cuopt_float_t* gc,
+                                     cuopt_float_t* glc,
+                                     cuopt_float_t* guc,
+                                     void* stream)
+{
+  if (!handle || !gx || !gc || !glc || !guc) return CUOPT_INVALID_ARGUMENT;
+  return guarded([&] { unrolled_backward(static_cast<UnrolledHandle*>(handle), gx, gy, gc, glc, guc, stream); });
+}

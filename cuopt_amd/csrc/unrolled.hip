@@ -1,0 +1,536 @@
+// unrolled.hip -- differentiable unrolled PDHG over a batch of objectives (gfx950).
+//
+// Forward: K steps of the PDHG iteration of the LP path (projections of
+// pdhg.cu:431-545, reflection x-bar = 2 x+ - x) for B right-hand sides at once,
+// with a ridge term lam/2 ||x||^2 whose proximal step divides by 1 + tau lam:
+//
+//   x+ = clamp((x - tau (c - A^T y)) / (1 + tau lam), l, u)
+//   y+ = sigma (t - clamp(t, -u_c, -l_c)),   t = y / sigma - A (2 x+ - x)
+//
+// tau = sigma = eta = 0.998 / sigma_max(A) (the power iteration of
+// compute_initial_step_size, pdlp.cu:2622-2738, on the unscaled matrix), no
+// restarts: x_K is a piecewise-linear function of (c, l_c, u_c).
+//
+// Backward: the vector-Jacobian product runs the steps in reverse with the
+// clamp states of x+ and y+ (transposed SpMV plus projection masks).  States
+// are checkpointed every `every` steps during the forward pass; each segment is
+// recomputed from its checkpoint before its reverse sweep.  The c and (l_c, u_c)
+// gradients are accumulated in FP64.
+//
+// Layout: batch-interleaved vectors, v[i * B + b].  SpMM kernels give one
+// wavefront to a matrix row and one lane to each batch member, so every nonzero
+// gathers B contiguous values of the operand (one 256-byte FP32 line for
+// B = 64) while the row's value / index stream is read once for the whole batch.
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "common.hpp"
+#include "pdlp_kernels.hpp"
+#include "unrolled.hpp"
+
+namespace cuopt_amd {
+
+namespace {
+
+constexpr int kUB = 256;              // 4 wavefronts: 4 rows per workgroup
+constexpr int kRowsPerWg = kUB / kWave;
+
+inline dim3 spmm_grid(int rows, int B) { return dim3((rows + kRowsPerWg - 1) / kRowsPerWg, (B + kWave - 1) / kWave); }
+
+// Row of this wavefront and batch member of this lane; false when out of range.
+__device__ inline bool row_lane(int rows, int B, int& r, int& b)
+{
+  r = blockIdx.x * kRowsPerWg + (int)threadIdx.x / kWave;
+  b = blockIdx.y * kWave + ((int)threadIdx.x & (kWave - 1));
+  return r < rows && b < B;
+}
+
+template <typename V>
+__device__ inline typename AccFor<V>::type row_dot(const int* __restrict__ ptr,
+                                                   const int* __restrict__ idx,
+                                                   const V* __restrict__ val,
+                                                   const V* __restrict__ vec,
+                                                   int r,
+                                                   int b,
+                                                   int B)
+{
+  using Acc = typename AccFor<V>::type;
+  Acc a     = Acc::zero();
+  for (int k = ptr[r]; k < ptr[r + 1]; ++k) a.add(Acc::prod(val[k], vec[(size_t)idx[k] * B + b]));
+  return a;
+}
+
+// ---- forward ----------------------------------------------------------------
+
+template <typename V>
+__global__ __launch_bounds__(kUB) void unr_primal(const int* __restrict__ tptr, const int* __restrict__ tidx,
+                                                  const V* __restrict__ tval, int n, int B, const V* __restrict__ y,
+                                                  const V* __restrict__ x, const V* __restrict__ c,
+                                                  const V* __restrict__ lb, const V* __restrict__ ub, V tau,
+                                                  V inv_prox, V* __restrict__ xn, V* __restrict__ xb)
+{
+  int j, b;
+  if (!row_lane(n, B, j, b)) return;
+  const V aty   = row_dot(tptr, tidx, tval, y, j, b, B).value();
+  const size_t o = (size_t)j * B + b;
+  const V xr    = x[o];
+  const V v     = (xr - tau * (c[o] - aty)) * inv_prox;
+  const V xnew  = mx(mn(v, ub[j]), lb[j]);
+  xn[o]         = xnew;
+  xb[o]         = V(2) * xnew - xr;
+}
+
+template <typename V>
+__global__ __launch_bounds__(kUB) void unr_dual(const int* __restrict__ aptr, const int* __restrict__ aidx,
+                                                const V* __restrict__ aval, int m, int B, const V* __restrict__ xb,
+                                                const V* __restrict__ y, const V* __restrict__ lc,
+                                                const V* __restrict__ uc, int ldb, V sigma, V* __restrict__ yn)
+{
+  int i, b;
+  if (!row_lane(m, B, i, b)) return;
+  const V ax     = row_dot(aptr, aidx, aval, xb, i, b, B).value();
+  const size_t o = (size_t)i * B + b;
+  const size_t q = (size_t)i * ldb + (ldb ? b : 0);
+  const V t      = y[o] / sigma - ax;
+  const V proj   = mx(-uc[q], mn(t, -lc[q]));
+  yn[o]          = (t - proj) * sigma;
+}
+
+// ---- backward ---------------------------------------------------------------
+
+// gt = sigma (1 - inside) ay for the step that produced y+ (its clamp states),
+// and the FP64 (l_c, u_c) gradient terms.
+template <typename V>
+__device__ inline V dual_adjoint(V ay, V yplus, V l, V u, V sigma, double& dl, double& du)
+{
+  const bool eq = l == u;
+  if (yplus == V(0) && !eq) {
+    dl = du = 0.0;
+    return V(0);
+  }
+  const double g = (double)sigma * (double)ay;
+  if (yplus >= V(0)) {  // also y+ == 0 on an equality row: either side, both are b
+    dl = g;
+    du = 0.0;
+  } else {
+    dl = 0.0;
+    du = g;
+  }
+  return sigma * ay;
+}
+
+// Start of the reverse sweep: gt for the last step from the upstream y-gradient.
+template <typename V>
+__global__ __launch_bounds__(kUB) void unr_adj_start(int m, int B, const V* ay,  // may alias gt
+                                                     const V* __restrict__ yplus, const V* __restrict__ lc,
+                                                     const V* __restrict__ uc, int ldb, V sigma, V* gt,
+                                                     double* __restrict__ glc, double* __restrict__ guc)
+{
+  const size_t tot = (size_t)m * B;
+  for (size_t o = blockIdx.x * (size_t)kUB + threadIdx.x; o < tot; o += (size_t)gridDim.x * kUB) {
+    const int i = (int)(o / B), b = (int)(o % B);
+    const size_t q = (size_t)i * ldb + (ldb ? b : 0);
+    double dl, du;
+    gt[o] = dual_adjoint(ay ? ay[o] : V(0), yplus[o], lc[q], uc[q], sigma, dl, du);
+    glc[o] += dl;
+    guc[o] += du;
+  }
+}
+
+// Rows j of A^T: gxb = -A^T gt; gw = inner(x+) (ax + 2 gxb) / (1 + tau lam);
+// ax <- gw - gxb (adjoint of x before the step); gc_acc += gw (FP64).
+template <typename V>
+__global__ __launch_bounds__(kUB) void unr_adj_primal(const int* __restrict__ tptr, const int* __restrict__ tidx,
+                                                      const V* __restrict__ tval, int n, int B,
+                                                      const V* __restrict__ gt, const V* __restrict__ xplus,
+                                                      const V* __restrict__ lb, const V* __restrict__ ub, V inv_prox,
+                                                      V* __restrict__ ax, V* __restrict__ gw,
+                                                      double* __restrict__ gc_acc)
+{
+  int j, b;
+  if (!row_lane(n, B, j, b)) return;
+  const V gxb    = -row_dot(tptr, tidx, tval, gt, j, b, B).value();
+  const size_t o = (size_t)j * B + b;
+  const V xp     = xplus[o];
+  const bool in  = xp > lb[j] && xp < ub[j];
+  const V w      = in ? (ax[o] + V(2) * gxb) * inv_prox : V(0);
+  gw[o]          = w;
+  ax[o]          = w - gxb;
+  gc_acc[o] += (double)w;
+}
+
+// Rows i of A: ay_prev = gt / sigma + tau A gw; then, if there is an earlier
+// step, its gt from ay_prev and the clamp states of its y+ (= y before this step).
+// gt is read and rewritten in place, element by element.
+template <typename V>
+__global__ __launch_bounds__(kUB) void unr_adj_dual(const int* __restrict__ aptr, const int* __restrict__ aidx,
+                                                    const V* __restrict__ aval, int m, int B,
+                                                    const V* __restrict__ gw, V tau, V sigma,
+                                                    const V* __restrict__ yprev, const V* __restrict__ lc,
+                                                    const V* __restrict__ uc, int ldb, V* __restrict__ gt,
+                                                    double* __restrict__ glc, double* __restrict__ guc)
+{
+  int i, b;
+  if (!row_lane(m, B, i, b)) return;
+  const V agw    = row_dot(aptr, aidx, aval, gw, i, b, B).value();
+  const size_t o = (size_t)i * B + b;
+  const V ay     = gt[o] / sigma + tau * agw;  // adjoint of y before this step
+  const size_t q = (size_t)i * ldb + (ldb ? b : 0);
+  double dl, du;
+  gt[o] = dual_adjoint(ay, yprev[o], lc[q], uc[q], sigma, dl, du);
+  glc[o] += dl;
+  guc[o] += du;
+}
+
+template <typename V>
+__global__ __launch_bounds__(kUB) void unr_convert_in(size_t n, const double* __restrict__ in, V* __restrict__ out)
+{
+  for (size_t i = blockIdx.x * (size_t)kUB + threadIdx.x; i < n; i += (size_t)gridDim.x * kUB) out[i] = (V)in[i];
+}
+template <typename V>
+__global__ __launch_bounds__(kUB) void unr_convert_out(size_t n, const V* __restrict__ in, double scale,
+                                                       double* __restrict__ out)
+{
+  for (size_t i = blockIdx.x * (size_t)kUB + threadIdx.x; i < n; i += (size_t)gridDim.x * kUB)
+    out[i] = scale * (double)in[i];
+}
+template <typename V>
+__global__ __launch_bounds__(kUB) void unr_init_x(int n, int B, const V* __restrict__ lb, const V* __restrict__ ub,
+                                                  V* __restrict__ x)
+{
+  const size_t tot = (size_t)n * B;
+  for (size_t o = blockIdx.x * (size_t)kUB + threadIdx.x; o < tot; o += (size_t)gridDim.x * kUB) {
+    const int j = (int)(o / B);
+    x[o]        = mn(mx(V(0), lb[j]), ub[j]);
+  }
+}
+__global__ __launch_bounds__(kUB) void unr_zero64(size_t n, double* __restrict__ v)
+{
+  for (size_t i = blockIdx.x * (size_t)kUB + threadIdx.x; i < n; i += (size_t)gridDim.x * kUB) v[i] = 0.0;
+}
+template <typename V>
+__global__ __launch_bounds__(kUB) void unr_zero(size_t n, V* __restrict__ v)
+{
+  for (size_t i = blockIdx.x * (size_t)kUB + threadIdx.x; i < n; i += (size_t)gridDim.x * kUB) v[i] = V(0);
+}
+// d = c / ||z|| style power step helpers (B = 1): z <- A A^T z, then norms on the host.
+__global__ __launch_bounds__(kUB) void unr_sumsq(int n, const double* __restrict__ v, double* __restrict__ out)
+{
+  __shared__ double lds[kUB];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += kUB) s += v[i] * v[i];
+  lds[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = kUB / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) lds[threadIdx.x] += lds[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = lds[0];
+}
+__global__ __launch_bounds__(kUB) void unr_spmv64(const int* __restrict__ ptr, const int* __restrict__ idx,
+                                                  const double* __restrict__ val, int rows, const double* __restrict__ x,
+                                                  double scale, double* __restrict__ y)
+{
+  const int r = blockIdx.x * kUB + threadIdx.x;
+  if (r >= rows) return;
+  double s = 0.0;
+  for (int k = ptr[r]; k < ptr[r + 1]; ++k) s += val[k] * (x[idx[k]] * scale);
+  y[r] = s;
+}
+
+inline int ew_blocks(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>((n + kUB - 1) / kUB, 4096)); }
+
+struct UnrolledImpl {
+  int m = 0, n = 0, precision = 1;
+  double lam = 0.0, eta = 1.0;
+  // structure (device)
+  DevBuf<int> aptr, aidx, tptr, tidx;
+  DevBuf<double> aval64, tval64;
+  DevBuf<float> aval32, tval32;
+  DevBuf<double> lb64, ub64;
+  DevBuf<float> lb32, ub32;
+  // last forward
+  int B = 0, K = 0, every = 1, ldb = 0;
+  DevBuf<unsigned char> work;  // typed views below
+  size_t ckpt_off = 0, seg_off = 0;
+};
+
+template <typename V>
+struct Views {
+  const int *aptr, *aidx, *tptr, *tidx;
+  const V *aval, *tval, *lb, *ub;
+};
+
+template <typename V>
+Views<V> views(UnrolledImpl& U);
+template <>
+Views<double> views<double>(UnrolledImpl& U)
+{
+  return {U.aptr.get(), U.aidx.get(), U.tptr.get(), U.tidx.get(), U.aval64.get(), U.tval64.get(), U.lb64.get(),
+          U.ub64.get()};
+}
+template <>
+Views<float> views<float>(UnrolledImpl& U)
+{
+  return {U.aptr.get(), U.aidx.get(), U.tptr.get(), U.tidx.get(), U.aval32.get(), U.tval32.get(), U.lb32.get(),
+          U.ub32.get()};
+}
+
+// Working storage of one forward/backward pair, carved from U.work:
+//   c, lc, uc (working precision), x[2], y[2], xb, checkpoints (x, y) per segment,
+//   segment states x+/y+ for every step of one segment, adjoints.
+template <typename V>
+struct Work {
+  V *c, *lc, *uc, *x[2], *y[2], *xb, *ck_x, *ck_y, *sx, *sy, *ax, *gt, *gw;
+  double *gc, *glc, *guc;
+};
+
+template <typename V>
+Work<V> carve(UnrolledImpl& U, int nseg, bool allocate)
+{
+  const size_t nB = (size_t)U.n * U.B, mB = (size_t)U.m * U.B, lcn = U.ldb ? mB : (size_t)U.m;
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  const size_t o_c = take(nB * sizeof(V)), o_lc = take(lcn * sizeof(V)), o_uc = take(lcn * sizeof(V));
+  const size_t o_x0 = take(nB * sizeof(V)), o_x1 = take(nB * sizeof(V)), o_y0 = take(mB * sizeof(V)),
+               o_y1 = take(mB * sizeof(V)), o_xb = take(nB * sizeof(V));
+  const size_t o_ckx = take((size_t)(nseg + 1) * nB * sizeof(V)), o_cky = take((size_t)(nseg + 1) * mB * sizeof(V));
+  const size_t o_sx = take((size_t)(U.every + 1) * nB * sizeof(V)), o_sy = take((size_t)(U.every + 1) * mB * sizeof(V));
+  const size_t o_ax = take(nB * sizeof(V)), o_gt = take(mB * sizeof(V)), o_gw = take(nB * sizeof(V));
+  const size_t o_gc = take(nB * sizeof(double)), o_glc = take(mB * sizeof(double)), o_guc = take(mB * sizeof(double));
+  if (allocate) U.work.resize(off);
+  unsigned char* p = U.work.get();
+  auto at = [&](size_t o) { return reinterpret_cast<V*>(p + o); };
+  auto atd = [&](size_t o) { return reinterpret_cast<double*>(p + o); };
+  return Work<V>{at(o_c), at(o_lc), at(o_uc), {at(o_x0), at(o_x1)}, {at(o_y0), at(o_y1)}, at(o_xb), at(o_ckx),
+                 at(o_cky), at(o_sx), at(o_sy), at(o_ax), at(o_gt), at(o_gw), atd(o_gc), atd(o_glc), atd(o_guc)};
+}
+
+// One PDHG step (x, y) -> (xn, yn) through the scratch xb.
+template <typename V>
+void step(const Views<V>& M, UnrolledImpl& U, const Work<V>& W, const V* x, const V* y, V* xn, V* yn, hipStream_t s)
+{
+  const V tau = (V)U.eta, sigma = (V)U.eta, inv_prox = (V)(1.0 / (1.0 + U.eta * U.lam));
+  hipLaunchKernelGGL(unr_primal<V>, spmm_grid(U.n, U.B), dim3(kUB), 0, s, M.tptr, M.tidx, M.tval, U.n, U.B, y, x,
+                     W.c, M.lb, M.ub, tau, inv_prox, xn, W.xb);
+  hipLaunchKernelGGL(unr_dual<V>, spmm_grid(U.m, U.B), dim3(kUB), 0, s, M.aptr, M.aidx, M.aval, U.m, U.B, W.xb, y,
+                     W.lc, W.uc, U.ldb, sigma, yn);
+}
+
+template <typename V>
+void forward_impl(UnrolledImpl& U, const double* c, const double* lc, const double* uc, double* x_out, double* y_out,
+                  hipStream_t s)
+{
+  const int nseg = (U.K + U.every - 1) / U.every;
+  Work<V> W      = carve<V>(U, nseg, true);
+  Views<V> M     = views<V>(U);
+  const size_t nB = (size_t)U.n * U.B, mB = (size_t)U.m * U.B, lcn = U.ldb ? mB : (size_t)U.m;
+  hipLaunchKernelGGL(unr_convert_in<V>, dim3(ew_blocks(nB)), dim3(kUB), 0, s, nB, c, W.c);
+  hipLaunchKernelGGL(unr_convert_in<V>, dim3(ew_blocks(lcn)), dim3(kUB), 0, s, lcn, lc, W.lc);
+  hipLaunchKernelGGL(unr_convert_in<V>, dim3(ew_blocks(lcn)), dim3(kUB), 0, s, lcn, uc, W.uc);
+  hipLaunchKernelGGL(unr_init_x<V>, dim3(ew_blocks(nB)), dim3(kUB), 0, s, U.n, U.B, M.lb, M.ub, W.x[0]);
+  hipLaunchKernelGGL(unr_zero<V>, dim3(ew_blocks(mB)), dim3(kUB), 0, s, mB, W.y[0]);
+  int cur = 0;
+  for (int k = 0; k < U.K; ++k) {
+    if (k % U.every == 0) {
+      const int sg = k / U.every;
+      CUOPT_HIP_CHECK(hipMemcpyAsync(W.ck_x + (size_t)sg * nB, W.x[cur], nB * sizeof(V), hipMemcpyDeviceToDevice, s));
+      CUOPT_HIP_CHECK(hipMemcpyAsync(W.ck_y + (size_t)sg * mB, W.y[cur], mB * sizeof(V), hipMemcpyDeviceToDevice, s));
+    }
+    step<V>(M, U, W, W.x[cur], W.y[cur], W.x[cur ^ 1], W.y[cur ^ 1], s);
+    cur ^= 1;
+  }
+  CUOPT_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(unr_convert_out<V>, dim3(ew_blocks(nB)), dim3(kUB), 0, s, nB, W.x[cur], 1.0, x_out);
+  if (y_out) hipLaunchKernelGGL(unr_convert_out<V>, dim3(ew_blocks(mB)), dim3(kUB), 0, s, mB, W.y[cur], 1.0, y_out);
+  // final state kept for the backward pass (segment states are recomputed)
+  CUOPT_HIP_CHECK(hipGetLastError());
+}
+
+template <typename V>
+void backward_impl(UnrolledImpl& U, const double* gx, const double* gy, double* gc, double* glc, double* guc,
+                   hipStream_t s)
+{
+  const int nseg  = (U.K + U.every - 1) / U.every;
+  Work<V> W       = carve<V>(U, nseg, false);
+  Views<V> M      = views<V>(U);
+  const size_t nB = (size_t)U.n * U.B, mB = (size_t)U.m * U.B;
+  const V tau = (V)U.eta, sigma = (V)U.eta, inv_prox = (V)(1.0 / (1.0 + U.eta * U.lam));
+  hipLaunchKernelGGL(unr_zero64, dim3(ew_blocks(nB)), dim3(kUB), 0, s, nB, W.gc);
+  hipLaunchKernelGGL(unr_zero64, dim3(ew_blocks(mB)), dim3(kUB), 0, s, mB, W.glc);
+  hipLaunchKernelGGL(unr_zero64, dim3(ew_blocks(mB)), dim3(kUB), 0, s, mB, W.guc);
+  hipLaunchKernelGGL(unr_convert_in<V>, dim3(ew_blocks(nB)), dim3(kUB), 0, s, nB, gx, W.ax);
+  // y-gradient: start value of ay (zero when absent), staged in gw's slot of size m*B? use gt as ay first.
+  if (gy) hipLaunchKernelGGL(unr_convert_in<V>, dim3(ew_blocks(mB)), dim3(kUB), 0, s, mB, gy, W.gt);
+  bool started = false;
+  for (int sg = nseg - 1; sg >= 0; --sg) {
+    const int k0 = sg * U.every, k1 = std::min(U.K, k0 + U.every), L = k1 - k0;
+    // recompute states x_{k0..k1}, y_{k0..k1} into the segment store (slot t holds step k0 + t)
+    CUOPT_HIP_CHECK(hipMemcpyAsync(W.sx, W.ck_x + (size_t)sg * nB, nB * sizeof(V), hipMemcpyDeviceToDevice, s));
+    CUOPT_HIP_CHECK(hipMemcpyAsync(W.sy, W.ck_y + (size_t)sg * mB, mB * sizeof(V), hipMemcpyDeviceToDevice, s));
+    for (int t = 0; t < L; ++t)
+      step<V>(M, U, W, W.sx + (size_t)t * nB, W.sy + (size_t)t * mB, W.sx + (size_t)(t + 1) * nB,
+              W.sy + (size_t)(t + 1) * mB, s);
+    for (int t = L - 1; t >= 0; --t) {
+      const V* xplus = W.sx + (size_t)(t + 1) * nB;
+      const V* yplus = W.sy + (size_t)(t + 1) * mB;
+      if (!started) {  // last step overall: gt from the upstream y-gradient
+        hipLaunchKernelGGL(unr_adj_start<V>, dim3(ew_blocks(mB)), dim3(kUB), 0, s, U.m, U.B,
+                           gy ? (const V*)W.gt : (const V*)nullptr, yplus, (const V*)W.lc, (const V*)W.uc, U.ldb,
+                           sigma, W.gt, W.glc, W.guc);
+        started = true;
+      }
+      hipLaunchKernelGGL(unr_adj_primal<V>, spmm_grid(U.n, U.B), dim3(kUB), 0, s, M.tptr, M.tidx, M.tval, U.n, U.B,
+                         (const V*)W.gt, xplus, M.lb, M.ub, inv_prox, W.ax, W.gw, W.gc);
+      // the first step starts from the constant y = 0: no earlier step to propagate into
+      if (sg > 0 || t > 0)
+        hipLaunchKernelGGL(unr_adj_dual<V>, spmm_grid(U.m, U.B), dim3(kUB), 0, s, M.aptr, M.aidx, M.aval, U.m, U.B,
+                           (const V*)W.gw, tau, sigma, (const V*)(W.sy + (size_t)t * mB), (const V*)W.lc,
+                           (const V*)W.uc, U.ldb, W.gt, W.glc, W.guc);
+    }
+  }
+  CUOPT_HIP_CHECK(hipGetLastError());
+  // gc = -tau / (1 + tau lam) * sum of gw: the kernel accumulated gw, already divided by (1 + tau lam).
+  hipLaunchKernelGGL(unr_convert_out<double>, dim3(ew_blocks(nB)), dim3(kUB), 0, s, nB, (const double*)W.gc,
+                     -U.eta, gc);
+  CUOPT_HIP_CHECK(hipMemcpyAsync(glc, W.glc, mB * sizeof(double), hipMemcpyDeviceToDevice, s));
+  CUOPT_HIP_CHECK(hipMemcpyAsync(guc, W.guc, mB * sizeof(double), hipMemcpyDeviceToDevice, s));
+  CUOPT_HIP_CHECK(hipGetLastError());
+}
+
+}  // namespace
+
+// ---- C++ entry points used by the C ABI ---------------------------------------------
+
+UnrolledHandle* unrolled_create(int m, int n, const int* rowptr, const int* colidx, const double* vals,
+                                const double* lb, const double* ub, double lam, int precision)
+{
+  auto* U = new UnrolledImpl();
+  try {
+    U->m = m;
+    U->n = n;
+    U->lam = lam;
+    U->precision = precision;
+    const int nnz = rowptr[m];
+    // A^T by a stable counting sort
+    std::vector<int> tp(n + 1, 0), ti(nnz);
+    std::vector<double> tv(nnz);
+    for (int k = 0; k < nnz; ++k) tp[colidx[k] + 1]++;
+    for (int j = 0; j < n; ++j) tp[j + 1] += tp[j];
+    {
+      std::vector<int> cur(tp.begin(), tp.end() - 1);
+      for (int i = 0; i < m; ++i)
+        for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+          const int d = cur[colidx[k]]++;
+          ti[d]       = i;
+          tv[d]       = vals[k];
+        }
+    }
+    hipStream_t s = nullptr;
+    U->aptr.resize(m + 1);
+    U->aptr.upload(rowptr, m + 1, s);
+    U->aidx.resize(std::max(nnz, 1));
+    U->aidx.upload(colidx, nnz, s);
+    U->tptr.resize(n + 1);
+    U->tptr.upload(tp.data(), n + 1, s);
+    U->tidx.resize(std::max(nnz, 1));
+    U->tidx.upload(ti.data(), nnz, s);
+    U->aval64.resize(std::max(nnz, 1));
+    U->aval64.upload(vals, nnz, s);
+    U->tval64.resize(std::max(nnz, 1));
+    U->tval64.upload(tv.data(), nnz, s);
+    U->lb64.resize(std::max(n, 1));
+    U->lb64.upload(lb, n, s);
+    U->ub64.resize(std::max(n, 1));
+    U->ub64.upload(ub, n, s);
+    std::vector<float> f, g, l, u;  // host sources of the async uploads, alive until the final sync
+    if (precision == CUOPT_AMD_PRECISION_FP32_MIXED) {
+      f.assign(vals, vals + nnz);
+      g.assign(tv.begin(), tv.end());
+      l.assign(lb, lb + n);
+      u.assign(ub, ub + n);
+      U->aval32.resize(std::max(nnz, 1));
+      U->aval32.upload(f, s);
+      U->tval32.resize(std::max(nnz, 1));
+      U->tval32.upload(g, s);
+      U->lb32.resize(std::max(n, 1));
+      U->lb32.upload(l, s);
+      U->ub32.resize(std::max(n, 1));
+      U->ub32.upload(u, s);
+    }
+    // power iteration on A A^T (FP64): z <- A (A^T z / ||z||), sigma_max^2 = ||z_new|| at convergence
+    DevBuf<double> z(std::max(m, 1)), w(std::max(n, 1)), nrm(1);
+    std::vector<double> z0(m);
+    for (int i = 0; i < m; ++i) z0[i] = 1.0 + 0.5 * std::sin(0.7 * i + 0.3);  // deterministic start
+    z.upload(z0, s);
+    double norm2 = 0.0, prev = 0.0, s2 = 0.0;
+    hipLaunchKernelGGL(unr_sumsq, dim3(1), dim3(kUB), 0, s, m, z.get(), nrm.get());
+    nrm.download(&norm2, 1, s);
+    CUOPT_HIP_CHECK(hipStreamSynchronize(s));
+    for (int it = 0; it < 2000 && m > 0; ++it) {
+      const double inv = 1.0 / std::sqrt(norm2);
+      hipLaunchKernelGGL(unr_spmv64, dim3((n + kUB - 1) / kUB), dim3(kUB), 0, s, U->tptr.get(), U->tidx.get(),
+                         U->tval64.get(), n, z.get(), inv, w.get());
+      hipLaunchKernelGGL(unr_spmv64, dim3((m + kUB - 1) / kUB), dim3(kUB), 0, s, U->aptr.get(), U->aidx.get(),
+                         U->aval64.get(), m, w.get(), 1.0, z.get());
+      hipLaunchKernelGGL(unr_sumsq, dim3(1), dim3(kUB), 0, s, m, z.get(), nrm.get());
+      nrm.download(&norm2, 1, s);
+      CUOPT_HIP_CHECK(hipStreamSynchronize(s));
+      s2 = std::sqrt(norm2);  // ||A A^T q|| -> sigma_max^2
+      if (it > 10 && std::fabs(s2 - prev) <= 1e-10 * s2) break;
+      prev = s2;
+    }
+    U->eta = s2 > 0.0 ? 0.998 / std::sqrt(s2) : 1.0;
+    CUOPT_HIP_CHECK(hipStreamSynchronize(s));
+  } catch (...) {
+    delete U;
+    throw;
+  }
+  return reinterpret_cast<UnrolledHandle*>(U);
+}
+
+void unrolled_destroy(UnrolledHandle* h) { delete reinterpret_cast<UnrolledImpl*>(h); }
+
+double unrolled_step_size(UnrolledHandle* h) { return reinterpret_cast<UnrolledImpl*>(h)->eta; }
+
+void unrolled_forward(UnrolledHandle* h, int B, int K, int every, const double* c, const double* lc, const double* uc,
+                      int bounds_per_batch, double* x, double* y, void* stream)
+{
+  auto* U = reinterpret_cast<UnrolledImpl*>(h);
+  if (B <= 0 || K < 0 || every <= 0) throw validation_error("batch and checkpoint interval must be positive");
+  U->B     = B;
+  U->K     = K;
+  U->every = every;
+  U->ldb   = bounds_per_batch ? B : 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (U->precision == CUOPT_AMD_PRECISION_FP32_MIXED)
+    forward_impl<float>(*U, c, lc, uc, x, y, s);
+  else
+    forward_impl<double>(*U, c, lc, uc, x, y, s);
+}
+
+void unrolled_backward(UnrolledHandle* h, const double* gx, const double* gy, double* gc, double* glc, double* guc,
+                       void* stream)
+{
+  auto* U = reinterpret_cast<UnrolledImpl*>(h);
+  if (U->B <= 0) throw validation_error("backward called before forward");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (U->K == 0) {
+    const size_t nB = (size_t)U->n * U->B, mB = (size_t)U->m * U->B;
+    hipLaunchKernelGGL(unr_zero64, dim3(ew_blocks(nB)), dim3(kUB), 0, s, nB, gc);
+    hipLaunchKernelGGL(unr_zero64, dim3(ew_blocks(mB)), dim3(kUB), 0, s, mB, glc);
+    hipLaunchKernelGGL(unr_zero64, dim3(ew_blocks(mB)), dim3(kUB), 0, s, mB, guc);
+    return;
+  }
+  if (U->precision == CUOPT_AMD_PRECISION_FP32_MIXED)
+    backward_impl<float>(*U, gx, gy, gc, glc, guc, s);
+  else
+    backward_impl<double>(*U, gx, gy, gc, glc, guc, s);
+}
+
+}  // namespace cuopt_amd

@@ -107,6 +107,52 @@ cuopt_int_t cuOptAmdBenchmarkSteps(cuOptOptimizationProblem problem,
 /* Number of GPUs visible to the library (0 when no HIP device is usable). */
 cuopt_int_t cuOptAmdDeviceCount(void);
 
+/* ---- differentiable unrolled PDHG ------------------------------------------------
+ * K steps of PDHG (no restarts) for B objectives at once on one constraint
+ * matrix, min c_b^T x + ridge/2 ||x||^2 s.t. lc <= A x <= uc, lb <= x <= ub,
+ * with step size 0.998 / sigma_max(A), and the vector-Jacobian product of
+ * (x_K, y_K) with respect to (c, lc, uc) through checkpointed recomputation.
+ * No reference counterpart (the reference LP path is not differentiable);
+ * it serves the torch.autograd.Function in cuopt_amd/autograd.py. */
+typedef void* cuOptAmdUnrolled;
+/* Host CSR, host bounds; precision as CUOPT_AMD_PDLP_PRECISION. */
+cuopt_int_t cuOptAmdUnrolledCreate(cuopt_int_t num_constraints,
+                                   cuopt_int_t num_variables,
+                                   const cuopt_int_t* row_offsets,
+                                   const cuopt_int_t* column_indices,
+                                   const cuopt_float_t* values,
+                                   const cuopt_float_t* lower_bounds,
+                                   const cuopt_float_t* upper_bounds,
+                                   cuopt_float_t ridge,
+                                   cuopt_int_t precision,
+                                   cuOptAmdUnrolled* handle_ptr);
+void cuOptAmdUnrolledDestroy(cuOptAmdUnrolled* handle_ptr);
+cuopt_int_t cuOptAmdUnrolledStepSize(cuOptAmdUnrolled handle, cuopt_float_t* eta);
+/* Device pointers, batch-interleaved (v[i * batch + b]).  c: n x batch;
+ * lc, uc: m x batch if bounds_per_batch else m; x_out: n x batch; y_out:
+ * m x batch or NULL.  Checkpoints every `checkpoint_every` steps.  Work is
+ * queued on `stream` (a hipStream_t, NULL for the default stream). */
+cuopt_int_t cuOptAmdUnrolledForward(cuOptAmdUnrolled handle,
+                                    cuopt_int_t batch,
+                                    cuopt_int_t steps,
+                                    cuopt_int_t checkpoint_every,
+                                    const cuopt_float_t* c,
+                                    const cuopt_float_t* lc,
+                                    const cuopt_float_t* uc,
+                                    cuopt_int_t bounds_per_batch,
+                                    cuopt_float_t* x_out,
+                                    cuopt_float_t* y_out,
+                                    void* stream);
+/* VJP of the last forward call: gx (n x batch), gy (m x batch or NULL) ->
+ * gc (n x batch), glc and guc (m x batch), accumulated in FP64. */
+cuopt_int_t cuOptAmdUnrolledBackward(cuOptAmdUnrolled handle,
+                                     const cuopt_float_t* gx,
+                                     const cuopt_float_t* gy,
+                                     cuopt_float_t* gc,
+                                     cuopt_float_t* glc,
+                                     cuopt_float_t* guc,
+                                     void* stream);
+
 /* Solves num_problems independent LPs together (the heterogeneous batch of the
  * reference's Python BatchSolve, cython_solve.cu:286-352) in one persistent
  * launch: one workgroup per LP runs setup and the whole Stable3 loop on the

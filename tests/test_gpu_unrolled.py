@@ -1,0 +1,112 @@
+"""GPU: differentiable unrolled PDHG (cuOptAmdUnrolled* through cuopt_amd.autograd)
+against the CPU restatement oracle/unrolled.py, central finite differences of the
+FP64 GPU map (h = 1e-6) and the envelope identities at convergence.
+
+Tolerances: FP64 forward / VJP vs oracle 1e-9 relative (summation order is the
+same, sequential per row); FP32 forward / VJP vs the FP64 oracle 2e-3 relative
+(FP32 iterates over K steps); FD vs FP64 VJP 1e-6 relative."""
+
+import numpy as np
+import pytest
+
+torch = pytest.importorskip("torch")
+
+from cuopt_amd import generators as G  # noqa: E402
+from cuopt_amd.autograd import UnrolledPDHG  # noqa: E402
+from oracle import unrolled as U  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+DEV = "cuda"
+
+
+def problem(seed=0, m=200, n=500):
+    lp = G.random_lp(m=m, n=n, seed=seed)
+    return lp, lp.to_scipy()
+
+
+def relerr(a, b):
+    return float(np.max(np.abs(a - b)) / (1.0 + np.max(np.abs(b))))
+
+
+@pytest.mark.parametrize("precision,tol", [("fp64", 1e-9), ("fp32_mixed", 2e-3)])
+def test_forward_and_vjp_match_oracle(precision, tol):
+    lp, A = problem(0)
+    B, K, every, lam = 8, 100, 7, 1e-3
+    op = UnrolledPDHG(lp, ridge=lam, steps=K, checkpoint_every=every, precision=precision)
+    eta = op.step_size
+    assert abs(eta - U.step_sizes(A)) <= 1e-6 * eta
+    rng = np.random.default_rng(0)
+    c = lp.c[None, :] + 0.1 * rng.normal(size=(B, lp.n))
+    b = lp.con_lb.copy()
+    ct = torch.tensor(c, device=DEV, requires_grad=True)
+    bt = torch.tensor(b, device=DEV, requires_grad=True)
+    x, y = op(ct, bt)
+    xo, yo, xs, ys = U.forward(A, c.T.copy(), b, b, lp.var_lb, lp.var_ub, lam, eta, K, keep=True)
+    assert relerr(x.detach().cpu().numpy().T, xo) < tol
+    assert relerr(y.detach().cpu().numpy().T, yo) < tol
+    target = rng.uniform(0, 1, size=(B, lp.n))
+    loss = ((x - torch.tensor(target, device=DEV)) ** 2).sum() + (y ** 2).sum()
+    loss.backward()
+    gc, glc, guc = U.backward(A, b, b, lp.var_lb, lp.var_ub, lam, eta, xs, ys, 2 * (xo - target.T), 2 * yo)
+    assert relerr(ct.grad.cpu().numpy().T, gc) < tol
+    assert relerr(bt.grad.cpu().numpy(), (glc + guc).sum(axis=1)) < tol
+
+
+def test_ranged_rows_vjp_matches_oracle():
+    lp, A = problem(1, m=120, n=300)
+    B, K, every, lam = 4, 60, 9, 1e-3
+    rng = np.random.default_rng(1)
+    lc = np.tile(lp.con_lb - rng.uniform(0, 0.5, size=lp.m), (B, 1))
+    uc = np.tile(lp.con_ub + rng.uniform(0, 0.5, size=lp.m), (B, 1))
+    op = UnrolledPDHG(lp, ridge=lam, steps=K, checkpoint_every=every, precision="fp64")
+    eta = op.step_size
+    c = lp.c[None, :] + 0.1 * rng.normal(size=(B, lp.n))
+    ct = torch.tensor(c, device=DEV, requires_grad=True)
+    lt = torch.tensor(lc, device=DEV, requires_grad=True)
+    ut = torch.tensor(uc, device=DEV, requires_grad=True)
+    x, _ = op(ct, lt, ut)
+    gx = rng.normal(size=(B, lp.n))
+    (x * torch.tensor(gx, device=DEV)).sum().backward()
+    xo, yo, xs, ys = U.forward(A, c.T.copy(), lc.T.copy(), uc.T.copy(), lp.var_lb, lp.var_ub, lam, eta, K, keep=True)
+    gc, glc, guc = U.backward(A, lc.T.copy(), uc.T.copy(), lp.var_lb, lp.var_ub, lam, eta, xs, ys, gx.T.copy())
+    assert relerr(ct.grad.cpu().numpy().T, gc) < 1e-9
+    assert relerr(lt.grad.cpu().numpy().T, glc) < 1e-9
+    assert relerr(ut.grad.cpu().numpy().T, guc) < 1e-9
+
+
+def test_vjp_matches_central_differences_fp64():
+    lp, A = problem(2, m=300, n=750)
+    B, K, lam, h = 2, 200, 1e-3, 1e-6
+    op = UnrolledPDHG(lp, ridge=lam, steps=K, checkpoint_every=14, precision="fp64")
+    rng = np.random.default_rng(2)
+    c = torch.tensor(lp.c[None, :] + 0.1 * rng.normal(size=(B, lp.n)), device=DEV, requires_grad=True)
+    b = torch.tensor(lp.con_lb, device=DEV, requires_grad=True)
+    target = torch.tensor(rng.uniform(0, 1, size=(B, lp.n)), device=DEV)
+
+    def loss(cc, bb):
+        x, _ = op(cc, bb)
+        return ((x - target) ** 2).sum()
+
+    loss(c, b).backward()
+    gc, gb = c.grad.clone(), b.grad.clone()
+    with torch.no_grad():
+        for _ in range(3):
+            vc = torch.tensor(rng.normal(size=(B, lp.n)), device=DEV)
+            fd = (loss(c + h * vc, b) - loss(c - h * vc, b)).item() / (2 * h)
+            assert abs(fd - (gc * vc).sum().item()) <= 1e-6 * (1 + abs(fd))
+            vb = torch.tensor(rng.normal(size=lp.m), device=DEV)
+            fd = (loss(c, b + h * vb) - loss(c, b - h * vb)).item() / (2 * h)
+            assert abs(fd - (gb * vb).sum().item()) <= 1e-6 * (1 + abs(fd))
+
+
+def test_envelope_gradients_at_convergence():
+    lp, A = problem(3, m=100, n=250)
+    lam, K = 1e-2, 20000
+    op = UnrolledPDHG(lp, ridge=lam, steps=K, checkpoint_every=141, precision="fp64")
+    c = torch.tensor(lp.c[None, :], device=DEV, requires_grad=True)
+    b = torch.tensor(lp.con_lb, device=DEV, requires_grad=True)
+    x, y = op(c, b)
+    obj = (c * x).sum() + 0.5 * lam * (x * x).sum()
+    obj.backward()
+    assert torch.allclose(c.grad, x.detach(), atol=1e-8)  # d obj / d c = x*
+    assert torch.allclose(b.grad, y.detach()[0], atol=1e-8 * (1 + y.abs().max().item()))  # d obj / d b = y*
