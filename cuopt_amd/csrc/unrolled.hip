@@ -472,7 +472,7 @@ UnrolledHandle* unrolled_create(int m, int n, const int* rowptr, const int* coli
     hipLaunchKernelGGL(unr_sumsq, dim3(1), dim3(kUB), 0, s, m, z.get(), nrm.get());
     nrm.download(&norm2, 1, s);
     CUOPT_HIP_CHECK(hipStreamSynchronize(s));
-    for (int it = 0; it < 2000 && m > 0; ++it) {
+    for (int it = 0; it < 2000 && m > 0 && n > 0; ++it) {
       const double inv = 1.0 / std::sqrt(norm2);
       hipLaunchKernelGGL(unr_spmv64, dim3((n + kUB - 1) / kUB), dim3(kUB), 0, s, U->tptr.get(), U->tidx.get(),
                          U->tval64.get(), n, z.get(), inv, w.get());
