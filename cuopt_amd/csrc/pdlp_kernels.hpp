@@ -82,6 +82,9 @@ struct PrimalStepEpi {
   V* aty_out;
   const StepScalars* sc;
   int j;
+  // Refinement rounds: store only the part (x-hat - x) / tau of the slack; the
+  // termination check adds the FP64 reduced cost of the corrected point.
+  bool small_slack;
   __device__ bool skip() const { return false; }
   struct Pre {
     V x, c, x0;
@@ -99,7 +102,7 @@ struct PrimalStepEpi {
     xbar[r]         = xb;
     if constexpr (MAJOR) {
       xhat[r]  = clamped;
-      slack[r] = (clamped - next) / tau;
+      slack[r] = small_slack ? (clamped - p.x) / tau : (clamped - next) / tau;
     }
     if constexpr (FPE) aty_out[r] = aty;
     xnext[r] = w * xb + (V(1) - w) * p.x0;
@@ -138,10 +141,13 @@ struct DualStepEpi {
     const V sigma = (V)sc->sigma;
     const V w     = halpern_weight<V>(sc, j);
     V nd;
-    if (kink) {  // the projection of the original problem, moved to the kink
-      const V t    = (p.y - p.k) / sigma - ax;
-      const V proj = mx(-p.b.y, mn(t, -p.b.x));
-      nd           = (t - proj) * sigma + p.k;
+    if (kink) {
+      // The projection of the original problem moved to the kink k, written as
+      // clamp(k, s + sigma l, s + sigma u) with s = y - sigma A x-bar: equal to
+      // k + sigma (t - clamp(t, -u, -l)) for t = (y - k) / sigma - A x-bar, but
+      // without forming y - k, whose rounding would undo the refinement.
+      const V s = p.y - sigma * ax;
+      nd        = mx(mn(p.k, s + sigma * p.b.y), s + sigma * p.b.x);
     } else {
       const V t    = p.y / sigma - ax;
       const V proj = mx(-p.b.y, mn(t, -p.b.x));
@@ -263,25 +269,36 @@ struct KktColsEpi {
   double* su_out;
   const double* xstar;  // refinement rounds, as in KktRowsEpi (else null)
   double inv_dp, inv_dd;
+  // Refinement rounds: scaled bounds of the correction problem.  A variable
+  // whose x-hat sits on one of them has the slack "FP64 reduced cost of the
+  // corrected point + stored small part"; any other variable has zero slack.
+  const typename Vec2<V>::type* bnd;
   __device__ bool skip() const { return false; }
   struct Pre {
     double dc, c;
     V xhat, slack, x0;
+    typename Vec2<V>::type b;
   };
   __device__ double product(double v, int col, int) const { return v * yu[col]; }
-  __device__ Pre load(int r) const { return Pre{Dc[r], c_orig[r], xhat[r], slack[r], x0[r]}; }
+  __device__ Pre load(int r) const
+  {
+    typename Vec2<V>::type bb{};
+    if (xstar) bb = bnd[r];
+    return Pre{Dc[r], c_orig[r], xhat[r], slack[r], x0[r], bb};
+  }
   __device__ void row(int r, double aty, const Pre& p, double (&red)[kMaxRed]) const
   {
     const double dc = p.dc;
     double xu       = ((double)p.xhat * dc) * inv_beta;
     double su       = dc != 0.0 ? (double)p.slack / dc : (double)p.slack;
     su              = su * inv_gamma;
-    if (xstar) {
-      xu = xstar[r] + xu * inv_dp;
-      su = su * inv_dd;
-    }
     const double cj = p.c;
     const double g  = cj - aty;
+    if (xstar) {
+      xu = xstar[r] + xu * inv_dp;
+      const bool active = p.xhat == p.b.x || p.xhat == p.b.y;
+      su                = active ? g + su * inv_dd : 0.0;
+    }
     const double rd = g - su;
     red[0] += rd * rd;
     red[1] += xu * cj;

@@ -13,6 +13,7 @@
 // Iterations between two termination checks are issued back to back on one
 // stream with no host synchronisation; the host reads one small record per check.
 #include <algorithm>
+#include <cstdio>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -949,7 +950,7 @@ void PdlpSolver<V>::evaluate_kkt()
   launch_reduce(red_.get(), A_.plan.red_blocks(), 3, scal_.get() + S_KKT_RP2, stream_);
   launch_tiled(AT_, at_orig_.get(), red_.get(),
                KktColsEpi<V>{yu_.get(), xhat_.get(), slack_.get(), Dc_.get(), ib, beta_, ig, gamma_, c_orig_.get(),
-                             x0_.get(), xu_.get(), su_.get(), xs, idp, idd},
+                             x0_.get(), xu_.get(), su_.get(), xs, idp, idd, bnd_w_.get()},
                stream_);
   launch_reduce(red_.get(), AT_.plan.red_blocks(), 4, scal_.get() + S_KKT_RD2, stream_);
   read_scalars();
@@ -978,7 +979,7 @@ void PdlpSolver<V>::launch_step(int j, bool major, bool fpe, double* ny_slot, do
   launch_tiled(AT_, at_w_.get(), nullptr,                                                                    \
                PrimalStepEpi<V, MAJ, FP>{y_[c].get(), x_[c].get(), c_w_.get(), bnd_w_.get(), x0_.get(),     \
                                          xbar_.get(), x_[nx].get(), xhat_.get(), slack_.get(), aty_.get(),  \
-                                         step_sc_.get(), j},                                                \
+                                         step_sc_.get(), j, refine_active_},                                \
                stream_)
 #define CUOPT_DUAL(MAJ, FP)                                                                                  \
   launch_tiled(A_, a_w_.get(), red_.get(),                                                                    \
@@ -1122,15 +1123,30 @@ SolveResult PdlpSolver<V>::run()
   for (double* t : {&loose.ap, &loose.rp, &loose.ad, &loose.rd, &loose.ag, &loose.rg}) *t = std::max(*t, kRefineFrom);
   int status = pdhg_loop(R, RoundSpec{loose});
   const long long first_iters = k_done_;
+  int failed = 0;  // consecutive rounds that did not reduce the error
   while (status == CUOPT_TERIMINATION_STATUS_OPTIMAL || status == kRoundDone) {
     if (is_optimal(user)) return finish_with(CUOPT_TERIMINATION_STATUS_OPTIMAL);
+    const double e_start = rel_kkt_error();
+    const double saved[6] = {l2p_, l2d_, pobj_, dobj_, gap_, absobj_};
     begin_refinement_round();
     ++rounds;
     RoundSpec spec{user};
     spec.refine = true;
-    spec.target = kRefineGain * rel_kkt_error();
-    spec.cap    = std::max<long long>(2 * first_iters, 2000);
+    spec.target = kRefineGain * e_start;
+    spec.cap    = std::max<long long>(2 * first_iters, 2000) << std::min(failed, 4);
     status      = pdhg_loop(R, spec);
+    if (status == kRoundDone && !(rel_kkt_error() < e_start)) {
+      // No progress: go back to the round's starting point (kept in xstar_ /
+      // ystar_) and try again with a longer round.
+      CUOPT_HIP_CHECK(hipMemcpyAsync(xu_.get(), xstar_.get(), (size_t)n_ * sizeof(double), hipMemcpyDeviceToDevice,
+                                     stream_));
+      CUOPT_HIP_CHECK(hipMemcpyAsync(yu_.get(), ystar_.get(), (size_t)m_ * sizeof(double), hipMemcpyDeviceToDevice,
+                                     stream_));
+      l2p_ = saved[0], l2d_ = saved[1], pobj_ = saved[2], dobj_ = saved[3], gap_ = saved[4], absobj_ = saved[5];
+      ++failed;
+    } else {
+      failed = 0;
+    }
   }
   return finish_with(status);
 }
@@ -1163,6 +1179,12 @@ void PdlpSolver<V>::begin_refinement_round()
     ax_.resize(m_);
     aty64_.resize(n_);
     kink_.resize(m_);
+  }
+  if (st_.log_to_console) {
+    std::printf("Refinement round: rel KKT %.3e (primal %.3e dual %.3e gap %.3e) -> Dp %.3e Dd %.3e omega %.3e\n",
+                rel_kkt_error(), l2p_ / (1.0 + b_norm_), l2d_ / (1.0 + c_norm_), gap_ / (1.0 + absobj_), dp, dd,
+                w_prev * ratio);
+    std::fflush(stdout);
   }
   refine_.dp = dp;
   refine_.dd = dd;

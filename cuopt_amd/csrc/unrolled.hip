@@ -91,7 +91,7 @@ __global__ __launch_bounds__(kUB) void unr_dual(const int* __restrict__ aptr, co
   if (!row_lane(m, B, i, b)) return;
   const V ax     = row_dot(aptr, aidx, aval, xb, i, b, B).value();
   const size_t o = (size_t)i * B + b;
-  const size_t q = (size_t)i * ldb + (ldb ? b : 0);
+  const size_t q = ldb ? (size_t)i * ldb + b : (size_t)i;  // per-batch bounds, or one per row
   const V t      = y[o] / sigma - ax;
   const V proj   = mx(-uc[q], mn(t, -lc[q]));
   yn[o]          = (t - proj) * sigma;
@@ -130,7 +130,7 @@ __global__ __launch_bounds__(kUB) void unr_adj_start(int m, int B, const V* ay, 
   const size_t tot = (size_t)m * B;
   for (size_t o = blockIdx.x * (size_t)kUB + threadIdx.x; o < tot; o += (size_t)gridDim.x * kUB) {
     const int i = (int)(o / B), b = (int)(o % B);
-    const size_t q = (size_t)i * ldb + (ldb ? b : 0);
+    const size_t q = ldb ? (size_t)i * ldb + b : (size_t)i;  // per-batch bounds, or one per row
     double dl, du;
     gt[o] = dual_adjoint(ay ? ay[o] : V(0), yplus[o], lc[q], uc[q], sigma, dl, du);
     glc[o] += dl;
@@ -176,7 +176,7 @@ __global__ __launch_bounds__(kUB) void unr_adj_dual(const int* __restrict__ aptr
   const V agw    = row_dot(aptr, aidx, aval, gw, i, b, B).value();
   const size_t o = (size_t)i * B + b;
   const V ay     = gt[o] / sigma + tau * agw;  // adjoint of y before this step
-  const size_t q = (size_t)i * ldb + (ldb ? b : 0);
+  const size_t q = ldb ? (size_t)i * ldb + b : (size_t)i;  // per-batch bounds, or one per row
   double dl, du;
   gt[o] = dual_adjoint(ay, yprev[o], lc[q], uc[q], sigma, dl, du);
   glc[o] += dl;
