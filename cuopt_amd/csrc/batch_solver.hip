@@ -162,7 +162,7 @@ __device__ inline void rows_pass(const int* __restrict__ ptr,
     Acc a = Acc::zero();
     for (int k = b + sub; k < e; k += G) a.add(epi.product(vals[k], idx[k], r));
     for (int off = G >> 1; off > 0; off >>= 1) a.merge(a.shfl(off));
-    if (sub == 0) epi.row(r, a.value(), pre, red);
+    if (sub == 0) epi.row(r, a, pre, red);
   }
 }
 
@@ -183,7 +183,7 @@ struct BScaleEpi {
     return fabs((v * dr) * dc);
   }
   __device__ Pre load(int) const { return Pre{}; }
-  __device__ void row(int r, double s, const Pre&, double (&)[kMaxRed]) const { out[r] = s; }
+  __device__ void row(int r, const Acc& s, const Pre&, double (&)[kMaxRed]) const { out[r] = s.value(); }
 };
 
 // Power iteration on the FP64 scaled matrix, values formed on the fly as in
@@ -199,7 +199,7 @@ struct BPowerAtEpi {
   double* atq;
   __device__ double product(double v, int col, int r) const { return ((v * Dc[r]) * Dr[col]) * (z[col] / nrm); }
   __device__ Pre load(int) const { return Pre{}; }
-  __device__ void row(int r, double s, const Pre&, double (&)[kMaxRed]) const { atq[r] = s; }
+  __device__ void row(int r, const Acc& s, const Pre&, double (&)[kMaxRed]) const { atq[r] = s.value(); }
 };
 struct BPowerAEpi {
   using Mat = double;
@@ -215,9 +215,10 @@ struct BPowerAEpi {
   double* znew;
   __device__ double product(double v, int col, int r) const { return ((v * Dr[r]) * Dc[col]) * atq[col]; }
   __device__ Pre load(int r) const { return Pre{zold[r]}; }
-  __device__ void row(int r, double s, const Pre& p, double (&red)[kMaxRed]) const
+  __device__ void row(int r, const Acc& sum, const Pre& p, double (&red)[kMaxRed]) const
   {
-    znew[r] = s;
+    const double s = sum.value();
+    znew[r]        = s;
     red[0] += (p.zold / nrm) * s;
   }
 };
@@ -514,7 +515,7 @@ __global__ __launch_bounds__(kBT) void batch_pdlp_kernel(BatchArgs<V> A)
             const double s   = err_sum * kISmoothB + err;
             err_sum          = s;
             const double de  = err - last_err;
-            const double w   = exp(kKpB * err + kKiB * s + kKdB * de) * cw;
+            const double w   = bound_primal_weight<V>(exp(kKpB * err + kKiB * s + kKdB * de) * cw);
             omega            = w;
             last_err         = err;
             tau              = eta / w;

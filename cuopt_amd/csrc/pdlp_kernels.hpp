@@ -9,6 +9,8 @@
 // operand.
 #pragma once
 
+#include <type_traits>
+
 #include "spmv_tiles.hpp"
 
 namespace cuopt_amd {
@@ -56,6 +58,21 @@ struct StepScalars {
   long long pad;
 };
 
+// Primal weight range of FP32-mixed solves.  In the normalised units of the
+// scaled problem (bound and objective rescaling put ||b||, ||c|| near one and
+// start at omega = 1) a weight outside [1e-3, 1e3] makes one of the step sizes
+// so small that its FP32 updates fall below the rounding of the iterate, which
+// then freezes while the PID keeps moving the weight away.  PDHG converges for
+// any fixed weight, so the bound only limits the adaptive heuristic.  FP64
+// solves use the reference's unbounded weight.
+constexpr double kOmegaMin32 = 1e-3, kOmegaMax32 = 1e3;
+template <typename V>
+__host__ __device__ inline double bound_primal_weight(double w)
+{
+  if constexpr (std::is_same<V, float>::value) return w < kOmegaMin32 ? kOmegaMin32 : (w > kOmegaMax32 ? kOmegaMax32 : w);
+  return w;
+}
+
 template <typename V>
 __device__ inline V halpern_weight(const StepScalars* sc, int j)
 {
@@ -82,7 +99,7 @@ struct PrimalStepEpi {
   V* aty_out;
   const StepScalars* sc;
   int j;
-  // Refinement rounds: store only the part (x-hat - x) / tau of the slack; the
+  // Recentred (refined) solves: store only the part (x-hat - x) / tau of the slack; the
   // termination check adds the FP64 reduced cost of the corrected point.
   bool small_slack;
   __device__ bool skip() const { return false; }
@@ -92,11 +109,11 @@ struct PrimalStepEpi {
   };
   __device__ typename Acc::Elem product(V v, int col, int) const { return Acc::prod(v, y[col]); }
   __device__ Pre load(int r) const { return Pre{x[r], c[r], x0[r], bnd[r]}; }
-  __device__ void row(int r, V aty, const Pre& p, double (&)[kMaxRed]) const
+  __device__ void row(int r, const Acc& aty, const Pre& p, double (&)[kMaxRed]) const
   {
     const V tau     = (V)sc->tau;
     const V w       = halpern_weight<V>(sc, j);
-    const V next    = p.x - tau * (p.c - aty);
+    const V next    = p.x - tau * aty.rsub(p.c);
     const V clamped = mx(mn(next, p.b.y), p.b.x);
     const V xb      = V(2) * clamped - p.x;
     xbar[r]         = xb;
@@ -104,7 +121,7 @@ struct PrimalStepEpi {
       xhat[r]  = clamped;
       slack[r] = small_slack ? (clamped - p.x) / tau : (clamped - next) / tau;
     }
-    if constexpr (FPE) aty_out[r] = aty;
+    if constexpr (FPE) aty_out[r] = aty.value();
     xnext[r] = w * xb + (V(1) - w) * p.x0;
   }
 };
@@ -125,7 +142,7 @@ struct DualStepEpi {
   V* ybar;
   const StepScalars* sc;
   int j;
-  // Refinement rounds only (else null): per-row position of the kink of the
+  // Recentred (refined) solves only (else null): per-row position of the kink of the
   // dual objective term, which sits at y = 0 in the original problem and at
   // -Dd * y* / Dr in the coordinates of a correction problem.
   const V* kink;
@@ -136,7 +153,7 @@ struct DualStepEpi {
   };
   __device__ typename Acc::Elem product(V v, int col, int) const { return Acc::prod(v, xbar[col]); }
   __device__ Pre load(int r) const { return Pre{y[r], y0[r], kink ? kink[r] : V(0), cb[r]}; }
-  __device__ void row(int r, V ax, const Pre& p, double (&red)[kMaxRed]) const
+  __device__ void row(int r, const Acc& ax, const Pre& p, double (&red)[kMaxRed]) const
   {
     const V sigma = (V)sc->sigma;
     const V w     = halpern_weight<V>(sc, j);
@@ -146,10 +163,10 @@ struct DualStepEpi {
       // clamp(k, s + sigma l, s + sigma u) with s = y - sigma A x-bar: equal to
       // k + sigma (t - clamp(t, -u, -l)) for t = (y - k) / sigma - A x-bar, but
       // without forming y - k, whose rounding would undo the refinement.
-      const V s = p.y - sigma * ax;
+      const V s = sigma * ax.rsub(p.y / sigma);
       nd        = mx(mn(p.k, s + sigma * p.b.y), s + sigma * p.b.x);
     } else {
-      const V t    = p.y / sigma - ax;
+      const V t    = ax.rsub(p.y / sigma);
       const V proj = mx(-p.b.y, mn(t, -p.b.x));
       nd           = (t - proj) * sigma;
     }
@@ -181,10 +198,10 @@ struct FpeEpi {
   };
   __device__ typename Acc::Elem product(V v, int col, int) const { return Acc::prod(v, ybar[col]); }
   __device__ Pre load(int r) const { return Pre{aty[r], xbar[r], x[r]}; }
-  __device__ void row(int, V atyb, const Pre& p, double (&red)[kMaxRed]) const
+  __device__ void row(int, const Acc& atyb, const Pre& p, double (&red)[kMaxRed]) const
   {
     const double dx = (double)p.xbar - (double)p.x;
-    const double t  = (double)atyb - (double)p.aty;
+    const double t  = (double)atyb.value() - (double)p.aty;
     red[0] += t * dx;
     red[1] += dx * dx;
   }
@@ -207,7 +224,7 @@ struct KktRowsEpi {
   const double2* cb_orig;  // unscaled constraint bounds
   const V* y0;
   double* yu_out;
-  // Refinement rounds (else null): the iterates are corrections around the
+  // Recentred (refined) solves (else null): the iterates are corrections around the
   // FP64 point (xstar, ystar) in units 1/inv_dp, 1/inv_dd; the residuals are
   // those of the original problem at the corrected point, and the iterates are
   // not rewritten.
@@ -226,8 +243,9 @@ struct KktRowsEpi {
     V yhat, y0;
   };
   __device__ Pre load(int r) const { return Pre{Dr[r], cb_orig[r], yhat[r], y0[r]}; }
-  __device__ void row(int r, double ax, const Pre& p, double (&red)[kMaxRed]) const
+  __device__ void row(int r, const Acc& axs, const Pre& p, double (&red)[kMaxRed]) const
   {
+    const double ax = axs.value();
     const double dr = p.dr;
     double yu       = ((double)p.yhat * dr) * inv_gamma;
     if (ystar) yu = ystar[r] + yu * inv_dd;
@@ -267,12 +285,14 @@ struct KktColsEpi {
   const V* x0;
   double* xu_out;
   double* su_out;
-  const double* xstar;  // refinement rounds, as in KktRowsEpi (else null)
+  const double* xstar;  // recentred solves, as in KktRowsEpi (else null)
   double inv_dp, inv_dd;
-  // Refinement rounds: scaled bounds of the correction problem.  A variable
+  // Recentred solves: scaled bounds of the correction problem.  A variable
   // whose x-hat sits on one of them has the slack "FP64 reduced cost of the
   // corrected point + stored small part"; any other variable has zero slack.
   const typename Vec2<V>::type* bnd;
+  const double* lb_orig;  // recentred solves: a variable on its correction bound
+  const double* ub_orig;  // is put exactly on the original bound
   __device__ bool skip() const { return false; }
   struct Pre {
     double dc, c;
@@ -286,8 +306,9 @@ struct KktColsEpi {
     if (xstar) bb = bnd[r];
     return Pre{Dc[r], c_orig[r], xhat[r], slack[r], x0[r], bb};
   }
-  __device__ void row(int r, double aty, const Pre& p, double (&red)[kMaxRed]) const
+  __device__ void row(int r, const Acc& atys, const Pre& p, double (&red)[kMaxRed]) const
   {
+    const double aty = atys.value();
     const double dc = p.dc;
     double xu       = ((double)p.xhat * dc) * inv_beta;
     double su       = dc != 0.0 ? (double)p.slack / dc : (double)p.slack;
@@ -295,9 +316,9 @@ struct KktColsEpi {
     const double cj = p.c;
     const double g  = cj - aty;
     if (xstar) {
-      xu = xstar[r] + xu * inv_dp;
-      const bool active = p.xhat == p.b.x || p.xhat == p.b.y;
-      su                = active ? g + su * inv_dd : 0.0;
+      const bool lo = p.xhat == p.b.x, hi = p.xhat == p.b.y;
+      xu            = lo ? lb_orig[r] : (hi ? ub_orig[r] : xstar[r] + xu * inv_dp);
+      su            = (lo || hi) ? g + su * inv_dd : 0.0;
     }
     const double rd = g - su;
     red[0] += rd * rd;
@@ -343,7 +364,7 @@ struct PowerAtEpi {  // atq = A^T (z / ||z||)
   struct Pre {};
   __device__ double product(double v, int col, int) const { return v * (z[col] / st->nrm); }
   __device__ Pre load(int) const { return Pre{}; }
-  __device__ void row(int r, double s, const Pre&, double (&)[kMaxRed]) const { atq[r] = s; }
+  __device__ void row(int r, const Acc& s, const Pre&, double (&)[kMaxRed]) const { atq[r] = s.value(); }
 };
 
 struct PowerAEpi {  // znew = A atq, red[0] = q . znew
@@ -360,8 +381,9 @@ struct PowerAEpi {  // znew = A atq, red[0] = q . znew
   };
   __device__ double product(double v, int col, int) const { return v * atq[col]; }
   __device__ Pre load(int r) const { return Pre{zold[r]}; }
-  __device__ void row(int r, double s, const Pre& p, double (&red)[kMaxRed]) const
+  __device__ void row(int r, const Acc& sum, const Pre& p, double (&red)[kMaxRed]) const
   {
+    const double s = sum.value();
     znew[r]        = s;
     const double q = p.zold / st->nrm;
     red[0] += q * s;
@@ -393,7 +415,7 @@ struct ScaleStatEpi {
   }
   struct Pre {};
   __device__ Pre load(int) const { return Pre{}; }
-  __device__ void row(int r, double s, const Pre&, double (&)[kMaxRed]) const { out[r] = s; }
+  __device__ void row(int r, const Acc& s, const Pre&, double (&)[kMaxRed]) const { out[r] = s.value(); }
 };
 
 // ---- plain y = A x (FP64 or compensated FP32) for the test entry point.
@@ -408,7 +430,7 @@ struct PlainEpi {
   struct Pre {};
   __device__ typename Acc::Elem product(V v, int col, int) const { return Acc::prod(v, x[col]); }
   __device__ Pre load(int) const { return Pre{}; }
-  __device__ void row(int r, V s, const Pre&, double (&)[kMaxRed]) const { y[r] = (double)s; }
+  __device__ void row(int r, const Acc& s, const Pre&, double (&)[kMaxRed]) const { y[r] = (double)s.value(); }
 };
 
 }  // namespace cuopt_amd

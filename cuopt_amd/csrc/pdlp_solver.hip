@@ -362,7 +362,7 @@ struct RestartCopyF {  // anchor = current = hat (cupdlpx_restart, pdlp_restart_
   }
 };
 
-// ---- refinement rounds (FP32-mixed solves refined in FP64) -----------------
+// ---- recentring (FP32-mixed solves refined in FP64) -----------------------
 
 // x* = the last FP64 primal point, clamped into the variable bounds so that the
 // zero correction is feasible for the shifted bounds.
@@ -537,22 +537,12 @@ constexpr double kSufficient      = 0.2;    // sufficient_reduction_for_restart
 constexpr double kNecessary       = 0.8;    // necessary_reduction_for_restart
 constexpr double kKp = 0.99, kKi = 0.01, kKd = 0.0, kISmooth = 0.3;  // restart PID (pdlp_hyper_params.cuh:51-54)
 constexpr int kRuizIters          = 10;
-// FP64 refinement of FP32-mixed solves: tolerance of the FP32 solve, and the
-// reduction of the relative KKT error that ends a correction round.
+// FP64 refinement of FP32-mixed solves: requested tolerances below this turn
+// it on, and restarts recentre once the relative KKT error is below it.
 constexpr double kRefineFrom      = 1e-4;
-constexpr double kRefineGain      = 1e-3;
-// pdhg_loop result that ends a correction round without an answer.
-constexpr int kRoundDone          = -1;
 
 struct Tolerances {
   double ap, rp, ad, rd, ag, rg;  // absolute / relative primal, dual, gap
-};
-
-struct RoundSpec {
-  Tolerances tol;                 // optimality test on the original problem
-  bool refine      = false;       // iterates are a correction around (x*, y*)
-  double target    = 0.0;         // correction round: stop at this relative KKT error
-  long long cap    = 0;           // correction round: iteration budget
 };
 constexpr double kPcAlpha         = 1.0;
 
@@ -626,8 +616,8 @@ class PdlpSolver {
   void evaluate_kkt();
   void launch_step(int j, bool major, bool fpe, double* ny_slot, double* fpe_slot);
   void finish(SolveResult& R, int status);
-  int pdhg_loop(SolveResult& R, const RoundSpec& spec);
-  void begin_refinement_round();
+  int pdhg_loop(SolveResult& R, const Tolerances& tol, bool refine);
+  void recentre();
   // Termination test of the last evaluation (termination_strategy.cu:311-352).
   bool is_optimal(const Tolerances& t) const
   {
@@ -670,13 +660,14 @@ class PdlpSolver {
   bool use_graphs_ = true;
   std::map<std::string, hipGraphExec_t> graphs_;  // captured iteration blocks by pattern
 
-  // refinement rounds: the FP64 point (x*, y*), A x*, A^T y*, dual kinks
+  // recentring: the FP64 point (x*, y*), A x*, A^T y*, dual kinks
   DevBuf<double> xstar_, ystar_, ax_, aty64_;
   DevBuf<V> kink_;
   struct {
     double dp = 0.0, dd = 0.0;  // primal / dual scale of the current correction
   } refine_;
   bool refine_active_ = false;
+  int recentrings_    = 0;
   long long k_done_   = 0;  // iterations of all finished loops
   int restarts_ = 0, checks_ = 0;
   double setup_time_ = 0.0;
@@ -950,7 +941,8 @@ void PdlpSolver<V>::evaluate_kkt()
   launch_reduce(red_.get(), A_.plan.red_blocks(), 3, scal_.get() + S_KKT_RP2, stream_);
   launch_tiled(AT_, at_orig_.get(), red_.get(),
                KktColsEpi<V>{yu_.get(), xhat_.get(), slack_.get(), Dc_.get(), ib, beta_, ig, gamma_, c_orig_.get(),
-                             x0_.get(), xu_.get(), su_.get(), xs, idp, idd, bnd_w_.get()},
+                             x0_.get(), xu_.get(), su_.get(), xs, idp, idd, bnd_w_.get(), lb_orig_.get(),
+                             ub_orig_.get()},
                stream_);
   launch_reduce(red_.get(), AT_.plan.red_blocks(), 4, scal_.get() + S_KKT_RD2, stream_);
   read_scalars();
@@ -1106,57 +1098,31 @@ SolveResult PdlpSolver<V>::run()
   const Tolerances user{st_.abs_primal, st_.rel_primal, st_.abs_dual, st_.rel_dual, st_.abs_gap, st_.rel_gap};
   const bool refine = !std::is_same<V, double>::value && st_.refinement != 0 && st_.bench_steps <= 0 &&
                       std::min({user.ap, user.rp, user.ad, user.rd, user.ag, user.rg}) < kRefineFrom;
-  int rounds = 0;
-  auto finish_with = [&](int status) {
-    finish(R, status);
-    R.info[CUOPT_AMD_INFO_STEPS]         = (double)k_done_;
-    R.info[CUOPT_AMD_INFO_RESTARTS]      = restarts_;
-    R.info[CUOPT_AMD_INFO_CHECKS]        = checks_;
-    R.info[CUOPT_AMD_INFO_LOOP_TIME]     = R.solve_time - setup_time_;
-    R.info[CUOPT_AMD_INFO_REFINE_ROUNDS] = rounds;
-    return R;
-  };
-  if (!refine) return finish_with(pdhg_loop(R, RoundSpec{user}));
-
-  // FP32 solve to the looser tolerance, then FP64 refinement rounds.
-  Tolerances loose = user;
-  for (double* t : {&loose.ap, &loose.rp, &loose.ad, &loose.rd, &loose.ag, &loose.rg}) *t = std::max(*t, kRefineFrom);
-  int status = pdhg_loop(R, RoundSpec{loose});
-  const long long first_iters = k_done_;
-  int failed = 0;  // consecutive rounds that did not reduce the error
-  while (status == CUOPT_TERIMINATION_STATUS_OPTIMAL || status == kRoundDone) {
-    if (is_optimal(user)) return finish_with(CUOPT_TERIMINATION_STATUS_OPTIMAL);
-    const double e_start = rel_kkt_error();
-    const double saved[6] = {l2p_, l2d_, pobj_, dobj_, gap_, absobj_};
-    begin_refinement_round();
-    ++rounds;
-    RoundSpec spec{user};
-    spec.refine = true;
-    spec.target = kRefineGain * e_start;
-    spec.cap    = std::max<long long>(2 * first_iters, 2000) << std::min(failed, 4);
-    status      = pdhg_loop(R, spec);
-    if (status == kRoundDone && !(rel_kkt_error() < e_start)) {
-      // No progress: go back to the round's starting point (kept in xstar_ /
-      // ystar_) and try again with a longer round.
-      CUOPT_HIP_CHECK(hipMemcpyAsync(xu_.get(), xstar_.get(), (size_t)n_ * sizeof(double), hipMemcpyDeviceToDevice,
-                                     stream_));
-      CUOPT_HIP_CHECK(hipMemcpyAsync(yu_.get(), ystar_.get(), (size_t)m_ * sizeof(double), hipMemcpyDeviceToDevice,
-                                     stream_));
-      l2p_ = saved[0], l2d_ = saved[1], pobj_ = saved[2], dobj_ = saved[3], gap_ = saved[4], absobj_ = saved[5];
-      ++failed;
-    } else {
-      failed = 0;
-    }
-  }
-  return finish_with(status);
+  finish(R, pdhg_loop(R, user, refine));
+  R.info[CUOPT_AMD_INFO_STEPS]         = (double)k_done_;
+  R.info[CUOPT_AMD_INFO_RESTARTS]      = restarts_;
+  R.info[CUOPT_AMD_INFO_CHECKS]        = checks_;
+  R.info[CUOPT_AMD_INFO_LOOP_TIME]     = R.solve_time - setup_time_;
+  R.info[CUOPT_AMD_INFO_REFINE_ROUNDS] = recentrings_;
+  return R;
 }
 
+// Recentring at a restart (FP64 iterative refinement of FP32-mixed solves).
+// At a restart the PDHG state is the single point (x-hat, y-hat), whose FP64
+// values in the original problem the check has just left in xu_ / yu_.  The
+// iterates are re-expressed as a correction problem around that point,
+//   d = Dp (x - x*) / Dc,   w = Dd (y - y*) / Dr,
+// which is the original LP exactly (objective Dd (c - A^T y*), bounds
+// Dp ([l, u] - x*), constraint bounds Dp ([l_c, u_c] - A x*), and the kink of
+// the dual term at y = 0 moved to w = -Dd y* / Dr), so the FP32 iterates only
+// carry the part of the solution not yet resolved.  Dp and Dd put the primal
+// and dual residuals of the correction at about one.  The point itself becomes
+// d = w = 0: the restart is applied here, and PDHG continues as if restarted
+// in the original coordinates (the PID error terms are scale-invariant; the
+// primal weight is converted to the new units).
 template <typename V>
-void PdlpSolver<V>::begin_refinement_round()
+void PdlpSolver<V>::recentre()
 {
-  // The last check left the FP64 point of the original problem in xu_ / yu_
-  // and its residual norms in l2p_, l2d_, gap_.  Scale factors put the
-  // residuals of the correction problem at about one.
   double dp = 1.0 / std::max(l2p_, 1e-14 * (1.0 + b_norm_));
   double dd = 1.0 / std::max(l2d_, 1e-14 * (1.0 + c_norm_));
   dp        = std::max(dp, 1.0);
@@ -1180,8 +1146,8 @@ void PdlpSolver<V>::begin_refinement_round()
     aty64_.resize(n_);
     kink_.resize(m_);
   }
-  if (st_.log_to_console) {
-    std::printf("Refinement round: rel KKT %.3e (primal %.3e dual %.3e gap %.3e) -> Dp %.3e Dd %.3e omega %.3e\n",
+  if (st_.log_to_console && std::getenv("CUOPT_AMD_LOG_REFINEMENT")) {
+    std::printf("Recentre: rel KKT %.3e (primal %.3e dual %.3e gap %.3e) -> Dp %.3e Dd %.3e omega %.3e\n",
                 rel_kkt_error(), l2p_ / (1.0 + b_norm_), l2d_ / (1.0 + c_norm_), gap_ / (1.0 + absobj_), dp, dd,
                 w_prev * ratio);
     std::fflush(stdout);
@@ -1212,30 +1178,31 @@ void PdlpSolver<V>::begin_refinement_round()
             stream_);
   launch_ew(n_, nullptr, ZeroF<V>{slack_.get()}, stream_);
   cur_   = 0;
-  omega_ = best_omega_ = w_prev * ratio;
+  omega_      = bound_primal_weight<V>(w_prev * ratio);
+  best_omega_ = bound_primal_weight<V>(best_omega_ * ratio);
+  ++recentrings_;
   tau_   = eta_ / omega_;
   sigma_ = eta_ * omega_;
 }
 
 template <typename V>
-int PdlpSolver<V>::pdhg_loop(SolveResult& R, const RoundSpec& spec)
+int PdlpSolver<V>::pdhg_loop(SolveResult& R, const Tolerances& tol, bool refine)
 {
-  const Tolerances& tol = spec.tol;
   const double nan    = std::numeric_limits<double>::quiet_NaN();
   double fpe = nan, fpe0 = nan, last_trial = std::numeric_limits<double>::infinity();
   double err_sum = 0.0, last_err = 0.0, best_gap = std::numeric_limits<double>::infinity();
   long long k        = 0;
   bool restarted_prev = false;   // a restart happened at the previous check
   bool periodic_prev  = false;   // the previous block ran a periodic FPE step
-  const long long k0  = k_done_;  // iterations of earlier rounds
+  const long long k0  = k_done_;  // iterations of earlier loops (none: one loop per solve)
   kr_ = 0;
-  refine_active_ = spec.refine;
+  refine_active_ = false;
 
   // Benchmark window (cuOptAmdBenchmarkSteps): [bench_w, bench_end) is timed on
   // the production path; the next `ev_iters` iterations run with direct
   // launches bracketed by HIP events for per-kernel times.  Window edges are
   // moved to check iterations, where the host synchronises anyway.
-  const bool bench = st_.bench_steps > 0 && !spec.refine;
+  const bool bench = st_.bench_steps > 0;
   auto round_up_check = [&](long long v) {
     while (!is_check(v)) ++v;
     return v;
@@ -1306,7 +1273,6 @@ int PdlpSolver<V>::pdhg_loop(SolveResult& R, const RoundSpec& spec)
       return done(CUOPT_TERIMINATION_STATUS_NUMERICAL_ERROR);
     if (elapsed() >= st_.time_limit) return done(CUOPT_TERIMINATION_STATUS_TIME_LIMIT);
     if (k0 + k >= st_.iteration_limit) return done(CUOPT_TERIMINATION_STATUS_ITERATION_LIMIT);
-    if (spec.refine && k > 1 && (rel_kkt_error() <= spec.target || k >= spec.cap)) return done(kRoundDone);
 
     bool restarted = false;
     if (is_major) {
@@ -1333,7 +1299,7 @@ int PdlpSolver<V>::pdhg_loop(SolveResult& R, const RoundSpec& spec)
           const double s   = err_sum * kISmooth + err;
           err_sum          = s;
           const double de  = err - last_err;
-          const double w   = std::exp(kKp * err + kKi * s + kKd * de) * cur;
+          const double w   = bound_primal_weight<V>(std::exp(kKp * err + kKi * s + kKd * de) * cur);
           omega_           = w;
           last_err         = err;
           tau_             = eta_ / w;
@@ -1350,8 +1316,13 @@ int PdlpSolver<V>::pdhg_loop(SolveResult& R, const RoundSpec& spec)
           best_gap    = g;
           best_omega_ = omega_;
         }
-        launch_ew(n_, nullptr, RestartCopyF<V>{xhat_.get(), x_[cur_].get(), x0_.get()}, stream_);
-        launch_ew(m_, nullptr, RestartCopyF<V>{yhat_.get(), y_[cur_].get(), y0_.get()}, stream_);
+        if (refine && (refine_active_ || rel_kkt_error() <= kRefineFrom)) {
+          recentre();  // the restart point becomes the origin of the new coordinates
+          refine_active_ = true;
+        } else {
+          launch_ew(n_, nullptr, RestartCopyF<V>{xhat_.get(), x_[cur_].get(), x0_.get()}, stream_);
+          launch_ew(m_, nullptr, RestartCopyF<V>{yhat_.get(), y_[cur_].get(), y0_.get()}, stream_);
+        }
         kr_        = 0;
         last_trial = std::numeric_limits<double>::infinity();
         restarted  = true;
@@ -1369,7 +1340,7 @@ int PdlpSolver<V>::pdhg_loop(SolveResult& R, const RoundSpec& spec)
     const bool evented = bench && k >= bench_end;
     std::string key;
     if (use_graphs_ && !evented) {
-      key = std::to_string(next - k) + (restarted ? "R" : "") + (spec.refine ? "K:" : ":");
+      key = std::to_string(next - k) + (restarted ? "R" : "") + (refine_active_ ? "K:" : ":");
       for (long long s = k; s < next; ++s) {
         const long long s2 = s + 2;
         if (((s + 1) % kMajor == 0) || (s2 % conditional_major(s2)) == 0) key += std::to_string(s - k) + ",";

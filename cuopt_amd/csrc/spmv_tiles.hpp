@@ -94,6 +94,7 @@ struct AccSum64 {
   __device__ void merge(const AccSum64& o) { s += o.s; }
   __device__ AccSum64 shfl(int m) const { return AccSum64{shfl_xor_d(s, m)}; }
   __device__ Val value() const { return s; }
+  __device__ Val rsub(Val x) const { return x - s; }  // x - sum
   __device__ uint64_t pack() const { return __double_as_longlong(s); }
   __device__ static AccSum64 unpack(uint64_t u) { return AccSum64{__longlong_as_double(u)}; }
 };
@@ -108,6 +109,7 @@ struct AccMax64 {
   __device__ void merge(const AccMax64& o) { s = (s < o.s) ? o.s : s; }
   __device__ AccMax64 shfl(int m) const { return AccMax64{shfl_xor_d(s, m)}; }
   __device__ Val value() const { return s; }
+  __device__ Val rsub(Val x) const { return x - s; }
   __device__ uint64_t pack() const { return __double_as_longlong(s); }
   __device__ static AccMax64 unpack(uint64_t u) { return AccMax64{__longlong_as_double(u)}; }
 };
@@ -140,6 +142,10 @@ struct AccComp32 {
   }
   __device__ AccComp32 shfl(int m) const { return AccComp32{shfl_xor_f(s, m), shfl_xor_f(c, m)}; }
   __device__ Val value() const { return s + c; }
+  // x - (s + c) without rounding s + c first: near a fixed point x and the sum
+  // cancel, and the rounded sum would leave an error of eps |x| in place of
+  // eps |x - sum|.
+  __device__ Val rsub(Val x) const { return (x - s) - c; }
   __device__ uint64_t pack() const
   {
     return (uint64_t)__float_as_uint(s) | ((uint64_t)__float_as_uint(c) << 32);
@@ -197,7 +203,7 @@ __device__ inline Acc block_merge(Acc a, uint64_t* lds)
 //   __device__ bool skip() const;                       uniform early exit
 //   __device__ typename Acc::Elem product(Mat v, int col, int k) const;
 //   __device__ Pre load(int r) const;                   issued before the products
-//   __device__ void row(int r, typename Acc::Val sum, const Pre& p, double (&red)[kMaxRed]) const;
+//   __device__ void row(int r, const Acc& sum, const Pre& p, double (&red)[kMaxRed]) const;
 template <class Epi, int TILE>
 __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restrict__ rowptr,
                                                             const int* __restrict__ colidx,
@@ -286,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
         Acc a = Acc::zero();
         for (int k = rb[i] + sub; k < re[i]; k += G) a.add(prod[k]);
         for (int off = G >> 1; off > 0; off >>= 1) a.merge(a.shfl(off));
-        if (sub == 0) epi.row(r, a.value(), pre[i], red);
+        if (sub == 0) epi.row(r, a, pre[i], red);
       }
     }
     // Tiles with more rows than kPreRows per lane group: the rest load as they go.
@@ -295,7 +301,7 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
       Acc a = Acc::zero();
       for (int k = b + sub; k < e; k += G) a.add(prod[k]);
       for (int off = G >> 1; off > 0; off >>= 1) a.merge(a.shfl(off));
-      if (sub == 0) epi.row(r, a.value(), epi.load(r), red);
+      if (sub == 0) epi.row(r, a, epi.load(r), red);
     }
   } else {
     // Chunk of a long row: only the workgroup partial is stored here; the
@@ -330,7 +336,7 @@ __global__ __launch_bounds__(kBlock) void long_rows_kernel(PlanDev plan,
   double red[kMaxRed];
 #pragma unroll
   for (int k = 0; k < kMaxRed; ++k) red[k] = 0.0;
-  if (threadIdx.x == 0) epi.row(lr.row, tot.value(), epi.load(lr.row), red);
+  if (threadIdx.x == 0) epi.row(lr.row, tot, epi.load(lr.row), red);
   if constexpr (Epi::NRED > 0)
     block_sum_store<Epi::NRED>(red, red_part + (size_t)(plan.ntiles + blockIdx.x) * Epi::NRED, red_lds);
 }

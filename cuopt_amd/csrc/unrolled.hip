@@ -72,10 +72,10 @@ __global__ __launch_bounds__(kUB) void unr_primal(const int* __restrict__ tptr, 
 {
   int j, b;
   if (!row_lane(n, B, j, b)) return;
-  const V aty   = row_dot(tptr, tidx, tval, y, j, b, B).value();
+  const auto aty = row_dot(tptr, tidx, tval, y, j, b, B);
   const size_t o = (size_t)j * B + b;
   const V xr    = x[o];
-  const V v     = (xr - tau * (c[o] - aty)) * inv_prox;
+  const V v     = (xr - tau * aty.rsub(c[o])) * inv_prox;
   const V xnew  = mx(mn(v, ub[j]), lb[j]);
   xn[o]         = xnew;
   xb[o]         = V(2) * xnew - xr;
@@ -89,10 +89,10 @@ __global__ __launch_bounds__(kUB) void unr_dual(const int* __restrict__ aptr, co
 {
   int i, b;
   if (!row_lane(m, B, i, b)) return;
-  const V ax     = row_dot(aptr, aidx, aval, xb, i, b, B).value();
+  const auto ax  = row_dot(aptr, aidx, aval, xb, i, b, B);
   const size_t o = (size_t)i * B + b;
   const size_t q = ldb ? (size_t)i * ldb + b : (size_t)i;  // per-batch bounds, or one per row
-  const V t      = y[o] / sigma - ax;
+  const V t      = ax.rsub(y[o] / sigma);
   const V proj   = mx(-uc[q], mn(t, -lc[q]));
   yn[o]          = (t - proj) * sigma;
 }
