@@ -67,6 +67,112 @@ def copy_calibration(mib: int) -> None:
     print(f"[bench] copy calibration: fill of {mib} MiB, then copy of {mib} MiB", file=sys.stderr, flush=True)
 
 
+def pmc_traffic(kernel: str):
+    """HBM bytes per launch of `kernel` from the rocprofv3 --pmc summary committed in
+    profiles/pmc_traffic.json (FETCH_SIZE and WRITE_SIZE passes scaled by a plain
+    copy kernel's counters), or None when no measurement is recorded."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        entry = json.load(f).get("kernels", {}).get(kernel)
+    return entry.get("hbm_bytes_per_launch") if entry else None
+
+
+def transport_extras(c_api, G, time_limit: float, cpu_simplex: bool) -> dict:
+    """Transportation LP (20 000 sources x 50 000 sinks, 50 arcs each: n = 1 M, m = 70 k):
+    FP32-mixed with FP64 refinement against plain FP64, to 1e-4 and 1e-8, objective error
+    against the HiGHS optimum committed in tests/golden/highs_optima.json."""
+    with open(os.path.join(ROOT, "tests", "golden", "highs_optima.json")) as f:
+        case = json.load(f)["cases"]["transport_full"]
+    lp = G.transportation_lp(**case["kwargs"])
+    obj = case["objective"]
+    out = {"workload": "transportation LP 20000x50000, 50 arcs/source, seed 0 (n=%d m=%d nnz=%d)" % (lp.n, lp.m, lp.nnz),
+           "oracle_objective": obj}
+    for tol in (1e-4, 1e-8):
+        for p in ("fp32_mixed", "fp64"):
+            r = c_api.solve(lp, tol=tol, precision=p, time_limit=time_limit)
+            out[f"{p}@{tol:g}"] = {"status": c_api.STATUS.get(r.status, r.status), "iterations": r.iterations,
+                                   "time_s": round(r.solve_time, 4), "recentrings": int(r.info["refine_rounds"]),
+                                   "obj_rel_err": abs(r.primal_objective - obj) / (1.0 + abs(obj))}
+            print(f"[bench] transport {p} tol {tol:g}: {out[f'{p}@{tol:g}']}", file=sys.stderr, flush=True)
+    if cpu_simplex:
+        try:
+            import numpy as np
+            from scipy.optimize import linprog
+
+            A = lp.to_scipy()
+            eq = lp.con_lb == lp.con_ub
+            up = np.isfinite(lp.con_ub) & ~eq
+            t = time.time()
+            res = linprog(lp.c, A_ub=A[up], b_ub=lp.con_ub[up], A_eq=A[eq], b_eq=lp.con_lb[eq], bounds=(0, None),
+                          method="highs-ds")
+            out["cpu_simplex"] = {"solver": "HiGHS dual simplex (SciPy), 1 thread", "time_s": round(time.time() - t, 3),
+                                  "status": int(res.status),
+                                  "obj_rel_err": abs(float(res.fun) - obj) / (1.0 + abs(obj))}
+            print(f"[bench] transport cpu simplex: {out['cpu_simplex']}", file=sys.stderr, flush=True)
+        except Exception as e:  # scipy absent on the host
+            out["cpu_simplex"] = {"error": repr(e)}
+    return out
+
+
+def batch_extras(c_api, G, count: int, sample: int, time_limit: float) -> dict:
+    """4 096 random equality LPs, m log-uniform in [50, 2000], n = 2.5 m, 8 nonzeros per
+    column, FP32-mixed to 1e-4: one persistent launch (cuOptAmdBatchSolve) against a
+    sequential loop of single-LP solves over the first `sample` LPs."""
+    t = time.time()
+    lps = G.batch_lps(count=count, seed=0)
+    gen = time.time() - t
+    t = time.time()
+    rs = c_api.batch_solve(lps, tol=1e-4, precision="fp32_mixed", time_limit=time_limit)
+    wall = time.time() - t
+    solved = sum(r.status == 1 for r in rs)
+    errs = [abs(r.primal_objective - lp.obj_opt) / (1 + abs(lp.obj_opt)) for r, lp in zip(rs, lps) if r.status == 1]
+    t = time.time()
+    seq = [c_api.solve(lp, tol=1e-4, precision="fp32_mixed", time_limit=time_limit) for lp in lps[:sample]]
+    seq_wall = time.time() - t
+    out = {"workload": f"{count} LPs m~logU[50,2000] n=2.5m 8 nnz/col (sum nnz {sum(lp.nnz for lp in lps)})",
+           "generation_s": round(gen, 2), "batch_wall_s": round(wall, 4), "batch_solve_time_s": round(rs[0].solve_time, 4),
+           "solved": solved, "lps_per_s": round(solved / wall, 1), "max_obj_rel_err": max(errs) if errs else None,
+           "sequential_sample": sample, "sequential_lps_per_s": round(sum(r.status == 1 for r in seq) / seq_wall, 2),
+           "total_iterations": int(sum(r.iterations for r in rs))}
+    out["speedup_vs_sequential"] = round(out["lps_per_s"] / max(out["sequential_lps_per_s"], 1e-9), 1)
+    print(f"[bench] batch: {out}", file=sys.stderr, flush=True)
+    return out
+
+
+def unrolled_extras(G, steps: int, batch: int) -> dict:
+    """Differentiable unrolled PDHG: m = 20 000, n = 50 000 (8 nonzeros per column),
+    ridge 1e-3, K steps in FP32 for a batch of objectives c + N(0, 0.1), VJP of
+    ||x_K - target||^2 with checkpoints every 45 steps."""
+    import numpy as np
+    import torch
+
+    from cuopt_amd.autograd import UnrolledPDHG
+
+    lp = G.random_lp(m=20_000, n=50_000, seed=0)
+    op = UnrolledPDHG(lp, ridge=1e-3, steps=steps, checkpoint_every=45, precision="fp32_mixed")
+    rng = np.random.default_rng(0)
+    c = torch.tensor(lp.c[None, :] + 0.1 * rng.normal(size=(batch, lp.n)), device="cuda", requires_grad=True)
+    b = torch.tensor(lp.con_lb, device="cuda", requires_grad=True)
+    target = torch.tensor(rng.uniform(0, 1, size=(batch, lp.n)), device="cuda")
+    x, _ = op(c, b)  # warm-up (allocations)
+    torch.cuda.synchronize()
+    t0 = time.time()
+    x, _ = op(c, b)
+    loss = ((x - target) ** 2).sum()
+    torch.cuda.synchronize()
+    t1 = time.time()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.time()
+    out = {"workload": f"m=20000 n=50000 nnz={lp.nnz} ridge 1e-3, K={steps}, batch {batch}, FP32, checkpoint 45",
+           "forward_ms": round((t1 - t0) * 1e3, 2), "backward_ms": round((t2 - t1) * 1e3, 2),
+           "forward_us_per_step": round((t1 - t0) * 1e6 / steps, 2), "eta": op.step_size}
+    print(f"[bench] unrolled: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,6 +186,11 @@ def main() -> int:
     ap.add_argument("--solve-time-limit", type=float, default=300.0)
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="also run one plain device-to-device copy of this many MiB (PMC calibration)")
+    ap.add_argument("--no-extras", action="store_true",
+                    help="skip the transportation, batched and unrolled workloads")
+    ap.add_argument("--no-cpu-simplex", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=4096)
+    ap.add_argument("--batch-sequential-sample", type=int, default=64)
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -139,7 +250,7 @@ def main() -> int:
     achieved = kb[dom] / (dom_ms * 1e-3) / 1e9
     roofline = {
         "bound": "hbm", "kernel": f"{dom}_step ({head})", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": pmc_traffic(f"{dom}_step_{head}"),
         "algorithmic_bytes_per_launch": kb[dom], "mean_launch_us": round(dom_ms * 1e3, 2),
         "iteration_GBps": round(kb["iteration"] / ((w["primal_kernel_ms"] + w["dual_kernel_ms"]) * 1e-3) / 1e9, 1),
     }
@@ -156,6 +267,18 @@ def main() -> int:
                          f"(setup {r['setup_time']:.1f}s untimed), {threads} OpenMP threads"}
         print(f"[bench] cpu baseline: {cpu}", file=sys.stderr, flush=True)
 
+    extras = {}
+    if not args.no_extras:
+        for name, fn in (("transport", lambda: transport_extras(c_api, G, 120.0, not args.no_cpu_simplex)),
+                         ("batch", lambda: batch_extras(c_api, G, args.batch_count, args.batch_sequential_sample,
+                                                        120.0)),
+                         ("unrolled", lambda: unrolled_extras(G, 2000, 64))):
+            try:
+                extras[name] = fn()
+            except Exception as e:  # an extra workload never hides the headline line
+                extras[name] = {"error": repr(e)}
+                print(f"[bench] {name} failed: {e!r}", file=sys.stderr, flush=True)
+
     other = {p: {"iters_per_s": round(1000.0 * b["iterations"] / b["window_ms"], 1),
                  "primal_kernel_us": round(b["primal_kernel_ms"] * 1e3, 2),
                  "dual_kernel_us": round(b["dual_kernel_ms"] * 1e3, 2)} for p, b in windows.items()}
@@ -169,7 +292,7 @@ def main() -> int:
         "config": {"workload": "skewed LP m=500000 n=1000000 nnz=%d, Zipf(1.1) rows, 1+Poisson(4) per column, "
                                "half = / half <=, seed 0" % lp.nnz, "precision": head, "parallelism": "1 GPU"},
         "windows": other, "time_to_tolerance": solves, "fp64_over_fp32_time_ratio": speedup,
-        "roofline": roofline, "cpu_baseline": cpu,
+        "roofline": roofline, "cpu_baseline": cpu, "other_workloads": extras,
     }
     print(json.dumps(out))
     return 0
