@@ -898,8 +898,25 @@ std::vector<SolveResult> batch_solve(const std::vector<const LpHost*>& lps, cons
     }
     active.push_back((int)a);
   }
-  if (st.precision == CUOPT_AMD_PRECISION_FP32_MIXED) return run_batch<float>(lps, active, lcs, ucs, st, std::move(R));
-  return run_batch<double>(lps, active, lcs, ucs, st, std::move(R));
+  // LPs answered on the host above keep their results if the device part fails.
+  std::vector<SolveResult> keep = R;
+  try {
+    if (st.precision == CUOPT_AMD_PRECISION_FP32_MIXED)
+      return run_batch<float>(lps, active, lcs, ucs, st, std::move(R));
+    return run_batch<double>(lps, active, lcs, ucs, st, std::move(R));
+  } catch (const std::exception& e) {
+    const std::string msg = e.what();
+    int code              = CUOPT_RUNTIME_ERROR;
+    if (dynamic_cast<const validation_error*>(&e)) code = CUOPT_VALIDATION_ERROR;
+    if (dynamic_cast<const std::bad_alloc*>(&e) || msg.find("out of memory") != std::string::npos)
+      code = CUOPT_OUT_OF_MEMORY;
+    for (int a : active) {
+      keep[a].error         = code;
+      keep[a].error_message = msg;
+      keep[a].status        = CUOPT_TERIMINATION_STATUS_NO_TERMINATION;
+    }
+    return keep;
+  }
 }
 
 }  // namespace cuopt_amd

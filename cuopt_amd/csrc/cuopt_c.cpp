@@ -565,7 +565,10 @@ cuopt_int_t cuOptAmdBatchSolve(cuopt_int_t num_problems,
   }
   try {
     std::vector<SolveResult> res = batch_solve(lps, conf);
-    for (size_t b = 0; b < res.size(); ++b) sols[where[b]]->res = std::move(res[b]);
+    for (size_t b = 0; b < res.size(); ++b) {
+      if (res[b].error != CUOPT_SUCCESS && first_error == CUOPT_SUCCESS) first_error = res[b].error;
+      sols[where[b]]->res = std::move(res[b]);
+    }
   } catch (const std::exception& e) {
     const std::string msg = e.what();
     int code              = CUOPT_RUNTIME_ERROR;

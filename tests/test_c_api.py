@@ -155,6 +155,41 @@ def test_solve_without_gpu_fails_cleanly():
     assert "HIP" in r.error_string
 
 
+def test_batch_without_gpu_fails_cleanly_and_keeps_host_results():
+    # crossing bounds and constraint-free LPs are answered on the host; the rest
+    # report the HIP error of the missing device, every handle filled and in order
+    if c_api.device_count() > 0:
+        pytest.skip("a GPU is present")
+    good = G.random_lp(m=20, n=50, seed=5)
+    crossing = G.random_lp(m=20, n=50, seed=6)
+    crossing.var_lb = crossing.var_lb.copy()
+    crossing.var_lb[0] = 11.0
+    rs = c_api.batch_solve([good, crossing], iteration_limit=10)
+    assert rs[0].error == c_api.CUOPT_RUNTIME_ERROR and "HIP" in rs[0].error_string
+    assert rs[1].status == 2 and len(rs[1].x) == crossing.n
+
+
+def test_batch_and_unrolled_argument_checks():
+    L = c_api.lib()
+    s = c_api.SolverSettings()
+    assert L.cuOptAmdBatchSolve(-1, None, s.handle, None) == c_api.CUOPT_INVALID_ARGUMENT
+    assert L.cuOptAmdBatchSolve(1, None, s.handle, None) == c_api.CUOPT_INVALID_ARGUMENT
+    assert L.cuOptAmdBatchSolve(0, None, s.handle, None) == c_api.CUOPT_SUCCESS
+    h = ctypes.c_void_p()
+    L.cuOptAmdUnrolledCreate.argtypes = [ctypes.c_int32, ctypes.c_int32] + [ctypes.c_void_p] * 5 + [
+        ctypes.c_double, ctypes.c_int32, ctypes.POINTER(ctypes.c_void_p)]
+    ptr = np.array([0, 1], np.int32)
+    idx = np.array([3], np.int32)  # column out of range for n = 2
+    val, lb, ub = np.ones(1), np.zeros(2), np.ones(2)
+    rc = L.cuOptAmdUnrolledCreate(1, 2, ptr.ctypes.data, idx.ctypes.data, val.ctypes.data, lb.ctypes.data,
+                                  ub.ctypes.data, 1e-3, 1, ctypes.byref(h))
+    assert rc == c_api.CUOPT_INVALID_ARGUMENT and not h.value
+    idx[0] = 1
+    rc = L.cuOptAmdUnrolledCreate(1, 2, ptr.ctypes.data, idx.ctypes.data, val.ctypes.data, lb.ctypes.data,
+                                  ub.ctypes.data, -1.0, 1, ctypes.byref(h))
+    assert rc == c_api.CUOPT_INVALID_ARGUMENT  # negative ridge
+
+
 # ---- MPS files through cuOptReadProblem / cuOptWriteProblem ---------------------
 GOLDEN_MPS = ["afiro_original.mps", "good-max.mps", "max_offset.mps", "lp-model-no-constraints.mps",
               "good-mps-fixed-ranges.mps", "lp_model_with_var_bounds.mps"]
