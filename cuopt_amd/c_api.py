@@ -45,6 +45,13 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise LibraryNotBuilt(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        # PyTorch ships its own HIP runtime under the same soname as the system one;
+        # the first to load serves the whole process.  Loading torch first keeps one
+        # runtime shared by torch tensors and this library (autograd path).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = ctypes.CDLL(LIB_PATH)
         i32, f64, vp = ctypes.c_int32, ctypes.c_double, ctypes.c_void_p
         pi32, pf64, pch = ctypes.POINTER(i32), ctypes.POINTER(f64), ctypes.c_char_p

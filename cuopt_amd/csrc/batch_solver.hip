@@ -52,7 +52,7 @@ struct BatchDesc {
 struct BatchOut {
   double l2p, l2d, pobj, dobj, gap, absobj, b_norm, c_norm;
   double eta, omega, sigma_max_sq, beta, gamma;
-  int status, iters, restarts, checks, power_iters, pad[3];
+  int status, iters, restarts, checks, power_iters, recentrings, pad[2];
 };
 
 template <typename V>
@@ -88,6 +88,10 @@ struct BatchArgs {
   double *pz, *pz2, *patq;
   const double* z0;
   BatchOut* out;
+  // recentring of FP32-mixed solves (FP64 refinement), as in the single-LP solver
+  int refine;
+  double *xstar, *ystar, *ax64, *aty64;
+  V* kink;
   double tol_ap, tol_rp, tol_ad, tol_rd, tol_ag, tol_rg;
   long long iter_limit;
   unsigned long long tick_limit;  // wall-clock ticks from kernel start (0 = none)
@@ -223,6 +227,18 @@ struct BPowerAEpi {
   }
 };
 
+// y = A x (FP64), row index passed to product() as in the other batch passes.
+struct BPlainEpi {
+  using Mat = double;
+  using Acc = AccSum64;
+  struct Pre {};
+  const double* x;
+  double* y;
+  __device__ double product(double v, int col, int) const { return v * x[col]; }
+  __device__ Pre load(int) const { return Pre{}; }
+  __device__ void row(int r, const Acc& s, const Pre&, double (&)[kMaxRed]) const { y[r] = s.value(); }
+};
+
 __device__ inline double bound_sq(double l, double u)
 {
   double t = 0.0;
@@ -268,6 +284,13 @@ __global__ __launch_bounds__(kBT) void batch_pdlp_kernel(BatchArgs<V> A)
   V *xhat = A.xhat + d.col0, *yhat = A.yhat + d.row0, *slack = A.slack + d.col0, *aty = A.aty + d.col0;
   double *xu = A.xu + d.col0, *yu = A.yu + d.row0, *su = A.su + d.col0;
   double *pz = A.pz + d.row0, *pz2 = A.pz2 + d.row0, *patq = A.patq + d.col0;
+  double *xstar = A.xstar + d.col0, *ystar = A.ystar + d.row0, *ax64 = A.ax64 + d.row0, *aty64 = A.aty64 + d.col0;
+  V* kink = A.kink + d.row0;
+  const double* lbv = lb;
+  const double* ubv = ub;
+  bool refine_active = false;
+  int recentrings    = 0;
+  double rdp = 0.0, rdd = 0.0;  // scales of the current correction problem (0: none yet)
   double red[kMaxRed];
 
   // ---- A^T: counting sort by column, then each column segment ordered by source position
@@ -360,7 +383,7 @@ __global__ __launch_bounds__(kBT) void batch_pdlp_kernel(BatchArgs<V> A)
     red[3] += bound_sq(lc[i], uc[i]);
   }
   wg_sum<4>(red, lds);
-  const double beta = 1.0 / (sqrt(red[2]) + 1.0), gamma = 1.0 / (sqrt(red[0]) + 1.0);
+  double beta = 1.0 / (sqrt(red[2]) + 1.0), gamma = 1.0 / (sqrt(red[0]) + 1.0);
   const double b_norm = sqrt(red[3]), c_norm = sqrt(red[1]);
   for (int j = tid; j < n; j += kBT) {
     const double dc = Dc[j];
@@ -442,16 +465,17 @@ __global__ __launch_bounds__(kBT) void batch_pdlp_kernel(BatchArgs<V> A)
     const bool is_major = ((k % kMajorB) == 0 && k > 0) || k <= 0;
     {
       zero_red(red);
+      const double idp = refine_active ? 1.0 / rdp : 1.0, idd = refine_active ? 1.0 / rdd : 1.0;
       rows_pass(aptr, aidx, aorig, m, d.lga,
-                KktRowsEpi<V>{xhat, Dc, 1.0 / beta, yhat, Dr, 1.0 / gamma, gamma, cb_orig, yanc, yu, nullptr,
-                              nullptr, 1.0, 1.0},
+                KktRowsEpi<V>{xhat, Dc, 1.0 / beta, yhat, Dr, 1.0 / gamma, gamma, cb_orig, yanc, yu,
+                              refine_active ? xstar : nullptr, refine_active ? ystar : nullptr, idp, idd},
                 red);
       wg_sum<3>(red, lds);
       const double rp2 = red[0], psum = red[1], dd = red[2];  // dd: ||y0 - yhat||^2
       zero_red(red);
       rows_pass(tptr, tidx, torig, n, d.lgt,
-                KktColsEpi<V>{yu, xhat, slack, Dc, 1.0 / beta, beta, 1.0 / gamma, gamma, c, xanc, xu, su, nullptr,
-                              1.0, 1.0},
+                KktColsEpi<V>{yu, xhat, slack, Dc, 1.0 / beta, beta, 1.0 / gamma, gamma, c, xanc, xu, su,
+                              refine_active ? xstar : nullptr, idp, idd, bnd_w, lbv, ubv},
                 red);
       wg_sum<4>(red, lds);
       const double dp = red[3];  // ||x0 - xhat||^2
@@ -532,15 +556,71 @@ __global__ __launch_bounds__(kBT) void batch_pdlp_kernel(BatchArgs<V> A)
             best_gap   = g;
             best_omega = omega;
           }
-          for (int j = tid; j < n; j += kBT) {
-            const V v  = xhat[j];
-            xanc[j]    = v;
-            xs[cur][j] = v;
-          }
-          for (int i = tid; i < m; i += kBT) {
-            const V v  = yhat[i];
-            yanc[i]    = v;
-            ys[cur][i] = v;
+          const double rel_err = fmax(fmax(l2p / (1.0 + b_norm), l2d / (1.0 + c_norm)), gap / (1.0 + absobj));
+          if (A.refine && (refine_active || rel_err <= 1e-2)) {
+            // Recentre at the restart point (see PdlpSolver::recentre in pdlp_solver.hip).
+            double sdp = 1.0 / fmax(l2p, 1e-14 * (1.0 + b_norm));
+            double sdd = 1.0 / fmax(l2d, 1e-14 * (1.0 + c_norm));
+            sdp        = fmax(sdp, 1.0);
+            sdd        = fmax(sdd, 1.0);
+            if (sdp * sdd * gap > 1.0) {
+              const double f = sqrt(1.0 / (sdp * sdd * gap));
+              sdp            = fmax(sdp * f, 1.0);
+              sdd            = fmax(sdd * f, 1.0);
+            }
+            const double p_prev = rdp > 0.0 ? rdp : beta, d_prev = rdd > 0.0 ? rdd : gamma;
+            const double wr     = (sdd / d_prev) / (sdp / p_prev);
+            omega               = bound_primal_weight<V>(omega * wr);
+            best_omega          = bound_primal_weight<V>(best_omega * wr);
+            tau                 = eta / omega;
+            sigma               = eta * omega;
+            rdp = sdp, rdd = sdd;
+            for (int j = tid; j < n; j += kBT) xstar[j] = fmin(fmax(xu[j], lbv[j]), ubv[j]);
+            for (int i = tid; i < m; i += kBT) ystar[i] = yu[i];
+            __syncthreads();
+            double rr[kMaxRed];
+            rows_pass(aptr, aidx, aorig, m, d.lga, BPlainEpi{xstar, ax64}, rr);
+            rows_pass(tptr, tidx, torig, n, d.lgt, BPlainEpi{ystar, aty64}, rr);
+            __syncthreads();
+            for (int j = tid; j < n; j += kBT) {
+              const double dc = Dc[j], x = xstar[j];
+              c_w[j]          = (V)((sdd * (c[j] - aty64[j])) * dc);
+              V2 bb;
+              bb.x      = dc != 0.0 ? (V)((sdp * (lbv[j] - x)) / dc) : V(0);
+              bb.y      = dc != 0.0 ? (V)((sdp * (ubv[j] - x)) / dc) : V(0);
+              bnd_w[j]  = bb;
+              xs[cur][j] = mn(mx(V(0), bb.x), bb.y);
+              xhat[j]    = V(0);
+              xanc[j]    = V(0);
+              slack[j]   = V(0);
+            }
+            for (int i = tid; i < m; i += kBT) {
+              const double r = Dr[i];
+              V2 bb;
+              bb.x       = (V)((sdp * (lc[i] - ax64[i])) * r);
+              bb.y       = (V)((sdp * (uc[i] - ax64[i])) * r);
+              cb_w[i]    = bb;
+              const double kk = -sdd * ystar[i];
+              kink[i]    = (V)(r != 0.0 ? kk / r : kk);
+              ys[cur][i] = V(0);
+              yhat[i]    = V(0);
+              yanc[i]    = V(0);
+            }
+            beta          = 1.0;
+            gamma         = 1.0;
+            refine_active = true;
+            ++recentrings;
+          } else {
+            for (int j = tid; j < n; j += kBT) {
+              const V v  = xhat[j];
+              xanc[j]    = v;
+              xs[cur][j] = v;
+            }
+            for (int i = tid; i < m; i += kBT) {
+              const V v  = yhat[i];
+              yanc[i]    = v;
+              ys[cur][i] = v;
+            }
           }
           kr         = 0;
           last_trial = inf;
@@ -566,11 +646,14 @@ __global__ __launch_bounds__(kBT) void batch_pdlp_kernel(BatchArgs<V> A)
         zero_red(red);
 #define CUOPT_BPRIMAL(MAJ, FP)                                                                              \
   rows_pass(tptr, tidx, tw, n, d.lgt,                                                                        \
-            PrimalStepEpi<V, MAJ, FP>{ys[ci], xs[ci], c_w, bnd_w, xanc, xbar, xs[nx], xhat, slack, aty, &sc, jj}, \
+            PrimalStepEpi<V, MAJ, FP>{ys[ci], xs[ci], c_w, bnd_w, xanc, xbar, xs[nx], xhat, slack, aty, &sc, jj,    \
+                                      refine_active},                                                         \
             red)
 #define CUOPT_BDUAL(MAJ, FP)                                                                                 \
   rows_pass(aptr, aidx, aw, m, d.lga,                                                                         \
-            DualStepEpi<V, MAJ, FP>{xbar, ys[ci], cb_w, yanc, ys[nx], yhat, ybar, &sc, jj, nullptr}, red)
+            DualStepEpi<V, MAJ, FP>{xbar, ys[ci], cb_w, yanc, ys[nx], yhat, ybar, &sc, jj,                         \
+                                    refine_active ? (const V*)kink : nullptr},                                  \
+            red)
         if (should_major) {
           if (fpe_step) CUOPT_BPRIMAL(true, true);
           else CUOPT_BPRIMAL(true, false);
@@ -634,6 +717,7 @@ __global__ __launch_bounds__(kBT) void batch_pdlp_kernel(BatchArgs<V> A)
     o.restarts = restarts;
     o.checks = checks;
     o.power_iters = piters;
+    o.recentrings = recentrings;
     A.out[A.order[blockIdx.x]] = o;
   }
 }
@@ -747,6 +831,9 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
   DevBuf<V2> bnd_w(N), cb_w(M);
   DevBuf<double2> cb_orig(M);
   DevBuf<double> xu(N), yu(M), su(N), pz(M), pz2(M), patq(N), dz0(z0.size());
+  const bool refine = std::is_same<V, float>::value && st.refinement != 0;
+  DevBuf<double> xstar(refine ? N : 1), ystar(refine ? M : 1), ax64(refine ? M : 1), aty64(refine ? N : 1);
+  DevBuf<V> kink(refine ? M : 1);
   DevBuf<BatchOut> out(std::max<size_t>(lps.size(), 1));
   d_desc.upload(desc, s);
   d_order.upload(order, s);
@@ -814,6 +901,12 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
   args.tol_rg = st.rel_gap;
   args.iter_limit = st.iteration_limit;
   args.tick_limit = 0;
+  args.refine     = refine ? 1 : 0;
+  args.xstar      = xstar.get();
+  args.ystar      = ystar.get();
+  args.ax64       = ax64.get();
+  args.aty64      = aty64.get();
+  args.kink       = kink.get();
   if (std::isfinite(st.time_limit)) {
     int dev = 0, khz = 0;
     CUOPT_HIP_CHECK(hipGetDevice(&dev));
@@ -869,7 +962,7 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
     r.info[CUOPT_AMD_INFO_GAMMA]        = o.gamma;
     r.info[CUOPT_AMD_INFO_PRECISION]    = std::is_same<V, double>::value ? 0 : 1;
     r.info[CUOPT_AMD_INFO_CHECKS]       = o.checks;
-    r.info[CUOPT_AMD_INFO_REFINE_ROUNDS] = 0;
+    r.info[CUOPT_AMD_INFO_REFINE_ROUNDS] = o.recentrings;
   }
   return R;
 }

@@ -463,7 +463,7 @@ struct PlanBufs {
 
 struct MatStruct {  // CSR structure shared by the FP64 and working-precision values
   int nrows = 0, ncols = 0, nnz = 0;
-  int tile = 2048;  // LDS capacity of the tiles (512, 1024 or 2048 nonzeros)
+  int tile = 1024;  // LDS capacity of the tiles (512, 1024 or 2048 nonzeros)
   std::vector<int> h_ptr;
   DevBuf<int> ptr, idx, rowid;
   PlanBufs plan;
@@ -537,9 +537,9 @@ constexpr double kSufficient      = 0.2;    // sufficient_reduction_for_restart
 constexpr double kNecessary       = 0.8;    // necessary_reduction_for_restart
 constexpr double kKp = 0.99, kKi = 0.01, kKd = 0.0, kISmooth = 0.3;  // restart PID (pdlp_hyper_params.cuh:51-54)
 constexpr int kRuizIters          = 10;
-// FP64 refinement of FP32-mixed solves: requested tolerances below this turn
-// it on, and restarts recentre once the relative KKT error is below it.
-constexpr double kRefineFrom      = 1e-4;
+// FP64 refinement of FP32-mixed solves: restarts recentre the iterates once
+// the relative KKT error is below this.
+constexpr double kRecentreFrom    = 1e-2;
 
 struct Tolerances {
   double ap, rp, ad, rd, ag, rg;  // absolute / relative primal, dual, gap
@@ -668,6 +668,7 @@ class PdlpSolver {
   } refine_;
   bool refine_active_ = false;
   int recentrings_    = 0;
+  static double bound_w(double w) { return bound_primal_weight<V>(w); }
   long long k_done_   = 0;  // iterations of all finished loops
   int restarts_ = 0, checks_ = 0;
   double setup_time_ = 0.0;
@@ -730,8 +731,8 @@ void PdlpSolver<V>::build_structures()
   AT_.ncols = m_;
   AT_.nnz   = nnz;
   AT_.h_ptr = tptr;
-  A_.tile  = tile_from_env("CUOPT_AMD_SPMV_TILE_A", 2048);
-  AT_.tile = tile_from_env("CUOPT_AMD_SPMV_TILE_AT", 2048);
+  A_.tile  = tile_from_env("CUOPT_AMD_SPMV_TILE_A", 1024);
+  AT_.tile = tile_from_env("CUOPT_AMD_SPMV_TILE_AT", 1024);
   for (MatStruct* M : {&A_, &AT_}) {
     M->ptr.resize(M->nrows + 1);
     M->ptr.upload(M->h_ptr, stream_);
@@ -1096,8 +1097,7 @@ SolveResult PdlpSolver<V>::run()
   R.info[CUOPT_AMD_INFO_SETUP_TIME] = setup_time_;
 
   const Tolerances user{st_.abs_primal, st_.rel_primal, st_.abs_dual, st_.rel_dual, st_.abs_gap, st_.rel_gap};
-  const bool refine = !std::is_same<V, double>::value && st_.refinement != 0 && st_.bench_steps <= 0 &&
-                      std::min({user.ap, user.rp, user.ad, user.rd, user.ag, user.rg}) < kRefineFrom;
+  const bool refine = !std::is_same<V, double>::value && st_.refinement != 0 && st_.bench_steps <= 0;
   finish(R, pdhg_loop(R, user, refine));
   R.info[CUOPT_AMD_INFO_STEPS]         = (double)k_done_;
   R.info[CUOPT_AMD_INFO_RESTARTS]      = restarts_;
@@ -1107,7 +1107,8 @@ SolveResult PdlpSolver<V>::run()
   return R;
 }
 
-// Recentring at a restart (FP64 iterative refinement of FP32-mixed solves).
+// Recentring at a restart (FP64 iterative refinement of FP32-mixed solves,
+// applied at every restart once the relative KKT error is below 1e-2).
 // At a restart the PDHG state is the single point (x-hat, y-hat), whose FP64
 // values in the original problem the check has just left in xu_ / yu_.  The
 // iterates are re-expressed as a correction problem around that point,
@@ -1178,8 +1179,8 @@ void PdlpSolver<V>::recentre()
             stream_);
   launch_ew(n_, nullptr, ZeroF<V>{slack_.get()}, stream_);
   cur_   = 0;
-  omega_      = bound_primal_weight<V>(w_prev * ratio);
-  best_omega_ = bound_primal_weight<V>(best_omega_ * ratio);
+  omega_      = bound_w(w_prev * ratio);
+  best_omega_ = bound_w(best_omega_ * ratio);
   ++recentrings_;
   tau_   = eta_ / omega_;
   sigma_ = eta_ * omega_;
@@ -1299,7 +1300,7 @@ int PdlpSolver<V>::pdhg_loop(SolveResult& R, const Tolerances& tol, bool refine)
           const double s   = err_sum * kISmooth + err;
           err_sum          = s;
           const double de  = err - last_err;
-          const double w   = bound_primal_weight<V>(std::exp(kKp * err + kKi * s + kKd * de) * cur);
+          const double w   = bound_w(std::exp(kKp * err + kKi * s + kKd * de) * cur);
           omega_           = w;
           last_err         = err;
           tau_             = eta_ / w;
@@ -1316,7 +1317,7 @@ int PdlpSolver<V>::pdhg_loop(SolveResult& R, const Tolerances& tol, bool refine)
           best_gap    = g;
           best_omega_ = omega_;
         }
-        if (refine && (refine_active_ || rel_kkt_error() <= kRefineFrom)) {
+        if (refine && (refine_active_ || rel_kkt_error() <= kRecentreFrom)) {
           recentre();  // the restart point becomes the origin of the new coordinates
           refine_active_ = true;
         } else {

@@ -23,12 +23,15 @@ extern "C" {
 #define CUOPT_AMD_PRECISION_FP32_MIXED 1
 
 /* Integer parameter (0/1, default 1) enabling FP64 iterative refinement of
- * FP32-mixed solves (no reference counterpart).  When the requested tolerances
- * are tighter than 1e-4, the FP32 solve stops at 1e-4 and FP32 PDHG then
- * re-solves correction problems: the original LP written in the coordinates
- * (x - x*) * Dp, (y - y*) * Dd around the current FP64 solution (x*, y*), with
- * Dp, Dd the inverse primal / dual residual norms.  Optimality is always
- * tested in FP64 on the original problem.  Ignored by FP64 solves. */
+ * FP32-mixed solves (no reference counterpart).  Once the relative KKT error is
+ * below 1e-2, every PDLP restart recentres the FP32 iterates: the solve
+ * continues on the original LP written in the coordinates (x - x*) * Dp,
+ * (y - y*) * Dd around the restart point's FP64 values (x*, y*), with Dp, Dd the
+ * inverse primal / dual residual norms.  FP32 iterates then carry only the
+ * unresolved part of the solution, so FP32-mixed solves reach FP64-level
+ * tolerances (e.g. 1e-8) and do not stall when a step size becomes small.
+ * Optimality is always tested in FP64 on the original problem.  Ignored by
+ * FP64 solves. */
 #define CUOPT_AMD_PDLP_REFINEMENT "pdlp_refinement"
 
 /* Slots of cuOptAmdGetTerminationInfo.  Slots 0-9 mirror
@@ -58,7 +61,7 @@ extern "C" {
 #define CUOPT_AMD_INFO_STRUCTURE_TIME      22 /* setup: CSR sort, transpose, tile plans, uploads (s) */
 #define CUOPT_AMD_INFO_SCALING_TIME        23 /* setup: Ruiz, Pock-Chambolle, bound/objective rescaling (s) */
 #define CUOPT_AMD_INFO_POWER_TIME          24 /* setup: power iteration for the step size (s) */
-#define CUOPT_AMD_INFO_REFINE_ROUNDS       25 /* FP64 refinement rounds run after the FP32 solve */
+#define CUOPT_AMD_INFO_REFINE_ROUNDS       25 /* FP64 recentrings of a FP32-mixed solve */
 #define CUOPT_AMD_INFO_COUNT               26
 
 /* Copies up to n slots of the last solve's statistics into info. */
