@@ -123,20 +123,26 @@ def batch_extras(c_api, G, count: int, sample: int, time_limit: float) -> dict:
     t = time.time()
     lps = G.batch_lps(count=count, seed=0)
     gen = time.time() - t
-    t = time.time()
-    rs = c_api.batch_solve(lps, tol=1e-4, precision="fp32_mixed", time_limit=time_limit)
-    wall = time.time() - t
-    solved = sum(r.status == 1 for r in rs)
-    errs = [abs(r.primal_objective - lp.obj_opt) / (1 + abs(lp.obj_opt)) for r, lp in zip(rs, lps) if r.status == 1]
+    out = {"workload": f"{count} LPs m~logU[50,2000] n=2.5m 8 nnz/col (sum nnz {sum(lp.nnz for lp in lps)})",
+           "generation_s": round(gen, 2)}
+    for prec in ("fp32_mixed", "fp64"):
+        t = time.time()
+        rs = c_api.batch_solve(lps, tol=1e-4, precision=prec, time_limit=time_limit)
+        wall = time.time() - t
+        solved = sum(r.status == 1 for r in rs)
+        errs = [abs(r.primal_objective - lp.obj_opt) / (1 + abs(lp.obj_opt)) for r, lp in zip(rs, lps)
+                if r.status == 1]
+        its = sorted(r.iterations for r in rs)
+        out[prec] = {"wall_s": round(wall, 4), "solve_time_s": round(rs[0].solve_time, 4), "solved": solved,
+                     "lps_per_s": round(solved / wall, 1), "max_obj_rel_err": max(errs) if errs else None,
+                     "iterations_p50_max": [its[len(its) // 2], its[-1]], "total_iterations": int(sum(its))}
+        print(f"[bench] batch {prec}: {out[prec]}", file=sys.stderr, flush=True)
     t = time.time()
     seq = [c_api.solve(lp, tol=1e-4, precision="fp32_mixed", time_limit=time_limit) for lp in lps[:sample]]
     seq_wall = time.time() - t
-    out = {"workload": f"{count} LPs m~logU[50,2000] n=2.5m 8 nnz/col (sum nnz {sum(lp.nnz for lp in lps)})",
-           "generation_s": round(gen, 2), "batch_wall_s": round(wall, 4), "batch_solve_time_s": round(rs[0].solve_time, 4),
-           "solved": solved, "lps_per_s": round(solved / wall, 1), "max_obj_rel_err": max(errs) if errs else None,
-           "sequential_sample": sample, "sequential_lps_per_s": round(sum(r.status == 1 for r in seq) / seq_wall, 2),
-           "total_iterations": int(sum(r.iterations for r in rs))}
-    out["speedup_vs_sequential"] = round(out["lps_per_s"] / max(out["sequential_lps_per_s"], 1e-9), 1)
+    out["sequential_sample"] = sample
+    out["sequential_lps_per_s"] = round(sum(r.status == 1 for r in seq) / seq_wall, 2)
+    out["speedup_vs_sequential"] = round(out["fp32_mixed"]["lps_per_s"] / max(out["sequential_lps_per_s"], 1e-9), 1)
     print(f"[bench] batch: {out}", file=sys.stderr, flush=True)
     return out
 
