@@ -23,6 +23,8 @@
 #include <string>
 #include <type_traits>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "common.hpp"
 #include "lp_types.hpp"
 #include "pdlp_kernels.hpp"
@@ -120,6 +122,56 @@ __global__ __launch_bounds__(1024) void power_finalize_res(const double* __restr
 // ---------------------------------------------------------------------------
 // Elementwise functors
 // ---------------------------------------------------------------------------
+
+// Lanczos steps on A~ A~^T (step size of FP32-mixed solves).
+struct LanczosAlphaF {  // w = z - beta_prev q_prev; red0 = q . w
+  static constexpr int NRED = 1;
+  const double* z;
+  const double* q;
+  const double* qprev;
+  double beta_prev;
+  double* w;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int i, double (&red)[kMaxRed]) const
+  {
+    const double v = z[i] - beta_prev * qprev[i];
+    w[i]           = v;
+    red[0] += q[i] * v;
+  }
+};
+struct LanczosOrthF {  // w -= alpha q; red0 = ||w||^2
+  static constexpr int NRED = 1;
+  const double* q;
+  double alpha;
+  double* w;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int i, double (&red)[kMaxRed]) const
+  {
+    const double v = w[i] - alpha * q[i];
+    w[i]           = v;
+    red[0] += v * v;
+  }
+};
+struct LanczosNextF {  // q_prev = q; q = w / beta
+  static constexpr int NRED = 0;
+  double* q;
+  double* qprev;
+  const double* w;
+  double inv_beta;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int i, double (&)[kMaxRed]) const
+  {
+    qprev[i] = q[i];
+    q[i]     = w[i] * inv_beta;
+  }
+};
+struct ScaleByF {  // v *= s
+  static constexpr int NRED = 0;
+  double* v;
+  double s;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int i, double (&)[kMaxRed]) const { v[i] *= s; }
+};
 
 struct SumSqF {  // red0 = sum v^2
   static constexpr int NRED = 1;
@@ -478,6 +530,84 @@ int tile_from_env(const char* name, int def)
   return (t == 512 || t == 1024 || t == 2048) ? t : def;
 }
 
+// ---- device CSR transpose -------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void rowid_kernel(const int* __restrict__ ptr, int m, int nnz,
+                                                       int* __restrict__ rowid)
+{
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < nnz; k += gridDim.x * kBlock) {
+    int lo = 0, hi = m;  // last row i with ptr[i] <= k
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (ptr[mid] <= k) lo = mid;
+      else hi = mid;
+    }
+    rowid[k] = lo;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void iota_kernel(int n, int* __restrict__ v)
+{
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) v[k] = k;
+}
+
+__global__ __launch_bounds__(kBlock) void transpose_gather_kernel(int nnz, const int* __restrict__ perm,
+                                                                  const int* __restrict__ rowid_a,
+                                                                  const double* __restrict__ val,
+                                                                  int* __restrict__ tidx, double* __restrict__ tval)
+{
+  for (int d = blockIdx.x * kBlock + threadIdx.x; d < nnz; d += gridDim.x * kBlock) {
+    const int k = perm[d];
+    tidx[d]     = rowid_a[k];
+    tval[d]     = val[k];
+  }
+}
+
+// tptr[j] = first position of column j among the sorted column keys (j in [0, n]).
+__global__ __launch_bounds__(kBlock) void column_ptr_kernel(int n, int nnz, const int* __restrict__ cols,
+                                                            int* __restrict__ tptr)
+{
+  for (int j = blockIdx.x * kBlock + threadIdx.x; j <= n; j += gridDim.x * kBlock) {
+    int lo = 0, hi = nnz;
+    while (lo < hi) {
+      const int mid = (lo + hi) >> 1;
+      if (cols[mid] < j) lo = mid + 1;
+      else hi = mid;
+    }
+    tptr[j] = lo;
+  }
+}
+
+inline int grid_for(long long n) { return (int)std::max<long long>(1, std::min<long long>((n + kBlock - 1) / kBlock, 4096)); }
+
+// A (CSR, device) -> A^T (CSR, device) plus the row index of every nonzero of both.
+void device_transpose(int m, int n, int nnz, const int* ptr, const int* idx, const double* val, int* tptr, int* tidx,
+                      double* tval, int* rowid_a, int* rowid_at, hipStream_t s)
+{
+  if (nnz == 0) {
+    hipLaunchKernelGGL(column_ptr_kernel, dim3(grid_for(n + 1)), dim3(kBlock), 0, s, n, 0, rowid_at, tptr);
+    CUOPT_HIP_CHECK(hipGetLastError());
+    return;
+  }
+  DevBuf<int> perm_in(nnz), perm(nnz);
+  hipLaunchKernelGGL(rowid_kernel, dim3(grid_for(nnz)), dim3(kBlock), 0, s, ptr, m, nnz, rowid_a);
+  hipLaunchKernelGGL(iota_kernel, dim3(grid_for(nnz)), dim3(kBlock), 0, s, nnz, perm_in.get());
+  CUOPT_HIP_CHECK(hipGetLastError());
+  int bits = 1;
+  while (bits < 31 && (1LL << bits) < n) ++bits;
+  size_t temp_bytes = 0;
+  CUOPT_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, temp_bytes, idx, rowid_at, perm_in.get(), perm.get(),
+                                            (size_t)nnz, 0, bits, s));
+  DevBuf<unsigned char> temp(std::max<size_t>(temp_bytes, 1));
+  CUOPT_HIP_CHECK(rocprim::radix_sort_pairs(temp.get(), temp_bytes, idx, rowid_at, perm_in.get(), perm.get(),
+                                            (size_t)nnz, 0, bits, s));
+  hipLaunchKernelGGL(transpose_gather_kernel, dim3(grid_for(nnz)), dim3(kBlock), 0, s, nnz, perm.get(), rowid_a,
+                     val, tidx, tval);
+  hipLaunchKernelGGL(column_ptr_kernel, dim3(grid_for(n + 1)), dim3(kBlock), 0, s, n, nnz, rowid_at, tptr);
+  CUOPT_HIP_CHECK(hipGetLastError());
+  CUOPT_HIP_CHECK(hipStreamSynchronize(s));  // temporaries are freed on return
+}
+
 void build_plan(MatStruct& M, hipStream_t s)
 {
   TilePlan p = plan_tiles(M.h_ptr.data(), M.nrows, M.tile - kVecPad, kMaxRowsPerTile, kBlock);
@@ -600,6 +730,7 @@ class PdlpSolver {
   void build_structures();
   void setup_scaling();
   void power_iteration();
+  double lanczos_sigma_max_sq(const std::vector<double>& z0, int& iters);
   void init_iterates();
   void read_scalars()
   {
@@ -637,9 +768,6 @@ class PdlpSolver {
   double* h_scal_     = nullptr;
   StepScalars* h_step_ = nullptr;
 
-  // host CSR (row-sorted) and its transpose
-  std::vector<int> a_idx_, at_idx_;
-  std::vector<double> a_val_, at_val_;
   MatStruct A_, AT_;
   DevBuf<double> a_orig_, at_orig_, a_s_, at_s_;
   DevBuf<V> a_w_, at_w_;
@@ -688,40 +816,32 @@ template <typename V>
 void PdlpSolver<V>::build_structures()
 {
   const int nnz = lp_.rowptr[m_];
-  // Row-sorted CSR (problem_t keeps sorted column indices).
-  a_idx_.assign(lp_.colidx.begin(), lp_.colidx.begin() + nnz);
-  a_val_.assign(lp_.vals.begin(), lp_.vals.begin() + nnz);
+  // Row-sorted CSR (problem_t keeps sorted column indices); rows that arrive
+  // unsorted are sorted here on the host, which callers rarely need.
+  const int* idx_src    = lp_.colidx.data();
+  const double* val_src = lp_.vals.data();
+  std::vector<int> sorted_idx;
+  std::vector<double> sorted_val;
   for (int i = 0; i < m_; ++i) {
     const int b = lp_.rowptr[i], e = lp_.rowptr[i + 1];
     bool sorted = true;
-    for (int k = b + 1; k < e && sorted; ++k) sorted = a_idx_[k - 1] <= a_idx_[k];
-    if (!sorted) {
-      std::vector<std::pair<int, double>> row;
-      for (int k = b; k < e; ++k) row.emplace_back(a_idx_[k], a_val_[k]);
-      std::stable_sort(row.begin(), row.end(), [](const auto& p, const auto& q) { return p.first < q.first; });
-      for (int k = b; k < e; ++k) {
-        a_idx_[k] = row[k - b].first;
-        a_val_[k] = row[k - b].second;
-      }
+    for (int k = b + 1; k < e && sorted; ++k) sorted = idx_src[k - 1] <= idx_src[k];
+    if (sorted) continue;
+    if (sorted_idx.empty()) {
+      sorted_idx.assign(lp_.colidx.begin(), lp_.colidx.begin() + nnz);
+      sorted_val.assign(lp_.vals.begin(), lp_.vals.begin() + nnz);
+    }
+    std::vector<std::pair<int, double>> row;
+    for (int k = b; k < e; ++k) row.emplace_back(sorted_idx[k], sorted_val[k]);
+    std::stable_sort(row.begin(), row.end(), [](const auto& p, const auto& q) { return p.first < q.first; });
+    for (int k = b; k < e; ++k) {
+      sorted_idx[k] = row[k - b].first;
+      sorted_val[k] = row[k - b].second;
     }
   }
-  // Stable counting-sort transpose (compute_transpose_of_problem, problem.cu:472-489).
-  std::vector<int> tptr(n_ + 1, 0);
-  for (int k = 0; k < nnz; ++k) tptr[a_idx_[k] + 1]++;
-  for (int j = 0; j < n_; ++j) tptr[j + 1] += tptr[j];
-  at_idx_.resize(nnz);
-  at_val_.resize(nnz);
-  std::vector<int> a_rowid(nnz), at_rowid(nnz);
-  {
-    std::vector<int> cur(tptr.begin(), tptr.end() - 1);
-    for (int i = 0; i < m_; ++i)
-      for (int k = lp_.rowptr[i]; k < lp_.rowptr[i + 1]; ++k) {
-        a_rowid[k]   = i;
-        const int d  = cur[a_idx_[k]]++;
-        at_idx_[d]   = i;
-        at_val_[d]   = a_val_[k];
-        at_rowid[d]  = a_idx_[k];
-      }
+  if (!sorted_idx.empty()) {
+    idx_src = sorted_idx.data();
+    val_src = sorted_val.data();
   }
   A_.nrows = m_;
   A_.ncols = n_;
@@ -730,28 +850,31 @@ void PdlpSolver<V>::build_structures()
   AT_.nrows = n_;
   AT_.ncols = m_;
   AT_.nnz   = nnz;
-  AT_.h_ptr = tptr;
   A_.tile  = tile_from_env("CUOPT_AMD_SPMV_TILE_A", 1024);
   AT_.tile = tile_from_env("CUOPT_AMD_SPMV_TILE_AT", 1024);
   for (MatStruct* M : {&A_, &AT_}) {
     M->ptr.resize(M->nrows + 1);
-    M->ptr.upload(M->h_ptr, stream_);
     M->idx.resize(nnz + kArrayPad);
-    M->idx.zero(stream_);
     M->rowid.resize(std::max(nnz, 1));
-    build_plan(*M, stream_);
+    CUOPT_HIP_CHECK(hipMemsetAsync(M->idx.get() + nnz, 0, kArrayPad * sizeof(int), stream_));
   }
-  A_.idx.upload(a_idx_.data(), nnz, stream_);
-  AT_.idx.upload(at_idx_.data(), nnz, stream_);
-  A_.rowid.upload(a_rowid.data(), nnz, stream_);
-  AT_.rowid.upload(at_rowid.data(), nnz, stream_);
   a_orig_.resize(nnz + kArrayPad);
   at_orig_.resize(nnz + kArrayPad);
-  a_orig_.zero(stream_);
-  at_orig_.zero(stream_);
-  a_orig_.upload(a_val_.data(), nnz, stream_);
-  at_orig_.upload(at_val_.data(), nnz, stream_);
-  CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));  // host vectors above are temporaries
+  CUOPT_HIP_CHECK(hipMemsetAsync(a_orig_.get() + nnz, 0, kArrayPad * sizeof(double), stream_));
+  CUOPT_HIP_CHECK(hipMemsetAsync(at_orig_.get() + nnz, 0, kArrayPad * sizeof(double), stream_));
+  A_.ptr.upload(A_.h_ptr, stream_);
+  A_.idx.upload(idx_src, nnz, stream_);
+  a_orig_.upload(val_src, nnz, stream_);
+  // A^T on the device (compute_transpose_of_problem, problem.cu:472-489): a
+  // stable radix sort of the nonzeros by column keeps each column's entries in
+  // row order, as the host counting sort of the reference does.
+  device_transpose(m_, n_, nnz, A_.ptr.get(), A_.idx.get(), a_orig_.get(), AT_.ptr.get(), AT_.idx.get(),
+                   at_orig_.get(), A_.rowid.get(), AT_.rowid.get(), stream_);
+  AT_.h_ptr.resize(n_ + 1);
+  AT_.ptr.download(AT_.h_ptr.data(), n_ + 1, stream_);
+  CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));  // host sources above are temporaries
+  build_plan(A_, stream_);
+  build_plan(AT_, stream_);
 }
 
 template <typename V>
@@ -846,6 +969,75 @@ void PdlpSolver<V>::setup_scaling()
             stream_);
 }
 
+// Largest eigenvalue of the symmetric tridiagonal matrix (a, b) by Sturm-sequence bisection.
+double tridiag_max_eig(const std::vector<double>& a, const std::vector<double>& b)
+{
+  const int k = (int)a.size();
+  double lo = 0.0, hi = 0.0;
+  for (int i = 0; i < k; ++i) {  // Gershgorin bounds
+    const double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i + 1 < k ? std::fabs(b[i]) : 0.0);
+    lo = std::min(lo, a[i] - r);
+    hi = std::max(hi, a[i] + r);
+  }
+  auto count_below = [&](double x) {  // eigenvalues < x
+    int c    = 0;
+    double d = 1.0;
+    for (int i = 0; i < k; ++i) {
+      d = a[i] - x - (i > 0 ? b[i - 1] * b[i - 1] / d : 0.0);
+      if (d == 0.0) d = -1e-300;
+      if (d < 0.0) ++c;
+    }
+    return c;
+  };
+  for (int it = 0; it < 200 && hi - lo > 1e-15 * std::max(std::fabs(hi), 1e-300); ++it) {
+    const double mid = 0.5 * (lo + hi);
+    if (count_below(mid) >= k) hi = mid;  // all eigenvalues below mid
+    else lo = mid;
+  }
+  return hi;
+}
+
+// sigma_max^2 of A~ by Lanczos on A~ A~^T from the power iteration's start
+// vector.  Used by FP32-mixed solves in place of the power iteration: both
+// estimate the same eigenvalue; Lanczos gets it to near machine precision in
+// tens of matrix products instead of hundreds.
+template <typename V>
+double PdlpSolver<V>::lanczos_sigma_max_sq(const std::vector<double>& z0, int& iters)
+{
+  DevBuf<double> q(m_), qprev(m_), w(m_), zb(m_), atq(n_);
+  q.upload(z0, stream_);
+  launch_ew(m_, nullptr, FillF{qprev.get(), 0.0}, stream_);
+  launch_ew(m_, red_.get(), SumSqF{q.get()}, stream_);
+  launch_reduce(red_.get(), ew_grid(m_), 1, scal_.get() + S_SETUP0, stream_);
+  read_scalars();
+  launch_ew(m_, nullptr, ScaleByF{q.get(), 1.0 / std::sqrt(h_scal_[S_SETUP0])}, stream_);
+  std::vector<double> alpha, beta;
+  double theta = 0.0, prev = -1.0, beta_prev = 0.0;
+  const int kmax = std::min(m_, 300);
+  for (iters = 0; iters < kmax;) {
+    launch_tiled(AT_, at_s_.get(), nullptr, PlainEpi<double>{q.get(), atq.get()}, stream_);
+    launch_tiled(A_, a_s_.get(), nullptr, PlainEpi<double>{atq.get(), zb.get()}, stream_);
+    launch_ew(m_, red_.get(), LanczosAlphaF{zb.get(), q.get(), qprev.get(), beta_prev, w.get()}, stream_);
+    launch_reduce(red_.get(), ew_grid(m_), 1, scal_.get() + S_SETUP0, stream_);
+    read_scalars();
+    const double a = h_scal_[S_SETUP0];
+    launch_ew(m_, red_.get(), LanczosOrthF{q.get(), a, w.get()}, stream_);
+    launch_reduce(red_.get(), ew_grid(m_), 1, scal_.get() + S_SETUP0, stream_);
+    read_scalars();
+    const double b = std::sqrt(h_scal_[S_SETUP0]);
+    alpha.push_back(a);
+    ++iters;
+    theta = tridiag_max_eig(alpha, beta);
+    if (iters > 2 && std::fabs(theta - prev) <= 1e-12 * theta) break;
+    if (!(b > 1e-13 * std::max(theta, 1e-300))) break;  // invariant subspace found
+    prev = theta;
+    beta.push_back(b);
+    beta_prev = b;
+    launch_ew(m_, nullptr, LanczosNextF{q.get(), qprev.get(), w.get(), 1.0 / b}, stream_);
+  }
+  return theta;
+}
+
 template <typename V>
 void PdlpSolver<V>::power_iteration()
 {
@@ -854,6 +1046,11 @@ void PdlpSolver<V>::power_iteration()
   std::mt19937 gen(1);
   std::normal_distribution<double> dist(0.0, 1.0);
   for (int i = 0; i < m_; ++i) z[i] = dist(gen);
+  if constexpr (std::is_same<V, float>::value) {
+    sigma_max_sq_ = lanczos_sigma_max_sq(z, power_iters_);
+    eta_          = 0.998 / std::sqrt(sigma_max_sq_);
+    return;
+  }
   DevBuf<double> za(m_), zb(m_), atq(n_);
   DevBuf<PowerState> st(1);
   za.upload(z, stream_);

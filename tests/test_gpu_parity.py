@@ -113,6 +113,46 @@ def test_setup_matches_oracle():
     assert abs(r.info["eta0"] - s["eta"]) < 1e-10 * s["eta"]
 
 
+def test_fp32_step_size_by_lanczos_matches_svd():
+    # FP32-mixed solves estimate sigma_max(A~)^2 by Lanczos on the FP64 scaled
+    # matrix (the oracle's scaled values here): checked against a dense SVD, and
+    # never below the reference power iteration's estimate
+    lp = G.random_lp(m=400, n=1000, seed=3)
+    r = c_api.solve(lp, tol=1e-4, iteration_limit=0, time_limit=TL, precision="fp32_mixed")
+    s = O.setup(lp)
+    import scipy.sparse as sp
+
+    At = sp.csr_matrix((s["scaled_values"], lp.col_indices, lp.row_offsets), shape=(lp.m, lp.n)).toarray()
+    smax2 = np.linalg.svd(At, compute_uv=False)[0] ** 2
+    assert abs(r.info["sigma_max_sq"] - smax2) <= 1e-10 * smax2
+    assert r.info["sigma_max_sq"] >= s["sigma_max_sq"] * (1 - 1e-12)
+    assert r.info["power_iterations"] < 100
+    assert abs(r.info["eta0"] - 0.998 / np.sqrt(smax2)) <= 1e-10 * r.info["eta0"]
+
+
+def test_device_transpose_unsorted_rows_and_empty_columns():
+    # rows given out of column order and columns without nonzeros: FP64 solve
+    # still matches the oracle (host row sort, device transpose)
+    lp = G.random_lp(m=200, n=500, seed=12)
+    n2 = lp.n + 3  # three empty columns at the end
+    perm_rows = []
+    idx, val = lp.col_indices.copy(), lp.values.copy()
+    rng = np.random.default_rng(0)
+    for i in range(lp.m):
+        b, e = lp.row_offsets[i], lp.row_offsets[i + 1]
+        p = rng.permutation(e - b)
+        idx[b:e], val[b:e] = idx[b:e][p], val[b:e][p]
+    import dataclasses
+
+    lp2 = dataclasses.replace(lp, n=n2, col_indices=idx, values=val, c=np.concatenate([lp.c, np.ones(3)]),
+                              var_lb=np.concatenate([lp.var_lb, np.zeros(3)]),
+                              var_ub=np.concatenate([lp.var_ub, np.full(3, 10.0)]))
+    r = c_api.solve(lp2, tol=1e-6, time_limit=TL, iteration_limit=100000)
+    o = O.solve(lp2, tol=1e-6)
+    assert r.status == 1 and r.iterations == o["iterations"]
+    assert rel(r.primal_objective, o["primal_objective"]) < 1e-9
+
+
 @pytest.mark.parametrize("tol", [1e-4, 1e-8])
 def test_solve_fp64_matches_oracle(tol):
     lp = G.random_lp(m=500, n=1250, seed=7)
