@@ -123,46 +123,52 @@ __global__ __launch_bounds__(1024) void power_finalize_res(const double* __restr
 // Elementwise functors
 // ---------------------------------------------------------------------------
 
-// Lanczos steps on A~ A~^T (step size of FP32-mixed solves).
-struct LanczosAlphaF {  // w = z - beta_prev q_prev; red0 = q . w
+// Lanczos steps on A~ A~^T (step size of FP32-mixed solves).  The
+// coefficients stay on the device (ab[2j] = alpha_j, ab[2j+1] = beta_j^2), so
+// the host reads them once per group of steps.
+struct LanczosAlphaF {  // w = z - beta_{j-1} q_{j-1}; red0 = q_j . w
   static constexpr int NRED = 1;
   const double* z;
   const double* q;
   const double* qprev;
-  double beta_prev;
+  const double* ab;
+  int j;
   double* w;
   __device__ bool skip() const { return false; }
   __device__ void operator()(int i, double (&red)[kMaxRed]) const
   {
-    const double v = z[i] - beta_prev * qprev[i];
-    w[i]           = v;
+    const double bp = j > 0 ? sqrt(ab[2 * j - 1]) : 0.0;
+    const double v  = z[i] - bp * qprev[i];
+    w[i]            = v;
     red[0] += q[i] * v;
   }
 };
-struct LanczosOrthF {  // w -= alpha q; red0 = ||w||^2
+struct LanczosOrthF {  // w -= alpha_j q_j; red0 = ||w||^2
   static constexpr int NRED = 1;
   const double* q;
-  double alpha;
+  const double* ab;
+  int j;
   double* w;
   __device__ bool skip() const { return false; }
   __device__ void operator()(int i, double (&red)[kMaxRed]) const
   {
-    const double v = w[i] - alpha * q[i];
+    const double v = w[i] - ab[2 * j] * q[i];
     w[i]           = v;
     red[0] += v * v;
   }
 };
-struct LanczosNextF {  // q_prev = q; q = w / beta
+struct LanczosNextF {  // q_{j-1} = q_j; q_{j+1} = w / beta_j (left as is once beta_j is zero)
   static constexpr int NRED = 0;
   double* q;
   double* qprev;
   const double* w;
-  double inv_beta;
-  __device__ bool skip() const { return false; }
+  const double* ab;
+  int j;
+  __device__ bool skip() const { return !(ab[2 * j + 1] > 0.0); }
   __device__ void operator()(int i, double (&)[kMaxRed]) const
   {
     qprev[i] = q[i];
-    q[i]     = w[i] * inv_beta;
+    q[i]     = w[i] / sqrt(ab[2 * j + 1]);
   }
 };
 struct ScaleByF {  // v *= s
@@ -1004,36 +1010,43 @@ double tridiag_max_eig(const std::vector<double>& a, const std::vector<double>& 
 template <typename V>
 double PdlpSolver<V>::lanczos_sigma_max_sq(const std::vector<double>& z0, int& iters)
 {
-  DevBuf<double> q(m_), qprev(m_), w(m_), zb(m_), atq(n_);
+  constexpr int kGroup = 8;  // steps between host reads of the coefficients
+  const int kmax = std::min(m_, 240);
+  DevBuf<double> q(m_), qprev(m_), w(m_), zb(m_), atq(n_), ab(2 * (size_t)kmax + 2);
   q.upload(z0, stream_);
   launch_ew(m_, nullptr, FillF{qprev.get(), 0.0}, stream_);
   launch_ew(m_, red_.get(), SumSqF{q.get()}, stream_);
   launch_reduce(red_.get(), ew_grid(m_), 1, scal_.get() + S_SETUP0, stream_);
   read_scalars();
   launch_ew(m_, nullptr, ScaleByF{q.get(), 1.0 / std::sqrt(h_scal_[S_SETUP0])}, stream_);
-  std::vector<double> alpha, beta;
-  double theta = 0.0, prev = -1.0, beta_prev = 0.0;
-  const int kmax = std::min(m_, 300);
-  for (iters = 0; iters < kmax;) {
-    launch_tiled(AT_, at_s_.get(), nullptr, PlainEpi<double>{q.get(), atq.get()}, stream_);
-    launch_tiled(A_, a_s_.get(), nullptr, PlainEpi<double>{atq.get(), zb.get()}, stream_);
-    launch_ew(m_, red_.get(), LanczosAlphaF{zb.get(), q.get(), qprev.get(), beta_prev, w.get()}, stream_);
-    launch_reduce(red_.get(), ew_grid(m_), 1, scal_.get() + S_SETUP0, stream_);
-    read_scalars();
-    const double a = h_scal_[S_SETUP0];
-    launch_ew(m_, red_.get(), LanczosOrthF{q.get(), a, w.get()}, stream_);
-    launch_reduce(red_.get(), ew_grid(m_), 1, scal_.get() + S_SETUP0, stream_);
-    read_scalars();
-    const double b = std::sqrt(h_scal_[S_SETUP0]);
-    alpha.push_back(a);
-    ++iters;
-    theta = tridiag_max_eig(alpha, beta);
-    if (iters > 2 && std::fabs(theta - prev) <= 1e-12 * theta) break;
-    if (!(b > 1e-13 * std::max(theta, 1e-300))) break;  // invariant subspace found
-    prev = theta;
-    beta.push_back(b);
-    beta_prev = b;
-    launch_ew(m_, nullptr, LanczosNextF{q.get(), qprev.get(), w.get(), 1.0 / b}, stream_);
+  std::vector<double> h_ab(2 * (size_t)kmax + 2), alpha, beta;
+  double theta = 0.0, prev = -1.0;
+  iters = 0;
+  while (iters < kmax) {
+    const int j0 = iters, j1 = std::min(kmax, iters + kGroup);
+    for (int j = j0; j < j1; ++j) {
+      launch_tiled(AT_, at_s_.get(), nullptr, PlainEpi<double>{q.get(), atq.get()}, stream_);
+      launch_tiled(A_, a_s_.get(), nullptr, PlainEpi<double>{atq.get(), zb.get()}, stream_);
+      launch_ew(m_, red_.get(), LanczosAlphaF{zb.get(), q.get(), qprev.get(), ab.get(), j, w.get()}, stream_);
+      launch_reduce(red_.get(), ew_grid(m_), 1, ab.get() + 2 * j, stream_);
+      launch_ew(m_, red_.get(), LanczosOrthF{q.get(), ab.get(), j, w.get()}, stream_);
+      launch_reduce(red_.get(), ew_grid(m_), 1, ab.get() + 2 * j + 1, stream_);
+      launch_ew(m_, nullptr, LanczosNextF{q.get(), qprev.get(), w.get(), ab.get(), j}, stream_);
+    }
+    ab.download(h_ab.data(), 2 * (size_t)j1, stream_);
+    CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
+    bool stop = false;
+    for (int j = j0; j < j1 && !stop; ++j) {
+      alpha.push_back(h_ab[2 * j]);
+      iters = j + 1;
+      theta = tridiag_max_eig(alpha, beta);
+      const double b2 = h_ab[2 * j + 1];
+      if (j > 2 && std::fabs(theta - prev) <= 1e-9 * theta) stop = true;  // converged
+      if (!(b2 > 1e-26 * theta * theta)) stop = true;                      // invariant subspace
+      prev = theta;
+      beta.push_back(std::sqrt(b2));
+    }
+    if (stop) break;
   }
   return theta;
 }
