@@ -309,7 +309,7 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
     Acc a = Acc::zero();
     for (int k = tid; k < nnz; k += kBlock) a.add(prod[k]);
     a = block_merge(a, acc_lds);
-    if (tid == 0) long_part[t] = a.pack();
+    if (tid == 0) long_part[tl.r1] = a.pack();  // the chunk's slot
   }
   if constexpr (Epi::NRED > 0) block_sum_store<Epi::NRED>(red, red_part + (size_t)t * Epi::NRED, red_lds);
 }

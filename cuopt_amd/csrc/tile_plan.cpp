@@ -28,22 +28,32 @@ int pick_lanes_log2(int rows, int maxlen, int block)
 
 }  // namespace
 
-TilePlan plan_tiles(const int* rowptr, int nrows, int tile_nnz, int max_rows, int block)
+TilePlan plan_tiles(const int* rowptr, int nrows, int tile_nnz, int max_rows, int block, const int* colidx, int ncols,
+                    int xcds)
 {
   TilePlan p;
-  int r = 0;
+  std::vector<int> chunk_xcd;  // per chunk tile: XCD owning its columns (-1 for row-run tiles)
+  int r     = 0;
+  int slots = 0;
   while (r < nrows) {
     const int len = rowptr[r + 1] - rowptr[r];
     if (len > tile_nnz) {
       const int nch = (len + tile_nnz - 1) / tile_nnz;
       const int lid = (int)p.longrows.size();
-      p.longrows.push_back(LongRow{r, (int)p.tiles.size(), nch, 0});
+      p.longrows.push_back(LongRow{r, slots, nch, 0});
       for (int c = 0; c < nch; ++c) {
         const int z0 = rowptr[r] + c * tile_nnz;
         const int z1 = std::min(z0 + tile_nnz, rowptr[r + 1]);
-        p.tiles.push_back(Tile{r, r + 1, z0, z1});
+        p.tiles.push_back(Tile{r, slots + c, z0, z1});
         p.meta.push_back(-(lid + 1));
+        int x = -1;
+        if (colidx && ncols > 0 && xcds > 1) {
+          const long long mid = colidx[(z0 + z1 - 1) / 2];
+          x                   = (int)std::min<long long>(xcds - 1, mid * xcds / ncols);
+        }
+        chunk_xcd.push_back(x);
       }
+      slots += nch;
       ++r;
       continue;
     }
@@ -57,8 +67,45 @@ TilePlan plan_tiles(const int* rowptr, int nrows, int tile_nnz, int max_rows, in
     }
     p.tiles.push_back(Tile{r0, r, z0, rowptr[r]});
     p.meta.push_back(pick_lanes_log2(r - r0, maxlen, block));
+    chunk_xcd.push_back(-1);
   }
-  return p;
+  if (!(colidx && ncols > 0 && xcds > 1) || p.longrows.empty()) return p;
+  // Interleave: position t prefers a chunk owned by XCD t % xcds.
+  std::vector<std::vector<int>> queue(xcds);
+  std::vector<int> free_tiles;
+  for (int t = 0; t < (int)p.tiles.size(); ++t) {
+    if (chunk_xcd[t] >= 0) queue[chunk_xcd[t]].push_back(t);
+    else free_tiles.push_back(t);
+  }
+  std::vector<size_t> head(xcds, 0);
+  size_t fh = 0;
+  std::vector<int> order;
+  order.reserve(p.tiles.size());
+  for (size_t t = 0; order.size() < p.tiles.size(); ++t) {
+    const int x = (int)(t % xcds);
+    if (head[x] < queue[x].size()) {
+      order.push_back(queue[x][head[x]++]);
+    } else if (fh < free_tiles.size()) {
+      order.push_back(free_tiles[fh++]);
+    } else {  // only other XCDs' chunks are left
+      for (int y = 1; y < xcds; ++y) {
+        const int z = (x + y) % xcds;
+        if (head[z] < queue[z].size()) {
+          order.push_back(queue[z][head[z]++]);
+          break;
+        }
+      }
+    }
+  }
+  TilePlan q;
+  q.longrows = std::move(p.longrows);
+  q.tiles.reserve(order.size());
+  q.meta.reserve(order.size());
+  for (int t : order) {
+    q.tiles.push_back(p.tiles[t]);
+    q.meta.push_back(p.meta[t]);
+  }
+  return q;
 }
 
 }  // namespace cuopt_amd

@@ -18,13 +18,13 @@
 namespace cuopt_amd {
 
 struct alignas(16) Tile {
-  int r0, r1;  // rows [r0, r1)
+  int r0, r1;  // rows [r0, r1); a chunk of a long row has r0 = the row, r1 = its partial-sum slot
   int z0, z1;  // nonzeros [z0, z1)
 };
 
 struct alignas(16) LongRow {
   int row;      // the long row
-  int first;    // index of its first chunk tile
+  int first;    // slot of its first chunk's partial sum (chunks occupy consecutive slots)
   int nchunks;  // number of chunk tiles
   int pad;
 };
@@ -38,6 +38,15 @@ struct TilePlan {
 };
 
 // Partition rows [0, nrows) of a CSR matrix with offsets rowptr.
-TilePlan plan_tiles(const int* rowptr, int nrows, int tile_nnz, int max_rows, int block);
+//
+// With colidx (column indices, sorted within each row) and ncols given, the
+// tile order is made XCD-aware: workgroups are dispatched to the XCDs round
+// robin (tile t runs on XCD t % xcds), and a chunk of a long row covers a
+// contiguous column range, so each chunk is placed on the XCD that owns the
+// 1/xcds slice of columns it gathers from; that XCD's L2 then holds its slice
+// of the dense operand.  Row-run tiles, whose columns are scattered, fill the
+// remaining positions.  The order of tiles does not change any sum.
+TilePlan plan_tiles(const int* rowptr, int nrows, int tile_nnz, int max_rows, int block, const int* colidx = nullptr,
+                    int ncols = 0, int xcds = 8);
 
 }  // namespace cuopt_amd

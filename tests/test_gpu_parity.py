@@ -124,10 +124,10 @@ def test_fp32_step_size_by_lanczos_matches_svd():
 
     At = sp.csr_matrix((s["scaled_values"], lp.col_indices, lp.row_offsets), shape=(lp.m, lp.n)).toarray()
     smax2 = np.linalg.svd(At, compute_uv=False)[0] ** 2
-    assert abs(r.info["sigma_max_sq"] - smax2) <= 1e-8 * smax2
+    assert abs(r.info["sigma_max_sq"] - smax2) <= 1e-5 * smax2
     assert r.info["sigma_max_sq"] >= s["sigma_max_sq"] * (1 - 1e-12)
     assert r.info["power_iterations"] < 100
-    assert abs(r.info["eta0"] - 0.998 / np.sqrt(smax2)) <= 1e-8 * r.info["eta0"]
+    assert abs(r.info["eta0"] - 0.998 / np.sqrt(smax2)) <= 1e-5 * r.info["eta0"]
 
 
 def test_device_transpose_unsorted_rows_and_empty_columns():
