@@ -171,6 +171,21 @@ struct LanczosNextF {  // q_{j-1} = q_j; q_{j+1} = w / beta_j (left as is once b
     q[i]     = w[i] / sqrt(ab[2 * j + 1]);
   }
 };
+struct HashStartF {  // deterministic start vector in [-1, 1) (integer hash of the index)
+  static constexpr int NRED = 0;
+  double* v;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int i, double (&)[kMaxRed]) const
+  {
+    uint32_t h = (uint32_t)i * 2654435761u + 0x9e3779b9u;
+    h ^= h >> 16;
+    h *= 0x85ebca6bu;
+    h ^= h >> 13;
+    h *= 0xc2b2ae35u;
+    h ^= h >> 16;
+    v[i] = (double)h * (2.0 / 4294967296.0) - 1.0;
+  }
+};
 struct ScaleByF {  // v *= s
   static constexpr int NRED = 0;
   double* v;
@@ -740,7 +755,7 @@ class PdlpSolver {
   void build_structures();
   void setup_scaling();
   void power_iteration();
-  double lanczos_sigma_max_sq(const std::vector<double>& z0, int& iters);
+  double lanczos_sigma_max_sq(int& iters);
   void init_iterates();
   void read_scalars()
   {
@@ -1007,17 +1022,16 @@ double tridiag_max_eig(const std::vector<double>& a, const std::vector<double>& 
   return hi;
 }
 
-// sigma_max^2 of A~ by Lanczos on A~ A~^T from the power iteration's start
-// vector.  Used by FP32-mixed solves in place of the power iteration: both
+// sigma_max^2 of A~ by Lanczos on A~ A~^T from a hashed start vector.  Used by FP32-mixed solves in place of the power iteration: both
 // estimate the same eigenvalue; Lanczos gets it to near machine precision in
 // tens of matrix products instead of hundreds.
 template <typename V>
-double PdlpSolver<V>::lanczos_sigma_max_sq(const std::vector<double>& z0, int& iters)
+double PdlpSolver<V>::lanczos_sigma_max_sq(int& iters)
 {
   constexpr int kGroup = 8;  // steps between host reads of the coefficients
   const int kmax = std::min(m_, 240);
   DevBuf<double> q(m_), qprev(m_), w(m_), zb(m_), atq(n_), ab(2 * (size_t)kmax + 2);
-  q.upload(z0, stream_);
+  launch_ew(m_, nullptr, HashStartF{q.get()}, stream_);
   launch_ew(m_, nullptr, FillF{qprev.get(), 0.0}, stream_);
   launch_ew(m_, red_.get(), SumSqF{q.get()}, stream_);
   launch_reduce(red_.get(), ew_grid(m_), 1, scal_.get() + S_SETUP0, stream_);
@@ -1061,15 +1075,15 @@ template <typename V>
 void PdlpSolver<V>::power_iteration()
 {
   // compute_initial_step_size, power-iteration branch (pdlp.cu:2622-2738).
+  if constexpr (std::is_same<V, float>::value) {
+    sigma_max_sq_ = lanczos_sigma_max_sq(power_iters_);
+    eta_          = 0.998 / std::sqrt(sigma_max_sq_);
+    return;
+  }
   std::vector<double> z(m_);
   std::mt19937 gen(1);
   std::normal_distribution<double> dist(0.0, 1.0);
   for (int i = 0; i < m_; ++i) z[i] = dist(gen);
-  if constexpr (std::is_same<V, float>::value) {
-    sigma_max_sq_ = lanczos_sigma_max_sq(z, power_iters_);
-    eta_          = 0.998 / std::sqrt(sigma_max_sq_);
-    return;
-  }
   DevBuf<double> za(m_), zb(m_), atq(n_);
   DevBuf<PowerState> st(1);
   za.upload(z, stream_);
