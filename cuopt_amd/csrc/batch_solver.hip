@@ -466,9 +466,14 @@ __global__ __launch_bounds__(kBT) void batch_pdlp_kernel(BatchArgs<V> A)
     {
       zero_red(red);
       const double idp = refine_active ? 1.0 / rdp : 1.0, idd = refine_active ? 1.0 / rdd : 1.0;
+      // xu carries the gathered primal point until the column pass writes x there.
+      const KktPointF<V> point{xhat, Dc, 1.0 / beta, refine_active ? xstar : nullptr, idp, xu};
+      double unused[kMaxRed];
+      for (int j = tid; j < n; j += kBT) point(j, unused);
+      __syncthreads();
       rows_pass(aptr, aidx, aorig, m, d.lga,
-                KktRowsEpi<V>{xhat, Dc, 1.0 / beta, yhat, Dr, 1.0 / gamma, gamma, cb_orig, yanc, yu,
-                              refine_active ? xstar : nullptr, refine_active ? ystar : nullptr, idp, idd},
+                KktRowsEpi<V>{xu, yhat, Dr, 1.0 / gamma, gamma, cb_orig, yanc, yu, refine_active ? xstar : nullptr,
+                              refine_active ? ystar : nullptr, idp, idd},
                 red);
       wg_sum<3>(red, lds);
       const double rp2 = red[0], psum = red[1], dd = red[2];  // dd: ||y0 - yhat||^2

@@ -207,6 +207,27 @@ struct FpeEpi {
   }
 };
 
+// Unscaled primal point x = Dc x-hat / beta (recentred: x* + that / dp), one
+// FP64 value per column, gathered by the row pass of the KKT evaluation: one
+// random 8-byte read per nonzero instead of three (x-hat, Dc, x*).  The same
+// expression, evaluated in the same order, as the column pass's own x.
+template <typename V>
+struct KktPointF {
+  static constexpr int NRED = 0;
+  const V* xhat;
+  const double* Dc;
+  double inv_beta;
+  const double* xstar;
+  double inv_dp;
+  double* xu;
+  __device__ bool skip() const { return false; }
+  __device__ void operator()(int j, double (&)[kMaxRed]) const
+  {
+    const double xs = ((double)xhat[j] * Dc[j]) * inv_beta;
+    xu[j]           = xstar ? xstar[j] + xs * inv_dp : xs;
+  }
+};
+
 // ---- termination evaluation on the unscaled problem (FP64 matrix).
 // Rows i of A: gathers unscaled x-hat on the fly, owns y-hat.
 // red: 0 ||Ax - clamp(Ax)||^2, 1 sum of primal slack terms, 2 ||y0 - yhat||^2 (rescaled)
@@ -215,9 +236,7 @@ struct KktRowsEpi {
   using Mat = double;
   using Acc = AccSum64;
   static constexpr int NRED = 3;
-  const V* xhat;     // gathered (scaled)
-  const double* Dc;  // gathered scaling
-  double inv_beta;
+  const double* xu;  // gathered: the unscaled primal point (KktPointF)
   V* yhat;
   const double* Dr;
   double inv_gamma, gamma;
@@ -232,11 +251,7 @@ struct KktRowsEpi {
   const double* ystar;
   double inv_dp, inv_dd;
   __device__ bool skip() const { return false; }
-  __device__ double product(double v, int col, int) const
-  {
-    const double xs = ((double)xhat[col] * Dc[col]) * inv_beta;
-    return v * (xstar ? xstar[col] + xs * inv_dp : xs);
-  }
+  __device__ double product(double v, int col, int) const { return v * xu[col]; }
   struct Pre {
     double dr;
     double2 b;

@@ -1165,9 +1165,11 @@ void PdlpSolver<V>::evaluate_kkt()
   const double* xs = rf ? xstar_.get() : nullptr;
   const double* ys = rf ? ystar_.get() : nullptr;
   const double idp = rf ? 1.0 / refine_.dp : 1.0, idd = rf ? 1.0 / refine_.dd : 1.0;
+  // xu_ holds the gathered point until the column pass writes the final x there.
+  launch_ew(n_, nullptr, KktPointF<V>{xhat_.get(), Dc_.get(), ib, xs, idp, xu_.get()}, stream_);
   launch_tiled(A_, a_orig_.get(), red_.get(),
-               KktRowsEpi<V>{xhat_.get(), Dc_.get(), ib, yhat_.get(), Dr_.get(), ig, gamma_, cb_orig_.get(),
-                             y0_.get(), yu_.get(), xs, ys, idp, idd},
+               KktRowsEpi<V>{xu_.get(), yhat_.get(), Dr_.get(), ig, gamma_, cb_orig_.get(), y0_.get(), yu_.get(), xs,
+                             ys, idp, idd},
                stream_);
   launch_reduce(red_.get(), A_.plan.red_blocks(), 3, scal_.get() + S_KKT_RP2, stream_);
   launch_tiled(AT_, at_orig_.get(), red_.get(),

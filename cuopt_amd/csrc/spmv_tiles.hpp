@@ -330,13 +330,15 @@ __global__ __launch_bounds__(kBlock) void long_rows_kernel(PlanDev plan,
   __shared__ uint64_t acc_lds[kBlock / kWave];
   if (epi.skip()) return;
   const LongRow lr = plan.longrows[blockIdx.x];
-  Acc tot          = Acc::zero();
+  typename Epi::Pre pre{};
+  if (threadIdx.x == 0) pre = epi.load(lr.row);  // in flight during the merge
+  Acc tot = Acc::zero();
   for (int ch = threadIdx.x; ch < lr.nchunks; ch += kBlock) tot.merge(Acc::unpack(long_part[lr.first + ch]));
   tot = block_merge(tot, acc_lds);
   double red[kMaxRed];
 #pragma unroll
   for (int k = 0; k < kMaxRed; ++k) red[k] = 0.0;
-  if (threadIdx.x == 0) epi.row(lr.row, tot, epi.load(lr.row), red);
+  if (threadIdx.x == 0) epi.row(lr.row, tot, pre, red);
   if constexpr (Epi::NRED > 0)
     block_sum_store<Epi::NRED>(red, red_part + (size_t)(plan.ntiles + blockIdx.x) * Epi::NRED, red_lds);
 }
