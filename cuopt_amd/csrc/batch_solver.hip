@@ -45,6 +45,8 @@ struct BatchDesc {
   int row0, col0;     // offsets in m- and n-sized arrays
   int aptr0, tptr0;   // offsets of the row-pointer arrays of A (m+1) and A^T (n+1)
   int lga, lgt;       // log2 lanes per row for A and A^T
+  int ablk0, nablk;   // row blocks of A: start rows at blk[ablk0 .. ablk0 + nablk]
+  int tblk0, ntblk;   // row blocks of A^T
   long long nz0;      // offset in nonzero arrays
   double obj_sf, obj_off;
 };
@@ -95,6 +97,8 @@ struct BatchArgs {
   double tol_ap, tol_rp, tol_ad, tol_rd, tol_ag, tol_rg;
   long long iter_limit;
   unsigned long long tick_limit;  // wall-clock ticks from kernel start (0 = none)
+  const int* blk;                 // row-block starts of every LP's A and A^T (plan_row_blocks)
+  int lds_bytes;                  // dynamic LDS of the launch: the stage, then the gathered vectors
 };
 
 constexpr int kMajorB = 200;
@@ -118,6 +122,17 @@ __device__ inline bool check_at(long long k)
 
 // ---- workgroup reductions (fixed order) -------------------------------------
 
+// x holds the same value in every lane: say so to the compiler, which then keeps
+// it (and what is computed from it: the restart and termination scalars) in
+// scalar registers instead of one vector register pair per value.
+__device__ inline double uniform_d(double x)
+{
+  const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b & 0xffffffffu));
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 template <int NR>
 __device__ inline void wg_sum(double (&v)[kMaxRed], double (*lds)[kMaxRed])
 {
@@ -134,7 +149,7 @@ __device__ inline void wg_sum(double (&v)[kMaxRed], double (*lds)[kMaxRed])
   for (int k = 0; k < NR; ++k) {
     double s = lds[0][k];
     for (int w = 1; w < kBT / kWave; ++w) s += lds[w][k];
-    v[k] = s;
+    v[k] = uniform_d(s);
   }
   __syncthreads();
 }
@@ -147,26 +162,158 @@ __device__ inline void zero_red(double (&r)[kMaxRed])
 
 // ---- row passes ---------------------------------------------------------------
 
+// Row passes walk an LP's CSR in row blocks planned on the host (plan_row_blocks):
+// a block holds at most kStage nonzeros and at most one row per lane group, or
+// is a single row longer than kStage.  For each block the workgroup
+//   1. copies the block's indices / values into LDS with global_load_lds (the
+//      data goes from memory to LDS without passing through registers) while
+//      each lane group loads the row bounds and epilogue operands of its row,
+//   2. lets each group of G lanes reduce its row from LDS and run the epilogue.
+// A block thus costs about one memory round trip, where a row-at-a-time walk
+// paid one per row step.  Each lane adds its row's products in increasing k and
+// groups merge by the same shuffle tree as before, so the sums do not depend on
+// the blocking.
+constexpr int kStage      = 2048;
+constexpr int kStageBytes = kStage * 12;  // values (8 bytes) then indices (4 bytes)
+
+struct Stage {               // LDS of one workgroup
+  int* idx;                  // kStage column indices
+  unsigned char* vals;       // kStage values, 8 bytes each (FP64 or FP32 matrices)
+  uint64_t* merge;           // kBT / kWave accumulator slots
+};
+
+struct RowBlocks {
+  const int* start;  // first row of each block; start[nblk] = number of rows
+  int nblk;
+};
+
+// Lanes per row and row blocks of one matrix (rows given by their lengths).
+// G keeps a row's share of a lane to about 16 nonzeros read from LDS.
+int plan_lanes(int nrows, long long nnz)
+{
+  const double avg = nrows > 0 ? (double)nnz / nrows : 0.0;
+  int lg           = 0;
+  while (lg < 6 && (double)(16 << lg) < avg) ++lg;
+  return lg;
+}
+
+template <class Len>
+void plan_row_blocks(int nrows, int lg, Len len, std::vector<int>& out)
+{
+  const int max_rows = kBT >> lg;
+  int r              = 0;
+  out.push_back(0);
+  while (r < nrows) {
+    long long z = len(r);
+    int e       = r + 1;
+    if (z <= kStage)
+      while (e < nrows && e - r < max_rows && z + len(e) <= kStage) z += len(e++);
+    out.push_back(e);
+    r = e;
+  }
+}
+
+// Copies the dwords [d0, d1) of src to dst[0, d1 - d0) in LDS: wave w moves
+// elements w*64 + i*256 + lane, and global_load_lds writes lane l's dword to the
+// wave-uniform LDS address given plus 4 l.  The caller's barrier waits for it.
+__device__ inline void copy_to_lds(const int* __restrict__ src, long long d0, long long d1, int* dst)
+{
+  const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+  const int cnt  = (int)(d1 - d0);
+  for (int base = w * kWave; base < cnt; base += kBT)
+    if (base + lane < cnt)
+      __builtin_amdgcn_global_load_lds(src + d0 + base + lane, dst + base, 4, 0, 0);
+}
+
+// Fixed-order workgroup merge of one accumulator (wave butterflies, then waves 0..3).
+template <class Acc>
+__device__ inline Acc wg_merge(Acc a, uint64_t* slots)
+{
+  const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) a.merge(a.shfl(off));
+  if (lane == 0) slots[wid] = a.pack();
+  __syncthreads();
+  Acc r = Acc::unpack(slots[0]);
+  for (int w = 1; w < kBT / kWave; ++w) r.merge(Acc::unpack(slots[w]));
+  __syncthreads();
+  return r;
+}
+
 // Rows [0, nrows) of one LP's CSR; Epi as in spmv_tiles.hpp, with the row index
 // passed as the third argument of product().
 template <class Epi>
 __device__ inline void rows_pass(const int* __restrict__ ptr,
                                  const int* __restrict__ idx,
                                  const typename Epi::Mat* __restrict__ vals,
-                                 int nrows,
                                  int lg,
+                                 RowBlocks blocks,
+                                 const Stage& st,
                                  const Epi& epi,
                                  double (&red)[kMaxRed])
 {
-  using Acc   = typename Epi::Acc;
-  const int G = 1 << lg, sub = threadIdx.x & (G - 1);
-  for (int r = threadIdx.x >> lg; r < nrows; r += kBT >> lg) {
-    const int b = ptr[r], e = ptr[r + 1];
-    const auto pre = epi.load(r);
-    Acc a = Acc::zero();
-    for (int k = b + sub; k < e; k += G) a.add(epi.product(vals[k], idx[k], r));
-    for (int off = G >> 1; off > 0; off >>= 1) a.merge(a.shfl(off));
-    if (sub == 0) epi.row(r, a, pre, red);
+  using Acc       = typename Epi::Acc;
+  using Mat       = typename Epi::Mat;
+  constexpr int W = (int)(sizeof(Mat) / 4);  // dwords per value
+  Mat* svals      = reinterpret_cast<Mat*>(st.vals);
+  const int G = 1 << lg, sub = threadIdx.x & (G - 1), grp = threadIdx.x >> lg;
+  if (blocks.nblk <= 0) return;
+  // Block bounds run one block ahead: the loads for block bi + 1 are issued
+  // while block bi is processed.
+  int r0 = blocks.start[0], r1 = blocks.start[1];
+  int z0 = ptr[r0], z1 = ptr[r1];
+  for (int bi = 0; bi < blocks.nblk; ++bi) {
+    int r2 = r1, z2 = z1;
+    if (bi + 2 <= blocks.nblk) {
+      r2 = blocks.start[bi + 2];
+      z2 = ptr[r2];
+    }
+    if (z1 - z0 > kStage) {  // one row longer than the stage: read it in place
+      const auto pre = epi.load(r0);
+      Acc a          = Acc::zero();
+      for (int k = z0 + (int)threadIdx.x; k < z1; k += kBT) a.add(epi.product(vals[k], idx[k], r0));
+      a = wg_merge(a, st.merge);
+      if (threadIdx.x == 0) epi.row(r0, a, pre, red);
+      r0 = r1;
+      z0 = z1;
+      r1 = r2;
+      z1 = z2;
+      continue;
+    }
+    const int r = r0 + grp;
+    int b = 0, e = 0;
+    typename Epi::Pre pre{};
+    if (r < r1) {
+      b   = ptr[r] - z0;
+      e   = ptr[r + 1] - z0;
+      pre = epi.load(r);
+    }
+    copy_to_lds(reinterpret_cast<const int*>(vals), (long long)W * z0, (long long)W * z1,
+                reinterpret_cast<int*>(st.vals));
+    copy_to_lds(idx, z0, z1, st.idx);
+    __syncthreads();
+    if (r < r1) {
+      Acc a = Acc::zero();
+      int k = b + sub;
+      for (; k + 3 * G < e; k += 4 * G) {  // four LDS reads in flight, added in order
+        const int c0 = st.idx[k], c1 = st.idx[k + G], c2 = st.idx[k + 2 * G], c3 = st.idx[k + 3 * G];
+        const Mat v0 = svals[k], v1 = svals[k + G], v2 = svals[k + 2 * G], v3 = svals[k + 3 * G];
+        const auto p0 = epi.product(v0, c0, r), p1 = epi.product(v1, c1, r);
+        const auto p2 = epi.product(v2, c2, r), p3 = epi.product(v3, c3, r);
+        a.add(p0);
+        a.add(p1);
+        a.add(p2);
+        a.add(p3);
+      }
+      for (; k < e; k += G) a.add(epi.product(svals[k], st.idx[k], r));
+      for (int off = G >> 1; off > 0; off >>= 1) a.merge(a.shfl(off));
+      if (sub == 0) epi.row(r, a, pre, red);
+    }
+    __syncthreads();
+    r0 = r1;
+    z0 = z1;
+    r1 = r2;
+    z1 = z2;
   }
 }
 
@@ -256,6 +403,8 @@ __global__ __launch_bounds__(kBT) void batch_pdlp_kernel(BatchArgs<V> A)
   __shared__ double lds[kBT / kWave][kMaxRed];
   __shared__ StepScalars sc;
   __shared__ int scan_lds[kBT];
+  __shared__ uint64_t merge_lds[kBT / kWave];
+  extern __shared__ __align__(16) unsigned char dyn_lds[];
   const unsigned long long t_start = wall_clock64();
   const int tid = threadIdx.x;
   const BatchDesc d = A.desc[A.order[blockIdx.x]];
@@ -281,6 +430,18 @@ __global__ __launch_bounds__(kBT) void batch_pdlp_kernel(BatchArgs<V> A)
   V* xs[2] = {A.x0 + d.col0, A.x1 + d.col0};
   V* ys[2] = {A.y0v + d.row0, A.y1 + d.row0};
   V *xanc = A.xa + d.col0, *yanc = A.ya + d.row0, *xbar = A.xbar + d.col0, *ybar = A.ybar + d.row0;
+  // The two vectors the PDHG step gathers -- y (primal step, both buffers) and
+  // x-bar (dual step) -- live in LDS when the LP fits: each gather is then an
+  // LDS read instead of a round trip to L2 / MALL.  Nothing reads them after
+  // the kernel (the solution is the unscaled xu / yu / su).
+  const Stage st{reinterpret_cast<int*>(dyn_lds + kStage * 8), dyn_lds, merge_lds};
+  const RowBlocks ablk{A.blk + d.ablk0, d.nablk}, tblk{A.blk + d.tblk0, d.ntblk};
+  if (kStageBytes + (long long)(2 * m + n) * (long long)sizeof(V) <= (long long)A.lds_bytes) {
+    V* base = reinterpret_cast<V*>(dyn_lds + kStageBytes);
+    ys[0]   = base;
+    ys[1]   = base + m;
+    xbar    = base + 2 * m;
+  }
   V *xhat = A.xhat + d.col0, *yhat = A.yhat + d.row0, *slack = A.slack + d.col0, *aty = A.aty + d.col0;
   double *xu = A.xu + d.col0, *yu = A.yu + d.row0, *su = A.su + d.col0;
   double *pz = A.pz + d.row0, *pz2 = A.pz2 + d.row0, *patq = A.patq + d.col0;
@@ -355,11 +516,11 @@ __global__ __launch_bounds__(kBT) void batch_pdlp_kernel(BatchArgs<V> A)
     const bool pc = it == 10;
     zero_red(red);
     if (pc) {
-      rows_pass(aptr, aidx, aorig, m, d.lga, BScaleEpi<false, false>{Dr, Dc, rowv}, red);
-      rows_pass(tptr, tidx, torig, n, d.lgt, BScaleEpi<true, false>{Dr, Dc, colv}, red);
+      rows_pass(aptr, aidx, aorig, d.lga, ablk, st, BScaleEpi<false, false>{Dr, Dc, rowv}, red);
+      rows_pass(tptr, tidx, torig, d.lgt, tblk, st, BScaleEpi<true, false>{Dr, Dc, colv}, red);
     } else {
-      rows_pass(aptr, aidx, aorig, m, d.lga, BScaleEpi<false, true>{Dr, Dc, rowv}, red);
-      rows_pass(tptr, tidx, torig, n, d.lgt, BScaleEpi<true, true>{Dr, Dc, colv}, red);
+      rows_pass(aptr, aidx, aorig, d.lga, ablk, st, BScaleEpi<false, true>{Dr, Dc, rowv}, red);
+      rows_pass(tptr, tidx, torig, d.lgt, tblk, st, BScaleEpi<true, true>{Dr, Dc, colv}, red);
     }
     __syncthreads();
     for (int i = tid; i < m; i += kBT) Dr[i] = rowv[i] > 0.0 ? Dr[i] / sqrt(rowv[i]) : Dr[i];
@@ -414,9 +575,9 @@ __global__ __launch_bounds__(kBT) void batch_pdlp_kernel(BatchArgs<V> A)
   double* zn = pz2;
   while (piters < 5000) {
     zero_red(red);
-    rows_pass(tptr, tidx, torig, n, d.lgt, BPowerAtEpi{Dr, Dc, zo, nrm, patq}, red);
+    rows_pass(tptr, tidx, torig, d.lgt, tblk, st, BPowerAtEpi{Dr, Dc, zo, nrm, patq}, red);
     __syncthreads();
-    rows_pass(aptr, aidx, aorig, m, d.lga, BPowerAEpi{Dr, Dc, patq, zo, nrm, zn}, red);
+    rows_pass(aptr, aidx, aorig, d.lga, ablk, st, BPowerAEpi{Dr, Dc, patq, zo, nrm, zn}, red);
     wg_sum<1>(red, lds);
     s2 = red[0];
     zero_red(red);
@@ -471,14 +632,14 @@ __global__ __launch_bounds__(kBT) void batch_pdlp_kernel(BatchArgs<V> A)
       double unused[kMaxRed];
       for (int j = tid; j < n; j += kBT) point(j, unused);
       __syncthreads();
-      rows_pass(aptr, aidx, aorig, m, d.lga,
+      rows_pass(aptr, aidx, aorig, d.lga, ablk, st,
                 KktRowsEpi<V>{xu, yhat, Dr, 1.0 / gamma, gamma, cb_orig, yanc, yu, refine_active ? xstar : nullptr,
                               refine_active ? ystar : nullptr, idp, idd},
                 red);
       wg_sum<3>(red, lds);
       const double rp2 = red[0], psum = red[1], dd = red[2];  // dd: ||y0 - yhat||^2
       zero_red(red);
-      rows_pass(tptr, tidx, torig, n, d.lgt,
+      rows_pass(tptr, tidx, torig, d.lgt, tblk, st,
                 KktColsEpi<V>{yu, xhat, slack, Dc, 1.0 / beta, beta, 1.0 / gamma, gamma, c, xanc, xu, su,
                               refine_active ? xstar : nullptr, idp, idd, bnd_w, lbv, ubv},
                 red);
@@ -584,8 +745,8 @@ __global__ __launch_bounds__(kBT) void batch_pdlp_kernel(BatchArgs<V> A)
             for (int i = tid; i < m; i += kBT) ystar[i] = yu[i];
             __syncthreads();
             double rr[kMaxRed];
-            rows_pass(aptr, aidx, aorig, m, d.lga, BPlainEpi{xstar, ax64}, rr);
-            rows_pass(tptr, tidx, torig, n, d.lgt, BPlainEpi{ystar, aty64}, rr);
+            rows_pass(aptr, aidx, aorig, d.lga, ablk, st, BPlainEpi{xstar, ax64}, rr);
+            rows_pass(tptr, tidx, torig, d.lgt, tblk, st, BPlainEpi{ystar, aty64}, rr);
             __syncthreads();
             for (int j = tid; j < n; j += kBT) {
               const double dc = Dc[j], x = xstar[j];
@@ -650,12 +811,12 @@ __global__ __launch_bounds__(kBT) void batch_pdlp_kernel(BatchArgs<V> A)
         const int jj = (int)kr;
         zero_red(red);
 #define CUOPT_BPRIMAL(MAJ, FP)                                                                              \
-  rows_pass(tptr, tidx, tw, n, d.lgt,                                                                        \
+  rows_pass(tptr, tidx, tw, d.lgt, tblk, st,                                                                        \
             PrimalStepEpi<V, MAJ, FP>{ys[ci], xs[ci], c_w, bnd_w, xanc, xbar, xs[nx], xhat, slack, aty, &sc, jj,    \
                                       refine_active},                                                         \
             red)
 #define CUOPT_BDUAL(MAJ, FP)                                                                                 \
-  rows_pass(aptr, aidx, aw, m, d.lga,                                                                         \
+  rows_pass(aptr, aidx, aw, d.lga, ablk, st,                                                                         \
             DualStepEpi<V, MAJ, FP>{xbar, ys[ci], cb_w, yanc, ys[nx], yhat, ybar, &sc, jj,                         \
                                     refine_active ? (const V*)kink : nullptr},                                  \
             red)
@@ -680,7 +841,7 @@ __global__ __launch_bounds__(kBT) void batch_pdlp_kernel(BatchArgs<V> A)
           wg_sum<1>(red, lds);
           const double ny = red[0];
           zero_red(red);
-          rows_pass(tptr, tidx, tw, n, d.lgt, FpeEpi<V>{ybar, aty, xbar, xs[ci]}, red);
+          rows_pass(tptr, tidx, tw, d.lgt, tblk, st, FpeEpi<V>{ybar, aty, xbar, xs[ci]}, red);
           wg_sum<2>(red, lds);
           if (post_restart) {
             fpe0_t[0] = ny;
@@ -727,14 +888,6 @@ __global__ __launch_bounds__(kBT) void batch_pdlp_kernel(BatchArgs<V> A)
   }
 }
 
-int lanes_log2(int nrows, long long nnz)
-{
-  const double avg = nrows > 0 ? (double)nnz / nrows : 0.0;
-  int lg           = 0;
-  while (lg < 6 && (double)(4 << lg) <= avg) ++lg;  // about two nonzeros per lane
-  return lg;
-}
-
 template <typename V>
 std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
                                    const std::vector<int>& active,
@@ -760,8 +913,8 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
     d.aptr0 = tot_ap;
     d.tptr0 = tot_tp;
     d.nz0 = tot_nnz;
-    d.lga = lanes_log2(lp.m, d.nnz);
-    d.lgt = lanes_log2(lp.n, d.nnz);
+    d.lga = plan_lanes(lp.m, d.nnz);
+    d.lgt = plan_lanes(lp.n, d.nnz);
     d.obj_sf  = lp.maximize ? -1.0 : 1.0;
     d.obj_off = lp.maximize ? -lp.offset : lp.offset;
     tot_m += lp.m;
@@ -810,6 +963,36 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
     for (int t = 0; t < nt; ++t) th.emplace_back(fill, t, nt);
     for (auto& t : th) t.join();
   }
+  // row blocks of A and A^T for each LP
+  std::vector<std::vector<int>> ablocks(lps.size()), tblocks(lps.size());
+  {
+    auto plan = [&](int t, int nt) {
+      std::vector<int> cnt;
+      for (int q = t; q < nb; q += nt) {
+        const int a      = active[q];
+        const LpHost& lp = *lps[a];
+        plan_row_blocks(lp.m, desc[a].lga, [&](int r) { return lp.rowptr[r + 1] - lp.rowptr[r]; }, ablocks[a]);
+        cnt.assign(lp.n, 0);
+        for (int k = 0; k < desc[a].nnz; ++k) ++cnt[lp.colidx[k]];
+        plan_row_blocks(lp.n, desc[a].lgt, [&](int j) { return cnt[j]; }, tblocks[a]);
+      }
+    };
+    const int nt = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t) th.emplace_back(plan, t, nt);
+    for (auto& t : th) t.join();
+  }
+  std::vector<int> h_blk;
+  for (int a : active) {
+    BatchDesc& d = desc[a];
+    d.ablk0      = (int)h_blk.size();
+    d.nablk      = (int)ablocks[a].size() - 1;
+    h_blk.insert(h_blk.end(), ablocks[a].begin(), ablocks[a].end());
+    d.tblk0 = (int)h_blk.size();
+    d.ntblk = (int)tblocks[a].size() - 1;
+    h_blk.insert(h_blk.end(), tblocks[a].begin(), tblocks[a].end());
+  }
+
   // processing order: largest first
   std::vector<int> order(active);
   std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return desc[x].nnz > desc[y].nnz; });
@@ -828,6 +1011,7 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
   } guard{s};
   const size_t M = std::max(tot_m, 1), N = std::max(tot_n, 1), Z = std::max<long long>(tot_nnz, 1);
   DevBuf<BatchDesc> d_desc(std::max<size_t>(desc.size(), 1));
+  DevBuf<int> d_blk(std::max<size_t>(h_blk.size(), 1));
   DevBuf<int> d_order(std::max(nb, 1)), a_ptr(std::max(tot_ap, 1)), a_idx(Z), t_ptr(std::max(tot_tp, 1)),
     t_idx(Z), t_pos(Z), t_fill(N);
   DevBuf<double> a_orig(Z), t_orig(Z), c(N), lb(N), ub(N), lc(M), uc(M), Dr(M), Dc(N), rowv(M), colv(N);
@@ -841,6 +1025,7 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
   DevBuf<V> kink(refine ? M : 1);
   DevBuf<BatchOut> out(std::max<size_t>(lps.size(), 1));
   d_desc.upload(desc, s);
+  if (!h_blk.empty()) d_blk.upload(h_blk, s);
   d_order.upload(order, s);
   a_ptr.upload(h_ptr, s);
   a_idx.upload(h_idx, s);
@@ -854,6 +1039,7 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
 
   BatchArgs<V> args{};
   args.desc = d_desc.get();
+  args.blk  = d_blk.get();
   args.order = d_order.get();
   args.a_ptr = a_ptr.get();
   args.a_idx = a_idx.get();
@@ -919,8 +1105,22 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
     const double ticks = std::max(1.0, st.time_limit * 1e3 * (double)khz);
     args.tick_limit    = ticks >= 1.8e19 ? 0ULL : (unsigned long long)ticks;
   }
+  {
+    // LDS: the row-block stage, then the gathered vectors of the largest LP when
+    // they fit in the per-workgroup limit and half a CU's LDS (two workgroups per CU).
+    int dev = 0, per_block = 0, per_cu = 0;
+    CUOPT_HIP_CHECK(hipGetDevice(&dev));
+    CUOPT_HIP_CHECK(hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    CUOPT_HIP_CHECK(hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
+    const long long static_lds = 4096;  // the kernel's own __shared__ arrays, rounded up
+    long long cap = std::min<long long>(per_block, per_cu / 2) - static_lds;
+    long long need = 0;
+    for (int a : active) need = std::max<long long>(need, (2LL * desc[a].m + desc[a].n) * (long long)sizeof(V));
+    if (cap < kStageBytes) throw std::runtime_error("batch solve: the device has too little LDS per workgroup");
+    args.lds_bytes = (int)std::min<long long>(kStageBytes + need, cap);
+  }
   if (nb > 0) {
-    hipLaunchKernelGGL(batch_pdlp_kernel<V>, dim3(nb), dim3(kBT), 0, s, args);
+    hipLaunchKernelGGL(batch_pdlp_kernel<V>, dim3(nb), dim3(kBT), (unsigned)args.lds_bytes, s, args);
     CUOPT_HIP_CHECK(hipGetLastError());
   }
   std::vector<BatchOut> h_out(std::max<size_t>(lps.size(), 1));
