@@ -197,6 +197,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-simplex", action="store_true")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--batch-sequential-sample", type=int, default=64)
+    ap.add_argument("--only", choices=("transport", "batch", "unrolled"),
+                    help="run just this extra workload and print its result (for profiling)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -213,6 +215,12 @@ def main() -> int:
         return 1
     if args.copy_calibration_mb:
         copy_calibration(args.copy_calibration_mb)
+    extra_fns = {"transport": lambda: transport_extras(c_api, G, 120.0, not args.no_cpu_simplex),
+                 "batch": lambda: batch_extras(c_api, G, args.batch_count, args.batch_sequential_sample, 120.0),
+                 "unrolled": lambda: unrolled_extras(G, 2000, 64)}
+    if args.only:
+        print(json.dumps({"workload": args.only, "result": extra_fns[args.only]()}))
+        return 0
     t = time.time()
     lp = G.skewed_lp(m=500_000, n=1_000_000, seed=0)
     gen_s = time.time() - t
@@ -275,10 +283,7 @@ def main() -> int:
 
     extras = {}
     if not args.no_extras:
-        for name, fn in (("transport", lambda: transport_extras(c_api, G, 120.0, not args.no_cpu_simplex)),
-                         ("batch", lambda: batch_extras(c_api, G, args.batch_count, args.batch_sequential_sample,
-                                                        120.0)),
-                         ("unrolled", lambda: unrolled_extras(G, 2000, 64))):
+        for name, fn in extra_fns.items():
             try:
                 extras[name] = fn()
             except Exception as e:  # an extra workload never hides the headline line

@@ -135,9 +135,10 @@ class Problem:
         c, lb, ub = _a(lp.c, np.float64), _a(lp.var_lb, np.float64), _a(lp.var_ub, np.float64)
         vt = np.frombuffer(b"C" * lp.n, dtype=np.uint8).copy() if lp.n else np.zeros(1, np.uint8)
         sense = CUOPT_MAXIMIZE if lp.maximize else CUOPT_MINIMIZE
+        sr = None
         if ranged is None:
             try:
-                lp.row_sense()
+                sr = lp.row_sense()
                 ranged = False
             except ValueError:
                 ranged = True
@@ -146,7 +147,7 @@ class Problem:
             rc = L.cuOptCreateRangedProblem(lp.m, lp.n, sense, lp.offset, _p(c), _p(ptr), _p(idx), _p(val), _p(lc),
                                             _p(uc), _p(lb), _p(ub), _p(vt), ctypes.byref(self.handle))
         else:
-            s, rhs = lp.row_sense()
+            s, rhs = sr if sr is not None else lp.row_sense()
             s = np.frombuffer(s.tobytes(), dtype=np.uint8).copy() if lp.m else np.zeros(1, np.uint8)
             rhs = _a(rhs, np.float64) if lp.m else np.zeros(1)
             rc = L.cuOptCreateProblem(lp.m, lp.n, sense, lp.offset, _p(c), _p(ptr), _p(idx), _p(val), _p(s), _p(rhs),

@@ -55,9 +55,41 @@ __device__ inline typename AccFor<V>::type row_dot(const int* __restrict__ ptr,
                                                    int b,
                                                    int B)
 {
-  using Acc = typename AccFor<V>::type;
-  Acc a     = Acc::zero();
-  for (int k = ptr[r]; k < ptr[r + 1]; ++k) a.add(Acc::prod(val[k], vec[(size_t)idx[k] * B + b]));
+  using Acc   = typename AccFor<V>::type;
+  Acc a       = Acc::zero();
+  const int e = ptr[r + 1];
+  int k       = ptr[r];
+  // Eight nonzeros at a time: their (wave-uniform) indices and values, then
+  // their eight operand loads, are all in flight before the first product is
+  // added; the products are still added in increasing k.
+  for (; k + 8 <= e; k += 8) {
+    int c[8];
+    V v[8], x[8];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      c[q] = idx[k + q];
+      v[q] = val[k + q];
+    }
+#pragma unroll
+    for (int q = 0; q < 8; ++q) x[q] = vec[(size_t)c[q] * B + b];
+#pragma unroll
+    for (int q = 0; q < 8; ++q) a.add(Acc::prod(v[q], x[q]));
+  }
+  if (k + 4 <= e) {
+    int c[4];
+    V v[4], x[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      c[q] = idx[k + q];
+      v[q] = val[k + q];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) x[q] = vec[(size_t)c[q] * B + b];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) a.add(Acc::prod(v[q], x[q]));
+    k += 4;
+  }
+  for (; k < e; ++k) a.add(Acc::prod(val[k], vec[(size_t)idx[k] * B + b]));
   return a;
 }
 
