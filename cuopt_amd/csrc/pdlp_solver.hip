@@ -1022,9 +1022,10 @@ double tridiag_max_eig(const std::vector<double>& a, const std::vector<double>& 
   return hi;
 }
 
-// sigma_max^2 of A~ by Lanczos on A~ A~^T from a hashed start vector.  Used by FP32-mixed solves in place of the power iteration: both
-// estimate the same eigenvalue; Lanczos gets it to near machine precision in
-// tens of matrix products instead of hundreds.
+// sigma_max^2 of A~ by Lanczos on A~ A~^T from a hashed start vector.  Used by
+// FP32-mixed solves in place of the power iteration: both estimate the same
+// eigenvalue; Lanczos gets it to near machine precision in tens of matrix
+// products instead of hundreds.
 template <typename V>
 double PdlpSolver<V>::lanczos_sigma_max_sq(int& iters)
 {

@@ -390,6 +390,41 @@ def test_batch_edge_cases_keep_order():
     assert np.array_equal(rs[0].x, rs[3].x)
 
 
+def _with_dense_row_and_column(lp, seed):
+    """lp plus one never-binding dense <= row and one dense column (a new variable in
+    [0, 10] with an entry in every row): rows and columns longer than the batch
+    kernel's 2 048-nonzero row blocks."""
+    import scipy.sparse as sp
+
+    rng = np.random.default_rng(seed)
+    A = sp.csr_matrix((lp.values, lp.col_indices, lp.row_offsets), shape=(lp.m, lp.n))
+    col = rng.standard_normal((lp.m, 1))
+    A = sp.hstack([A, sp.csr_matrix(col)]).tocsr()
+    row = rng.standard_normal(lp.n + 1)
+    A = sp.vstack([A, sp.csr_matrix(row)]).tocsr()
+    A.sort_indices()
+    ub = np.append(lp.var_ub, 10.0)
+    return G.LP(m=lp.m + 1, n=lp.n + 1, row_offsets=A.indptr.astype(np.int32), col_indices=A.indices.astype(np.int32),
+                values=A.data.astype(np.float64), c=np.append(lp.c, rng.uniform(-1.0, 1.0)),
+                con_lb=np.append(lp.con_lb, -np.inf), con_ub=np.append(lp.con_ub, np.abs(row) @ ub + 1.0),
+                var_lb=np.append(lp.var_lb, 0.0), var_ub=ub, name=lp.name + "+dense")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
+def test_batch_rows_longer_than_a_block(precision):
+    # A row of 6 001 and a column of 2 501 nonzeros take the batch kernel's
+    # in-place path (longer than its 2 048-nonzero row blocks); the single-LP
+    # solver is the reference.
+    lps = [_with_dense_row_and_column(G.random_lp(m=2500, n=6000, seed=11), 1),
+           G.random_lp(m=80, n=200, seed=12)]
+    assert np.diff(lps[0].row_offsets).max() == 6001
+    rs = c_api.batch_solve(lps, tol=1e-6, precision=precision, time_limit=TL, iteration_limit=500000)
+    for lp, r in zip(lps, rs):
+        s = c_api.solve(lp, tol=1e-6, precision=precision, time_limit=TL, iteration_limit=500000)
+        assert r.status == s.status == 1, (lp.name, r.status, s.status)
+        assert rel(r.primal_objective, s.primal_objective) < 1e-5, lp.name
+
+
 def test_batch_iteration_limit():
     lps = [G.random_lp(m=100, n=250, seed=s) for s in range(4)]
     rs = c_api.batch_solve(lps, tol=0.0, time_limit=TL, iteration_limit=50)
