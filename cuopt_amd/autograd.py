@@ -39,6 +39,7 @@ def _lib():
         L.cuOptAmdUnrolledStepSize.argtypes = [vp, ctypes.POINTER(f64)]
         L.cuOptAmdUnrolledForward.argtypes = [vp, i32, i32, i32, vp, vp, vp, i32, vp, vp, vp]
         L.cuOptAmdUnrolledBackward.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+        L.cuOptAmdUnrolledOptimalBatchSize.argtypes = [vp, i32, ctypes.POINTER(i32)]
         L._unrolled_bound = True
     return L
 
@@ -73,6 +74,15 @@ class UnrolledPDHG:
         eta = ctypes.c_double()
         c_api._check(_lib().cuOptAmdUnrolledStepSize(self.handle, ctypes.byref(eta)))
         return eta.value
+
+    def optimal_batch_size(self, max_batch: int) -> int:
+        """Batch size in [1, max_batch] with the least SpMM time per batch member on this
+        matrix (cuOptAmdUnrolledOptimalBatchSize); larger batches are best run in chunks
+        of that size."""
+        out = ctypes.c_int32()
+        c_api._check(_lib().cuOptAmdUnrolledOptimalBatchSize(self.handle, int(max_batch), ctypes.byref(out)),
+                     "cuOptAmdUnrolledOptimalBatchSize")
+        return out.value
 
     def __call__(self, c: torch.Tensor, lc: torch.Tensor, uc: torch.Tensor | None = None):
         return _UnrolledFn.apply(c, lc, lc if uc is None else uc, self)

@@ -353,6 +353,35 @@ def batch_solve(lps, tol: float = 1e-4, precision: str = "fp64", iteration_limit
     return out
 
 
+_BATCH_COST = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_int32, ctypes.c_void_p)
+
+
+def batch_size_search(max_batch: int, cost) -> int:
+    """cuOptAmdBatchSizeSearch: the batch-size search over ``cost(b)`` (time per batch
+    member at batch size b); see cuOptAmdUnrolledOptimalBatchSize for the device-timed form."""
+    L = lib()
+    if not getattr(L, "_bss_bound", False):
+        L.cuOptAmdBatchSizeSearch.argtypes = [ctypes.c_int32, _BATCH_COST, ctypes.c_void_p,
+                                              ctypes.POINTER(ctypes.c_int32)]
+        L._bss_bound = True
+    errors = []
+
+    def cb(b, _ctx):
+        try:
+            return float(cost(int(b)))
+        except Exception as e:  # an exception cannot cross the C boundary
+            errors.append(e)
+            return float("inf")
+
+    fn = _BATCH_COST(cb)
+    out = ctypes.c_int32()
+    rc = L.cuOptAmdBatchSizeSearch(int(max_batch), fn, None, ctypes.byref(out))
+    if errors:
+        raise errors[0]
+    _check(rc, "cuOptAmdBatchSizeSearch")
+    return out.value
+
+
 def spmv(row_offsets, col_indices, values, x, ncols: int, precision: str = "fp64") -> np.ndarray:
     """y = A x on the GPU with the solver's tiled SpMV kernel."""
     ptr, idx, val = _a(row_offsets, np.int32), _a(col_indices, np.int32), _a(values, np.float64)

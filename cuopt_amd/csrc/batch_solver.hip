@@ -309,12 +309,16 @@ __device__ inline void rows_pass(const int* __restrict__ ptr,
       for (int off = G >> 1; off > 0; off >>= 1) a.merge(a.shfl(off));
       if (sub == 0) epi.row(r, a, pre, red);
     }
-    __syncthreads();
+    // Every wave has consumed its LDS reads of the stage: the next block may
+    // overwrite it.  A bare barrier -- the epilogue's stores stay in flight and
+    // are waited for by the next block's barrier, together with its copy.
+    __builtin_amdgcn_s_barrier();
     r0 = r1;
     z0 = z1;
     r1 = r2;
     z1 = z2;
   }
+  __syncthreads();  // the pass's stores are visible to the whole workgroup
 }
 
 // Ruiz / Pock-Chambolle statistics of |(a * D_row-side) * D_col-side| in the

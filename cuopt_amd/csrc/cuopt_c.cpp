@@ -22,6 +22,7 @@
 #include "lp_types.hpp"
 #include "mps_io.hpp"
 #include "settings.hpp"
+#include "batch_size.hpp"
 #include "unrolled.hpp"
 
 using namespace cuopt_amd;
@@ -797,6 +798,23 @@ cuopt_int_t cuOptAmdUnrolledStepSize(cuOptAmdUnrolled handle, cuopt_float_t* eta
   if (!handle || !eta) return CUOPT_INVALID_ARGUMENT;
   *eta = unrolled_step_size(static_cast<UnrolledHandle*>(handle));
   return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptAmdUnrolledOptimalBatchSize(cuOptAmdUnrolled handle, cuopt_int_t max_batch_size,
+                                             cuopt_int_t* batch_size)
+{
+  if (!handle || !batch_size || max_batch_size < 1) return CUOPT_INVALID_ARGUMENT;
+  return guarded(
+    [&] { *batch_size = unrolled_optimal_batch_size(static_cast<UnrolledHandle*>(handle), max_batch_size); });
+}
+
+cuopt_int_t cuOptAmdBatchSizeSearch(cuopt_int_t max_batch_size,
+                                    cuopt_float_t (*cost)(cuopt_int_t, void*),
+                                    void* context,
+                                    cuopt_int_t* batch_size)
+{
+  if (!cost || !batch_size || max_batch_size < 1) return CUOPT_INVALID_ARGUMENT;
+  return guarded([&] { *batch_size = search_batch_size(max_batch_size, [&](int b) { return cost(b, context); }); });
 }
 
 cuopt_int_t cuOptAmdUnrolledForward(cuOptAmdUnrolled handle,

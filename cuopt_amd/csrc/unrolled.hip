@@ -25,6 +25,7 @@
 #include <cstring>
 #include <vector>
 
+#include "batch_size.hpp"
 #include "common.hpp"
 #include "pdlp_kernels.hpp"
 #include "unrolled.hpp"
@@ -269,6 +270,17 @@ __global__ __launch_bounds__(kUB) void unr_spmv64(const int* __restrict__ ptr, c
   double s = 0.0;
   for (int k = ptr[r]; k < ptr[r + 1]; ++k) s += val[k] * (x[idx[k]] * scale);
   y[r] = s;
+}
+
+// Y = A X, batch-interleaved (the SpMM timed by the batch-size search).
+template <typename V>
+__global__ __launch_bounds__(kUB) void unr_spmm(const int* __restrict__ ptr, const int* __restrict__ idx,
+                                                const V* __restrict__ val, int rows, int B, const V* __restrict__ x,
+                                                V* __restrict__ y)
+{
+  int r, b;
+  if (!row_lane(rows, B, r, b)) return;
+  y[(size_t)r * B + b] = row_dot(ptr, idx, val, x, r, b, B).value();
 }
 
 inline int ew_blocks(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>((n + kUB - 1) / kUB, 4096)); }
@@ -563,6 +575,63 @@ void unrolled_backward(UnrolledHandle* h, const double* gx, const double* gy, do
     backward_impl<float>(*U, gx, gy, gc, glc, guc, s);
   else
     backward_impl<double>(*U, gx, gy, gc, glc, guc, s);
+}
+
+template <typename V>
+int optimal_batch_size_impl(UnrolledImpl& U, int max_batch)
+{
+  const Views<V> M = views<V>(U);
+  hipStream_t s;
+  CUOPT_HIP_CHECK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  DevBuf<V> x, y;
+  auto cleanup = [&]() {
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(s);
+  };
+  try {
+    CUOPT_HIP_CHECK(hipEventCreate(&e0));
+    CUOPT_HIP_CHECK(hipEventCreate(&e1));
+    auto cost = [&](int B) {
+      const size_t nB = (size_t)std::max(U.n, 1) * B, mB = (size_t)std::max(U.m, 1) * B;
+      if (x.size() < nB) x.resize(nB);
+      if (y.size() < mB) y.resize(mB);
+      hipLaunchKernelGGL(unr_zero<V>, dim3(ew_blocks(nB)), dim3(kUB), 0, s, nB, x.get());
+      hipLaunchKernelGGL(unr_zero<V>, dim3(ew_blocks(mB)), dim3(kUB), 0, s, mB, y.get());
+      auto run = [&]() {
+        if (U.m > 0)
+          hipLaunchKernelGGL(unr_spmm<V>, spmm_grid(U.m, B), dim3(kUB), 0, s, M.aptr, M.aidx, M.aval, U.m, B,
+                             (const V*)x.get(), y.get());
+        if (U.n > 0)
+          hipLaunchKernelGGL(unr_spmm<V>, spmm_grid(U.n, B), dim3(kUB), 0, s, M.tptr, M.tidx, M.tval, U.n, B,
+                             (const V*)y.get(), x.get());
+      };
+      run();
+      CUOPT_HIP_CHECK(hipEventRecord(e0, s));
+      constexpr int kRuns = 5;
+      for (int i = 0; i < kRuns; ++i) run();
+      CUOPT_HIP_CHECK(hipEventRecord(e1, s));
+      CUOPT_HIP_CHECK(hipGetLastError());
+      CUOPT_HIP_CHECK(hipEventSynchronize(e1));
+      float ms = 0.f;
+      CUOPT_HIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+      return (double)ms / kRuns / B;
+    };
+    const int best = search_batch_size(max_batch, cost);
+    cleanup();
+    return best;
+  } catch (...) {
+    cleanup();
+    throw;
+  }
+}
+
+int unrolled_optimal_batch_size(UnrolledHandle* h, int max_batch)
+{
+  auto& U = *reinterpret_cast<UnrolledImpl*>(h);
+  if (U.precision == CUOPT_AMD_PRECISION_FP32_MIXED) return optimal_batch_size_impl<float>(U, max_batch);
+  return optimal_batch_size_impl<double>(U, max_batch);
 }
 
 }  // namespace cuopt_amd

@@ -20,5 +20,9 @@ void unrolled_forward(UnrolledHandle* h, int B, int K, int every, const double* 
 // VJP of the last forward: gx (n x B), gy (m x B or null) -> gc (n x B), glc, guc (m x B).
 void unrolled_backward(UnrolledHandle* h, const double* gx, const double* gy, double* gc, double* glc, double* guc,
                        void* stream);
+// Batch size in [1, max_batch] with the least SpMM time per batch member
+// (A X plus A^T Y on the handle's matrices, search_batch_size), timed on the
+// device: one untimed run, then the mean of five.
+int unrolled_optimal_batch_size(UnrolledHandle* h, int max_batch);
 
 }  // namespace cuopt_amd

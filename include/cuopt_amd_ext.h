@@ -156,6 +156,24 @@ cuopt_int_t cuOptAmdUnrolledBackward(cuOptAmdUnrolled handle,
                                      cuopt_float_t* guc,
                                      void* stream);
 
+/* Batch size for a same-matrix batch on this handle's matrix: times
+ * Y = A X plus X = A^T Y (batch-interleaved SpMM, one untimed run and the mean
+ * of five) at the sizes visited by the search of the reference's batch-size
+ * handler (optimal_batch_size_handler.cu:196-435, called from solve.cu:788):
+ * start at the largest power of two <= min(128, max_batch_size), move by
+ * factors of two in the direction whose time per batch member falls, at most
+ * four steps, then try the size halfway back.  *batch_size gets a value in
+ * [1, max_batch_size]; callers split larger batches into chunks of that size. */
+cuopt_int_t cuOptAmdUnrolledOptimalBatchSize(cuOptAmdUnrolled handle,
+                                             cuopt_int_t max_batch_size,
+                                             cuopt_int_t* batch_size);
+/* The same search over a caller-supplied cost per batch member, cost(b, context)
+ * (for callers that time their own SpMM, and for tests of the search). */
+cuopt_int_t cuOptAmdBatchSizeSearch(cuopt_int_t max_batch_size,
+                                    cuopt_float_t (*cost)(cuopt_int_t batch_size, void* context),
+                                    void* context,
+                                    cuopt_int_t* batch_size);
+
 /* Solves num_problems independent LPs together (the heterogeneous batch of the
  * reference's Python BatchSolve, cython_solve.cu:286-352) in one persistent
  * launch: one workgroup per LP runs setup and the whole Stable3 loop on the

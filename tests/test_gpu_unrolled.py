@@ -113,3 +113,19 @@ def test_envelope_gradients_at_convergence():
     obj.backward()
     assert torch.allclose(c.grad, x.detach(), atol=1e-8)  # d obj / d c = x*
     assert torch.allclose(b.grad, y.detach()[0], atol=1e-8 * (1 + y.abs().max().item()))  # d obj / d b = y*
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
+def test_optimal_batch_size_on_device(precision):
+    # The device-timed search returns a size the search can visit; one member is
+    # always one, and the forward pass still matches the oracle at that size.
+    lp, A = problem(1, m=1000, n=2500)
+    op = UnrolledPDHG(lp, ridge=1e-3, steps=20, checkpoint_every=5, precision=precision)
+    assert op.optimal_batch_size(1) == 1
+    b = op.optimal_batch_size(512)
+    visited = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64, 96, 128, 192, 256, 384, 512}
+    assert b in visited, b
+    c = np.tile(lp.c, (b, 1))
+    x, _ = op(torch.tensor(c, device=DEV), torch.tensor(lp.con_lb, device=DEV), torch.tensor(lp.con_ub, device=DEV))
+    xo, _ = U.forward(A, c.T, lp.con_lb, lp.con_ub, lp.var_lb, lp.var_ub, 1e-3, op.step_size, 20)
+    assert relerr(x.cpu().numpy(), xo.T) < (1e-9 if precision == "fp64" else 2e-3)
