@@ -35,6 +35,12 @@ struct ProblemHandle {
 
 struct SettingsHandle {
   SolverSettingsRegistry reg;
+  // MIP-only settings, kept as the reference keeps them; an LP solve ignores them.
+  std::vector<std::vector<double>> mip_starts;
+  cuOptMIPGetSolutionCallback mip_get_cb = nullptr;
+  cuOptMIPSetSolutionCallback mip_set_cb = nullptr;
+  void* mip_get_data = nullptr;
+  void* mip_set_data = nullptr;
 };
 
 struct SolutionHandle {
@@ -254,6 +260,73 @@ cuopt_int_t cuOptCreateRangedProblem(cuopt_int_t num_constraints,
   return CUOPT_SUCCESS;
 }
 
+namespace {
+// Nonzeros of a quadratic objective matrix given in CSR over n rows, or -1
+// when the arguments cannot describe one.
+long long quadratic_nnz(cuopt_int_t n, const cuopt_int_t* q_ptr)
+{
+  if (n < 0 || !q_ptr || q_ptr[0] != 0) return -1;
+  return q_ptr[n] < 0 ? -1 : q_ptr[n];
+}
+}  // namespace
+
+cuopt_int_t cuOptCreateQuadraticProblem(cuopt_int_t num_constraints,
+                                        cuopt_int_t num_variables,
+                                        cuopt_int_t objective_sense,
+                                        cuopt_float_t objective_offset,
+                                        const cuopt_float_t* objective_coefficients,
+                                        const cuopt_int_t* q_ptr,
+                                        const cuopt_int_t* q_idx,
+                                        const cuopt_float_t* q_val,
+                                        const cuopt_int_t* row_offsets,
+                                        const cuopt_int_t* column_indices,
+                                        const cuopt_float_t* values,
+                                        const char* constraint_sense,
+                                        const cuopt_float_t* rhs,
+                                        const cuopt_float_t* lower_bounds,
+                                        const cuopt_float_t* upper_bounds,
+                                        cuOptOptimizationProblem* problem_ptr)
+{
+  (void)q_idx;
+  (void)q_val;
+  if (!problem_ptr) return CUOPT_INVALID_ARGUMENT;
+  *problem_ptr = nullptr;
+  if (quadratic_nnz(num_variables, q_ptr) != 0) return CUOPT_INVALID_ARGUMENT;  // a QP: barrier, not served here
+  const std::vector<char> types(std::max(num_variables, 1), CUOPT_CONTINUOUS);
+  return cuOptCreateProblem(num_constraints, num_variables, objective_sense, objective_offset, objective_coefficients,
+                            row_offsets, column_indices, values, constraint_sense, rhs, lower_bounds, upper_bounds,
+                            types.data(), problem_ptr);
+}
+
+cuopt_int_t cuOptCreateQuadraticRangedProblem(cuopt_int_t num_constraints,
+                                              cuopt_int_t num_variables,
+                                              cuopt_int_t objective_sense,
+                                              cuopt_float_t objective_offset,
+                                              const cuopt_float_t* objective_coefficients,
+                                              const cuopt_int_t* q_ptr,
+                                              const cuopt_int_t* q_idx,
+                                              const cuopt_float_t* q_val,
+                                              const cuopt_int_t* row_offsets,
+                                              const cuopt_int_t* column_indices,
+                                              const cuopt_float_t* values,
+                                              const cuopt_float_t* constraint_lower_bounds,
+                                              const cuopt_float_t* constraint_upper_bounds,
+                                              const cuopt_float_t* variable_lower_bounds,
+                                              const cuopt_float_t* variable_upper_bounds,
+                                              cuOptOptimizationProblem* problem_ptr)
+{
+  (void)q_idx;
+  (void)q_val;
+  if (!problem_ptr) return CUOPT_INVALID_ARGUMENT;
+  *problem_ptr = nullptr;
+  if (quadratic_nnz(num_variables, q_ptr) != 0) return CUOPT_INVALID_ARGUMENT;
+  const std::vector<char> types(std::max(num_variables, 1), CUOPT_CONTINUOUS);
+  return cuOptCreateRangedProblem(num_constraints, num_variables, objective_sense, objective_offset,
+                                  objective_coefficients, row_offsets, column_indices, values, constraint_lower_bounds,
+                                  constraint_upper_bounds, variable_lower_bounds, variable_upper_bounds, types.data(),
+                                  problem_ptr);
+}
+
 void cuOptDestroyProblem(cuOptOptimizationProblem* problem_ptr)
 {
   if (!problem_ptr || !*problem_ptr) return;
@@ -464,6 +537,35 @@ cuopt_int_t cuOptSetInitialDualSolution(cuOptSolverSettings settings, const cuop
   return CUOPT_SUCCESS;
 }
 
+cuopt_int_t cuOptSetMIPGetSolutionCallback(cuOptSolverSettings settings, cuOptMIPGetSolutionCallback callback,
+                                           void* user_data)
+{
+  if (!settings || !callback) return CUOPT_INVALID_ARGUMENT;
+  as_settings(settings)->mip_get_cb   = callback;
+  as_settings(settings)->mip_get_data = user_data;
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptSetMIPSetSolutionCallback(cuOptSolverSettings settings, cuOptMIPSetSolutionCallback callback,
+                                           void* user_data)
+{
+  if (!settings || !callback) return CUOPT_INVALID_ARGUMENT;
+  as_settings(settings)->mip_set_cb   = callback;
+  as_settings(settings)->mip_set_data = user_data;
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptAddMIPStart(cuOptSolverSettings settings, const cuopt_float_t* solution, cuopt_int_t num_variables)
+{
+  if (!settings || !solution || num_variables <= 0) return CUOPT_INVALID_ARGUMENT;
+  try {
+    as_settings(settings)->mip_starts.emplace_back(solution, solution + num_variables);
+  } catch (...) {
+    return CUOPT_INVALID_ARGUMENT;
+  }
+  return CUOPT_SUCCESS;
+}
+
 cuopt_int_t cuOptIsMIP(cuOptOptimizationProblem problem, cuopt_int_t* is_mip_ptr)
 {
   if (!problem || !is_mip_ptr) return CUOPT_INVALID_ARGUMENT;
@@ -640,6 +742,21 @@ cuopt_int_t cuOptGetDualSolution(cuOptSolution solution, cuopt_float_t* out)
 {
   if (!solution || !out) return CUOPT_INVALID_ARGUMENT;
   return copy_out(as_solution(solution)->res.y, out);
+}
+
+// Solutions of this engine are LP solutions: no MIP gap, no solution bound.
+cuopt_int_t cuOptGetMIPGap(cuOptSolution solution, cuopt_float_t* mip_gap_ptr)
+{
+  (void)solution;
+  (void)mip_gap_ptr;
+  return CUOPT_INVALID_ARGUMENT;
+}
+
+cuopt_int_t cuOptGetSolutionBound(cuOptSolution solution, cuopt_float_t* solution_bound_ptr)
+{
+  (void)solution;
+  (void)solution_bound_ptr;
+  return CUOPT_INVALID_ARGUMENT;
 }
 
 cuopt_int_t cuOptGetDualObjectiveValue(cuOptSolution solution, cuopt_float_t* out)

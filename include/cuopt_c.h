@@ -168,6 +168,44 @@ cuopt_int_t cuOptCreateRangedProblem(cuopt_int_t num_constraints,
                                      const char* variable_types,
                                      cuOptOptimizationProblem* problem_ptr);
 
+/* Quadratic-objective forms (cuopt_c.h:309-325 and 391-407).  The reference
+ * sends problems with a nonzero quadratic matrix to its barrier method
+ * (solve.cu:1142-1152), which this LP engine does not run: they are refused
+ * with CUOPT_INVALID_ARGUMENT.  An empty quadratic matrix (no nonzeros) gives
+ * the plain LP, with continuous variables. */
+cuopt_int_t cuOptCreateQuadraticProblem(cuopt_int_t num_constraints,
+                                        cuopt_int_t num_variables,
+                                        cuopt_int_t objective_sense,
+                                        cuopt_float_t objective_offset,
+                                        const cuopt_float_t* objective_coefficients,
+                                        const cuopt_int_t* quadratic_objective_matrix_row_offsets,
+                                        const cuopt_int_t* quadratic_objective_matrix_column_indices,
+                                        const cuopt_float_t* quadratic_objective_matrix_coefficent_values,
+                                        const cuopt_int_t* constraint_matrix_row_offsets,
+                                        const cuopt_int_t* constraint_matrix_column_indices,
+                                        const cuopt_float_t* constraint_matrix_coefficent_values,
+                                        const char* constraint_sense,
+                                        const cuopt_float_t* rhs,
+                                        const cuopt_float_t* lower_bounds,
+                                        const cuopt_float_t* upper_bounds,
+                                        cuOptOptimizationProblem* problem_ptr);
+cuopt_int_t cuOptCreateQuadraticRangedProblem(cuopt_int_t num_constraints,
+                                              cuopt_int_t num_variables,
+                                              cuopt_int_t objective_sense,
+                                              cuopt_float_t objective_offset,
+                                              const cuopt_float_t* objective_coefficients,
+                                              const cuopt_int_t* quadratic_objective_matrix_row_offsets,
+                                              const cuopt_int_t* quadratic_objective_matrix_column_indices,
+                                              const cuopt_float_t* quadratic_objective_matrix_coefficent_values,
+                                              const cuopt_int_t* constraint_matrix_row_offsets,
+                                              const cuopt_int_t* constraint_matrix_column_indices,
+                                              const cuopt_float_t* constraint_matrix_coefficients,
+                                              const cuopt_float_t* constraint_lower_bounds,
+                                              const cuopt_float_t* constraint_upper_bounds,
+                                              const cuopt_float_t* variable_lower_bounds,
+                                              const cuopt_float_t* variable_upper_bounds,
+                                              cuOptOptimizationProblem* problem_ptr);
+
 /* cuopt_c.h:414 */
 void cuOptDestroyProblem(cuOptOptimizationProblem* problem_ptr);
 
@@ -220,6 +258,27 @@ cuopt_int_t cuOptSetInitialDualSolution(cuOptSolverSettings settings,
                                         const cuopt_float_t* dual_solution,
                                         cuopt_int_t num_constraints);
 
+/* MIP-only settings (cuopt_c.h:711-765 callbacks, :818 MIP start).  As in the
+ * reference, they are accepted and kept with the settings (same argument
+ * checks, CUOPT_SUCCESS) and an LP solve never uses them. */
+typedef void (*cuOptMIPGetSolutionCallback)(const cuopt_float_t* solution,
+                                            const cuopt_float_t* objective_value,
+                                            const cuopt_float_t* solution_bound,
+                                            void* user_data);
+typedef void (*cuOptMIPSetSolutionCallback)(cuopt_float_t* solution,
+                                            cuopt_float_t* objective_value,
+                                            const cuopt_float_t* solution_bound,
+                                            void* user_data);
+cuopt_int_t cuOptSetMIPGetSolutionCallback(cuOptSolverSettings settings,
+                                           cuOptMIPGetSolutionCallback callback,
+                                           void* user_data);
+cuopt_int_t cuOptSetMIPSetSolutionCallback(cuOptSolverSettings settings,
+                                           cuOptMIPSetSolutionCallback callback,
+                                           void* user_data);
+cuopt_int_t cuOptAddMIPStart(cuOptSolverSettings settings,
+                             const cuopt_float_t* solution,
+                             cuopt_int_t num_variables);
+
 /* ---- solve (cuopt_c.h:831-853) -------------------------------------------- */
 cuopt_int_t cuOptIsMIP(cuOptOptimizationProblem problem, cuopt_int_t* is_mip_ptr);
 /* Returns the solution's error status (cuopt_c.cpp:896-897). */
@@ -238,6 +297,11 @@ cuopt_int_t cuOptGetPrimalSolution(cuOptSolution solution, cuopt_float_t* soluti
 cuopt_int_t cuOptGetObjectiveValue(cuOptSolution solution, cuopt_float_t* objective_value_ptr);
 cuopt_int_t cuOptGetSolveTime(cuOptSolution solution, cuopt_float_t* solve_time_ptr);
 cuopt_int_t cuOptGetDualSolution(cuOptSolution solution, cuopt_float_t* dual_solution_ptr);
+/* MIP solution getters (cuopt_c.h:933, 944): an LP solution has neither, and
+ * both return CUOPT_INVALID_ARGUMENT for it, as in the reference
+ * (cuopt_c.cpp:1050-1082). */
+cuopt_int_t cuOptGetMIPGap(cuOptSolution solution, cuopt_float_t* mip_gap_ptr);
+cuopt_int_t cuOptGetSolutionBound(cuOptSolution solution, cuopt_float_t* solution_bound_ptr);
 cuopt_int_t cuOptGetDualObjectiveValue(cuOptSolution solution, cuopt_float_t* dual_objective_value_ptr);
 cuopt_int_t cuOptGetReducedCosts(cuOptSolution solution, cuopt_float_t* reduced_cost_ptr);
 

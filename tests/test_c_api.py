@@ -237,3 +237,84 @@ def test_read_mps_errors(tmp_path):
     p = c_api.Problem.read_mps(os.path.join(GOLDEN, "afiro_original.mps"))
     assert p.write_mps(tmp_path / "x.mps", fmt=7) == c_api.CUOPT_INVALID_ARGUMENT
     assert L.cuOptWriteProblem(p.handle, b"", 0) == c_api.CUOPT_INVALID_ARGUMENT
+
+
+def _qp_args(lp, q_ptr, q_idx, q_val):
+    s, rhs = lp.row_sense()
+    keep = [np.ascontiguousarray(a) for a in (lp.c, q_ptr, q_idx, q_val, lp.row_offsets.astype(np.int32),
+                                              lp.col_indices.astype(np.int32), lp.values, np.frombuffer(s.tobytes(), np.uint8).copy(),
+                                              rhs, lp.var_lb, lp.var_ub)]
+    return keep, [k.ctypes.data for k in keep]
+
+
+def test_quadratic_create_refuses_qp_and_accepts_empty_q():
+    # The reference sends QPs to barrier (solve.cu:1142-1152); with no quadratic
+    # nonzeros the problem is the LP itself, with continuous variables.
+    L = c_api.lib()
+    vp, i32, f64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_double
+    L.cuOptCreateQuadraticProblem.argtypes = [i32, i32, i32, f64] + [vp] * 11 + [ctypes.POINTER(vp)]
+    L.cuOptCreateQuadraticRangedProblem.argtypes = [i32, i32, i32, f64] + [vp] * 11 + [ctypes.POINTER(vp)]
+    lp = G.random_lp(m=12, n=30, seed=7)
+    h = ctypes.c_void_p()
+    empty = (np.zeros(31, np.int32), np.zeros(1, np.int32), np.zeros(1))
+    keep, ptrs = _qp_args(lp, *empty)
+    assert L.cuOptCreateQuadraticProblem(12, 30, 0, ctypes.c_double(0.0), *ptrs, ctypes.byref(h)) == 0 and h.value
+    nz, mip = ctypes.c_int32(), ctypes.c_int32()
+    assert L.cuOptGetNumNonZeros(h, ctypes.byref(nz)) == 0 and nz.value == lp.nnz
+    assert L.cuOptIsMIP(h, ctypes.byref(mip)) == 0 and mip.value == 0
+    L.cuOptDestroyProblem(ctypes.byref(h))
+    q_ptr = np.zeros(31, np.int32)
+    q_ptr[1:] = 1  # one diagonal entry in row 0
+    keep, ptrs = _qp_args(lp, q_ptr, np.zeros(1, np.int32), np.ones(1))
+    h = ctypes.c_void_p()
+    assert L.cuOptCreateQuadraticProblem(12, 30, 0, ctypes.c_double(0.0), *ptrs, ctypes.byref(h)) == \
+        c_api.CUOPT_INVALID_ARGUMENT
+    assert not h.value
+    # ranged form: same rule
+    lc, uc = np.ascontiguousarray(lp.con_lb), np.ascontiguousarray(lp.con_ub)
+    args = [lp.c, empty[0], empty[1], empty[2], lp.row_offsets.astype(np.int32), lp.col_indices.astype(np.int32),
+            lp.values, lc, uc, lp.var_lb, lp.var_ub]
+    args = [np.ascontiguousarray(a) for a in args]
+    h = ctypes.c_void_p()
+    assert L.cuOptCreateQuadraticRangedProblem(12, 30, 0, ctypes.c_double(0.0), *[a.ctypes.data for a in args],
+                                               ctypes.byref(h)) == 0 and h.value
+    L.cuOptDestroyProblem(ctypes.byref(h))
+    args[1] = q_ptr
+    h = ctypes.c_void_p()
+    assert L.cuOptCreateQuadraticRangedProblem(12, 30, 0, ctypes.c_double(0.0), *[a.ctypes.data for a in args],
+                                               ctypes.byref(h)) == c_api.CUOPT_INVALID_ARGUMENT
+
+
+def test_mip_only_settings_and_getters():
+    # As cuopt_c.cpp:758-836: callbacks and MIP starts are kept with the
+    # settings (argument checks only); an LP solution has no MIP gap or bound
+    # (cuopt_c.cpp:1050-1082).
+    L = c_api.lib()
+    vp = ctypes.c_void_p
+    L.cuOptAddMIPStart.argtypes = [vp, vp, ctypes.c_int32]
+    L.cuOptGetMIPGap.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    L.cuOptGetSolutionBound.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+    L.cuOptSolve.argtypes = [vp, vp, ctypes.POINTER(vp)]
+    s = c_api.SolverSettings()
+    x = np.ones(5)
+    assert L.cuOptAddMIPStart(s.handle, x.ctypes.data, 5) == 0
+    assert L.cuOptAddMIPStart(s.handle, x.ctypes.data, 0) == c_api.CUOPT_INVALID_ARGUMENT
+    assert L.cuOptAddMIPStart(None, x.ctypes.data, 5) == c_api.CUOPT_INVALID_ARGUMENT
+    cb_t = ctypes.CFUNCTYPE(None, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p)
+    cb = cb_t(lambda a, b, c, d: None)
+    L.cuOptSetMIPGetSolutionCallback.argtypes = [ctypes.c_void_p, cb_t, ctypes.c_void_p]
+    L.cuOptSetMIPSetSolutionCallback.argtypes = [ctypes.c_void_p, cb_t, ctypes.c_void_p]
+    assert L.cuOptSetMIPGetSolutionCallback(s.handle, cb, None) == 0
+    assert L.cuOptSetMIPSetSolutionCallback(s.handle, cb, None) == 0
+    assert L.cuOptSetMIPGetSolutionCallback(None, cb, None) == c_api.CUOPT_INVALID_ARGUMENT
+    # a host-answered LP solution (crossing bounds)
+    lp = G.random_lp(m=10, n=20, seed=3)
+    lp.var_lb = lp.var_lb.copy()
+    lp.var_lb[4] = 11.0
+    p = c_api.Problem(lp)
+    sol = ctypes.c_void_p()
+    assert L.cuOptSolve(p.handle, c_api.make_settings().handle, ctypes.byref(sol)) == 0
+    v = ctypes.c_double()
+    assert L.cuOptGetMIPGap(sol, ctypes.byref(v)) == c_api.CUOPT_INVALID_ARGUMENT
+    assert L.cuOptGetSolutionBound(sol, ctypes.byref(v)) == c_api.CUOPT_INVALID_ARGUMENT
+    L.cuOptDestroySolution(ctypes.byref(sol))
