@@ -744,6 +744,45 @@ cuopt_int_t cuOptGetDualSolution(cuOptSolution solution, cuopt_float_t* out)
   return copy_out(as_solution(solution)->res.y, out);
 }
 
+namespace {
+cuopt_int_t copy_name(const std::string& s, char* buf, cuopt_int_t size)
+{
+  if (!buf || size < 1) return CUOPT_INVALID_ARGUMENT;
+  const size_t n = std::min<size_t>(s.size(), (size_t)size - 1);
+  std::memcpy(buf, s.data(), n);
+  buf[n] = '\0';
+  return CUOPT_SUCCESS;
+}
+}  // namespace
+
+cuopt_int_t cuOptAmdGetVariableName(cuOptOptimizationProblem problem, cuopt_int_t index, char* buf, cuopt_int_t size)
+{
+  if (!problem) return CUOPT_INVALID_ARGUMENT;
+  const LpHost& lp = as_problem(problem)->lp;
+  if (index < 0 || index >= lp.n) return CUOPT_INVALID_ARGUMENT;
+  return copy_name((size_t)index < lp.col_names.size() ? lp.col_names[index] : std::string(), buf, size);
+}
+
+cuopt_int_t cuOptAmdGetConstraintName(cuOptOptimizationProblem problem, cuopt_int_t index, char* buf, cuopt_int_t size)
+{
+  if (!problem) return CUOPT_INVALID_ARGUMENT;
+  const LpHost& lp = as_problem(problem)->lp;
+  if (index < 0 || index >= lp.m) return CUOPT_INVALID_ARGUMENT;
+  return copy_name((size_t)index < lp.row_names.size() ? lp.row_names[index] : std::string(), buf, size);
+}
+
+cuopt_int_t cuOptAmdGetObjectiveName(cuOptOptimizationProblem problem, char* buf, cuopt_int_t size)
+{
+  if (!problem) return CUOPT_INVALID_ARGUMENT;
+  return copy_name(as_problem(problem)->lp.objective_name, buf, size);
+}
+
+cuopt_int_t cuOptAmdGetProblemName(cuOptOptimizationProblem problem, char* buf, cuopt_int_t size)
+{
+  if (!problem) return CUOPT_INVALID_ARGUMENT;
+  return copy_name(as_problem(problem)->lp.problem_name, buf, size);
+}
+
 // Solutions of this engine are LP solutions: no MIP gap, no solution bound.
 cuopt_int_t cuOptGetMIPGap(cuOptSolution solution, cuopt_float_t* mip_gap_ptr)
 {

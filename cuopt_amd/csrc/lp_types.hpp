@@ -27,6 +27,9 @@ struct LpHost {
   std::vector<double> lc, uc;  // ranged form
   std::vector<double> lb, ub;
   std::vector<char> var_types;
+  // Names read from an MPS file (empty for problems built from arrays).
+  std::vector<std::string> row_names, col_names;
+  std::string objective_name, problem_name;
 };
 
 // pdlp_solver_settings_t fields used by the LP path (pdlp/solver_settings.hpp:193-262).

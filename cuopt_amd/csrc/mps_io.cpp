@@ -62,6 +62,7 @@ LpHost read_mps(const std::string& path)
     double val;
   };
   std::vector<Entry> entries;
+  std::string problem_name;
   std::vector<double> cost, lb, ub;
   std::vector<char> has_bound;
   std::unordered_map<int, double> rhs, range;
@@ -95,6 +96,7 @@ LpHost read_mps(const std::string& path)
       const std::string h = upper(t[0]);
       if (h == "NAME") {
         sec = NAME;
+        if (t.size() > 1) problem_name = t[1];
       } else if (h == "OBJSENSE" || h == "OBJSENCE") {
         sec = OBJSENSE;
         if (t.size() > 1) maximize = upper(t[1]).rfind("MAX", 0) == 0;
@@ -234,6 +236,10 @@ LpHost read_mps(const std::string& path)
   LpHost lp;
   lp.m        = (int)row_names.size();
   lp.n        = (int)col_names.size();
+  lp.row_names      = row_names;
+  lp.col_names      = col_names;
+  lp.objective_name = obj_row;
+  lp.problem_name   = problem_name;
   lp.maximize = maximize;
   lp.offset   = offset;
   lp.c        = cost;

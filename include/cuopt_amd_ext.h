@@ -107,6 +107,23 @@ cuopt_int_t cuOptAmdBenchmarkSteps(cuOptOptimizationProblem problem,
                                    cuopt_float_t* out,
                                    cuopt_int_t n);
 
+/* Names of a problem read from an MPS file (the reference's data model keeps
+ * them: data_model.py get_variable_names / get_row_names / get_objective_name /
+ * get_problem_name, mps_parser.cpp).  The name is copied NUL-terminated into
+ * name_buffer, truncated to buffer_size - 1 characters; a problem built from
+ * arrays has empty names.  CUOPT_INVALID_ARGUMENT for a null pointer, an index
+ * out of range or buffer_size < 1. */
+cuopt_int_t cuOptAmdGetVariableName(cuOptOptimizationProblem problem,
+                                    cuopt_int_t index,
+                                    char* name_buffer,
+                                    cuopt_int_t buffer_size);
+cuopt_int_t cuOptAmdGetConstraintName(cuOptOptimizationProblem problem,
+                                      cuopt_int_t index,
+                                      char* name_buffer,
+                                      cuopt_int_t buffer_size);
+cuopt_int_t cuOptAmdGetObjectiveName(cuOptOptimizationProblem problem, char* name_buffer, cuopt_int_t buffer_size);
+cuopt_int_t cuOptAmdGetProblemName(cuOptOptimizationProblem problem, char* name_buffer, cuopt_int_t buffer_size);
+
 /* Number of GPUs visible to the library (0 when no HIP device is usable). */
 cuopt_int_t cuOptAmdDeviceCount(void);
 
