@@ -9,7 +9,7 @@ LIB = os.path.join(HERE, "libcuopt_amd.so")
 OBJ = os.path.join(HERE, "_build")
 
 HIP_SOURCES = ["pdlp_solver.hip", "batch_solver.hip", "unrolled.hip"]
-CXX_SOURCES = ["tile_plan.cpp", "settings.cpp", "mps_io.cpp", "batch_size.cpp", "cuopt_c.cpp"]
+CXX_SOURCES = ["tile_plan.cpp", "settings.cpp", "mps_io.cpp", "batch_size.cpp", "presolve.cpp", "cuopt_c.cpp"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unused-result"]
 # gfx950 only; packed FP32 instructions disabled (they are never emitted by hand here).

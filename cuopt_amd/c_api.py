@@ -21,6 +21,7 @@ CUOPT_SUCCESS, CUOPT_INVALID_ARGUMENT, CUOPT_MPS_FILE_ERROR, CUOPT_MPS_PARSE_ERR
 CUOPT_VALIDATION_ERROR, CUOPT_OUT_OF_MEMORY, CUOPT_RUNTIME_ERROR = 4, 5, 6
 CUOPT_MINIMIZE, CUOPT_MAXIMIZE = 1, -1
 CUOPT_METHOD_CONCURRENT, CUOPT_METHOD_PDLP, CUOPT_METHOD_DUAL_SIMPLEX, CUOPT_METHOD_BARRIER = 0, 1, 2, 3
+CUOPT_PRESOLVE_DEFAULT, CUOPT_PRESOLVE_OFF, CUOPT_PRESOLVE_PAPILO, CUOPT_PRESOLVE_PSLP = -1, 0, 1, 2
 STATUS = {0: "NoTermination", 1: "Optimal", 2: "PrimalInfeasible", 3: "DualInfeasible", 4: "IterationLimit",
           5: "TimeLimit", 6: "NumericalError", 7: "PrimalFeasible", 9: "ConcurrentLimit"}
 INFO_NAMES = [
@@ -305,14 +306,14 @@ def _take_result(sol, lp) -> Result:
 
 def make_settings(tol: float = 1e-4, precision: str = "fp64", iteration_limit: int | None = None,
                   time_limit: float | None = None, method: int = CUOPT_METHOD_PDLP, log: bool = False,
-                  refinement: bool | None = None) -> SolverSettings:
-    """Settings with all six tolerances at ``tol``, presolve off and the given limits."""
+                  refinement: bool | None = None, presolve: bool = False) -> SolverSettings:
+    """Settings with all six tolerances at ``tol``, presolve off (unless asked for) and the given limits."""
     s = SolverSettings()
     for name in ("absolute_primal_tolerance", "relative_primal_tolerance", "absolute_dual_tolerance",
                  "relative_dual_tolerance", "absolute_gap_tolerance", "relative_gap_tolerance"):
         _check(s.set(name, float(tol)), name)
     _check(s.set("method", method))
-    _check(s.set("presolve", 0))
+    _check(s.set("presolve", CUOPT_PRESOLVE_PSLP if presolve else 0))
     _check(s.set("pdlp_precision", PRECISION[precision]))
     _check(s.set("log_to_console", bool(log)))
     if iteration_limit is not None:
@@ -326,10 +327,10 @@ def make_settings(tol: float = 1e-4, precision: str = "fp64", iteration_limit: i
 
 def solve(lp, tol: float = 1e-4, precision: str = "fp64", iteration_limit: int | None = None,
           time_limit: float | None = None, x_init=None, y_init=None, method: int = CUOPT_METHOD_PDLP,
-          log: bool = False, refinement: bool | None = None) -> Result:
+          log: bool = False, refinement: bool | None = None, presolve: bool = False) -> Result:
     """Solve ``lp`` (a generators.LP) through the C ABI with all six tolerances set to ``tol``."""
     prob = Problem(lp)
-    s = make_settings(tol, precision, iteration_limit, time_limit, method, log, refinement)
+    s = make_settings(tol, precision, iteration_limit, time_limit, method, log, refinement, presolve)
     if x_init is not None:
         s.set_initial_primal(x_init)
     if y_init is not None:

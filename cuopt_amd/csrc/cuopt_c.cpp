@@ -8,7 +8,8 @@
 //
 // Documented differences (see the solve entry): MIP problems, and the
 // dual-simplex / barrier methods, are rejected with CUOPT_INVALID_ARGUMENT;
-// every PDLP solver mode runs the Stable3 method; presolve is not applied.
+// every PDLP solver mode runs the Stable3 method; every presolve value but Off
+// runs this engine's own presolve (presolve.hpp).
 #include <cstdio>
 #include <cstring>
 #include <exception>

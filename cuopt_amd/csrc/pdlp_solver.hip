@@ -25,6 +25,8 @@
 
 #include <rocprim/device/device_radix_sort.hpp>
 
+#include "presolve.hpp"
+
 #include "common.hpp"
 #include "lp_types.hpp"
 #include "pdlp_kernels.hpp"
@@ -1664,6 +1666,65 @@ void constraint_bounds(const LpHost& lp, std::vector<double>& lc, std::vector<do
   }
 }
 
+namespace {
+
+// The solve after the crossing-bounds check, on bounds lc/uc/lb/ub of lp.
+SolveResult solve_checked(const LpHost& lp, const std::vector<double>& lc, const std::vector<double>& uc,
+                          const std::vector<double>& lb, const std::vector<double>& ub, const PdlpSettings& st,
+                          std::chrono::steady_clock::time_point t0);
+
+// Presolve (presolve.hpp) around solve_checked: the reference's default
+// pipeline presolves LPs before PDLP and postsolves the solution
+// (solve.cu:1183-1325).
+SolveResult solve_presolved(const LpHost& lp, const std::vector<double>& lc, const std::vector<double>& uc,
+                            const std::vector<double>& lb, const std::vector<double>& ub, const PdlpSettings& st,
+                            std::chrono::steady_clock::time_point t0)
+{
+  const Presolved P = presolve(lp, lc, uc, lb, ub);
+  SolveResult R;
+  auto seconds = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+  if (P.status != Presolved::REDUCED) {
+    R.status = P.status == Presolved::INFEASIBLE ? CUOPT_TERIMINATION_STATUS_INFEASIBLE
+                                                 : CUOPT_TERIMINATION_STATUS_UNBOUNDED;
+    R.x.assign(lp.n, 0.0);
+    R.y.assign(lp.m, 0.0);
+    R.rc.assign(lp.n, 0.0);
+    R.solve_time = seconds();
+    return R;
+  }
+  if (P.reduced.m == 0) {  // presolve solved the problem
+    R.status = CUOPT_TERIMINATION_STATUS_OPTIMAL;
+    postsolve(P, lp, R);
+    R.dual_objective = R.primal_objective;
+    R.info[CUOPT_AMD_INFO_STEPS] = 0;
+    R.solve_time = seconds();
+    R.info[CUOPT_AMD_INFO_SOLVE_TIME] = R.solve_time;
+    return R;
+  }
+  PdlpSettings rs = st;
+  if (!st.initial_primal.empty()) {
+    rs.initial_primal.clear();
+    for (int j : P.col_map) rs.initial_primal.push_back(j < (int)st.initial_primal.size() ? st.initial_primal[j] : 0.0);
+  }
+  if (!st.initial_dual.empty()) {
+    rs.initial_dual.clear();
+    for (int i : P.row_map) rs.initial_dual.push_back(i < (int)st.initial_dual.size() ? st.initial_dual[i] : 0.0);
+  }
+  R = solve_checked(P.reduced, P.lc, P.uc, P.lb, P.ub, rs, t0);
+  if (R.error == CUOPT_SUCCESS && R.x.size() == (size_t)P.reduced.n && R.y.size() == (size_t)P.reduced.m) {
+    postsolve(P, lp, R);
+  } else {
+    R.x.assign(lp.n, 0.0);
+    R.y.assign(lp.m, 0.0);
+    R.rc.assign(lp.n, 0.0);
+  }
+  R.solve_time = seconds();
+  R.info[CUOPT_AMD_INFO_SOLVE_TIME] = R.solve_time;
+  return R;
+}
+
+}  // namespace
+
 SolveResult solve_lp(const LpHost& lp, const PdlpSettings& st)
 {
   const auto t0 = std::chrono::steady_clock::now();
@@ -1688,6 +1749,22 @@ SolveResult solve_lp(const LpHost& lp, const PdlpSettings& st)
     R.rc.assign(n, 0.0);
     return R;
   }
+  // presolve: Default, Papilo and PSLP all select this engine's presolve; Off skips it
+  if (st.presolve != CUOPT_PRESOLVE_OFF && st.bench_steps <= 0) return solve_presolved(lp, lc, uc, lb, ub, st, t0);
+  return solve_checked(lp, lc, uc, lb, ub, st, t0);
+}
+
+namespace {
+
+SolveResult solve_checked(const LpHost& lp, const std::vector<double>& lc, const std::vector<double>& uc,
+                          const std::vector<double>& lb, const std::vector<double>& ub, const PdlpSettings& st,
+                          std::chrono::steady_clock::time_point t0)
+{
+  SolveResult R;
+  const int m = lp.m, n = lp.n;
+  auto seconds = [&]() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  };
   if (m == 0) {
     R.solve_time = seconds();
     R.x.assign(n, 0.0);
@@ -1726,6 +1803,8 @@ SolveResult solve_lp(const LpHost& lp, const PdlpSettings& st)
   PdlpSolver<double> S(lp, lc, uc, lb, ub, st);
   return S.run();
 }
+
+}  // namespace
 
 // ---------------------------------------------------------------------------
 // Test entry point: y = A x

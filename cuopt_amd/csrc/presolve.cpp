@@ -1,0 +1,221 @@
+// presolve.cpp -- see presolve.hpp.
+#include "presolve.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+
+namespace cuopt_amd {
+
+namespace {
+constexpr double kInf = std::numeric_limits<double>::infinity();
+}  // namespace
+
+Presolved presolve(const LpHost& lp, const std::vector<double>& lc0, const std::vector<double>& uc0,
+                   const std::vector<double>& lb0, const std::vector<double>& ub0)
+{
+  Presolved P;
+  const int m = lp.m, n = lp.n;
+  const int nnz = m > 0 ? lp.rowptr[m] : 0;
+  std::vector<double> lc(lc0), uc(uc0), lb(lb0), ub(ub0);
+  // Columns of A (CSC) for the substitution of fixed columns.
+  std::vector<int> cptr(n + 1, 0), crow(nnz);
+  std::vector<double> cval(nnz);
+  for (int k = 0; k < nnz; ++k) cptr[lp.colidx[k] + 1]++;
+  for (int j = 0; j < n; ++j) cptr[j + 1] += cptr[j];
+  {
+    std::vector<int> cur(cptr.begin(), cptr.end() - 1);
+    for (int i = 0; i < m; ++i)
+      for (int k = lp.rowptr[i]; k < lp.rowptr[i + 1]; ++k) {
+        const int d = cur[lp.colidx[k]]++;
+        crow[d]     = i;
+        cval[d]     = lp.vals[k];
+      }
+  }
+  std::vector<char> row_on(m, 1), col_on(n, 1);
+  std::vector<int> rcount(m), ccount(n);
+  for (int i = 0; i < m; ++i) rcount[i] = lp.rowptr[i + 1] - lp.rowptr[i];
+  for (int j = 0; j < n; ++j) ccount[j] = cptr[j + 1] - cptr[j];
+  P.x_removed.assign(n, 0.0);
+  P.lb_from.assign(n, -1);
+  P.ub_from.assign(n, -1);
+  double offset = lp.offset;
+  auto cost = [&](int j) { return lp.maximize ? -lp.c[j] : lp.c[j]; };  // minimisation sense
+
+  std::vector<int> rq, cq;  // rows / columns to look at again
+  for (int i = 0; i < m; ++i)
+    if (rcount[i] <= 1) rq.push_back(i);
+  for (int j = 0; j < n; ++j)
+    if (ccount[j] == 0 || lb[j] == ub[j]) cq.push_back(j);
+
+  auto remove_col = [&](int j, double v) {  // x_j = v everywhere
+    col_on[j]      = 0;
+    P.x_removed[j] = v;
+    ++P.removed_cols;
+    offset += lp.c[j] * v;
+    for (int k = cptr[j]; k < cptr[j + 1]; ++k) {
+      const int i = crow[k];
+      if (!row_on[i]) continue;
+      const double d = cval[k] * v;
+      if (std::isfinite(lc[i])) lc[i] -= d;
+      if (std::isfinite(uc[i])) uc[i] -= d;
+      if (--rcount[i] <= 1) rq.push_back(i);
+    }
+  };
+  while (!rq.empty() || !cq.empty()) {
+    while (!rq.empty()) {
+      const int i = rq.back();
+      rq.pop_back();
+      if (!row_on[i] || rcount[i] > 1) continue;
+      if (rcount[i] == 0) {  // empty row
+        const double tol = 1e-9 * (1.0 + std::max(std::fabs(std::isfinite(lc[i]) ? lc[i] : 0.0),
+                                                  std::fabs(std::isfinite(uc[i]) ? uc[i] : 0.0)));
+        if (lc[i] > tol || uc[i] < -tol) {
+          P.status = Presolved::INFEASIBLE;
+          return P;
+        }
+        row_on[i] = 0;
+        ++P.removed_rows;
+        continue;
+      }
+      int j = -1;  // singleton row: its one active column
+      double a = 0.0;
+      for (int k = lp.rowptr[i]; k < lp.rowptr[i + 1]; ++k)
+        if (col_on[lp.colidx[k]]) {
+          if (j == lp.colidx[k]) {
+            a += lp.vals[k];  // a repeated column index sums
+          } else {
+            j = lp.colidx[k];
+            a = lp.vals[k];
+          }
+        }
+      if (j < 0 || a == 0.0) {  // only zero coefficients left: an empty row in effect
+        rcount[i] = 0;
+        rq.push_back(i);
+        continue;
+      }
+      const int s    = (int)P.singletons.size();
+      P.singletons.push_back({i, j, a});
+      const double lo = a > 0 ? lc[i] / a : uc[i] / a;
+      const double hi = a > 0 ? uc[i] / a : lc[i] / a;
+      if (lo > lb[j]) {
+        lb[j]        = lo;
+        P.lb_from[j] = s;
+      }
+      if (hi < ub[j]) {
+        ub[j]        = hi;
+        P.ub_from[j] = s;
+      }
+      row_on[i] = 0;
+      ++P.removed_rows;
+      if (lb[j] > ub[j]) {
+        const double tol = 1e-9 * (1.0 + std::fabs(lb[j]) + std::fabs(ub[j]));
+        if (lb[j] - ub[j] > tol) {
+          P.status = Presolved::INFEASIBLE;
+          return P;
+        }
+        lb[j] = ub[j] = 0.5 * (lb[j] + ub[j]);
+      }
+      if (--ccount[j] == 0 || lb[j] == ub[j]) cq.push_back(j);
+    }
+    while (!cq.empty()) {
+      const int j = cq.back();
+      cq.pop_back();
+      if (!col_on[j]) continue;
+      if (lb[j] == ub[j]) {  // fixed
+        remove_col(j, lb[j]);
+        break;  // its rows may have become singletons
+      }
+      if (ccount[j] != 0) continue;
+      const double cj = cost(j);  // empty column: the bound its cost prefers
+      double v        = 0.0;
+      if (cj > 0.0) v = lb[j];
+      else if (cj < 0.0) v = ub[j];
+      else v = std::min(std::max(0.0, lb[j]), ub[j]);
+      if (!std::isfinite(v)) {
+        P.status = Presolved::UNBOUNDED;
+        return P;
+      }
+      remove_col(j, v);
+    }
+  }
+
+  // The reduced problem (ranged form, continuous variables).
+  std::vector<int> new_col(n, -1);
+  for (int j = 0; j < n; ++j)
+    if (col_on[j]) {
+      new_col[j] = (int)P.col_map.size();
+      P.col_map.push_back(j);
+    }
+  LpHost& R = P.reduced;
+  R.maximize = lp.maximize;
+  R.offset   = offset;
+  R.ranged   = true;
+  R.n        = (int)P.col_map.size();
+  R.rowptr.push_back(0);
+  for (int i = 0; i < m; ++i) {
+    if (!row_on[i]) continue;
+    P.row_map.push_back(i);
+    for (int k = lp.rowptr[i]; k < lp.rowptr[i + 1]; ++k) {
+      const int j = lp.colidx[k];
+      if (!col_on[j]) continue;
+      R.colidx.push_back(new_col[j]);
+      R.vals.push_back(lp.vals[k]);
+    }
+    R.rowptr.push_back((int)R.colidx.size());
+    P.lc.push_back(lc[i]);
+    P.uc.push_back(uc[i]);
+  }
+  R.m = (int)P.row_map.size();
+  for (int j : P.col_map) {
+    R.c.push_back(lp.c[j]);
+    P.lb.push_back(lb[j]);
+    P.ub.push_back(ub[j]);
+  }
+  R.lc = P.lc;
+  R.uc = P.uc;
+  R.lb = P.lb;
+  R.ub = P.ub;
+  R.var_types.assign(R.n, CUOPT_CONTINUOUS);
+  return P;
+}
+
+void postsolve(const Presolved& P, const LpHost& lp, SolveResult& R)
+{
+  const int m = lp.m, n = lp.n;
+  std::vector<double> x(P.x_removed), y(m, 0.0);
+  for (size_t j = 0; j < P.col_map.size() && j < R.x.size(); ++j) x[P.col_map[j]] = R.x[j];
+  for (size_t i = 0; i < P.row_map.size() && i < R.y.size(); ++i) y[P.row_map[i]] = R.y[i];
+  // z = c - A^T y over the original rows (singleton rows still at y = 0)
+  auto reduced_costs = [&](std::vector<double>& z) {
+    z.assign(n, 0.0);
+    for (int j = 0; j < n; ++j) z[j] = lp.c[j];
+    for (int i = 0; i < m; ++i)
+      if (y[i] != 0.0)
+        for (int k = lp.rowptr[i]; k < lp.rowptr[i + 1]; ++k) z[lp.colidx[k]] -= lp.vals[k] * y[i];
+  };
+  std::vector<double> z;
+  reduced_costs(z);
+  // A singleton row whose bound holds x_j takes x_j's reduced cost (latest first).
+  for (int s = (int)P.singletons.size() - 1; s >= 0; --s) {
+    const auto& S = P.singletons[s];
+    const int j   = S.col;
+    const bool at_lo = P.lb_from[j] == s, at_hi = P.ub_from[j] == s;
+    if (!at_lo && !at_hi) continue;
+    const double zj = lp.maximize ? -z[j] : z[j];  // minimisation sense
+    // at a lower bound the reduced cost is >= 0, at an upper bound <= 0
+    const bool use = (at_lo && zj > 0.0) || (at_hi && zj < 0.0);
+    if (!use) continue;
+    const double dy = z[j] / S.a;
+    y[S.row] += dy;
+    for (int k = lp.rowptr[S.row]; k < lp.rowptr[S.row + 1]; ++k) z[lp.colidx[k]] -= lp.vals[k] * dy;
+  }
+  double obj = lp.offset;
+  for (int j = 0; j < n; ++j) obj += lp.c[j] * x[j];
+  R.x  = std::move(x);
+  R.y  = std::move(y);
+  R.rc = std::move(z);
+  R.primal_objective = obj;
+}
+
+}  // namespace cuopt_amd

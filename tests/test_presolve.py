@@ -1,0 +1,153 @@
+"""LP presolve and postsolve (presolve.cpp).
+
+The reference presolves with a third-party library (PSLP, solve.cu:1183-1325)
+whose reductions are not reproducible here; what is pinned is the contract:
+the postsolved (x, y, reduced costs) is an optimal primal-dual pair of the
+ORIGINAL problem (checked independently in numpy: feasibility, sign
+conditions, complementary slackness, objective), infeasible / unbounded
+detections are right, and solves with and without presolve agree.  Problems
+that presolve solves completely need no GPU."""
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from cuopt_amd import c_api
+from cuopt_amd import generators as G
+
+
+def kkt_ok(lp, r, tol=1e-7):
+    """Optimality of (x, y, z) for lp in numpy (minimisation sense after the max flip)."""
+    A = sp.csr_matrix((lp.values, lp.col_indices, lp.row_offsets), shape=(lp.m, lp.n))
+    x, y, z = r.x, r.y, r.reduced_cost
+    sgn = -1.0 if lp.maximize else 1.0
+    ax = A @ x
+    scale = 1.0 + np.abs(ax).max(initial=0.0)
+    assert np.all(ax >= lp.con_lb - tol * scale) and np.all(ax <= lp.con_ub + tol * scale)
+    assert np.all(x >= lp.var_lb - tol) and np.all(x <= lp.var_ub + tol)
+    assert np.allclose(z, lp.c - A.T @ y, atol=1e-9 * (1 + np.abs(lp.c).max(initial=0)))
+    zs, ys = sgn * z, sgn * y
+    zt = tol * (1 + np.abs(lp.c).max(initial=0.0))
+    for j in range(lp.n):
+        at_lo = x[j] <= lp.var_lb[j] + tol
+        at_hi = x[j] >= lp.var_ub[j] - tol
+        if not at_lo:
+            assert zs[j] <= zt, ("reduced cost", j, zs[j])
+        if not at_hi:
+            assert zs[j] >= -zt, ("reduced cost", j, zs[j])
+    for i in range(lp.m):
+        lo = ax[i] <= lp.con_lb[i] + tol * scale
+        hi = ax[i] >= lp.con_ub[i] - tol * scale
+        if not lo:
+            assert ys[i] <= zt, ("row dual", i, ys[i])
+        if not hi:
+            assert ys[i] >= -zt, ("row dual", i, ys[i])
+    return True
+
+
+def lp_from(A, c, lc, uc, lb, ub, maximize=False):
+    A = sp.csr_matrix(A)
+    A.sort_indices()
+    return G.LP(m=A.shape[0], n=A.shape[1], row_offsets=A.indptr.astype(np.int32),
+                col_indices=A.indices.astype(np.int32), values=A.data.astype(np.float64), c=np.asarray(c, float),
+                con_lb=np.asarray(lc, float), con_ub=np.asarray(uc, float), var_lb=np.asarray(lb, float),
+                var_ub=np.asarray(ub, float), maximize=maximize, name="presolve_case")
+
+
+INF = np.inf
+
+
+def fully_reducible():
+    # min x0 + 2 x1 - x2 + 0 x3 with
+    #   r0: 2 x0 >= 4 (singleton), r1: -x1 <= -1 (singleton, negative coefficient),
+    #   r2: empty row 0 = 0, r3: x2 <= 5 (singleton), r4: x0 + x3 >= 4 with x3 fixed at 3.
+    # Optimum x = (2, 1, 5, 3), objective -1; duals y = (1/2, -2, 0, -1, 0), z = 0.
+    A = np.array([[2, 0, 0, 0], [0, -1, 0, 0], [0, 0, 0, 0], [0, 0, 1, 0], [1, 0, 0, 1]], float)
+    return lp_from(A, [1, 2, -1, 0], [4, -INF, 0, -INF, 4], [INF, -1, 0, 5, INF], [0, 0, 0, 3], [INF, INF, INF, 3])
+
+
+def test_presolve_solves_singletons_fixed_and_empty_rows_without_gpu():
+    lp = fully_reducible()
+    r = c_api.solve(lp, tol=1e-8, presolve=True)
+    assert r.error == 0 and r.status == 1
+    assert np.allclose(r.x, [2, 1, 5, 3]) and abs(r.primal_objective - (-1.0)) < 1e-12
+    assert np.allclose(r.y, [0.5, -2, 0, -1, 0]) and np.allclose(r.reduced_cost, 0)
+    assert kkt_ok(lp, r)
+
+
+def test_presolve_maximisation_and_empty_columns():
+    # max 3 x0 - x1 + 2 x2 s.t. x0 <= 4 (singleton), x1 >= -2 via -x1 <= 2; x2 in no row, in [0, 7].
+    A = sp.csr_matrix(np.array([[1.0, 0, 0], [0, -1.0, 0]]))
+    lp = lp_from(A, [3, -1, 2], [-INF, -INF], [4, 2], [-INF, -INF, 0], [INF, INF, 7], maximize=True)
+    r = c_api.solve(lp, tol=1e-8, presolve=True)
+    assert r.status == 1 and np.allclose(r.x, [4, -2, 7])
+    assert abs(r.primal_objective - (12 + 2 + 14)) < 1e-12
+    assert kkt_ok(lp, r)
+
+
+def test_presolve_detects_infeasible_and_unbounded():
+    # empty equality row with b = 1
+    lp = lp_from(np.array([[0.0, 0.0], [1.0, 1.0]]), [1, 1], [1, 0], [1, 10], [0, 0], [5, 5])
+    assert c_api.solve(lp, presolve=True).status == 2
+    # singleton rows giving x0 >= 3 and x0 <= 2
+    lp = lp_from(np.array([[1.0, 0.0], [1.0, 0.0], [1.0, 1.0]]), [1, 1], [3, -INF, 0], [INF, 2, 10], [0, 0], [9, 9])
+    assert c_api.solve(lp, presolve=True).status == 2
+    # x1 in no row, cost -1, no upper bound: unbounded (as the reference reports it: DualInfeasible)
+    lp = lp_from(np.array([[1.0, 0.0]]), [1, -1], [0], [1], [0, 0], [5, INF])
+    assert c_api.solve(lp, presolve=True).status == 3
+
+
+def test_presolve_off_keeps_the_no_constraint_rule():
+    lp = fully_reducible()
+    r = c_api.solve(lp, presolve=False)
+    if c_api.device_count() == 0:
+        assert r.error == c_api.CUOPT_RUNTIME_ERROR  # the reduced problem would need the GPU
+    else:
+        assert r.status == 1
+
+
+# ---- GPU: problems with a part left for PDLP ---------------------------------------
+
+def augmented(seed):
+    """A generated LP with singleton rows, a fixed column, empty rows and columns added."""
+    base = G.random_lp(m=120, n=300, seed=seed)
+    A = sp.csr_matrix((base.values, base.col_indices, base.row_offsets), shape=(base.m, base.n)).tolil()
+    m, n = base.m, base.n
+    rng = np.random.default_rng(seed)
+    A.resize((m + 4, n + 2))
+    lc = list(base.con_lb) + [0.0, -INF, 0.0, 0.5]
+    uc = list(base.con_ub) + [0.0, INF, INF, 1.5]
+    A[m + 1, 5] = 2.0          # singleton: 2 x5 in (-inf, inf): a free row
+    A[m + 2, 7] = -1.0         # singleton: -x7 >= 0 -> x7 <= 0, with x7 >= 0: fixes x7 at 0
+    A[m + 3, n] = 1.0          # singleton on a new column: x_n in [0.5, 1.5]
+    A[m + 3, n] = 1.0
+    c = np.append(base.c, [1.0, -0.5])  # x_n (cost 1) goes to 0.5; x_{n+1} (in no row) to its upper bound 4
+    lb = np.append(base.var_lb, [0.0, 0.0])
+    ub = np.append(base.var_ub, [10.0, 4.0])
+    return lp_from(A.tocsr(), c, lc, uc, lb, ub)
+
+
+gpu = pytest.mark.gpu
+
+
+@gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
+def test_presolve_matches_the_plain_solve(precision):
+    for seed in (1, 2):
+        lp = augmented(seed)
+        a = c_api.solve(lp, tol=1e-8, precision=precision, time_limit=120.0, presolve=False)
+        b = c_api.solve(lp, tol=1e-8, precision=precision, time_limit=120.0, presolve=True)
+        assert a.status == b.status == 1
+        assert abs(a.primal_objective - b.primal_objective) <= 1e-6 * (1 + abs(a.primal_objective))
+        assert kkt_ok(lp, b, tol=1e-5)
+
+
+@gpu
+def test_presolve_on_skewed_lp_with_empty_rows():
+    lp = G.skewed_lp(m=4000, n=8000, seed=3)  # many empty rows
+    a = c_api.solve(lp, tol=1e-8, time_limit=120.0, presolve=False)
+    b = c_api.solve(lp, tol=1e-8, time_limit=120.0, presolve=True)
+    assert a.status == b.status == 1
+    assert abs(b.primal_objective - lp.obj_opt) <= 1e-6 * (1 + abs(lp.obj_opt))
+    assert abs(a.primal_objective - b.primal_objective) <= 1e-6 * (1 + abs(lp.obj_opt))
+    assert b.x.shape == (lp.n,) and b.y.shape == (lp.m,)
