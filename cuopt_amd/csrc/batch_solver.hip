@@ -39,6 +39,10 @@ namespace cuopt_amd {
 namespace {
 
 constexpr int kBT = 256;  // lanes per LP workgroup
+// Workgroups per CU the kernel is built for (its register budget, __launch_bounds__)
+// and the LDS share of each.  Three (at most 168 VGPRs) spilled ~100 registers
+// to scratch and ran 10-40 % slower than two.
+constexpr int kWgPerCu = 2;
 
 struct BatchDesc {
   int m, n, nnz;
@@ -401,7 +405,7 @@ __device__ inline double bound_sq(double l, double u)
 // ---- the kernel ----------------------------------------------------------------
 
 template <typename V>
-__global__ __launch_bounds__(kBT) void batch_pdlp_kernel(BatchArgs<V> A)
+__global__ __launch_bounds__(kBT, kWgPerCu) void batch_pdlp_kernel(BatchArgs<V> A)
 {
   using V2 = typename Vec2<V>::type;
   __shared__ double lds[kBT / kWave][kMaxRed];
@@ -1111,13 +1115,13 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
   }
   {
     // LDS: the row-block stage, then the gathered vectors of the largest LP when
-    // they fit in the per-workgroup limit and half a CU's LDS (two workgroups per CU).
+    // they fit in the per-workgroup limit and a CU's LDS divided by kWgPerCu.
     int dev = 0, per_block = 0, per_cu = 0;
     CUOPT_HIP_CHECK(hipGetDevice(&dev));
     CUOPT_HIP_CHECK(hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     CUOPT_HIP_CHECK(hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
     const long long static_lds = 4096;  // the kernel's own __shared__ arrays, rounded up
-    long long cap = std::min<long long>(per_block, per_cu / 2) - static_lds;
+    long long cap = std::min<long long>(per_block, per_cu / kWgPerCu) - static_lds;
     long long need = 0;
     for (int a : active) need = std::max<long long>(need, (2LL * desc[a].m + desc[a].n) * (long long)sizeof(V));
     if (cap < kStageBytes) throw std::runtime_error("batch solve: the device has too little LDS per workgroup");
