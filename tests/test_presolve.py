@@ -151,3 +151,60 @@ def test_presolve_on_skewed_lp_with_empty_rows():
     assert abs(b.primal_objective - lp.obj_opt) <= 1e-6 * (1 + abs(lp.obj_opt))
     assert abs(a.primal_objective - b.primal_objective) <= 1e-6 * (1 + abs(lp.obj_opt))
     assert b.x.shape == (lp.n,) and b.y.shape == (lp.m,)
+
+
+def _random_reducible(seed):
+    """Random LPs made only of singleton rows (several per variable, both coefficient
+    signs, equality / <= / >= / ranged), empty rows, fixed and free columns: presolve
+    solves them completely, so no GPU is needed."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(2, 9))
+    lb = rng.uniform(-10, 0, n)
+    ub = lb + rng.uniform(0, 20, n)
+    fixed = rng.random(n) < 0.2
+    ub[fixed] = lb[fixed]
+    xs = lb + rng.random(n) * (ub - lb)  # a point the rows keep feasible, unless shifted below
+    rows, lc, uc = [], [], []
+    for _ in range(int(rng.integers(1, 3 * n))):
+        j = int(rng.integers(0, n))
+        a = float(rng.choice([-1, 1]) * rng.uniform(0.5, 3.0))
+        kind = rng.integers(0, 4)
+        v = a * xs[j] + (float(rng.uniform(1, 3)) if rng.random() < 0.1 else 0.0)
+        d1, d2 = float(rng.uniform(0, 2)), float(rng.uniform(0, 2))
+        lo, hi = {0: (v, v), 1: (-INF, v + d1), 2: (v - d1, INF), 3: (v - d1, v + d2)}[int(kind)]
+        r = np.zeros(n)
+        r[j] = a
+        rows.append(r)
+        lc.append(lo)
+        uc.append(hi)
+    for _ in range(int(rng.integers(0, 3))):  # empty rows that hold
+        rows.append(np.zeros(n))
+        lc.append(-float(rng.uniform(0, 1)))
+        uc.append(float(rng.uniform(0, 1)))
+    c = rng.normal(size=n)
+    return lp_from(np.array(rows), c, lc, uc, lb, ub, maximize=bool(rng.random() < 0.3))
+
+
+def test_presolve_random_reducible_lps_against_highs():
+    from scipy.optimize import linprog
+
+    solved = infeasible = 0
+    for seed in range(300):
+        lp = _random_reducible(seed)
+        A = sp.csr_matrix((lp.values, lp.col_indices, lp.row_offsets), shape=(lp.m, lp.n)).toarray()
+        sgn = -1.0 if lp.maximize else 1.0
+        fin_u, fin_l = np.isfinite(lp.con_ub), np.isfinite(lp.con_lb)
+        A_ub = np.vstack([A[fin_u], -A[fin_l]]) if (fin_u.any() or fin_l.any()) else None
+        b_ub = np.concatenate([lp.con_ub[fin_u], -lp.con_lb[fin_l]]) if A_ub is not None else None
+        ref = linprog(sgn * lp.c, A_ub=A_ub, b_ub=b_ub, bounds=list(zip(lp.var_lb, lp.var_ub)), method="highs")
+        r = c_api.solve(lp, tol=1e-9, presolve=True)
+        assert r.error == 0, (seed, r.error_string)
+        if ref.status == 2:
+            assert r.status == 2, seed
+            infeasible += 1
+            continue
+        assert ref.status == 0 and r.status == 1, (seed, ref.status, r.status)
+        assert abs(r.primal_objective - sgn * ref.fun) <= 1e-9 * (1 + abs(ref.fun)), seed
+        assert kkt_ok(lp, r, tol=1e-9), seed
+        solved += 1
+    assert solved > 150 and infeasible > 10
