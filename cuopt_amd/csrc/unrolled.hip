@@ -347,7 +347,7 @@ Work<V> carve(UnrolledImpl& U, int nseg, bool allocate)
   const size_t o_sx = take((size_t)(U.every + 1) * nB * sizeof(V)), o_sy = take((size_t)(U.every + 1) * mB * sizeof(V));
   const size_t o_ax = take(nB * sizeof(V)), o_gt = take(mB * sizeof(V)), o_gw = take(nB * sizeof(V));
   const size_t o_gc = take(nB * sizeof(double)), o_glc = take(mB * sizeof(double)), o_guc = take(mB * sizeof(double));
-  if (allocate) U.work.resize(off);
+  if (allocate && U.work.size() < off) U.work.resize(off);  // kept across calls: no free / malloc per forward
   unsigned char* p = U.work.get();
   auto at = [&](size_t o) { return reinterpret_cast<V*>(p + o); };
   auto atd = [&](size_t o) { return reinterpret_cast<double*>(p + o); };
