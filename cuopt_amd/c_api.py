@@ -110,7 +110,8 @@ def _a(x, dtype):
 
 
 def _p(a):
-    return a.ctypes.data_as(ctypes.c_void_p)
+    """Address of a contiguous numpy array, for c_void_p arguments (the caller keeps a alive)."""
+    return a.ctypes.data
 
 
 class CuOptError(RuntimeError):
