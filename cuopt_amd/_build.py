@@ -22,6 +22,12 @@ def _hipcc():
             return c
 
 
+def _rocm_include():
+    root = os.environ.get("ROCM_PATH") or os.path.dirname(os.path.dirname(os.path.realpath(_hipcc()))) \
+        if os.path.isabs(_hipcc()) else "/opt/rocm"
+    return os.path.join(root, "include")
+
+
 def build_library(verbose: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     hipcc = _hipcc()
@@ -33,7 +39,9 @@ def build_library(verbose: bool = False) -> str:
         objs.append(o)
     for src in CXX_SOURCES:
         o = os.path.join(OBJ, src + ".o")
-        jobs.append([hipcc, *COMMON, "-D__HIP_PLATFORM_AMD__", "-c", os.path.join(CSRC, src), "-o", o])
+        # host-only C++ (hipcc would otherwise compile .cpp files as HIP for a default GPU)
+        jobs.append([hipcc, *COMMON, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I" + _rocm_include(), "-c",
+                     os.path.join(CSRC, src), "-o", o])
         objs.append(o)
     procs = [(j, subprocess.Popen(j, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)) for j in jobs]
     for j, p in procs:

@@ -3,13 +3,8 @@
 
 #include <algorithm>
 #include <cmath>
-#include <limits>
 
 namespace cuopt_amd {
-
-namespace {
-constexpr double kInf = std::numeric_limits<double>::infinity();
-}  // namespace
 
 Presolved presolve(const LpHost& lp, const std::vector<double>& lc0, const std::vector<double>& uc0,
                    const std::vector<double>& lb0, const std::vector<double>& ub0)
