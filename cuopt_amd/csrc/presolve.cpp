@@ -118,6 +118,10 @@ Presolved presolve(const LpHost& lp, const std::vector<double>& lc0, const std::
       cq.pop_back();
       if (!col_on[j]) continue;
       if (lb[j] == ub[j]) {  // fixed
+        if (!std::isfinite(lb[j])) {  // fixed at an infinite value: no finite point exists
+          P.status = Presolved::INFEASIBLE;
+          return P;
+        }
         remove_col(j, lb[j]);
         break;  // its rows may have become singletons
       }

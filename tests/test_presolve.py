@@ -92,6 +92,9 @@ def test_presolve_detects_infeasible_and_unbounded():
     # singleton rows giving x0 >= 3 and x0 <= 2
     lp = lp_from(np.array([[1.0, 0.0], [1.0, 0.0], [1.0, 1.0]]), [1, 1], [3, -INF, 0], [INF, 2, 10], [0, 0], [9, 9])
     assert c_api.solve(lp, presolve=True).status == 2
+    # a variable fixed at +inf has no finite value
+    lp = lp_from(np.array([[1.0, 1.0]]), [1, 1], [0], [10], [0, INF], [5, INF])
+    assert c_api.solve(lp, presolve=True).status == 2
     # x1 in no row, cost -1, no upper bound: unbounded (as the reference reports it: DualInfeasible)
     lp = lp_from(np.array([[1.0, 0.0]]), [1, -1], [0], [1], [0, 0], [5, INF])
     assert c_api.solve(lp, presolve=True).status == 3
