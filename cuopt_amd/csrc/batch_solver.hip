@@ -168,21 +168,21 @@ __device__ inline void zero_red(double (&r)[kMaxRed])
 
 // Row passes walk an LP's CSR in row blocks planned on the host (plan_row_blocks):
 // a block holds at most kStage nonzeros and at most one row per lane group, or
-// is a single row longer than kStage.  For each block the workgroup
-//   1. copies the block's indices / values into LDS with global_load_lds (the
-//      data goes from memory to LDS without passing through registers) while
-//      each lane group loads the row bounds and epilogue operands of its row,
-//   2. lets each group of G lanes reduce its row from LDS and run the epilogue.
-// A block thus costs about one memory round trip, where a row-at-a-time walk
-// paid one per row step.  Each lane adds its row's products in increasing k and
-// groups merge by the same shuffle tree as before, so the sums do not depend on
-// the blocking.
+// is a single row longer than kStage.  The stage has two buffers; block b uses
+// buffer b & 1.  While the workgroup reduces block b from LDS it
+//   - copies block b + 1's indices / values into the other buffer with
+//     global_load_lds (memory to LDS without passing through registers),
+//   - loads the row bounds of its lane group's row in block b + 1,
+// so the copy's round trip overlaps the reduction (one barrier per block).
+// Each group of G lanes reduces one row and runs the epilogue.
+// Each lane adds its row's products in increasing k and groups merge by a
+// fixed shuffle tree, so the sums do not depend on the blocking.
 constexpr int kStage      = 2048;
-constexpr int kStageBytes = kStage * 12;  // values (8 bytes) then indices (4 bytes)
+constexpr int kStageBytes = 2 * kStage * 12;  // two buffers of values (8 bytes) and indices (4 bytes)
 
 struct Stage {               // LDS of one workgroup
-  int* idx;                  // kStage column indices
-  unsigned char* vals;       // kStage values, 8 bytes each (FP64 or FP32 matrices)
+  int* idx;                  // 2 x kStage column indices (buffer b at idx + b * kStage)
+  unsigned char* vals;       // 2 x kStage values of 8 bytes (FP64 or FP32 matrices), same order
   uint64_t* merge;           // kBT / kWave accumulator slots
 };
 
@@ -259,48 +259,57 @@ __device__ inline void rows_pass(const int* __restrict__ ptr,
   using Acc       = typename Epi::Acc;
   using Mat       = typename Epi::Mat;
   constexpr int W = (int)(sizeof(Mat) / 4);  // dwords per value
-  Mat* svals      = reinterpret_cast<Mat*>(st.vals);
   const int G = 1 << lg, sub = threadIdx.x & (G - 1), grp = threadIdx.x >> lg;
-  if (blocks.nblk <= 0) return;
-  // Block bounds run one block ahead: the loads for block bi + 1 are issued
-  // while block bi is processed.
+  const int nb = blocks.nblk;
+  if (nb <= 0) return;
+  // Buffer addresses are computed, not indexed from an array: a runtime-indexed
+  // pointer array would live in scratch, and its load would wait for the copy
+  // in flight.
+  auto stage = [&](int za, int zb, int buf) {
+    copy_to_lds(reinterpret_cast<const int*>(vals), (long long)W * za, (long long)W * zb,
+                reinterpret_cast<int*>(st.vals + buf * (kStage * 8)));
+    copy_to_lds(idx, za, zb, st.idx + buf * kStage);
+  };
+  // (r0, r1) are block bi's rows, (r1, r2) block bi + 1's, z the matching
+  // nonzero offsets; r3 / z3 (block bi + 2's end) are loaded during block bi.
   int r0 = blocks.start[0], r1 = blocks.start[1];
   int z0 = ptr[r0], z1 = ptr[r1];
-  for (int bi = 0; bi < blocks.nblk; ++bi) {
-    int r2 = r1, z2 = z1;
-    if (bi + 2 <= blocks.nblk) {
-      r2 = blocks.start[bi + 2];
-      z2 = ptr[r2];
-    }
+  int r2 = r1, z2 = z1;
+  if (nb >= 2) {
+    r2 = blocks.start[2];
+    z2 = ptr[r2];
+  }
+  if (z1 - z0 <= kStage) stage(z0, z1, 0);
+  // This lane group's row of the current block: its ptr entries.  The loads
+  // are clamped into range instead of guarded, so that no branch waits for them
+  // (a wait for a load also waits for every copy issued before it).
+  int pb = ptr[min(r0 + grp, r1)], pe = ptr[min(r0 + grp + 1, r1)];
+  for (int bi = 0; bi < nb; ++bi) {
+    const int cur = bi & 1;
+    // Every wave's copy of block bi has landed (the barrier waits for each
+    // wave's memory operations), and block bi - 1's reads of the other buffer
+    // are done, so it may be refilled.
+    __syncthreads();
+    const int b = pb - z0, e = pe - z0;
+    const int r3          = bi + 3 <= nb ? blocks.start[bi + 3] : r2;
+    const bool stage_next = bi + 1 < nb && z2 - z1 <= kStage;
+    if (stage_next) stage(z1, z2, cur ^ 1);
+    const int npb = ptr[min(r1 + grp, r2)], npe = ptr[min(r1 + grp + 1, r2)];
     if (z1 - z0 > kStage) {  // one row longer than the stage: read it in place
       const auto pre = epi.load(r0);
       Acc a          = Acc::zero();
       for (int k = z0 + (int)threadIdx.x; k < z1; k += kBT) a.add(epi.product(vals[k], idx[k], r0));
       a = wg_merge(a, st.merge);
       if (threadIdx.x == 0) epi.row(r0, a, pre, red);
-      r0 = r1;
-      z0 = z1;
-      r1 = r2;
-      z1 = z2;
-      continue;
-    }
-    const int r = r0 + grp;
-    int b = 0, e = 0;
-    typename Epi::Pre pre{};
-    if (r < r1) {
-      b   = ptr[r] - z0;
-      e   = ptr[r + 1] - z0;
-      pre = epi.load(r);
-    }
-    copy_to_lds(reinterpret_cast<const int*>(vals), (long long)W * z0, (long long)W * z1,
-                reinterpret_cast<int*>(st.vals));
-    copy_to_lds(idx, z0, z1, st.idx);
-    __syncthreads();
-    if (r < r1) {
+    } else if (r0 + grp < r1) {
+      const int r       = r0 + grp;
+      const auto pre    = epi.load(r);
+      const int* sidx   = st.idx + cur * kStage;
+      const Mat* svals  = reinterpret_cast<const Mat*>(st.vals + cur * (kStage * 8));
       Acc a = Acc::zero();
       int k = b + sub;
       for (; k + 3 * G < e; k += 4 * G) {  // four LDS reads in flight, added in order
-        const int c0 = st.idx[k], c1 = st.idx[k + G], c2 = st.idx[k + 2 * G], c3 = st.idx[k + 3 * G];
+        const int c0 = sidx[k], c1 = sidx[k + G], c2 = sidx[k + 2 * G], c3 = sidx[k + 3 * G];
         const Mat v0 = svals[k], v1 = svals[k + G], v2 = svals[k + 2 * G], v3 = svals[k + 3 * G];
         const auto p0 = epi.product(v0, c0, r), p1 = epi.product(v1, c1, r);
         const auto p2 = epi.product(v2, c2, r), p3 = epi.product(v3, c3, r);
@@ -309,18 +318,19 @@ __device__ inline void rows_pass(const int* __restrict__ ptr,
         a.add(p2);
         a.add(p3);
       }
-      for (; k < e; k += G) a.add(epi.product(svals[k], st.idx[k], r));
+      for (; k < e; k += G) a.add(epi.product(svals[k], sidx[k], r));
       for (int off = G >> 1; off > 0; off >>= 1) a.merge(a.shfl(off));
       if (sub == 0) epi.row(r, a, pre, red);
     }
-    // Every wave has consumed its LDS reads of the stage: the next block may
-    // overwrite it.  A bare barrier -- the epilogue's stores stay in flight and
-    // are waited for by the next block's barrier, together with its copy.
-    __builtin_amdgcn_s_barrier();
+    const int z3 = bi + 3 <= nb ? ptr[r3] : z2;
     r0 = r1;
     z0 = z1;
     r1 = r2;
     z1 = z2;
+    r2 = r3;
+    z2 = z3;
+    pb = npb;
+    pe = npe;
   }
   __syncthreads();  // the pass's stores are visible to the whole workgroup
 }
@@ -442,7 +452,7 @@ __global__ __launch_bounds__(kBT, kWgPerCu) void batch_pdlp_kernel(BatchArgs<V> 
   // x-bar (dual step) -- live in LDS when the LP fits: each gather is then an
   // LDS read instead of a round trip to L2 / MALL.  Nothing reads them after
   // the kernel (the solution is the unscaled xu / yu / su).
-  const Stage st{reinterpret_cast<int*>(dyn_lds + kStage * 8), dyn_lds, merge_lds};
+  const Stage st{reinterpret_cast<int*>(dyn_lds + 2 * kStage * 8), dyn_lds, merge_lds};
   const RowBlocks ablk{A.blk + d.ablk0, d.nablk}, tblk{A.blk + d.tblk0, d.ntblk};
   if (kStageBytes + (long long)(2 * m + n) * (long long)sizeof(V) <= (long long)A.lds_bytes) {
     V* base = reinterpret_cast<V*>(dyn_lds + kStageBytes);
