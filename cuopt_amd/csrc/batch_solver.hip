@@ -167,8 +167,8 @@ __device__ inline void zero_red(double (&r)[kMaxRed])
 // ---- row passes ---------------------------------------------------------------
 
 // Row passes walk an LP's CSR in row blocks planned on the host (plan_row_blocks):
-// a block holds at most kStage nonzeros and at most one row per lane group, or
-// is a single row longer than kStage.  The stage has two buffers; block b uses
+// a block holds at most kBlockNnz nonzeros and at most one row per lane group,
+// or is a single row longer than that.  The stage has two buffers; block b uses
 // buffer b & 1.  While the workgroup reduces block b from LDS it
 //   - copies block b + 1's indices / values into the other buffer with
 //     global_load_lds (memory to LDS without passing through registers),
@@ -177,8 +177,10 @@ __device__ inline void zero_red(double (&r)[kMaxRed])
 // Each group of G lanes reduces one row and runs the epilogue.
 // Each lane adds its row's products in increasing k and groups merge by a
 // fixed shuffle tree, so the sums do not depend on the blocking.
-constexpr int kStage      = 2048;
-constexpr int kStageBytes = 2 * kStage * 12;  // two buffers of values (8 bytes) and indices (4 bytes)
+constexpr int kStage      = 2048;                // entries per stage buffer
+constexpr int kBlockNnz   = kStage - 8;          // nonzeros per staged block (room for the copy's slack)
+constexpr int kStageBytes = 2 * kStage * 12;     // two buffers of values (8 bytes) and indices (4 bytes)
+constexpr int kNnzPad     = 4;                   // extra elements after each nonzero array (16 bytes or more)
 
 struct Stage {               // LDS of one workgroup
   int* idx;                  // 2 x kStage column indices (buffer b at idx + b * kStage)
@@ -210,23 +212,37 @@ void plan_row_blocks(int nrows, int lg, Len len, std::vector<int>& out)
   while (r < nrows) {
     long long z = len(r);
     int e       = r + 1;
-    if (z <= kStage)
-      while (e < nrows && e - r < max_rows && z + len(e) <= kStage) z += len(e++);
+    if (z <= kBlockNnz)
+      while (e < nrows && e - r < max_rows && z + len(e) <= kBlockNnz) z += len(e++);
     out.push_back(e);
     r = e;
   }
 }
 
-// Copies the dwords [d0, d1) of src to dst[0, d1 - d0) in LDS: wave w moves
-// elements w*64 + i*256 + lane, and global_load_lds writes lane l's dword to the
-// wave-uniform LDS address given plus 4 l.  The caller's barrier waits for it.
+// Dword offset of p within its 16-byte line.
+__device__ inline int dword_mis(const void* p)
+{
+  return (int)((reinterpret_cast<uintptr_t>(p) >> 2) & 3);
+}
+
+// Copies the dwords [d0, d1) of src into LDS in 16-byte pieces: the copy starts
+// at the 16-byte boundary at or below src + d0, so dword d0 lands at
+// dst[dword_mis(src + d0)], and up to 3 dwords past d1 come along (the nonzero
+// arrays carry kNnzPad elements of padding).  Wave w moves pieces
+// w*64 + i*256 + lane; global_load_lds with 16 bytes writes lane l's piece to
+// the wave-uniform LDS address given plus 16 l.  One 16-byte request moves four
+// times the data of a dword request, and the copy is bound by requests.  The
+// caller's barrier waits for it.
 __device__ inline void copy_to_lds(const int* __restrict__ src, long long d0, long long d1, int* dst)
 {
+  if (d1 <= d0) return;
   const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
-  const int cnt  = (int)(d1 - d0);
-  for (int base = w * kWave; base < cnt; base += kBT)
-    if (base + lane < cnt)
-      __builtin_amdgcn_global_load_lds(src + d0 + base + lane, dst + base, 4, 0, 0);
+  const int mis  = dword_mis(src + d0);
+  const int* s0  = src + d0 - mis;
+  const int npc  = (int)((d1 - d0 + mis + 3) >> 2);
+  for (int base = w * kWave; base < npc; base += kBT)
+    if (base + lane < npc)
+      __builtin_amdgcn_global_load_lds(s0 + 4 * (base + lane), dst + 4 * base, 16, 0, 0);
 }
 
 // Fixed-order workgroup merge of one accumulator (wave butterflies, then waves 0..3).
@@ -279,7 +295,7 @@ __device__ inline void rows_pass(const int* __restrict__ ptr,
     r2 = blocks.start[2];
     z2 = ptr[r2];
   }
-  if (z1 - z0 <= kStage) stage(z0, z1, 0);
+  if (z1 - z0 <= kBlockNnz) stage(z0, z1, 0);
   // This lane group's row of the current block: its ptr entries.  The loads
   // are clamped into range instead of guarded, so that no branch waits for them
   // (a wait for a load also waits for every copy issued before it).
@@ -292,10 +308,10 @@ __device__ inline void rows_pass(const int* __restrict__ ptr,
     __syncthreads();
     const int b = pb - z0, e = pe - z0;
     const int r3          = bi + 3 <= nb ? blocks.start[bi + 3] : r2;
-    const bool stage_next = bi + 1 < nb && z2 - z1 <= kStage;
+    const bool stage_next = bi + 1 < nb && z2 - z1 <= kBlockNnz;
     if (stage_next) stage(z1, z2, cur ^ 1);
     const int npb = ptr[min(r1 + grp, r2)], npe = ptr[min(r1 + grp + 1, r2)];
-    if (z1 - z0 > kStage) {  // one row longer than the stage: read it in place
+    if (z1 - z0 > kBlockNnz) {  // one row longer than a block: read it in place
       const auto pre = epi.load(r0);
       Acc a          = Acc::zero();
       for (int k = z0 + (int)threadIdx.x; k < z1; k += kBT) a.add(epi.product(vals[k], idx[k], r0));
@@ -304,8 +320,11 @@ __device__ inline void rows_pass(const int* __restrict__ ptr,
     } else if (r0 + grp < r1) {
       const int r       = r0 + grp;
       const auto pre    = epi.load(r);
-      const int* sidx   = st.idx + cur * kStage;
-      const Mat* svals  = reinterpret_cast<const Mat*>(st.vals + cur * (kStage * 8));
+      // where the copy put nonzero z0 (values are 8-byte aligned when W = 2,
+      // so their dword offset is even)
+      const int* sidx  = st.idx + cur * kStage + dword_mis(idx + z0);
+      const Mat* svals = reinterpret_cast<const Mat*>(st.vals + cur * (kStage * 8)) +
+                         dword_mis(reinterpret_cast<const int*>(vals) + (long long)W * z0) / W;
       Acc a = Acc::zero();
       int k = b + sub;
       for (; k + 3 * G < e; k += 4 * G) {  // four LDS reads in flight, added in order
@@ -1030,10 +1049,13 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
   const size_t M = std::max(tot_m, 1), N = std::max(tot_n, 1), Z = std::max<long long>(tot_nnz, 1);
   DevBuf<BatchDesc> d_desc(std::max<size_t>(desc.size(), 1));
   DevBuf<int> d_blk(std::max<size_t>(h_blk.size(), 1));
-  DevBuf<int> d_order(std::max(nb, 1)), a_ptr(std::max(tot_ap, 1)), a_idx(Z), t_ptr(std::max(tot_tp, 1)),
-    t_idx(Z), t_pos(Z), t_fill(N);
-  DevBuf<double> a_orig(Z), t_orig(Z), c(N), lb(N), ub(N), lc(M), uc(M), Dr(M), Dc(N), rowv(M), colv(N);
-  DevBuf<V> a_w(Z), t_w(Z), c_w(N), x0(N), x1(N), y0(M), y1(M), xa(N), ya(M), xbar(N), ybar(M), xhat(N), yhat(M),
+  // The nonzero arrays read by row passes carry kNnzPad elements of padding:
+  // a block's 16-byte copy may read up to 3 dwords past its last nonzero.
+  const size_t Zp = Z + kNnzPad;
+  DevBuf<int> d_order(std::max(nb, 1)), a_ptr(std::max(tot_ap, 1)), a_idx(Zp), t_ptr(std::max(tot_tp, 1)),
+    t_idx(Zp), t_pos(Z), t_fill(N);
+  DevBuf<double> a_orig(Zp), t_orig(Zp), c(N), lb(N), ub(N), lc(M), uc(M), Dr(M), Dc(N), rowv(M), colv(N);
+  DevBuf<V> a_w(Zp), t_w(Zp), c_w(N), x0(N), x1(N), y0(M), y1(M), xa(N), ya(M), xbar(N), ybar(M), xhat(N), yhat(M),
     slack(N), aty(N);
   DevBuf<V2> bnd_w(N), cb_w(M);
   DevBuf<double2> cb_orig(M);

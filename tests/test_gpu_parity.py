@@ -413,7 +413,7 @@ def _with_dense_row_and_column(lp, seed):
 @pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
 def test_batch_rows_longer_than_a_block(precision):
     # A row of 6 001 and a column of 2 501 nonzeros take the batch kernel's
-    # in-place path (longer than its 2 048-nonzero row blocks); the single-LP
+    # in-place path (longer than its 2 040-nonzero row blocks); the single-LP
     # solver is the reference.
     lps = [_with_dense_row_and_column(G.random_lp(m=2500, n=6000, seed=11), 1),
            G.random_lp(m=80, n=200, seed=12)]
