@@ -177,7 +177,7 @@ __device__ inline void zero_red(double (&r)[kMaxRed])
 // Each group of G lanes reduces one row and runs the epilogue.
 // Each lane adds its row's products in increasing k and groups merge by a
 // fixed shuffle tree, so the sums do not depend on the blocking.
-constexpr int kStage      = 2048;                // entries per stage buffer
+constexpr int kStage      = 2560;                // entries per stage buffer
 constexpr int kBlockNnz   = kStage - 8;          // nonzeros per staged block (room for the copy's slack)
 constexpr int kStageBytes = 2 * kStage * 12;     // two buffers of values (8 bytes) and indices (4 bytes)
 constexpr int kNnzPad     = 4;                   // extra elements after each nonzero array (16 bytes or more)

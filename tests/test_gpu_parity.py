@@ -393,7 +393,7 @@ def test_batch_edge_cases_keep_order():
 def _with_dense_row_and_column(lp, seed):
     """lp plus one never-binding dense <= row and one dense column (a new variable in
     [0, 10] with an entry in every row): rows and columns longer than the batch
-    kernel's 2 048-nonzero row blocks."""
+    kernel's 2 552-nonzero row blocks when lp has more than 2 552 rows and columns."""
     import scipy.sparse as sp
 
     rng = np.random.default_rng(seed)
@@ -412,10 +412,10 @@ def _with_dense_row_and_column(lp, seed):
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
 def test_batch_rows_longer_than_a_block(precision):
-    # A row of 6 001 and a column of 2 501 nonzeros take the batch kernel's
-    # in-place path (longer than its 2 040-nonzero row blocks); the single-LP
+    # A row of 6 001 and a column of 3 001 nonzeros take the batch kernel's
+    # in-place path (longer than its 2 552-nonzero row blocks); the single-LP
     # solver is the reference.
-    lps = [_with_dense_row_and_column(G.random_lp(m=2500, n=6000, seed=11), 1),
+    lps = [_with_dense_row_and_column(G.random_lp(m=3000, n=6000, seed=11), 1),
            G.random_lp(m=80, n=200, seed=12)]
     assert np.diff(lps[0].row_offsets).max() == 6001
     rs = c_api.batch_solve(lps, tol=1e-6, precision=precision, time_limit=TL, iteration_limit=500000)
