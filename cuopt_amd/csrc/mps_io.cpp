@@ -48,10 +48,12 @@ LpHost read_mps(const std::string& path)
   std::ifstream in(path);
   if (!in) throw mps_file_error("Error opening MPS file: " + path);
 
-  enum Section { NONE, NAME, OBJSENSE, ROWS, COLUMNS, RHS, RANGES, BOUNDS };
+  enum Section { NONE, NAME, OBJSENSE, OBJNAME, ROWS, COLUMNS, RHS, RANGES, BOUNDS };
   Section sec   = NONE;
   bool maximize = false, integer_block = false;
-  std::string obj_row;
+  bool seen_rows = false, seen_columns = false, seen_rhs = false;
+  std::string obj_row;                           // the objective: the first N row, or the OBJNAME row
+  std::unordered_set<std::string> ignored_rows;  // further N rows: their entries are dropped
   std::vector<std::string> row_names;
   std::vector<char> row_type;
   std::unordered_map<std::string, int> row_id, col_id;
@@ -87,11 +89,18 @@ LpHost read_mps(const std::string& path)
     return it->second;
   };
 
+  auto set_objname = [&](const std::string& name) {
+    // OBJNAME must come before ROWS (mps_parser.cpp:1274-1278)
+    if (!obj_row.empty()) throw mps_parse_error("OBJNAME section should appear before ROWS section");
+    obj_row = name;
+  };
   std::string line;
+  int last_col = -1;
   while (std::getline(in, line)) {
     if (!line.empty() && line.back() == '\r') line.pop_back();
-    const auto t = tokens(line);
-    if (t.empty() || t[0][0] == '*') continue;
+    auto t = tokens(line);
+    // comment lines start with '*' or '$' (mps_parser.cpp:594-596)
+    if (t.empty() || line[0] == '*' || line[0] == '$') continue;
     if (!std::isspace((unsigned char)line[0])) {  // section header
       const std::string h = upper(t[0]);
       if (h == "NAME") {
@@ -100,12 +109,18 @@ LpHost read_mps(const std::string& path)
       } else if (h == "OBJSENSE" || h == "OBJSENCE") {
         sec = OBJSENSE;
         if (t.size() > 1) maximize = upper(t[1]).rfind("MAX", 0) == 0;
+      } else if (h == "OBJNAME") {
+        sec = OBJNAME;
+        if (t.size() > 1) set_objname(t[1]);
       } else if (h == "ROWS") {
-        sec = ROWS;
+        sec       = ROWS;
+        seen_rows = true;
       } else if (h == "COLUMNS") {
-        sec = COLUMNS;
+        sec          = COLUMNS;
+        seen_columns = true;
       } else if (h == "RHS") {
-        sec = RHS;
+        sec      = RHS;
+        seen_rhs = true;
       } else if (h == "RANGES") {
         sec = RANGES;
       } else if (h == "BOUNDS") {
@@ -121,13 +136,22 @@ LpHost read_mps(const std::string& path)
       }
       continue;
     }
+    // In a data line a field starting with '$' begins a comment that runs to
+    // the end of the line (mps_parser.cpp:978, 1082, 1154, 1196).
+    for (size_t p = 1; p < t.size(); ++p)
+      if (t[p][0] == '$') {
+        t.resize(p);
+        break;
+      }
     switch (sec) {
       case OBJSENSE: maximize = upper(t[0]).rfind("MAX", 0) == 0; break;
+      case OBJNAME: set_objname(t[0]); break;
       case ROWS: {
         if (t.size() < 2) throw mps_parse_error("bad ROWS line: " + line);
         const std::string ty = upper(t[0]);
-        if (ty == "N") {
-          if (obj_row.empty()) obj_row = t[1];  // further N rows are free rows, dropped
+        if (ty == "N") {  // the first N row (or the one OBJNAME names) is the objective
+          if (obj_row.empty()) obj_row = t[1];
+          else if (t[1] != obj_row) ignored_rows.insert(t[1]);
           break;
         }
         if (ty != "E" && ty != "L" && ty != "G") throw mps_parse_error("bad row type in line: " + line);
@@ -144,12 +168,19 @@ LpHost read_mps(const std::string& path)
           if (mk.find("INTEND") != std::string::npos) integer_block = false;
           break;
         }
-        if (t.size() < 3 || t.size() % 2 == 0) throw mps_parse_error("bad COLUMNS line: " + line);
-        const int j = col_of(t[0]);
+        if (t.size() % 2 == 0) throw mps_parse_error("bad COLUMNS line: " + line);
+        // a column's entries must be contiguous (mps_parser.cpp:911-918)
+        if (last_col < 0 || col_names[last_col] != t[0]) {
+          if (col_id.count(t[0]))
+            throw mps_parse_error("All rows for the column (" + t[0] + ") should occur contiguously! line=" + line);
+          last_col = col_of(t[0]);
+        }
+        const int j = last_col;
         for (size_t p = 1; p + 1 < t.size(); p += 2) {
           const double v = number(t[p + 1], line);
+          if (ignored_rows.count(t[p])) continue;
           if (t[p] == obj_row) {
-            cost[j] += v;
+            cost[j] = v;
           } else {
             auto it = row_id.find(t[p]);
             if (it == row_id.end()) {
@@ -162,12 +193,17 @@ LpHost read_mps(const std::string& path)
       }
       case RHS:
       case RANGES: {
-        // optional set name: an odd token count means the first token names the set
-        const size_t first = (t.size() % 2 == 1) ? 1 : 0;
-        if (t.size() - first < 2) throw mps_parse_error("bad RHS/RANGES line: " + line);
+        // Optional set name.  RHS: there is none when the first field is a row
+        // or the objective (mps_parser.cpp:1052-1056); RANGES: an odd field count.
+        size_t first = (t.size() % 2 == 1) ? 1 : 0;
+        if (sec == RHS) first = (t[0] == obj_row || row_id.count(t[0]) || ignored_rows.count(t[0])) ? 0 : 1;
+        if (t.size() - first == 1 || (t.size() - first) % 2 != 0)
+          throw mps_parse_error("bad RHS/RANGES line: " + line);
         for (size_t p = first; p + 1 < t.size(); p += 2) {
           const double v = number(t[p + 1], line);
-          if (sec == RHS && t[p] == obj_row) {
+          if (sec == RHS && ignored_rows.count(t[p])) {
+            continue;
+          } else if (sec == RHS && t[p] == obj_row) {
             offset = -v;
           } else if (sec == RHS) {
             rhs[row_of(t[p], line)] = v;
@@ -180,16 +216,15 @@ LpHost read_mps(const std::string& path)
       case BOUNDS: {
         const std::string bt = upper(t[0]);
         const bool valueless = bt == "FR" || bt == "MI" || bt == "PL" || bt == "BV";
-        // records: type [set] column [value]
-        std::string name;
-        double v = 0.0;
-        if (valueless) {
-          if (t.size() < 2) throw mps_parse_error("bad BOUNDS line: " + line);
-          name = t.size() >= 3 ? t[2] : t[1];
-        } else {
-          if (t.size() < 3) throw mps_parse_error("bad BOUNDS line: " + line);
-          name = t.size() >= 4 ? t[2] : t[1];
-          v    = number(t.size() >= 4 ? t[3] : t[2], line);
+        // records: type [set] column [value]; there is no set name when the
+        // second field is a column (mps_parser.cpp:1147-1151)
+        if (t.size() < 2) throw mps_parse_error("bad BOUNDS line: " + line);
+        const size_t ci = (t.size() >= 3 && !col_id.count(t[1])) ? 2 : 1;
+        const std::string name = t[ci];
+        double v               = 0.0;
+        if (!valueless) {
+          if (t.size() < ci + 2) throw mps_parse_error("bad BOUNDS line: " + line);
+          v = number(t[ci + 1], line);
         }
         const int j       = col_of(name);
         const bool first  = !has_bound[j];
@@ -232,6 +267,18 @@ LpHost read_mps(const std::string& path)
       default: throw mps_parse_error("data line outside a section: " + line);
     }
   }
+
+  // mps_parser.cpp:772-780
+  if (obj_row.empty()) throw mps_parse_error("No objective found!");
+  if (!seen_rows) throw mps_parse_error("ROWS section is missing");
+  if (!seen_columns) throw mps_parse_error("COLUMNS section is missing");
+  if (!seen_rhs) throw mps_parse_error("RHS section is missing");
+  // an integer column without a BOUNDS entry is binary (mps_parser.cpp:801-807)
+  for (size_t j = 0; j < col_names.size(); ++j)
+    if (vtype[j] == 'I' && !has_bound[j]) {
+      lb[j] = 0.0;
+      ub[j] = 1.0;
+    }
 
   LpHost lp;
   lp.m        = (int)row_names.size();

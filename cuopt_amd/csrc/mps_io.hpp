@@ -1,14 +1,27 @@
 // mps_io.hpp -- MPS reader and writer behind cuOptReadProblem / cuOptWriteProblem.
 //
-// Reading follows the semantics of the reference parser
-// (cpp/libmps_parser/src/mps_parser.cpp): whitespace-separated (free) records,
-// which also covers fixed-format files whose names hold no blanks; OBJSENSE as a
-// one-line or two-line section; the right-hand side of the objective row is the
-// negated objective offset; integer markers in COLUMNS; RANGES by row type
-// (E: sign picks [b, b+|R|] or [b-|R|, b]; L: [b-|R|, b]; G: [b, b+|R|]);
-// BOUNDS types UP LO FX FR MI PL BV LI UI with the CPLEX rule that a negative
-// UP / UI on a variable without an earlier bound sets its lower bound to -inf
-// (mps_parser.cpp:1370-1440).  Default variable bounds are [0, +inf).
+// Reading follows the semantics of the reference parser in the free format that
+// cuOptReadProblem selects (cpp/libmps_parser/src/mps_parser.cpp, cuopt_c.cpp:103):
+//   - whitespace-separated records, which also covers fixed-format files whose
+//     names hold no blanks;
+//   - comment lines start with '*' or '$', and a data field starting with '$'
+//     comments out the rest of its line;
+//   - OBJSENSE and OBJNAME as one-line or two-line sections; OBJNAME must come
+//     before ROWS; without it the first N row is the objective and the entries of
+//     further N rows are dropped;
+//   - ROWS, COLUMNS, RHS and an objective are required; a column's entries must
+//     be contiguous; row names are unique;
+//   - the right-hand side of the objective row is the negated objective offset;
+//   - integer markers in COLUMNS; an integer column without a BOUNDS entry is
+//     binary;
+//   - RANGES by row type (E: sign picks [b, b+|R|] or [b-|R|, b]; L: [b-|R|, b];
+//     G: [b, b+|R|]);
+//   - BOUNDS types UP LO FX FR MI PL BV LI UI with the CPLEX rule that a negative
+//     UP / UI on a variable without an earlier bound sets its lower bound to -inf
+//     (mps_parser.cpp:1370-1440).  Default variable bounds are [0, +inf).
+// One deliberate difference: a row named by OBJNAME keeps its COLUMNS entries
+// (the reference files every N row met after OBJNAME among the ignored ones,
+// its own included, mps_parser.cpp:851-856 and 995, which drops them).
 #pragma once
 
 #include <stdexcept>

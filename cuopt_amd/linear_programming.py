@@ -588,9 +588,11 @@ def _name(getter, handle, *idx):
 
 
 def ParseMps(mps_file_path, fixed_mps_format=False):
-    """Reads an MPS file (cuOptReadProblem, free or fixed format) into a DataModel with
-    its names (cuopt_mps_parser/parser.py:10)."""
-    del fixed_mps_format  # the reader accepts both formats
+    """Reads an MPS file (cuOptReadProblem) into a DataModel with its names
+    (cuopt_mps_parser/parser.py:10).  The reader follows the free-format rules that
+    cuOptReadProblem uses (cuopt_c.cpp:103): fields are separated by blanks, so
+    fixed-format names with embedded spaces are rejected."""
+    del fixed_mps_format
     L = lib()
     for f in ("cuOptAmdGetVariableName", "cuOptAmdGetConstraintName"):
         getattr(L, f).argtypes = [ctypes.c_void_p, ctypes.c_int32, ctypes.c_char_p, ctypes.c_int32]
