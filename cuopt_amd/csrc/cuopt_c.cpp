@@ -784,6 +784,48 @@ cuopt_int_t cuOptAmdGetProblemName(cuOptOptimizationProblem problem, char* buf, 
   return copy_name(as_problem(problem)->lp.problem_name, buf, size);
 }
 
+namespace {
+cuopt_int_t set_names(const char* const* names, cuopt_int_t count, int expected, std::vector<std::string>& out)
+{
+  if (!names || count != expected) return CUOPT_INVALID_ARGUMENT;
+  std::vector<std::string> v((size_t)count);
+  for (cuopt_int_t i = 0; i < count; ++i) {
+    if (!names[i]) return CUOPT_INVALID_ARGUMENT;
+    v[(size_t)i] = names[i];
+  }
+  out = std::move(v);
+  return CUOPT_SUCCESS;
+}
+}  // namespace
+
+cuopt_int_t cuOptAmdSetVariableNames(cuOptOptimizationProblem problem, const char* const* names, cuopt_int_t count)
+{
+  if (!problem) return CUOPT_INVALID_ARGUMENT;
+  LpHost& lp = as_problem(problem)->lp;
+  return set_names(names, count, lp.n, lp.col_names);
+}
+
+cuopt_int_t cuOptAmdSetConstraintNames(cuOptOptimizationProblem problem, const char* const* names, cuopt_int_t count)
+{
+  if (!problem) return CUOPT_INVALID_ARGUMENT;
+  LpHost& lp = as_problem(problem)->lp;
+  return set_names(names, count, lp.m, lp.row_names);
+}
+
+cuopt_int_t cuOptAmdSetObjectiveName(cuOptOptimizationProblem problem, const char* name)
+{
+  if (!problem || !name) return CUOPT_INVALID_ARGUMENT;
+  as_problem(problem)->lp.objective_name = name;
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptAmdSetProblemName(cuOptOptimizationProblem problem, const char* name)
+{
+  if (!problem || !name) return CUOPT_INVALID_ARGUMENT;
+  as_problem(problem)->lp.problem_name = name;
+  return CUOPT_SUCCESS;
+}
+
 // Solutions of this engine are LP solutions: no MIP gap, no solution bound.
 cuopt_int_t cuOptGetMIPGap(cuOptSolution solution, cuopt_float_t* mip_gap_ptr)
 {

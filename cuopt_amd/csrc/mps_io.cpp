@@ -5,6 +5,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <fstream>
 #include <limits>
 #include <sstream>
@@ -357,15 +358,38 @@ void write_mps(const LpHost& lp, const std::string& path)
       hi[i]          = lp.sense[i] == 'G' ? kInf : b;
     }
   }
-  std::fprintf(f, "NAME cuopt_amd\n");
+  // The problem's names when they can be read back (non-empty, no blanks, unique,
+  // not a '$' comment, not a set name this writer uses); R<i> / C<j> / OBJ otherwise.
+  const std::unordered_set<std::string> reserved{"RHS", "RNG", "BND", "MARKER", "'MARKER'"};
+  auto readable = [&](const std::string& s) {
+    if (s.empty() || s[0] == '$' || reserved.count(s)) return false;
+    for (char ch : s)
+      if (std::isspace((unsigned char)ch)) return false;
+    return true;
+  };
+  auto pick = [&](const std::vector<std::string>& v, int count) {
+    if ((int)v.size() != count) return false;
+    std::unordered_set<std::string> seen;
+    for (const auto& s : v)
+      if (!readable(s) || !seen.insert(s).second) return false;
+    return true;
+  };
+  const bool row_names = pick(lp.row_names, lp.m), col_names = pick(lp.col_names, lp.n);
+  std::string obj = "OBJ";
+  if (readable(lp.objective_name)) obj = lp.objective_name;
+  if (row_names && std::find(lp.row_names.begin(), lp.row_names.end(), obj) != lp.row_names.end()) obj = "OBJ";
+  const bool rows_ok = row_names && std::find(lp.row_names.begin(), lp.row_names.end(), obj) == lp.row_names.end();
+  auto rname = [&](int i) { return rows_ok ? lp.row_names[i] : "R" + std::to_string(i); };
+  auto cname = [&](int j) { return col_names ? lp.col_names[j] : "C" + std::to_string(j); };
+  std::fprintf(f, "NAME %s\n", readable(lp.problem_name) ? lp.problem_name.c_str() : "cuopt_amd");
   if (lp.maximize) std::fprintf(f, "OBJSENSE\n    MAX\n");
-  std::fprintf(f, "ROWS\n N OBJ\n");
+  std::fprintf(f, "ROWS\n N %s\n", obj.c_str());
   std::vector<char> ty(lp.m);
   for (int i = 0; i < lp.m; ++i) {
     if (lo[i] == hi[i]) ty[i] = 'E';
     else if (std::isinf(lo[i]) && lo[i] < 0) ty[i] = 'L';
     else ty[i] = 'G';  // finite lower bound; a finite upper bound becomes a range
-    std::fprintf(f, " %c R%d\n", ty[i], i);
+    std::fprintf(f, " %c %s\n", ty[i], rname(i).c_str());
   }
   // columns: transpose the CSR rows
   std::vector<int> cptr(lp.n + 1, 0);
@@ -391,42 +415,76 @@ void write_mps(const LpHost& lp, const std::string& path)
       std::fprintf(f, "    MARKER 'MARKER' '%s'\n", is_int ? "INTORG" : "INTEND");
       in_int = is_int;
     }
-    std::fprintf(f, "    C%d OBJ %s\n", j, num(lp.c[j]).c_str());
-    for (int k = cptr[j]; k < cptr[j + 1]; ++k) std::fprintf(f, "    C%d R%d %s\n", j, crow[k], num(cval[k]).c_str());
+    const std::string cj = cname(j);
+    std::fprintf(f, "    %s %s %s\n", cj.c_str(), obj.c_str(), num(lp.c[j]).c_str());
+    for (int k = cptr[j]; k < cptr[j + 1]; ++k)
+      std::fprintf(f, "    %s %s %s\n", cj.c_str(), rname(crow[k]).c_str(), num(cval[k]).c_str());
   }
   if (in_int) std::fprintf(f, "    MARKER 'MARKER' 'INTEND'\n");
   std::fprintf(f, "RHS\n");
-  if (lp.offset != 0.0) std::fprintf(f, "    RHS OBJ %s\n", num(-lp.offset).c_str());
+  if (lp.offset != 0.0) std::fprintf(f, "    RHS %s %s\n", obj.c_str(), num(-lp.offset).c_str());
   for (int i = 0; i < lp.m; ++i) {
     const double b = ty[i] == 'L' ? hi[i] : lo[i];
-    if (b != 0.0) std::fprintf(f, "    RHS R%d %s\n", i, num(b).c_str());
+    if (b != 0.0) std::fprintf(f, "    RHS %s %s\n", rname(i).c_str(), num(b).c_str());
   }
   bool any_range = false;
   for (int i = 0; i < lp.m; ++i)
     if (ty[i] == 'G' && std::isfinite(hi[i])) {
       if (!any_range) std::fprintf(f, "RANGES\n");
       any_range = true;
-      std::fprintf(f, "    RNG R%d %s\n", i, num(hi[i] - lo[i]).c_str());
+      std::fprintf(f, "    RNG %s %s\n", rname(i).c_str(), num(hi[i] - lo[i]).c_str());
     }
   std::fprintf(f, "BOUNDS\n");
   for (int j = 0; j < lp.n; ++j) {
     const double l = lp.lb[j], u = lp.ub[j];
+    const std::string cj = cname(j);
+    const char* c = cj.c_str();
     if (l == u) {
-      std::fprintf(f, " FX BND C%d %s\n", j, num(l).c_str());
+      std::fprintf(f, " FX BND %s %s\n", c, num(l).c_str());
       continue;
     }
     if (std::isinf(l) && std::isinf(u)) {
-      std::fprintf(f, " FR BND C%d\n", j);
+      std::fprintf(f, " FR BND %s\n", c);
       continue;
     }
-    if (std::isinf(l)) std::fprintf(f, " MI BND C%d\n", j);
-    else if (l != 0.0) std::fprintf(f, " LO BND C%d %s\n", j, num(l).c_str());
-    if (std::isfinite(u)) std::fprintf(f, " UP BND C%d %s\n", j, num(u).c_str());
+    bool written = false;
+    if (std::isinf(l)) {
+      std::fprintf(f, " MI BND %s\n", c);
+      written = true;
+    } else if (l != 0.0) {
+      std::fprintf(f, " LO BND %s %s\n", c, num(l).c_str());
+      written = true;
+    }
+    if (std::isfinite(u)) {
+      std::fprintf(f, " UP BND %s %s\n", c, num(u).c_str());
+      written = true;
+    }
+    // an integer column without a BOUNDS entry would read back as binary
+    if (!written && !lp.var_types.empty() && lp.var_types[j] == 'I') std::fprintf(f, " PL BND %s\n", c);
   }
   std::fprintf(f, "ENDATA\n");
   const bool ok = std::ferror(f) == 0;
   std::fclose(f);
   if (!ok) throw mps_file_error("Error writing MPS file: " + path);
+}
+
+void write_sol_file(const std::string& path, const LpHost& lp, const SolveResult& R)
+{
+  FILE* f = std::fopen(path.c_str(), "w");
+  if (!f) {
+    std::fprintf(stderr, "Could not open file: %s for solution output\n", path.c_str());
+    return;
+  }
+  const char* status = R.status == CUOPT_TERIMINATION_STATUS_OPTIMAL           ? "Optimal"
+                       : R.status == CUOPT_TERIMINATION_STATUS_PRIMAL_FEASIBLE ? "Primal Feasible"
+                                                                               : "Infeasible";
+  std::fprintf(f, "# Status: %s\n", status);
+  if (std::strcmp(status, "Infeasible") != 0) {
+    std::fprintf(f, "# Objective value: %.18g\n", R.primal_objective);
+    for (size_t j = 0; j < lp.col_names.size() && j < R.x.size(); ++j)
+      std::fprintf(f, "%s %.18g\n", lp.col_names[j].c_str(), R.x[j]);
+  }
+  std::fclose(f);
 }
 
 }  // namespace cuopt_amd

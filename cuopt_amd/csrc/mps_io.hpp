@@ -42,7 +42,18 @@ struct mps_parse_error : std::runtime_error {
 // ranged form (lc / uc) is what the solver uses, and sense / rhs stay readable.
 LpHost read_mps(const std::string& path);
 
-// Free-format MPS with generated names R<i> / C<j>; values printed to round-trip.
+// Free-format MPS; values printed to round-trip.  Rows, columns, the objective and
+// the problem keep their names when every name of the kind can be read back
+// (non-empty, no blanks, unique, not a '$' comment, not RHS / RNG / BND / MARKER);
+// otherwise the kind gets generated names R<i> / C<j> / OBJ.
 void write_mps(const LpHost& lp, const std::string& path);
+
+// The solution file of the solution_file parameter (solve.cu:1327-1330,
+// solution_writer.cu:16-40): "# Status: Optimal" or "Primal Feasible" (any other
+// status is written as "Infeasible"), then, unless infeasible, "# Objective
+// value: <primal objective>" and one "<name> <value>" line per named variable,
+// 18 significant digits.  A file that cannot be opened is reported on stderr and
+// skipped, as the reference does.
+void write_sol_file(const std::string& path, const LpHost& lp, const SolveResult& R);
 
 }  // namespace cuopt_amd

@@ -25,6 +25,7 @@
 
 #include <rocprim/device/device_radix_sort.hpp>
 
+#include "mps_io.hpp"
 #include "presolve.hpp"
 
 #include "common.hpp"
@@ -1723,9 +1724,8 @@ SolveResult solve_presolved(const LpHost& lp, const std::vector<double>& lc, con
   return R;
 }
 
-}  // namespace
-
-SolveResult solve_lp(const LpHost& lp, const PdlpSettings& st)
+// solve_lp without the file parameters.
+SolveResult solve_lp_core(const LpHost& lp, const PdlpSettings& st)
 {
   const auto t0 = std::chrono::steady_clock::now();
   SolveResult R;
@@ -1752,6 +1752,25 @@ SolveResult solve_lp(const LpHost& lp, const PdlpSettings& st)
   // presolve: Default, Papilo and PSLP all select this engine's presolve; Off skips it
   if (st.presolve != CUOPT_PRESOLVE_OFF && st.bench_steps <= 0) return solve_presolved(lp, lc, uc, lb, ub, st, t0);
   return solve_checked(lp, lc, uc, lb, ub, st, t0);
+}
+
+}  // namespace
+
+SolveResult solve_lp(const LpHost& lp, const PdlpSettings& st)
+{
+  // The user problem is written before the solve (solve.cu:1281-1284); a file
+  // that cannot be written is a validation error, as in the reference's writer
+  // (mps_writer.cpp:32).
+  if (!st.user_problem_file.empty()) {
+    try {
+      write_mps(lp, st.user_problem_file);
+    } catch (const mps_file_error& e) {
+      throw validation_error(e.what());
+    }
+  }
+  SolveResult R = solve_lp_core(lp, st);
+  if (!st.solution_file.empty()) write_sol_file(st.solution_file, lp, R);  // solve.cu:1327-1330
+  return R;
 }
 
 namespace {

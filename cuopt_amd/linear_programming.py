@@ -288,22 +288,28 @@ class DataModel:
         return any(t == "I" for t in self.variable_types)
 
 
+class _InvalidModel(ValueError):
+    """Incomplete or inconsistent problem data: Solve reports it as a Solution with
+    ErrorStatus.ValidationError, as the reference does (test_lp_solver.py:309-368)."""
+
+
 def _to_problem(dm: DataModel) -> ctypes.c_void_p:
-    """A C problem handle for dm (the caller destroys it); raises ValueError on inconsistent data."""
+    """A C problem handle for dm (the caller destroys it); raises _InvalidModel on
+    incomplete or inconsistent data and ValueError on a quadratic objective."""
     m, n = dm._sizes()
     if len(dm.A_offsets) == 0:
-        raise ValueError("set_csr_constraint_matrix must be called before solving")
+        raise _InvalidModel("set_csr_constraint_matrix must be called before solving")
     nnz = int(dm.A_offsets[-1])
     if len(dm.A_values) != nnz or len(dm.A_indices) != nnz:
-        raise ValueError("A_values and A_indices must hold A_offsets[-1] entries")
+        raise _InvalidModel("A_values and A_indices must hold A_offsets[-1] entries")
     lb = dm.variable_lower_bounds if len(dm.variable_lower_bounds) else np.zeros(n)
     ub = dm.variable_upper_bounds if len(dm.variable_upper_bounds) else np.full(n, np.inf)
     for name, v, k in (("variable_lower_bounds", lb, n), ("variable_upper_bounds", ub, n)):
         if len(v) != k:
-            raise ValueError(f"{name} must have {k} entries")
+            raise _InvalidModel(f"{name} must have {k} entries")
     vt = dm.variable_types if len(dm.variable_types) else np.array(["C"] * n)
     if len(vt) != n:
-        raise ValueError(f"variable_types must have {n} entries")
+        raise _InvalidModel(f"variable_types must have {n} entries")
     vt_b = np.frombuffer("".join(vt).encode(), np.uint8).copy() if n else np.zeros(1, np.uint8)
     if len(dm.Q_offsets) and int(dm.Q_offsets[-1]) > 0:
         raise ValueError("quadratic objectives are solved by the reference's barrier method, which this LP "
@@ -321,22 +327,45 @@ def _to_problem(dm: DataModel) -> ctypes.c_void_p:
     if len(dm.constraint_lower_bounds) and len(dm.constraint_upper_bounds):
         lc, uc = _a(dm.constraint_lower_bounds, np.float64), _a(dm.constraint_upper_bounds, np.float64)
         if len(lc) != m or len(uc) != m:
-            raise ValueError(f"constraint bounds must have {m} entries")
+            raise _InvalidModel(f"constraint bounds must have {m} entries")
         rc = L.cuOptCreateRangedProblem(m, n, sense, dm.objective_offset, _p(c), _p(ptr), _p(idx), _p(val),
                                         _p(lc if m else np.zeros(1)), _p(uc if m else np.zeros(1)), _p(lb), _p(ub),
                                         _p(vt_b), ctypes.byref(h))
     else:
         rt = dm.row_types
         if len(rt) != m or len(dm.b) != m:
-            raise ValueError("give row types and constraint bounds b (or constraint lower and upper bounds) "
-                             f"for the {m} constraints")
+            raise _InvalidModel("give row types and constraint bounds b (or constraint lower and upper bounds) "
+                                f"for the {m} constraints")
         s = np.frombuffer("".join(rt).encode(), np.uint8).copy() if m else np.zeros(1, np.uint8)
         b = _a(dm.b, np.float64) if m else np.zeros(1)
         rc = L.cuOptCreateProblem(m, n, sense, dm.objective_offset, _p(c), _p(ptr), _p(idx), _p(val), _p(s), _p(b),
                                   _p(lb), _p(ub), _p(vt_b), ctypes.byref(h))
     if rc != c_api.CUOPT_SUCCESS:
-        raise ValueError(f"invalid problem data (cuOpt error {rc})")
+        raise _InvalidModel(f"invalid problem data (cuOpt error {rc})")
+    try:
+        _set_names(L, h, dm, m, n)
+    except _InvalidModel:
+        L.cuOptDestroyProblem(ctypes.byref(h))
+        raise
     return h
+
+
+def _set_names(L, h, dm: DataModel, m: int, n: int):
+    """Hands the data model's names to the C problem (they name the solution_file
+    variables and the rows and columns of user_problem_file)."""
+    for setter, names, k, what in ((L.cuOptAmdSetVariableNames, dm.variable_names, n, "variable"),
+                                   (L.cuOptAmdSetConstraintNames, dm.row_names, m, "row")):
+        if len(names) == 0:
+            continue
+        if len(names) != k:
+            raise _InvalidModel(f"{what} names must have {k} entries")
+        arr = (ctypes.c_char_p * k)(*[str(v).encode() for v in names])
+        setter.argtypes = [ctypes.c_void_p, ctypes.POINTER(ctypes.c_char_p), ctypes.c_int32]
+        _check(setter(h, arr, k), setter.__name__)
+    for setter, name in ((L.cuOptAmdSetObjectiveName, dm.objective_name), (L.cuOptAmdSetProblemName, dm.problem_name)):
+        if name:
+            setter.argtypes = [ctypes.c_void_p, ctypes.c_char_p]
+            _check(setter(h, str(name).encode()), setter.__name__)
 
 
 class SolverSettings:
@@ -515,6 +544,12 @@ def _solution(dm: DataModel, r: c_api.Result) -> Solution:
                     gap=abs(obj - dobj), nb_iterations=int(r.iterations), solved_by_pdlp=True)
 
 
+def _invalid_solution(message: str) -> Solution:
+    return Solution(ProblemCategory.LP, {}, primal_solution=np.zeros(0), dual_solution=np.zeros(0),
+                    reduced_cost=np.zeros(0), termination_status=LPTerminationStatus.NoTermination,
+                    error_status=ErrorStatus.ValidationError, error_message=message)
+
+
 def _prepare(dm: DataModel, settings: SolverSettings | None):
     if settings is None:
         settings = SolverSettings()
@@ -534,7 +569,10 @@ def Solve(data_model, solver_settings=None):
         s.set_initial_primal(data_model.initial_primal_solution)
     if len(data_model.initial_dual_solution):
         s.set_initial_dual(data_model.initial_dual_solution)
-    p = _to_problem(data_model)
+    try:
+        p = _to_problem(data_model)
+    except _InvalidModel as e:
+        return _invalid_solution(str(e))
     sol = ctypes.c_void_p()
     L = lib()
     try:
@@ -555,23 +593,31 @@ def BatchSolve(data_model_list, solver_settings=None):
     for dm in data_model_list:
         _prepare(dm, settings)
     s = settings._handle()
-    handles = [_to_problem(dm) for dm in data_model_list]
     L = lib()
-    k = len(handles)
+    handles, invalid = [], {}  # models with incomplete data get a ValidationError solution
     try:
-        hs = (ctypes.c_void_p * max(k, 1))(*[h.value for h in handles])
-        sols = (ctypes.c_void_p * max(k, 1))()
-        L.cuOptAmdBatchSolve(k, hs, s.handle, sols)
-        out = []
         for i, dm in enumerate(data_model_list):
-            if not sols[i]:
+            try:
+                handles.append((i, _to_problem(dm)))
+            except _InvalidModel as e:
+                invalid[i] = _invalid_solution(str(e))
+        k = len(handles)
+        hs = (ctypes.c_void_p * max(k, 1))(*[h.value for _, h in handles])
+        sols = (ctypes.c_void_p * max(k, 1))()
+        if k:
+            L.cuOptAmdBatchSolve(k, hs, s.handle, sols)
+        out = dict(invalid)
+        for slot, (i, _) in enumerate(handles):
+            if not sols[slot]:
                 raise RuntimeError("cuOptAmdBatchSolve returned no solution")
-            m, n = dm._sizes()
-            out.append(_solution(dm, c_api._take_result(ctypes.c_void_p(sols[i]), _LpShape(m, n))))
+            m, n = data_model_list[i]._sizes()
+            out[i] = _solution(data_model_list[i], c_api._take_result(ctypes.c_void_p(sols[slot]), _LpShape(m, n)))
     finally:
-        for h in handles:
+        for _, h in handles:
             L.cuOptDestroyProblem(ctypes.byref(h))
-    return out, (out[0].get_solve_time() if out else 0.0)
+    solved = [out[i] for i in range(len(data_model_list))]
+    times = [out[i].get_solve_time() for i, _ in handles]
+    return solved, (times[0] if times else 0.0)
 
 
 class _LpShape:

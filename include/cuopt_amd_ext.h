@@ -124,6 +124,17 @@ cuopt_int_t cuOptAmdGetConstraintName(cuOptOptimizationProblem problem,
 cuopt_int_t cuOptAmdGetObjectiveName(cuOptOptimizationProblem problem, char* name_buffer, cuopt_int_t buffer_size);
 cuopt_int_t cuOptAmdGetProblemName(cuOptOptimizationProblem problem, char* name_buffer, cuopt_int_t buffer_size);
 
+/* Sets the names of a problem built from arrays, as the reference's data model
+ * does (data_model.py set_variable_names / set_row_names / set_objective_name /
+ * set_problem_name).  They name the variables of the solution_file output and
+ * the rows and columns of user_problem_file and cuOptWriteProblem.  count must
+ * be the number of variables (constraints) and every entry non-null; the
+ * strings are copied.  CUOPT_INVALID_ARGUMENT otherwise. */
+cuopt_int_t cuOptAmdSetVariableNames(cuOptOptimizationProblem problem, const char* const* names, cuopt_int_t count);
+cuopt_int_t cuOptAmdSetConstraintNames(cuOptOptimizationProblem problem, const char* const* names, cuopt_int_t count);
+cuopt_int_t cuOptAmdSetObjectiveName(cuOptOptimizationProblem problem, const char* name);
+cuopt_int_t cuOptAmdSetProblemName(cuOptOptimizationProblem problem, const char* name);
+
 /* Number of GPUs visible to the library (0 when no HIP device is usable). */
 cuopt_int_t cuOptAmdDeviceCount(void);
 

@@ -83,6 +83,90 @@ def test_inconsistent_models_raise():
         dm.set_csr_constraint_matrix(np.array(["a"]), np.array([0]), np.array([0, 1]))
 
 
+def _validity_steps():
+    """The model of test_lp_solver.py:309-368, completed step by step: every model
+    before the row types are set is incomplete."""
+    dm = LPM.DataModel()
+    yield dm  # nothing set
+    dm.set_csr_constraint_matrix(np.array([1.0]), np.array([0], np.int32), np.array([0, 1], np.int32))
+    yield dm  # no b
+    dm.set_constraint_bounds(np.array([1.0]))
+    yield dm  # no c
+    dm.set_objective_coefficients(np.array([1.0]))
+    dm.set_maximize(True)
+    yield dm  # no row types
+    dm.set_maximize(False)
+    yield dm  # still no row types
+
+
+def test_incomplete_models_give_validation_error_solutions():
+    # test_lp_solver.py:309-346: Solve reports incomplete data as a Solution with
+    # ErrorStatus.ValidationError instead of raising (no device is touched)
+    for dm in _validity_steps():
+        sol = LPM.Solve(dm)
+        assert sol.get_error_status() == LPM.ErrorStatus.ValidationError
+        assert sol.get_termination_status() == LPM.LPTerminationStatus.NoTermination
+        assert sol.get_error_message()
+
+
+def _named_model():
+    dm = LPM.DataModel()
+    dm.set_csr_constraint_matrix(np.array([1.0, 2.0, 3.0]), np.array([0, 1, 1], np.int32),
+                                 np.array([0, 2, 3], np.int32))
+    dm.set_objective_coefficients(np.array([1.0, -1.0]))
+    dm.set_row_types(np.array(["L", "G"]))
+    dm.set_constraint_bounds(np.array([4.0, 1.0]))
+    dm.set_variable_types(np.array(["I", "C"]))
+    dm.set_variable_names(["x", "y"])
+    dm.set_row_names(["cap", "need"])
+    dm.set_objective_name("profit")
+    dm.set_problem_name("tiny")
+    return dm
+
+
+def test_names_reach_the_written_mps(tmp_path):
+    # the data model's names reach the C problem (cuOptAmdSet*Names) and the
+    # written MPS, which reads back with them; an integer column without bounds
+    # is written with an explicit bound so that it does not read back as binary
+    f = tmp_path / "named.mps"
+    _named_model().writeMPS(str(f))
+    back = LPM.ParseMps(str(f))
+    assert list(back.get_variable_names()) == ["x", "y"]
+    assert list(back.get_row_names()) == ["cap", "need"]
+    assert back.get_objective_name() == "profit" and back.get_problem_name() == "tiny"
+    assert list(back.get_variable_types()) == ["I", "C"]
+    assert back.get_variable_upper_bounds().tolist() == [np.inf, np.inf]
+    assert rows_equal(back, _named_model())
+
+
+def rows_equal(a, b):
+    return (np.array_equal(a.get_constraint_matrix_offsets(), b.get_constraint_matrix_offsets())
+            and np.array_equal(a.get_constraint_matrix_indices(), b.get_constraint_matrix_indices())
+            and np.array_equal(a.get_constraint_matrix_values(), b.get_constraint_matrix_values()))
+
+
+def test_unreadable_names_fall_back_to_generated_ones(tmp_path):
+    dm = _named_model()
+    dm.set_variable_names(["x 1", "y"])  # a blank: the columns get C0, C1
+    dm.set_row_names(["cap", "cap"])     # not unique: the rows get R0, R1
+    f = tmp_path / "gen.mps"
+    dm.writeMPS(str(f))
+    back = LPM.ParseMps(str(f))
+    assert list(back.get_variable_names()) == ["C0", "C1"]
+    assert list(back.get_row_names()) == ["R0", "R1"]
+    assert rows_equal(back, dm)
+
+
+def test_name_count_must_match():
+    dm = _named_model()
+    dm.set_variable_types(np.array(["C", "C"]))
+    dm.set_variable_names(["x"])
+    with pytest.raises(ValueError):
+        LPM._to_problem(dm)
+    sol = LPM.Solve(dm)
+    assert sol.get_error_status() == LPM.ErrorStatus.ValidationError
+
+
 @pytest.mark.parametrize("fname", ["afiro_original.mps", "good-mps-fixed-ranges.mps", "lp_model_with_var_bounds.mps"])
 def test_parse_mps_matches_python_reader_and_keeps_names(fname):
     path = os.path.join(GOLDEN, fname)
@@ -207,3 +291,57 @@ def test_batch_solve_returns_solutions_and_batch_time():
         assert sol.get_termination_reason() == "Optimal"
         assert abs(sol.get_primal_objective() - lp.obj_opt) < 1e-4 * (1 + abs(lp.obj_opt))
         assert sol.get_solve_time() == t
+
+
+@gpu
+def test_completed_model_solves():
+    # test_lp_solver.py:348-368: with row types (and then ranged bounds) the model solves
+    *_, dm = _validity_steps()
+    dm.set_row_types(np.array(["E"]))
+    sol = LPM.Solve(dm)
+    assert sol.get_error_status() == LPM.ErrorStatus.Success
+    assert sol.get_termination_status() == LPM.LPTerminationStatus.Optimal
+    assert abs(sol.get_primal_objective() - 1.0) < 1e-6
+    dm.set_constraint_lower_bounds(np.array([1.0]))
+    dm.set_constraint_upper_bounds(np.array([1.0]))
+    assert LPM.Solve(dm).get_error_status() == LPM.ErrorStatus.Success
+
+
+@gpu
+def test_user_problem_file_and_solution_file(tmp_path):
+    # test_lp_solver.py:691-722: user_problem_file writes the problem before the
+    # solve (solve.cu:1281-1284), solution_file the .sol after it (solve.cu:1327-1330)
+    afiro = LPM.ParseMps(os.path.join(GOLDEN, "afiro_original.mps"))
+    s = LPM.SolverSettings()
+    s.set_optimality_tolerance(1e-8)
+    s.set_parameter("time_limit", 120.0)
+    out = tmp_path / "afiro_out.mps"
+    s.set_parameter("user_problem_file", str(out))
+    LPM.Solve(afiro, s)
+    assert out.is_file()
+    again = LPM.ParseMps(str(out))
+    np.testing.assert_array_equal(again.get_objective_coefficients(), afiro.get_objective_coefficients())
+    s.set_parameter("user_problem_file", "")
+    sol_file = tmp_path / "afiro.sol"
+    s.set_parameter("solution_file", str(sol_file))
+    sol = LPM.Solve(afiro, s)
+    assert sol.get_termination_status() == LPM.LPTerminationStatus.Optimal
+    assert abs(sol.get_primal_objective() - (-464.7531)) < 1e-3
+    lines = sol_file.read_text().splitlines()
+    assert lines[0] == "# Status: Optimal"
+    assert abs(float(lines[1].split(":")[1]) - sol.get_primal_objective()) < 1e-9
+    values = dict(line.split() for line in lines[2:])
+    assert len(values) == 32 and abs(float(values["X01"]) - 80.0) < 1e-3
+
+
+@gpu
+def test_batch_solve_reports_an_incomplete_model_in_place():
+    lps = [G.random_lp(m=m, n=int(2.5 * m), seed=30 + m) for m in (60, 90)]
+    s = LPM.SolverSettings()
+    s.set_parameter("time_limit", 120.0)
+    sols, t = LPM.BatchSolve([_model(lps[0]), LPM.DataModel(), _model(lps[1])], s)
+    assert len(sols) == 3 and t > 0.0
+    assert sols[1].get_error_status() == LPM.ErrorStatus.ValidationError
+    for lp, sol in zip(lps, (sols[0], sols[2])):
+        assert sol.get_termination_reason() == "Optimal"
+        assert abs(sol.get_primal_objective() - lp.obj_opt) < 1e-3 * (1 + abs(lp.obj_opt))
