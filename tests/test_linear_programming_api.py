@@ -345,3 +345,65 @@ def test_batch_solve_reports_an_incomplete_model_in_place():
     for lp, sol in zip(lps, (sols[0], sols[2])):
         assert sol.get_termination_reason() == "Optimal"
         assert abs(sol.get_primal_objective() - lp.obj_opt) < 1e-3 * (1 + abs(lp.obj_opt))
+
+
+@gpu
+@pytest.mark.parametrize("presolve", [0, 1])
+def test_tiny_lp_of_the_reference_solver_test(presolve):
+    # test_lp_solver.py:46-79: min x s.t. x <= 1 twice, x >= 0.  The reference runs
+    # it in Stable2 ("Stable3 infinite-loops on this sample trivial problem"); here
+    # Stable3 reaches the optimum x = 0.
+    dm = LPM.DataModel()
+    dm.set_csr_constraint_matrix(np.array([1.0, 1.0]), np.array([0, 0]), np.array([0, 1, 2]))
+    dm.set_constraint_bounds(np.array([1.0, 1.0]))
+    dm.set_objective_coefficients(np.array([1.0]))
+    dm.set_row_types(np.array(["L", "L"]))
+    s = LPM.SolverSettings()
+    s.set_optimality_tolerance(1e-2)
+    s.set_parameter("method", LPM.SolverMethod.PDLP)
+    s.set_parameter("presolve", presolve)
+    s.set_parameter("time_limit", 60.0)
+    sol = LPM.Solve(dm, s)
+    assert sol.get_termination_reason() == "Optimal"
+    assert sol.get_primal_solution()[0] == pytest.approx(0.0)
+    assert sol.get_primal_objective() == pytest.approx(0.0) and sol.get_dual_objective() == pytest.approx(0.0)
+    st = sol.get_lp_stats()
+    assert st["primal_residual"] == pytest.approx(0.0) and st["dual_residual"] == pytest.approx(0.0)
+    assert st["gap"] == pytest.approx(0.0)
+    assert sol.get_solved_by_pdlp()
+
+
+@gpu
+def test_parser_and_solver():
+    # test_lp_solver.py:82-89
+    s = LPM.SolverSettings()
+    s.set_optimality_tolerance(1e-2)
+    s.set_parameter("time_limit", 60.0)
+    sol = LPM.Solve(LPM.ParseMps(os.path.join(GOLDEN, "mps", "good-mps-1.mps")), s)
+    assert sol.get_termination_reason() == "Optimal"
+
+
+@gpu
+def test_very_low_tolerance_afiro():
+    # test_lp_solver.py:92-112 (afiro to 1e-12)
+    s = LPM.SolverSettings()
+    s.set_optimality_tolerance(1e-12)
+    s.set_parameter("time_limit", 120.0)
+    sol = LPM.Solve(LPM.ParseMps(os.path.join(GOLDEN, "afiro_original.mps")), s)
+    assert sol.get_termination_status() == LPM.LPTerminationStatus.Optimal
+    assert sol.get_primal_objective() == pytest.approx(-464.7531)
+
+
+@gpu
+def test_iteration_limit_returns_the_iterate():
+    # test_lp_solver.py:115-133 on afiro (savsched1 is a downloaded dataset): the
+    # iteration limit wins over a large time limit, and the returned point is the
+    # iterate, not zeros
+    s = LPM.SolverSettings()
+    s.set_optimality_tolerance(1e-12)
+    s.set_parameter("iteration_limit", 1)
+    s.set_parameter("time_limit", 99999999)
+    sol = LPM.Solve(LPM.ParseMps(os.path.join(GOLDEN, "afiro_original.mps")), s)
+    assert sol.get_termination_status() == LPM.LPTerminationStatus.IterationLimit
+    assert sol.get_primal_objective() != 0.0
+    assert np.any(sol.get_primal_solution())
