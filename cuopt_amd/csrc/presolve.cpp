@@ -139,6 +139,26 @@ Presolved presolve(const LpHost& lp, const std::vector<double>& lc0, const std::
     }
   }
 
+  // A row whose activity range over the (tightened) column bounds misses
+  // [lc, uc] cannot hold: the problem is infeasible.
+  for (int i = 0; i < m; ++i) {
+    if (!row_on[i]) continue;
+    double lo = 0.0, hi = 0.0;  // -inf / +inf or finite: zero coefficients are skipped
+    for (int k = lp.rowptr[i]; k < lp.rowptr[i + 1]; ++k) {
+      const int j    = lp.colidx[k];
+      const double a = lp.vals[k];
+      if (!col_on[j] || a == 0.0) continue;
+      lo += a > 0.0 ? a * lb[j] : a * ub[j];
+      hi += a > 0.0 ? a * ub[j] : a * lb[j];
+    }
+    auto fin = [](double v) { return std::isfinite(v) ? std::fabs(v) : 0.0; };
+    const double tol = 1e-9 * (1.0 + std::max(fin(lc[i]), fin(uc[i])) + std::max(fin(lo), fin(hi)));
+    if (lo > uc[i] + tol || hi < lc[i] - tol) {
+      P.status = Presolved::INFEASIBLE;
+      return P;
+    }
+  }
+
   // The reduced problem (ranged form, continuous variables).
   std::vector<int> new_col(n, -1);
   for (int j = 0; j < n; ++j)

@@ -11,6 +11,8 @@
 //     its y takes the reduced cost of x_j when x_j sits on a bound it supplied;
 //   - an empty column: set to the bound its cost prefers (unbounded if that
 //     bound is infinite and the cost nonzero), reduced cost = cost.
+// Afterwards a row whose activity range over the column bounds misses its
+// [lc, uc] (beyond 1e-9 relative) reports the problem infeasible.
 // Everything runs in FP64 in the user's objective sense; the reduced problem
 // is handed to the solver in ranged form.
 #pragma once

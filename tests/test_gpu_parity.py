@@ -212,6 +212,37 @@ def test_fixtures_fp64(highs, fname):
     assert np.allclose(r.y, o["y"], atol=1e-7)
 
 
+def test_reference_ranged_problem():
+    # c_api_tests.cpp:105-112 / c_api_test.c:927-1040: max 5x + 8y s.t. 2x + 3y <= 12,
+    # 3x + y <= 6, 2 <= x + 2y <= 8, 0 <= x, y <= 10; optimum 32 at (0, 4)
+    lp = G.LP(m=3, n=2, row_offsets=np.array([0, 2, 4, 6], np.int32), col_indices=np.array([0, 1] * 3, np.int32),
+              values=np.array([2.0, 3.0, 3.0, 1.0, 1.0, 2.0]), c=np.array([5.0, 8.0]),
+              con_lb=np.array([-np.inf, -np.inf, 2.0]), con_ub=np.array([12.0, 6.0, 8.0]),
+              var_lb=np.zeros(2), var_ub=np.full(2, 10.0), maximize=True, name="ranged")
+    r = c_api.solve(lp, time_limit=TL, method=c_api.CUOPT_METHOD_CONCURRENT)
+    assert r.status == 1
+    assert abs(r.primal_objective - 32.0) < 1e-3
+
+
+@pytest.mark.parametrize("method", ["CONCURRENT", "PDLP"])
+def test_reference_maximize_problem_dual_variables(method):
+    # c_api_tests.cpp:149-166 / c_api_test.c:1461-1580: max 4x1 + x2 + 5x3 + 3x4 s.t. three <= rows,
+    # x >= 0.  The reference asks objective == dual objective within 1e-2 for PDLP at the
+    # default tolerances; the HiGHS optimum is 29 with row duals (11, 0, 6) for the max problem.
+    A = np.array([[1, -1, -1, 3], [5, 1, 3, 8], [-1, 2, 3, -5]], float)
+    lp = G.LP(m=3, n=4, row_offsets=np.array([0, 4, 8, 12], np.int32), col_indices=np.tile(np.arange(4, dtype=np.int32), 3),
+              values=A.ravel(), c=np.array([4.0, 1.0, 5.0, 3.0]), con_lb=np.full(3, -np.inf),
+              con_ub=np.array([1.0, 55.0, 3.0]), var_lb=np.zeros(4), var_ub=np.full(4, np.inf), maximize=True,
+              name="maxdual")
+    r = c_api.solve(lp, time_limit=TL, method=getattr(c_api, f"CUOPT_METHOD_{method}"))
+    assert r.status == 1
+    assert abs(r.primal_objective - r.dual_objective) < 1e-2
+    assert abs(r.primal_objective - 29.0) < 1e-2
+    np.testing.assert_allclose(r.y, [11.0, 0.0, 6.0], atol=5e-2)
+    # reduced costs are the dual slack of the iterate: c - A^T y up to the 1e-4 tolerance
+    np.testing.assert_allclose(r.reduced_cost, lp.c - A.T @ r.y, atol=1e-3)
+
+
 def test_transport_matches_highs(highs):
     case = highs["transport_small"]
     lp = G.transportation_lp(**case["kwargs"])

@@ -70,6 +70,9 @@ def test_presolve_solves_singletons_fixed_and_empty_rows_without_gpu():
     lp = fully_reducible()
     r = c_api.solve(lp, tol=1e-8, presolve=True)
     assert r.error == 0 and r.status == 1
+    # termination information of the original problem, no NaN
+    assert r.info["l2_primal_residual"] < 1e-12 and r.info["l2_dual_residual"] == 0.0
+    assert r.info["gap"] == 0.0 and r.info["primal_objective"] == r.primal_objective
     assert np.allclose(r.x, [2, 1, 5, 3]) and abs(r.primal_objective - (-1.0)) < 1e-12
     assert np.allclose(r.y, [0.5, -2, 0, -1, 0]) and np.allclose(r.reduced_cost, 0)
     assert kkt_ok(lp, r)
@@ -98,6 +101,27 @@ def test_presolve_detects_infeasible_and_unbounded():
     # x1 in no row, cost -1, no upper bound: unbounded (as the reference reports it: DualInfeasible)
     lp = lp_from(np.array([[1.0, 0.0]]), [1, -1], [0], [1], [0, 0], [5, INF])
     assert c_api.solve(lp, presolve=True).status == 3
+
+
+def test_reference_infeasible_problem_by_row_activity():
+    # c_api_test.c:791-900 (solved there by dual simplex, status Infeasible): x4 >= 5
+    # and x5 <= 2 are singleton rows, and then 3 x4 - x5 >= 13 > 2 in row 5
+    ptr = [0, 2, 4, 6, 7, 9, 10, 12, 15, 17]
+    idx = [0, 1, 0, 1, 0, 1, 3, 2, 3, 2, 0, 3, 0, 1, 2, 1, 2]
+    val = [-0.5, 1.0, 2.0, -1.0, 3.0, 1.0, 1.0, 3.0, -1.0, 1.0, 1.0, 1.0, 1.0, 2.0, 1.0, 1.0, 1.0]
+    rhs = np.array([0.5, 3.0, 6.0, 2.0, 2.0, 5.0, 10.0, 14.0, 1.0])
+    ge = np.array([1, 1, 0, 0, 0, 1, 0, 0, 1], bool)
+    A = sp.csr_matrix((val, idx, ptr), shape=(9, 4))
+    lp = lp_from(A, [0, 0, 0, 0], np.where(ge, rhs, -INF), np.where(ge, INF, rhs), [0] * 4, [INF] * 4)
+    r = c_api.solve(lp, presolve=True)
+    assert r.error == 0 and r.status == 2
+    # the activity test alone: one two-variable row that its bounds cannot reach
+    lp = lp_from(np.array([[1.0, 1.0]]), [1, 1], [30], [INF], [0, 0], [10, 10])
+    assert c_api.solve(lp, presolve=True).status == 2
+    # ... and one that they just reach is kept (solved on the GPU when present)
+    lp = lp_from(np.array([[1.0, 1.0]]), [1, 1], [20], [INF], [0, 0], [10, 10])
+    r = c_api.solve(lp, presolve=True)
+    assert r.status != 2
 
 
 def test_presolve_off_keeps_the_no_constraint_rule():
