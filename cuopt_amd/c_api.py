@@ -22,6 +22,11 @@ CUOPT_VALIDATION_ERROR, CUOPT_OUT_OF_MEMORY, CUOPT_RUNTIME_ERROR = 4, 5, 6
 CUOPT_MINIMIZE, CUOPT_MAXIMIZE = 1, -1
 CUOPT_METHOD_CONCURRENT, CUOPT_METHOD_PDLP, CUOPT_METHOD_DUAL_SIMPLEX, CUOPT_METHOD_BARRIER = 0, 1, 2, 3
 CUOPT_PRESOLVE_DEFAULT, CUOPT_PRESOLVE_OFF, CUOPT_PRESOLVE_PAPILO, CUOPT_PRESOLVE_PSLP = -1, 0, 1, 2
+# termination statuses (cuopt_c.h; the reference's spelling)
+CUOPT_TERIMINATION_STATUS_NO_TERMINATION, CUOPT_TERIMINATION_STATUS_OPTIMAL = 0, 1
+CUOPT_TERIMINATION_STATUS_INFEASIBLE, CUOPT_TERIMINATION_STATUS_UNBOUNDED = 2, 3
+CUOPT_TERIMINATION_STATUS_ITERATION_LIMIT, CUOPT_TERIMINATION_STATUS_TIME_LIMIT = 4, 5
+CUOPT_TERIMINATION_STATUS_NUMERICAL_ERROR, CUOPT_TERIMINATION_STATUS_PRIMAL_FEASIBLE = 6, 7
 STATUS = {0: "NoTermination", 1: "Optimal", 2: "PrimalInfeasible", 3: "DualInfeasible", 4: "IterationLimit",
           5: "TimeLimit", 6: "NumericalError", 7: "PrimalFeasible", 9: "ConcurrentLimit"}
 INFO_NAMES = [
@@ -340,10 +345,13 @@ def solve(lp, tol: float = 1e-4, precision: str = "fp64", iteration_limit: int |
 
 
 def batch_solve(lps, tol: float = 1e-4, precision: str = "fp64", iteration_limit: int | None = None,
-                time_limit: float | None = None) -> list[Result]:
-    """cuOptAmdBatchSolve over ``lps``: one persistent launch for the whole list."""
+                time_limit: float | None = None, **params) -> list[Result]:
+    """cuOptAmdBatchSolve over ``lps``: one persistent launch for the whole list.  Further
+    keyword arguments are settings by parameter name."""
     probs = [Problem(lp) for lp in lps]
     s = make_settings(tol, precision, iteration_limit, time_limit)
+    for name, v in params.items():
+        _check(s.set(name, v), name)
     hs = (ctypes.c_void_p * max(len(probs), 1))(*[p.handle.value for p in probs])
     sols = (ctypes.c_void_p * max(len(probs), 1))()
     lib().cuOptAmdBatchSolve(len(probs), hs, s.handle, sols)

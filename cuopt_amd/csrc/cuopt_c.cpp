@@ -662,6 +662,11 @@ cuopt_int_t cuOptAmdBatchSolve(cuopt_int_t num_problems,
       fail(sols[i]->res, CUOPT_INVALID_ARGUMENT, "MIP problems are not handled by this LP engine");
     else if (conf.method == CUOPT_METHOD_DUAL_SIMPLEX || conf.method == CUOPT_METHOD_BARRIER)
       fail(sols[i]->res, CUOPT_INVALID_ARGUMENT, "only the PDLP method (or concurrent, which runs PDLP) is available");
+    else if (conf.per_constraint_residual || conf.first_primal_feasible || conf.save_best_primal_so_far)
+      // as the reference's batch mode (convergence_information.cu:393-395, pdlp.cu:931-933)
+      fail(sols[i]->res, CUOPT_VALIDATION_ERROR,
+           "per_constraint_residual, first_primal_feasible and save_best_primal_so_far are not supported in batch "
+           "mode");
     else {
       lps.push_back(&lp);
       where.push_back(i);

@@ -250,6 +250,7 @@ struct KktRowsEpi {
   const double* xstar;
   const double* ystar;
   double inv_dp, inv_dd;
+  double* rp_out = nullptr;  // per_constraint_residual: the row's primal residual (else null)
   __device__ bool skip() const { return false; }
   __device__ double product(double v, int col, int) const { return v * xu[col]; }
   struct Pre {
@@ -268,6 +269,7 @@ struct KktRowsEpi {
     const double cl = mx(b.x, mn(ax, b.y));
     const double rp = ax - cl;
     red[0] += rp * rp;
+    if (rp_out) rp_out[r] = rp;
     red[1] += mx(yu, 0.0) * fin0(b.x) + mn(yu, 0.0) * fin0(b.y);
     yu_out[r]     = yu;
     if (ystar) {
@@ -308,6 +310,7 @@ struct KktColsEpi {
   const typename Vec2<V>::type* bnd;
   const double* lb_orig;  // recentred solves: a variable on its correction bound
   const double* ub_orig;  // is put exactly on the original bound
+  double* rd_out = nullptr;  // per_constraint_residual: the column's dual residual (else null)
   __device__ bool skip() const { return false; }
   struct Pre {
     double dc, c;
@@ -337,6 +340,7 @@ struct KktColsEpi {
     }
     const double rd = g - su;
     red[0] += rd * rd;
+    if (rd_out) rd_out[r] = rd;
     red[1] += xu * cj;
     red[2] += su * xu;
     xu_out[r] = xu;

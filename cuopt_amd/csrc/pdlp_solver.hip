@@ -61,6 +61,39 @@ __global__ __launch_bounds__(1024) void reduce_partials_kernel(const double* __r
   }
 }
 
+// per_constraint_residual (convergence_information.cu:390-416, 453-470, utils.cuh:578-590):
+// the largest |r_i| - rel * w_i of a residual vector, with w_i the larger finite
+// bound magnitude of row i (cb given, combine_finite_abs_bounds) or |c_i|, and
+// 0 as the neutral element.  One workgroup; the order of the max is fixed.
+__global__ __launch_bounds__(1024) void linf_rel_kernel(const double* __restrict__ r,
+                                                        const double2* __restrict__ cb,
+                                                        const double* __restrict__ c,
+                                                        int n,
+                                                        double rel,
+                                                        double* __restrict__ out)
+{
+  __shared__ double lds[1024];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    double w = 0.0;
+    if (cb) {
+      const double2 b = cb[i];
+      if (isfinite(b.y)) w = fmax(w, fabs(b.y));
+      if (isfinite(b.x)) w = fmax(w, fabs(b.x));
+    } else {
+      w = fabs(c[i]);
+    }
+    v = fmax(v, fabs(r[i]) - rel * w);
+  }
+  lds[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) lds[threadIdx.x] = fmax(lds[threadIdx.x], lds[threadIdx.x + off]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = lds[0];
+}
+
 template <class F>
 __global__ __launch_bounds__(kBlock) void elementwise_kernel(int n, double* __restrict__ red_part, F f)
 {
@@ -717,6 +750,7 @@ enum Slot {
   S_FPE_NY, S_FPE_I, S_FPE_NX,                   // periodic fixed-point error terms
   S_FPE0_NY, S_FPE0_I, S_FPE0_NX,                // post-restart fixed-point error terms
   S_SETUP0, S_SETUP1,                            // setup reductions
+  S_LINF_P, S_LINF_D,                            // per_constraint_residual maxima
   S_COUNT
 };
 
@@ -777,11 +811,23 @@ class PdlpSolver {
   void finish(SolveResult& R, int status);
   int pdhg_loop(SolveResult& R, const Tolerances& tol, bool refine);
   void recentre();
-  // Termination test of the last evaluation (termination_strategy.cu:311-352).
+  // Termination tests of the last evaluation (termination_strategy.cu:311-352); with
+  // per_constraint_residual the residual tests are the relative L-inf ones
+  // (:327-345): max_i |r_i| - rel w_i <= the absolute tolerance.
+  bool primal_feasible(const Tolerances& t) const
+  {
+    return st_.per_constraint_residual ? linf_p_ <= t.ap : l2p_ <= t.ap + t.rp * b_norm_;
+  }
   bool is_optimal(const Tolerances& t) const
   {
-    return l2d_ <= t.ad + t.rd * c_norm_ && l2p_ <= t.ap + t.rp * b_norm_ && gap_ <= t.ag + t.rg * absobj_;
+    const bool dual_ok = st_.per_constraint_residual ? linf_d_ <= t.ad : l2d_ <= t.ad + t.rd * c_norm_;
+    return dual_ok && primal_feasible(t) && gap_ <= t.ag + t.rg * absobj_;
   }
+  // save_best_primal_so_far (pdlp.cu:508-625): the best unscaled point met at a
+  // check -- primal feasible beats infeasible, then the better objective, then
+  // the smaller L2 primal residual -- is returned at a time or iteration limit.
+  void record_best(const Tolerances& t);
+  void restore_best();
   double rel_kkt_error() const
   {
     return std::max({l2p_ / (1.0 + b_norm_), l2d_ / (1.0 + c_norm_), gap_ / (1.0 + absobj_)});
@@ -810,6 +856,11 @@ class PdlpSolver {
   DevBuf<V> x_[2], y_[2], x0_, y0_, xbar_, ybar_, xhat_, yhat_, slack_, aty_;
   DevBuf<double> xu_, yu_, su_;
   DevBuf<double> red_, scal_;
+  DevBuf<double> rp_res_, rd_res_;          // per_constraint_residual: per-row / per-column residuals
+  double linf_p_ = 0.0, linf_d_ = 0.0;
+  DevBuf<double> best_x_, best_y_, best_s_;  // save_best_primal_so_far
+  bool have_best_ = false, best_feasible_ = false;
+  double best_l2p_ = 0, best_l2d_ = 0, best_pobj_ = 0, best_dobj_ = 0, best_gap_ = 0, best_absobj_ = 0;
   DevBuf<StepScalars> step_sc_;
   int cur_ = 0;
   hipEvent_t* ev_ = nullptr;  // events of the current timed iteration (benchmark window)
@@ -1171,17 +1222,28 @@ void PdlpSolver<V>::evaluate_kkt()
   const double idp = rf ? 1.0 / refine_.dp : 1.0, idd = rf ? 1.0 / refine_.dd : 1.0;
   // xu_ holds the gathered point until the column pass writes the final x there.
   launch_ew(n_, nullptr, KktPointF<V>{xhat_.get(), Dc_.get(), ib, xs, idp, xu_.get()}, stream_);
+  const bool per = st_.per_constraint_residual;
+  if (per && rp_res_.size() < (size_t)std::max(m_, 1)) rp_res_.resize(std::max(m_, 1));
+  if (per && rd_res_.size() < (size_t)std::max(n_, 1)) rd_res_.resize(std::max(n_, 1));
   launch_tiled(A_, a_orig_.get(), red_.get(),
                KktRowsEpi<V>{xu_.get(), yhat_.get(), Dr_.get(), ig, gamma_, cb_orig_.get(), y0_.get(), yu_.get(), xs,
-                             ys, idp, idd},
+                             ys, idp, idd, per ? rp_res_.get() : nullptr},
                stream_);
   launch_reduce(red_.get(), A_.plan.red_blocks(), 3, scal_.get() + S_KKT_RP2, stream_);
   launch_tiled(AT_, at_orig_.get(), red_.get(),
                KktColsEpi<V>{yu_.get(), xhat_.get(), slack_.get(), Dc_.get(), ib, beta_, ig, gamma_, c_orig_.get(),
                              x0_.get(), xu_.get(), su_.get(), xs, idp, idd, bnd_w_.get(), lb_orig_.get(),
-                             ub_orig_.get()},
+                             ub_orig_.get(), per ? rd_res_.get() : nullptr},
                stream_);
   launch_reduce(red_.get(), AT_.plan.red_blocks(), 4, scal_.get() + S_KKT_RD2, stream_);
+  if (per) {
+    hipLaunchKernelGGL(linf_rel_kernel, dim3(1), dim3(1024), 0, stream_, rp_res_.get(), cb_orig_.get(), nullptr, m_,
+                       st_.rel_primal, scal_.get() + S_LINF_P);
+    CUOPT_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(linf_rel_kernel, dim3(1), dim3(1024), 0, stream_, rd_res_.get(), nullptr, c_orig_.get(), n_,
+                       st_.rel_dual, scal_.get() + S_LINF_D);
+    CUOPT_HIP_CHECK(hipGetLastError());
+  }
   read_scalars();
   const double* s = h_scal_;
   l2p_            = std::sqrt(s[S_KKT_RP2]);
@@ -1198,6 +1260,54 @@ void PdlpSolver<V>::evaluate_kkt()
   absobj_ = std::fabs(pobj) + std::fabs(dobj);
   dp_     = s[S_KKT_DP];
   dd_     = s[S_KKT_DD];
+  if (per) {
+    linf_p_ = s[S_LINF_P];
+    linf_d_ = s[S_LINF_D];
+  }
+}
+
+template <typename V>
+void PdlpSolver<V>::record_best(const Tolerances& t)
+{
+  const bool feas = primal_feasible(t);
+  bool better     = !have_best_;
+  if (have_best_) {
+    if (feas != best_feasible_) better = feas;
+    else if (feas) better = lp_.maximize ? pobj_ > best_pobj_ : pobj_ < best_pobj_;
+    else better = l2p_ < best_l2p_;
+  }
+  if (!better) return;
+  if (!have_best_) {
+    best_x_.resize(std::max(n_, 1));
+    best_y_.resize(std::max(m_, 1));
+    best_s_.resize(std::max(n_, 1));
+  }
+  CUOPT_HIP_CHECK(hipMemcpyAsync(best_x_.get(), xu_.get(), n_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  CUOPT_HIP_CHECK(hipMemcpyAsync(best_y_.get(), yu_.get(), m_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  CUOPT_HIP_CHECK(hipMemcpyAsync(best_s_.get(), su_.get(), n_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  have_best_     = true;
+  best_feasible_ = feas;
+  best_l2p_      = l2p_;
+  best_l2d_      = l2d_;
+  best_pobj_     = pobj_;
+  best_dobj_     = dobj_;
+  best_gap_      = gap_;
+  best_absobj_   = absobj_;
+}
+
+template <typename V>
+void PdlpSolver<V>::restore_best()
+{
+  if (!have_best_) return;
+  CUOPT_HIP_CHECK(hipMemcpyAsync(xu_.get(), best_x_.get(), n_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  CUOPT_HIP_CHECK(hipMemcpyAsync(yu_.get(), best_y_.get(), m_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  CUOPT_HIP_CHECK(hipMemcpyAsync(su_.get(), best_s_.get(), n_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+  l2p_    = best_l2p_;
+  l2d_    = best_l2d_;
+  pobj_   = best_pobj_;
+  dobj_   = best_dobj_;
+  gap_    = best_gap_;
+  absobj_ = best_absobj_;
 }
 
 template <typename V>
@@ -1504,12 +1614,22 @@ int PdlpSolver<V>::pdhg_loop(SolveResult& R, const Tolerances& tol, bool refine)
     const size_t rec = R.trace.size();
     R.trace.insert(R.trace.end(), {(double)(k0 + k), l2p_, l2d_, pobj_, dobj_, gap_, 0.0, omega_, fpe, (double)kr_});
     if (k > 1 && is_optimal(tol)) return done(CUOPT_TERIMINATION_STATUS_OPTIMAL);
+    // first_primal_feasible (pdlp.cu:928-983): a primal feasible, not optimal iterate ends the solve
+    if (k > 1 && st_.first_primal_feasible && !bench && primal_feasible(tol))
+      return done(CUOPT_TERIMINATION_STATUS_PRIMAL_FEASIBLE);
     // Divergence guard: a NaN residual can never satisfy the test again, so stop
     // with a numerical error instead of running into the limits.
     if (std::isnan(l2p_) || std::isnan(l2d_) || std::isnan(gap_))
       return done(CUOPT_TERIMINATION_STATUS_NUMERICAL_ERROR);
-    if (elapsed() >= st_.time_limit) return done(CUOPT_TERIMINATION_STATUS_TIME_LIMIT);
-    if (k0 + k >= st_.iteration_limit) return done(CUOPT_TERIMINATION_STATUS_ITERATION_LIMIT);
+    if (st_.save_best_primal_so_far && !bench) record_best(tol);  // pdlp.cu:1171-1175
+    if (elapsed() >= st_.time_limit) {
+      if (st_.save_best_primal_so_far && !bench) restore_best();  // pdlp.cu:341-347
+      return done(CUOPT_TERIMINATION_STATUS_TIME_LIMIT);
+    }
+    if (k0 + k >= st_.iteration_limit) {
+      if (st_.save_best_primal_so_far && !bench) restore_best();  // pdlp.cu:394-401
+      return done(CUOPT_TERIMINATION_STATUS_ITERATION_LIMIT);
+    }
 
     bool restarted = false;
     if (is_major) {

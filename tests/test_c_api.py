@@ -318,3 +318,12 @@ def test_mip_only_settings_and_getters():
     assert L.cuOptGetMIPGap(sol, ctypes.byref(v)) == c_api.CUOPT_INVALID_ARGUMENT
     assert L.cuOptGetSolutionBound(sol, ctypes.byref(v)) == c_api.CUOPT_INVALID_ARGUMENT
     L.cuOptDestroySolution(ctypes.byref(sol))
+
+
+@pytest.mark.parametrize("name", ["per_constraint_residual", "first_primal_feasible", "save_best_primal_so_far"])
+def test_batch_rejects_single_lp_termination_options(name):
+    # the reference's batch mode rejects them (convergence_information.cu:393-395,
+    # pdlp.cu:931-933); no device work is done for a rejected problem
+    lps = [G.random_lp(m=60, n=150, seed=44)]
+    (r,) = c_api.batch_solve(lps, tol=1e-4, time_limit=60.0, **{name: True})
+    assert r.error == c_api.CUOPT_VALIDATION_ERROR

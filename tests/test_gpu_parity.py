@@ -243,6 +243,87 @@ def test_reference_maximize_problem_dual_variables(method):
     np.testing.assert_allclose(r.reduced_cost, lp.c - A.T @ r.y, atol=1e-3)
 
 
+def _solve_with(lp, tol, **params):
+    prob = c_api.Problem(lp)
+    st = c_api.make_settings(tol, "fp64", params.pop("iteration_limit", 200000), TL)
+    for name, v in params.items():
+        assert st.set(name, v) == 0, name
+    return c_api.solve_problem(prob, st)
+
+
+def _residuals(lp, r):
+    import scipy.sparse as sp
+
+    A = sp.csr_matrix((lp.values, lp.col_indices, lp.row_offsets), shape=(lp.m, lp.n))
+    ax = A @ r.x
+    viol = np.maximum(lp.con_lb - ax, 0.0) + np.maximum(ax - lp.con_ub, 0.0)
+    w = np.maximum(np.where(np.isfinite(lp.con_lb), np.abs(lp.con_lb), 0.0),
+                   np.where(np.isfinite(lp.con_ub), np.abs(lp.con_ub), 0.0))
+    return viol, w
+
+
+def test_per_constraint_residual():
+    # termination_strategy.cu:327-345: with per_constraint_residual the residual tests are
+    # max_i |r_p,i| - rel w_i <= abs (w_i the larger finite bound magnitude of row i) and
+    # max_j |r_d,j| - rel |c_j| <= abs.  The returned point meets both (checked in numpy);
+    # the L2-terminated point of the same LP does not, so the option changes the rule.
+    import scipy.sparse as sp
+
+    lp = G.random_lp(m=3000, n=7500, seed=41)
+    tol = 1e-4
+    A = sp.csr_matrix((lp.values, lp.col_indices, lp.row_offsets), shape=(lp.m, lp.n))
+
+    def linf(r):
+        viol, w = _residuals(lp, r)
+        rd = lp.c - A.T @ r.y - r.reduced_cost
+        return np.max(viol - tol * w, initial=0.0), np.max(np.abs(rd) - tol * np.abs(lp.c), initial=0.0)
+
+    plain = _solve_with(lp, tol)
+    per = _solve_with(lp, tol, per_constraint_residual=True)
+    assert plain.status == per.status == 1
+    p_inf, d_inf = linf(per)
+    assert p_inf <= tol * (1 + 1e-6) and d_inf <= tol * (1 + 1e-6)
+    assert max(linf(plain)) > tol
+    assert rel(per.primal_objective, lp.obj_opt) < 1e-4
+
+
+def test_first_primal_feasible():
+    # pdlp.cu:928-983: the first check whose iterate passes the primal test, but not
+    # the optimality test, ends the solve with PrimalFeasible.  The primal tolerances
+    # are looser than the dual and gap ones here, so the primal test passes first.
+    lp = G.random_lp(m=1000, n=2500, seed=42)
+    tol, ptol = 1e-8, 1e-4
+    loose = dict(absolute_primal_tolerance=ptol, relative_primal_tolerance=ptol)
+    full = _solve_with(lp, tol, **loose)
+    first = _solve_with(lp, tol, first_primal_feasible=True, **loose)
+    assert full.status == 1
+    assert first.status == c_api.CUOPT_TERIMINATION_STATUS_PRIMAL_FEASIBLE
+    assert first.iterations < full.iterations
+    viol, _ = _residuals(lp, first)
+    b_norm = np.sqrt(np.sum(np.where(np.isfinite(lp.con_lb) & (lp.con_lb != lp.con_ub), lp.con_lb, 0.0) ** 2)
+                     + np.sum(np.where(np.isfinite(lp.con_ub), lp.con_ub, 0.0) ** 2))
+    assert np.linalg.norm(viol) <= (ptol + ptol * b_norm) * (1 + 1e-6)
+    # the previous check was not primal feasible
+    tr = first.trace
+    assert tr.shape[0] >= 2 and tr[-2, 1] > ptol + ptol * b_norm
+
+
+def test_save_best_primal_so_far():
+    # pdlp.cu:508-625, 394-401: at the iteration limit the best point met at a check
+    # is returned (no check is primal feasible this early: the smallest L2 primal
+    # residual wins), with status IterationLimit
+    lp = G.skewed_lp(m=20000, n=40000, seed=43)
+    last = _solve_with(lp, 1e-8, iteration_limit=1500)
+    best = _solve_with(lp, 1e-8, iteration_limit=1500, save_best_primal_so_far=True)
+    assert last.status == best.status == c_api.CUOPT_TERIMINATION_STATUS_ITERATION_LIMIT
+    np.testing.assert_array_equal(last.trace, best.trace)  # the same run
+    l2p = best.trace[:, 1]
+    assert best.info["l2_primal_residual"] == l2p.min()
+    viol, _ = _residuals(lp, best)
+    assert abs(np.linalg.norm(viol) - l2p.min()) <= 1e-6 * (1 + l2p.min())
+    assert best.info["l2_primal_residual"] <= last.info["l2_primal_residual"]
+
+
 def test_transport_matches_highs(highs):
     case = highs["transport_small"]
     lp = G.transportation_lp(**case["kwargs"])
