@@ -609,6 +609,14 @@ cuopt_int_t cuOptSolve(cuOptOptimizationProblem problem, cuOptSolverSettings set
                     R.primal_objective, (int)R.info[CUOPT_AMD_INFO_STEPS], R.solve_time);
         std::fflush(stdout);
       }
+      // log_file: the same line, appended to the file (solve.cu:1136, logger.cpp:149-172)
+      if (!conf.log_file.empty()) {
+        if (FILE* f = std::fopen(conf.log_file.c_str(), "a")) {
+          std::fprintf(f, "Status: %s   Objective: %.8e  Iterations: %d  Time: %.3fs\n", status_string(R.status),
+                       R.primal_objective, (int)R.info[CUOPT_AMD_INFO_STEPS], R.solve_time);
+          std::fclose(f);
+        }
+      }
     } catch (const validation_error& e) {
       fail(CUOPT_VALIDATION_ERROR, e.what());
     } catch (const std::bad_alloc& e) {

@@ -407,3 +407,16 @@ def test_iteration_limit_returns_the_iterate():
     assert sol.get_termination_status() == LPM.LPTerminationStatus.IterationLimit
     assert sol.get_primal_objective() != 0.0
     assert np.any(sol.get_primal_solution())
+
+
+@gpu
+def test_log_file_gets_the_status_line(tmp_path):
+    # log_file (solve.cu:1136): the final status line goes to the file as well
+    log = tmp_path / "solve.log"
+    s = LPM.SolverSettings()
+    s.set_parameter("time_limit", 60.0)
+    s.set_parameter("log_file", str(log))
+    LPM.Solve(LPM.ParseMps(os.path.join(GOLDEN, "afiro_original.mps")), s)
+    LPM.Solve(LPM.ParseMps(os.path.join(GOLDEN, "good-max.mps")), s)
+    lines = log.read_text().splitlines()
+    assert len(lines) == 2 and all(line.startswith("Status: Optimal") for line in lines)
