@@ -186,6 +186,7 @@ struct Stage {               // LDS of one workgroup
   int* idx;                  // 2 x kStage column indices (buffer b at idx + b * kStage)
   unsigned char* vals;       // 2 x kStage values of 8 bytes (FP64 or FP32 matrices), same order
   uint64_t* merge;           // kBT / kWave accumulator slots
+  bool dbl;                  // two buffers; else one (block b's copy is waited for before b)
 };
 
 struct RowBlocks {
@@ -295,13 +296,13 @@ __device__ inline void rows_pass(const int* __restrict__ ptr,
     r2 = blocks.start[2];
     z2 = ptr[r2];
   }
-  if (z1 - z0 <= kBlockNnz) stage(z0, z1, 0);
+  if (st.dbl && z1 - z0 <= kBlockNnz) stage(z0, z1, 0);
   // This lane group's row of the current block: its ptr entries.  The loads
   // are clamped into range instead of guarded, so that no branch waits for them
   // (a wait for a load also waits for every copy issued before it).
   int pb = ptr[min(r0 + grp, r1)], pe = ptr[min(r0 + grp + 1, r1)];
   for (int bi = 0; bi < nb; ++bi) {
-    const int cur = bi & 1;
+    const int cur = st.dbl ? bi & 1 : 0;
     // Every wave's copy of block bi has landed (the barrier waits for each
     // wave's memory operations), and block bi - 1's reads of the other buffer
     // are done, so it may be refilled.
@@ -309,7 +310,11 @@ __device__ inline void rows_pass(const int* __restrict__ ptr,
     const int b = pb - z0, e = pe - z0;
     const int r3          = bi + 3 <= nb ? blocks.start[bi + 3] : r2;
     const bool stage_next = bi + 1 < nb && z2 - z1 <= kBlockNnz;
-    if (stage_next) stage(z1, z2, cur ^ 1);
+    if (st.dbl && stage_next) stage(z1, z2, cur ^ 1);
+    if (!st.dbl && z1 - z0 <= kBlockNnz) {  // one buffer: copy this block now and wait for it
+      stage(z0, z1, 0);
+      __syncthreads();
+    }
     const int npb = ptr[min(r1 + grp, r2)], npe = ptr[min(r1 + grp + 1, r2)];
     if (z1 - z0 > kBlockNnz) {  // one row longer than a block: read it in place
       const auto pre = epi.load(r0);
@@ -471,10 +476,17 @@ __global__ __launch_bounds__(kBT, kWgPerCu) void batch_pdlp_kernel(BatchArgs<V> 
   // x-bar (dual step) -- live in LDS when the LP fits: each gather is then an
   // LDS read instead of a round trip to L2 / MALL.  Nothing reads them after
   // the kernel (the solution is the unscaled xu / yu / su).
-  const Stage st{reinterpret_cast<int*>(dyn_lds + 2 * kStage * 8), dyn_lds, merge_lds};
+  // A double-buffered stage when the vectors fit beside it (or do not fit at
+  // all); a single buffer when that lets them fit: gathers from LDS outweigh
+  // the copy overlap.
+  const long long vec_bytes = (long long)(2 * m + n) * (long long)sizeof(V);
+  const bool dbl = kStageBytes + vec_bytes <= (long long)A.lds_bytes ||
+                   kStageBytes / 2 + vec_bytes > (long long)A.lds_bytes;
+  const int stage_bytes = dbl ? kStageBytes : kStageBytes / 2;
+  const Stage st{reinterpret_cast<int*>(dyn_lds + (dbl ? 2 : 1) * kStage * 8), dyn_lds, merge_lds, dbl};
   const RowBlocks ablk{A.blk + d.ablk0, d.nablk}, tblk{A.blk + d.tblk0, d.ntblk};
-  if (kStageBytes + (long long)(2 * m + n) * (long long)sizeof(V) <= (long long)A.lds_bytes) {
-    V* base = reinterpret_cast<V*>(dyn_lds + kStageBytes);
+  if (stage_bytes + vec_bytes <= (long long)A.lds_bytes) {
+    V* base = reinterpret_cast<V*>(dyn_lds + stage_bytes);
     ys[0]   = base;
     ys[1]   = base + m;
     xbar    = base + 2 * m;
