@@ -1816,6 +1816,10 @@ SolveResult solve_presolved(const LpHost& lp, const std::vector<double>& lc, con
   if (P.reduced.m == 0) {  // presolve solved the problem
     R.status = CUOPT_TERIMINATION_STATUS_OPTIMAL;
     postsolve(P, lp, R);
+    if (!st.dual_postsolve) {
+      R.y.clear();
+      R.rc.clear();
+    }
     R.dual_objective = R.primal_objective;
     R.info[CUOPT_AMD_INFO_STEPS] = 0;
     // Termination information on the original problem: the primal residual of x
@@ -1854,6 +1858,10 @@ SolveResult solve_presolved(const LpHost& lp, const std::vector<double>& lc, con
   R = solve_checked(P.reduced, P.lc, P.uc, P.lb, P.ub, rs, t0);
   if (R.error == CUOPT_SUCCESS && R.x.size() == (size_t)P.reduced.n && R.y.size() == (size_t)P.reduced.m) {
     postsolve(P, lp, R);
+    if (!st.dual_postsolve) {  // no dual solution after presolve (pdlp_test.cu:903-930)
+      R.y.clear();
+      R.rc.clear();
+    }
   } else {
     R.x.assign(lp.n, 0.0);
     R.y.assign(lp.m, 0.0);

@@ -78,6 +78,18 @@ def test_presolve_solves_singletons_fixed_and_empty_rows_without_gpu():
     assert kkt_ok(lp, r)
 
 
+def test_dual_postsolve_off_returns_no_duals():
+    # pdlp_test.cu:903-930: with presolve and dual_postsolve false the solution has no
+    # duals (the C getters then fill zeros); the primal solution is unchanged
+    lp = fully_reducible()
+    prob = c_api.Problem(lp)
+    st = c_api.make_settings(1e-8, presolve=True)
+    assert st.set("dual_postsolve", False) == 0
+    r = c_api.solve_problem(prob, st)
+    assert r.status == 1 and np.allclose(r.x, [2, 1, 5, 3])
+    assert not np.any(r.y) and not np.any(r.reduced_cost)
+
+
 def test_presolve_maximisation_and_empty_columns():
     # max 3 x0 - x1 + 2 x2 s.t. x0 <= 4 (singleton), x1 >= -2 via -x1 <= 2; x2 in no row, in [0, 7].
     A = sp.csr_matrix(np.array([[1.0, 0, 0], [0, -1.0, 0]]))
