@@ -417,8 +417,11 @@ class SolverSettings:
             self.settings_dict[p] = eps_optimal
 
     def set_pdlp_warm_start_data(self, pdlp_warm_start_data):
-        raise NotImplementedError("PDLP warm-start data is not supported by this engine; use "
-                                  "DataModel.set_initial_primal_solution / set_initial_dual_solution")
+        """Stored as in the reference.  Every solver mode runs Stable3 here, where the
+        reference does not support PDLP warm starts: Solve then returns a Solution with
+        LPTerminationStatus.NoTermination (pdlp_test.cu:880-900), and BatchSolve raises
+        (solver_wrapper.pyx:528).  Initial primal / dual solutions are the supported form."""
+        self.pdlp_warm_start_data = pdlp_warm_start_data
 
     def get_pdlp_warm_start_data(self):
         return self.pdlp_warm_start_data
@@ -555,14 +558,16 @@ def _prepare(dm: DataModel, settings: SolverSettings | None):
         settings = SolverSettings()
     if dm._is_mip():
         raise ValueError("MIP problems are not handled by this LP engine")
-    if settings.get_pdlp_warm_start_data() is not None:
-        raise ValueError("PDLP warm-start data is not supported by this engine")
     return settings
 
 
 def Solve(data_model, solver_settings=None):
     """Solves the LP of data_model and returns a Solution (solver.py:12-105)."""
     settings = _prepare(data_model, solver_settings)
+    if settings.get_pdlp_warm_start_data() is not None:  # Stable3: no PDLP warm start (pdlp_test.cu:880-900)
+        return Solution(ProblemCategory.LP, {}, primal_solution=np.zeros(0), dual_solution=np.zeros(0),
+                        reduced_cost=np.zeros(0), termination_status=LPTerminationStatus.NoTermination,
+                        error_message="PDLP warm-start data is not supported in Stable3 mode")
     s = settings._handle()
     m, n = data_model._sizes()
     if len(data_model.initial_primal_solution):
@@ -590,6 +595,8 @@ def BatchSolve(data_model_list, solver_settings=None):
     """Solves every LP of data_model_list in one persistent launch; returns
     (solutions, solve_time) with the engine time of the whole batch (solver.py:108-150)."""
     settings = solver_settings if solver_settings is not None else SolverSettings()
+    if settings.get_pdlp_warm_start_data() is not None:
+        raise ValueError("Cannot use warmstart data with Batch Solve")  # solver_wrapper.pyx:528
     for dm in data_model_list:
         _prepare(dm, settings)
     s = settings._handle()

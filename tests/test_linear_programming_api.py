@@ -214,8 +214,13 @@ def test_solver_settings_by_name():
     s.set_parameter("iteration_limit", -3)
     with pytest.raises(ValueError):
         s._handle()
-    with pytest.raises(NotImplementedError):
-        s.set_pdlp_warm_start_data(object())
+    # PDLP warm-start data: Stable3 returns NoTermination (pdlp_test.cu:880-900), batch raises
+    s.set_parameter("iteration_limit", 100)
+    s.set_pdlp_warm_start_data(object())
+    sol = LPM.Solve(LPM.ParseMps(os.path.join(GOLDEN, "afiro_original.mps")), s)
+    assert sol.get_termination_status() == LPM.LPTerminationStatus.NoTermination
+    with pytest.raises(ValueError):
+        LPM.BatchSolve([LPM.ParseMps(os.path.join(GOLDEN, "afiro_original.mps"))], s)
 
 
 def test_enums_follow_the_c_constants():
