@@ -425,3 +425,27 @@ def test_log_file_gets_the_status_line(tmp_path):
     LPM.Solve(LPM.ParseMps(os.path.join(GOLDEN, "good-max.mps")), s)
     lines = log.read_text().splitlines()
     assert len(lines) == 2 and all(line.startswith("Status: Optimal") for line in lines)
+
+
+def test_reference_solver_settings_sequence():
+    # test_lp_solver.py:254-308, step by step (the out-of-range solver mode is caught
+    # when Solve applies the settings, before any device work)
+    s = LPM.SolverSettings()
+    names = ["absolute_dual_tolerance", "relative_dual_tolerance", "absolute_primal_tolerance",
+             "relative_primal_tolerance", "absolute_gap_tolerance", "relative_gap_tolerance",
+             "primal_infeasible_tolerance", "dual_infeasible_tolerance"]
+    for name in names:
+        s.set_parameter(name, 1e-5)
+    assert all(s.get_parameter(name) == 1e-5 for name in names)
+    assert s.get_parameter("time_limit") == float("inf")
+    s.set_parameter("iteration_limit", 10)
+    s.set_parameter("time_limit", 10.2)
+    assert s.get_parameter("iteration_limit") == 10 and s.get_parameter("time_limit") == 10.2
+    s.set_parameter("infeasibility_detection", False)
+    assert not s.get_parameter("infeasibility_detection")
+    assert s.get_parameter("pdlp_solver_mode") == int(LPM.PDLPSolverMode.Stable3)
+    with pytest.raises(ValueError):
+        s.set_parameter("pdlp_solver_mode", 10)
+        LPM.Solve(LPM.ParseMps(os.path.join(GOLDEN, "mps", "good-mps-1.mps")), s)
+    s.set_parameter("pdlp_solver_mode", LPM.PDLPSolverMode.Methodical1)
+    assert s.get_parameter("pdlp_solver_mode") == int(LPM.PDLPSolverMode.Methodical1)
