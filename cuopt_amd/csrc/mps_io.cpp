@@ -374,13 +374,15 @@ void write_mps(const LpHost& lp, const std::string& path)
       if (!readable(s) || !seen.insert(s).second) return false;
     return true;
   };
-  const bool row_names = pick(lp.row_names, lp.m), col_names = pick(lp.col_names, lp.n);
-  std::string obj = "OBJ";
-  if (readable(lp.objective_name)) obj = lp.objective_name;
-  if (row_names && std::find(lp.row_names.begin(), lp.row_names.end(), obj) != lp.row_names.end()) obj = "OBJ";
-  const bool rows_ok = row_names && std::find(lp.row_names.begin(), lp.row_names.end(), obj) == lp.row_names.end();
-  auto rname = [&](int i) { return rows_ok ? lp.row_names[i] : "R" + std::to_string(i); };
-  auto cname = [&](int j) { return col_names ? lp.col_names[j] : "C" + std::to_string(j); };
+  const bool rows_readable = pick(lp.row_names, lp.m), cols_readable = pick(lp.col_names, lp.n);
+  auto is_row = [&](const std::string& s) {
+    return std::find(lp.row_names.begin(), lp.row_names.end(), s) != lp.row_names.end();
+  };
+  std::string obj = readable(lp.objective_name) ? lp.objective_name : "OBJ";
+  if (rows_readable && is_row(obj)) obj = "OBJ";         // the objective row needs a name of its own
+  const bool use_rows = rows_readable && !is_row(obj);  // a row named OBJ: generated row names
+  auto rname = [&](int i) { return use_rows ? lp.row_names[i] : "R" + std::to_string(i); };
+  auto cname = [&](int j) { return cols_readable ? lp.col_names[j] : "C" + std::to_string(j); };
   std::fprintf(f, "NAME %s\n", readable(lp.problem_name) ? lp.problem_name.c_str() : "cuopt_amd");
   if (lp.maximize) std::fprintf(f, "OBJSENSE\n    MAX\n");
   std::fprintf(f, "ROWS\n N %s\n", obj.c_str());
