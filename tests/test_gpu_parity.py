@@ -371,6 +371,37 @@ def test_fp32_refinement_skewed_1e8():
     assert rp <= 1e-8 * (1.0 + bound_norm(lp)) * 1.01 and bv <= 1e-12
 
 
+def test_full_size_skewed_lp_fp32_mixed_and_fp64():
+    # the headline LP at its full size (m = 500 000, n = 1 000 000, about 4.75 M
+    # nonzeros) through size-independent properties: both precisions reach 1e-4
+    # relative KKT, the objective is the built-in optimum to 1e-3, an independent
+    # numpy check confirms the primal residual, and the two precisions agree to 1e-6
+    lp = G.skewed_lp()
+    assert lp.m == 500_000 and lp.n == 1_000_000
+    r32 = c_api.solve(lp, tol=1e-4, precision="fp32_mixed", time_limit=TL, iteration_limit=200000)
+    r64 = c_api.solve(lp, tol=1e-4, precision="fp64", time_limit=TL, iteration_limit=200000)
+    assert r32.status == r64.status == 1
+    for r in (r32, r64):
+        assert rel(r.primal_objective, lp.obj_opt) < 1e-3
+        rp, bv = primal_violation(lp, r.x)
+        assert rp <= 1e-4 * (1.0 + bound_norm(lp)) * 1.01 and bv <= 1e-12
+    assert abs(r32.primal_objective - r64.primal_objective) <= 1e-6 * abs(r64.primal_objective)
+
+
+def test_full_size_transport_fp32_refined_to_1e8(highs):
+    # the transportation LP at its full size (20 000 x 50 000, n = 1 000 000 arcs,
+    # m = 70 000): FP32-mixed with FP64 refinement to 1e-8 against the HiGHS optimum
+    case = highs["transport_full"]
+    lp = G.transportation_lp(**case["kwargs"])
+    assert lp.n == 1_000_000 and lp.m == 70_000
+    r = c_api.solve(lp, tol=1e-8, precision="fp32_mixed", time_limit=TL, iteration_limit=2_000_000)
+    assert r.status == 1, (r.status, r.info)
+    assert r.info["refine_rounds"] >= 1
+    assert rel(r.primal_objective, case["objective"]) < 1e-7
+    rp, bv = primal_violation(lp, r.x)
+    assert rp <= 1e-8 * (1.0 + bound_norm(lp)) * 1.01 and bv <= 1e-12
+
+
 def test_fp32_refinement_off_is_plain_loop():
     # default FP32-mixed solves recentre at restarts once the relative KKT error
     # is below 1e-2; with refinement off the loop is the plain one and never does
