@@ -568,6 +568,26 @@ def test_batch_rows_longer_than_a_block(precision):
         assert rel(r.primal_objective, s.primal_objective) < 1e-5, lp.name
 
 
+@pytest.mark.parametrize("precision", ["fp32_mixed", "fp64"])
+def test_full_size_batch_of_4096_lps(precision):
+    # the batched workload at its full size (4 096 LPs, m log-uniform in [50, 2 000])
+    # in one launch: every LP meets the 1e-4 relative KKT test it reports, and a
+    # sample checked in numpy has its primal residual within the tolerance and its
+    # objective near the built-in optimum
+    lps = G.batch_lps(count=4096, seed=0)
+    rs = c_api.batch_solve(lps, tol=1e-4, precision=precision, time_limit=TL)
+    assert all(r.status == 1 for r in rs)
+    for r in rs:
+        assert r.info["l2_relative_primal_residual"] <= 1e-4 * (1 + 1e-9)
+        assert r.info["l2_relative_dual_residual"] <= 1e-4 * (1 + 1e-9)
+        assert r.info["relative_gap"] <= 1e-4 * (1 + 1e-9)
+    for i in np.random.default_rng(0).choice(len(lps), 32, replace=False):
+        lp, r = lps[i], rs[i]
+        rp, bv = primal_violation(lp, r.x)
+        assert rp <= 1e-4 * (1.0 + bound_norm(lp)) * 1.01 and bv <= 1e-12
+        assert abs(r.primal_objective - lp.obj_opt) <= 1e-2 * (1 + abs(lp.obj_opt))
+
+
 def test_batch_iteration_limit():
     lps = [G.random_lp(m=100, n=250, seed=s) for s in range(4)]
     rs = c_api.batch_solve(lps, tol=0.0, time_limit=TL, iteration_limit=50)
