@@ -57,7 +57,21 @@ Presolved presolve(const LpHost& lp, const std::vector<double>& lc0, const std::
       if (--rcount[i] <= 1) rq.push_back(i);
     }
   };
-  while (!rq.empty() || !cq.empty()) {
+  // A column whose remaining entries all sit in removed rows is empty: recount
+  // over the active rows until no such column is left.
+  auto recount_columns = [&]() {
+    std::vector<int> live(n, 0);
+    for (int i = 0; i < m; ++i)
+      if (row_on[i])
+        for (int k = lp.rowptr[i]; k < lp.rowptr[i + 1]; ++k) ++live[lp.colidx[k]];
+    for (int j = 0; j < n; ++j)
+      if (col_on[j] && live[j] == 0 && ccount[j] != 0) {
+        ccount[j] = 0;
+        cq.push_back(j);
+      }
+    return !cq.empty();
+  };
+  while (!rq.empty() || !cq.empty() || recount_columns()) {
     while (!rq.empty()) {
       const int i = rq.back();
       rq.pop_back();
@@ -85,6 +99,10 @@ Presolved presolve(const LpHost& lp, const std::vector<double>& lc0, const std::
           }
         }
       if (j < 0 || a == 0.0) {  // only zero coefficients left: an empty row in effect
+        for (int k = lp.rowptr[i]; k < lp.rowptr[i + 1]; ++k) {
+          const int jj = lp.colidx[k];
+          if (col_on[jj] && --ccount[jj] == 0) cq.push_back(jj);
+        }
         rcount[i] = 0;
         rq.push_back(i);
         continue;

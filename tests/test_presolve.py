@@ -78,6 +78,17 @@ def test_presolve_solves_singletons_fixed_and_empty_rows_without_gpu():
     assert kkt_ok(lp, r)
 
 
+def test_rows_of_explicit_zeros_free_their_columns():
+    # row 0 holds only an explicit zero coefficient on x0 (an empty row in effect);
+    # x0, then in no row, must take the bound its cost prefers (5), not 0
+    A = sp.csr_matrix((np.array([0.0, 1.0]), np.array([0, 1]), np.array([0, 1, 2])), shape=(2, 2))
+    lp = lp_from(A, [1, 1], [-1, 2], [1, INF], [5, 0], [9, 10])
+    r = c_api.solve(lp, tol=1e-8, presolve=True)
+    assert r.error == 0 and r.status == 1
+    assert np.allclose(r.x, [5, 2]) and abs(r.primal_objective - 7.0) < 1e-12
+    assert kkt_ok(lp, r)
+
+
 def test_dual_postsolve_off_returns_no_duals():
     # pdlp_test.cu:903-930: with presolve and dual_postsolve false the solution has no
     # duals (the C getters then fill zeros); the primal solution is unchanged
