@@ -89,6 +89,27 @@ def test_rows_of_explicit_zeros_free_their_columns():
     assert kkt_ok(lp, r)
 
 
+def test_file_parameters_without_gpu(tmp_path):
+    # solve.cu:1281-1284 and 1327-1330 on a problem that presolve solves: the user
+    # problem is written as MPS before the solve and reads back as the same LP; the
+    # .sol file has the reference's layout (solution_writer.cu:16-40), without
+    # variable lines when the problem has no names
+    from cuopt_amd.mps import read_mps
+
+    lp = fully_reducible()
+    prob = c_api.Problem(lp)
+    st = c_api.make_settings(1e-8, presolve=True)
+    assert st.set("user_problem_file", str(tmp_path / "user.mps")) == 0
+    assert st.set("solution_file", str(tmp_path / "out.sol")) == 0
+    r = c_api.solve_problem(prob, st)
+    assert r.status == 1
+    back = read_mps(str(tmp_path / "user.mps"))
+    assert np.allclose(back.c, lp.c) and np.allclose(back.values, lp.values)
+    assert np.allclose(back.var_lb, lp.var_lb) and np.allclose(back.var_ub, lp.var_ub)
+    lines = (tmp_path / "out.sol").read_text().splitlines()
+    assert lines == ["# Status: Optimal", "# Objective value: -1"]
+
+
 def test_dual_postsolve_off_returns_no_duals():
     # pdlp_test.cu:903-930: with presolve and dual_postsolve false the solution has no
     # duals (the C getters then fill zeros); the primal solution is unchanged
