@@ -279,3 +279,50 @@ def test_presolve_random_reducible_lps_against_highs():
         assert kkt_ok(lp, r, tol=1e-9), seed
         solved += 1
     assert solved > 150 and infeasible > 10
+
+
+def _random_activity_case(seed):
+    """Small LPs with two- and three-variable rows and box bounds, about half of them
+    made infeasible by a row whose bound lies beyond its activity range."""
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(2, 7))
+    lb = rng.uniform(-5, 0, n)
+    ub = lb + rng.uniform(0.5, 5, n)
+    rows, lc, uc = [], [], []
+    for _ in range(int(rng.integers(1, 6))):
+        r = np.zeros(n)
+        cols = rng.choice(n, size=min(n, int(rng.integers(2, 4))), replace=False)
+        r[cols] = rng.choice([-1, 1], size=len(cols)) * rng.uniform(0.5, 2.0, size=len(cols))
+        amin = np.sum(np.where(r > 0, r * lb, r * ub))
+        amax = np.sum(np.where(r > 0, r * ub, r * lb))
+        if rng.random() < 0.5:  # reachable: a band inside [amin, amax]
+            t = rng.uniform(amin, amax)
+            lo, hi = t - rng.uniform(0, 1), t + rng.uniform(0, 1)
+        elif rng.random() < 0.5:  # beyond the top of the activity range
+            lo, hi = amax + rng.uniform(1e-3, 2), INF
+        else:  # below the bottom
+            lo, hi = -INF, amin - rng.uniform(1e-3, 2)
+        rows.append(r)
+        lc.append(lo)
+        uc.append(hi)
+    return lp_from(np.array(rows), rng.normal(size=n), lc, uc, lb, ub)
+
+
+def test_presolve_infeasibility_is_sound_against_highs():
+    # whenever presolve reports infeasibility (no GPU involved) HiGHS agrees
+    from scipy.optimize import linprog
+
+    detected = 0
+    for seed in range(300):
+        lp = _random_activity_case(seed)
+        r = c_api.solve(lp, tol=1e-9, presolve=True)
+        if r.status != 2:
+            continue
+        A = sp.csr_matrix((lp.values, lp.col_indices, lp.row_offsets), shape=(lp.m, lp.n)).toarray()
+        fin_u, fin_l = np.isfinite(lp.con_ub), np.isfinite(lp.con_lb)
+        A_ub = np.vstack([A[fin_u], -A[fin_l]])
+        b_ub = np.concatenate([lp.con_ub[fin_u], -lp.con_lb[fin_l]])
+        ref = linprog(lp.c, A_ub=A_ub, b_ub=b_ub, bounds=list(zip(lp.var_lb, lp.var_ub)), method="highs")
+        assert ref.status == 2, seed
+        detected += 1
+    assert detected > 100
