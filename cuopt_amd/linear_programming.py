@@ -588,7 +588,14 @@ def Solve(data_model, solver_settings=None):
         r = c_api._take_result(sol, shim)
     finally:
         L.cuOptDestroyProblem(ctypes.byref(p))
-    return _solution(data_model, r)
+    sol = _solution(data_model, r)
+    # presolve without dual postsolve gives no duals (pdlp_test.cu:903-930)
+    dual_post = settings.get_parameter("dual_postsolve")
+    if isinstance(dual_post, str):
+        dual_post = dual_post.strip().lower() in ("1", "true")
+    if int(settings.get_parameter("presolve")) != 0 and not dual_post:
+        sol.dual_solution, sol.reduced_cost = np.zeros(0), np.zeros(0)
+    return sol
 
 
 def BatchSolve(data_model_list, solver_settings=None):

@@ -449,3 +449,17 @@ def test_reference_solver_settings_sequence():
         LPM.Solve(LPM.ParseMps(os.path.join(GOLDEN, "mps", "good-mps-1.mps")), s)
     s.set_parameter("pdlp_solver_mode", LPM.PDLPSolverMode.Methodical1)
     assert s.get_parameter("pdlp_solver_mode") == int(LPM.PDLPSolverMode.Methodical1)
+
+
+@gpu
+def test_dual_postsolve_off_gives_an_empty_dual_solution():
+    # pdlp_test.cu:903-930: with presolve on and dual_postsolve off the solution has no duals
+    dm = LPM.ParseMps(os.path.join(GOLDEN, "afiro_original.mps"))
+    s = LPM.SolverSettings()
+    s.set_parameter("time_limit", 60.0)
+    sol = LPM.Solve(dm, s)
+    assert len(sol.get_dual_solution()) == 27
+    s.set_parameter("dual_postsolve", False)
+    sol = LPM.Solve(dm, s)
+    assert sol.get_termination_status() == LPM.LPTerminationStatus.Optimal
+    assert len(sol.get_dual_solution()) == 0 and len(sol.get_primal_solution()) == 32
