@@ -180,3 +180,55 @@ def test_empty_matrix():
     assert dm.get_objective_coefficients().tolist() == [-3.0]
     assert dm.get_variable_lower_bounds().tolist() == [-INF]
     assert dm.get_variable_upper_bounds().tolist() == [2.0]
+
+
+def _counts(L, h):
+    m, n = ctypes.c_int32(), ctypes.c_int32()
+    assert L.cuOptGetNumConstraints(h, ctypes.byref(m)) == c_api.CUOPT_SUCCESS
+    assert L.cuOptGetNumVariables(h, ctypes.byref(n)) == c_api.CUOPT_SUCCESS
+    return m.value, n.value
+
+
+def test_mutated_files_parse_or_fail_cleanly(tmp_path):
+    # Seeded mutations of the fixtures (a changed byte, a truncated file, a cleared
+    # or repeated line) either parse or give MPS_PARSE_ERROR; a file that parses
+    # writes out and reads back with the same dimensions.
+    rng = np.random.default_rng(7)
+    L = c_api.lib()
+    src_files = sorted(f for f in os.listdir(MPS) if f.endswith(".mps"))
+    path, out = tmp_path / "fuzz.mps", tmp_path / "out.mps"
+    parsed = 0
+    for name in src_files:
+        src = open(os.path.join(MPS, name), "rb").read()
+        for _ in range(25):
+            s = bytearray(src)
+            if not s:
+                break
+            pos = int(rng.integers(len(s)))
+            kind = int(rng.integers(4))
+            end = s.find(b"\n", pos)
+            if kind == 0:
+                s[pos] = b" \n$*-+.eE0123456789ABCXYZ"[int(rng.integers(25))]
+            elif kind == 1:
+                del s[pos:]
+            elif kind == 2 and end >= 0:
+                del s[pos:end]
+            elif end >= 0:
+                s[pos:pos] = s[pos:end + 1]
+            path.write_bytes(bytes(s))
+            h = ctypes.c_void_p()
+            rc = L.cuOptReadProblem(str(path).encode(), ctypes.byref(h))
+            assert rc in (c_api.CUOPT_SUCCESS, c_api.CUOPT_MPS_PARSE_ERROR), (name, bytes(s))
+            if rc != c_api.CUOPT_SUCCESS:
+                assert h.value is None
+                continue
+            parsed += 1
+            try:
+                assert L.cuOptWriteProblem(h, str(out).encode(), 0) == c_api.CUOPT_SUCCESS
+                h2 = ctypes.c_void_p()
+                assert L.cuOptReadProblem(str(out).encode(), ctypes.byref(h2)) == c_api.CUOPT_SUCCESS
+                assert _counts(L, h2) == _counts(L, h), (name, bytes(s))
+                L.cuOptDestroyProblem(ctypes.byref(h2))
+            finally:
+                L.cuOptDestroyProblem(ctypes.byref(h))
+    assert parsed > 100
