@@ -105,11 +105,13 @@ __global__ __launch_bounds__(kUB) void unr_primal(const int* __restrict__ tptr, 
 {
   int j, b;
   if (!row_lane(n, B, j, b)) return;
-  const auto aty = row_dot(tptr, tidx, tval, y, j, b, B);
   const size_t o = (size_t)j * B + b;
-  const V xr    = x[o];
-  const V v     = (xr - tau * aty.rsub(c[o])) * inv_prox;
-  const V xnew  = mx(mn(v, ub[j]), lb[j]);
+  // The lane's own operands are loaded before the row's gathers, so their
+  // latency overlaps the gathers'.
+  const V xr = x[o], cr = c[o], lo = lb[j], hi = ub[j];
+  const auto aty = row_dot(tptr, tidx, tval, y, j, b, B);
+  const V v     = (xr - tau * aty.rsub(cr)) * inv_prox;
+  const V xnew  = mx(mn(v, hi), lo);
   xn[o]         = xnew;
   xb[o]         = V(2) * xnew - xr;
 }
@@ -122,11 +124,12 @@ __global__ __launch_bounds__(kUB) void unr_dual(const int* __restrict__ aptr, co
 {
   int i, b;
   if (!row_lane(m, B, i, b)) return;
-  const auto ax  = row_dot(aptr, aidx, aval, xb, i, b, B);
   const size_t o = (size_t)i * B + b;
   const size_t q = ldb ? (size_t)i * ldb + b : (size_t)i;  // per-batch bounds, or one per row
-  const V t      = ax.rsub(y[o] / sigma);
-  const V proj   = mx(-uc[q], mn(t, -lc[q]));
+  const V yo = y[o], lo = lc[q], hi = uc[q];  // loaded before the gathers
+  const auto ax  = row_dot(aptr, aidx, aval, xb, i, b, B);
+  const V t      = ax.rsub(yo / sigma);
+  const V proj   = mx(-hi, mn(t, -lo));
   yn[o]          = (t - proj) * sigma;
 }
 
@@ -183,14 +186,16 @@ __global__ __launch_bounds__(kUB) void unr_adj_primal(const int* __restrict__ tp
 {
   int j, b;
   if (!row_lane(n, B, j, b)) return;
-  const V gxb    = -row_dot(tptr, tidx, tval, gt, j, b, B).value();
   const size_t o = (size_t)j * B + b;
-  const V xp     = xplus[o];
-  const bool in  = xp > lb[j] && xp < ub[j];
-  const V w      = in ? (ax[o] + V(2) * gxb) * inv_prox : V(0);
-  gw[o]          = w;
-  ax[o]          = w - gxb;
-  gc_acc[o] += (double)w;
+  // Element operands first (gt, which the row gathers, is none of them).
+  const V xp      = xplus[o], axo = ax[o];
+  const bool in   = xp > lb[j] && xp < ub[j];
+  const double gc = gc_acc[o];
+  const V gxb     = -row_dot(tptr, tidx, tval, gt, j, b, B).value();
+  const V w       = in ? (axo + V(2) * gxb) * inv_prox : V(0);
+  gw[o]           = w;
+  ax[o]           = w - gxb;
+  gc_acc[o]       = gc + (double)w;
 }
 
 // Rows i of A: ay_prev = gt / sigma + tau A gw; then, if there is an earlier
@@ -206,14 +211,17 @@ __global__ __launch_bounds__(kUB) void unr_adj_dual(const int* __restrict__ aptr
 {
   int i, b;
   if (!row_lane(m, B, i, b)) return;
-  const V agw    = row_dot(aptr, aidx, aval, gw, i, b, B).value();
   const size_t o = (size_t)i * B + b;
-  const V ay     = gt[o] / sigma + tau * agw;  // adjoint of y before this step
   const size_t q = ldb ? (size_t)i * ldb + b : (size_t)i;  // per-batch bounds, or one per row
+  // Element operands first (gw, which the row gathers, is none of them).
+  const V gto = gt[o], yp = yprev[o], lo = lc[q], hi = uc[q];
+  const double glo = glc[o], guo = guc[o];
+  const V agw    = row_dot(aptr, aidx, aval, gw, i, b, B).value();
+  const V ay     = gto / sigma + tau * agw;  // adjoint of y before this step
   double dl, du;
-  gt[o] = dual_adjoint(ay, yprev[o], lc[q], uc[q], sigma, dl, du);
-  glc[o] += dl;
-  guc[o] += du;
+  gt[o]  = dual_adjoint(ay, yp, lo, hi, sigma, dl, du);
+  glc[o] = glo + dl;
+  guc[o] = guo + du;
 }
 
 template <typename V>
