@@ -101,6 +101,27 @@ def test_spmv_fp32_compensated(case):
         assert abs(y[longest] - ref[longest]) * 8 < abs(float(naive) - ref[longest]) + EPS32 * abs(ref[longest])
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
+def test_spmv_full_size_skewed(precision):
+    # The skewed LP at its full size (m = 500 000, n = 1 000 000, 4.75 M nonzeros,
+    # rows of up to about 474 000 nonzeros split over many tiles), with the
+    # tolerances of the cases above.
+    lp = G.skewed_lp()
+    assert lp.m == 500_000 and lp.n == 1_000_000
+    x = np.random.default_rng(21).standard_normal(lp.n)
+    y = c_api.spmv(lp.row_offsets, lp.col_indices, lp.values, x, lp.n, precision)
+    if precision == "fp64":
+        ref = O.spmv(lp.row_offsets, lp.col_indices, lp.values, x, lp.n)
+        bound = 64 * EPS64 * abs_products(lp, x) + 1e-300
+    else:
+        v32 = lp.values.astype(np.float32).astype(np.float64)
+        x32 = x.astype(np.float32).astype(np.float64)
+        ref = O.spmv(lp.row_offsets, lp.col_indices, v32, x32, lp.n)
+        bound = EPS32 * np.abs(ref) + 64 * EPS32 * EPS32 * abs_products(lp, x32) + 1e-30
+    worst = int(np.argmax(np.abs(y - ref) - bound))
+    assert np.all(np.abs(y - ref) <= bound), (worst, y[worst], ref[worst], bound[worst])
+
+
 def test_setup_matches_oracle():
     lp = G.random_lp(m=400, n=1000, seed=3)
     r = c_api.solve(lp, tol=1e-4, iteration_limit=0, time_limit=TL)
