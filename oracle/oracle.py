@@ -74,12 +74,12 @@ def transpose(ptr, idx, val, ncols):
     return tp, ti, tv
 
 
-def setup(lp, mode: int = 0) -> dict:
+def setup(lp, mode: int = 0, L=None) -> dict:
     """Scaling vectors and initial step/weight (mode 0 = Stable3, 1 = Methodical1)."""
     a = _arrays(lp)
     Dr, Dc, out = np.zeros(lp.m), np.zeros(lp.n), np.zeros(8)
     sv = np.zeros(lp.nnz)
-    rc = lib().oracle_setup(ctypes.c_int(lp.m), ctypes.c_int(lp.n), *[_ptr(v) for v in a],
+    rc = _fn_lib(L).oracle_setup(ctypes.c_int(lp.m), ctypes.c_int(lp.n), *[_ptr(v) for v in a],
                             ctypes.c_int(int(lp.maximize)), ctypes.c_int(mode), _ptr(Dr), _ptr(Dc), _ptr(out), _ptr(sv))
     if rc != 0:
         raise ValueError("crossing bounds")
@@ -133,3 +133,38 @@ def solve(lp, tol: float = 1e-4, iteration_limit: int = 2**31 - 1, time_limit: f
     if trace_cap:
         d["trace"] = trace[: tlen.value].copy()
     return d
+
+
+def _fn_lib(L):
+    return lib() if L is None else L
+
+
+def pid_update(state: dict, primal_distance: float, dual_distance: float, rel_dual: float, rel_primal: float,
+               step_size: float, L=None) -> dict:
+    """One PID primal-weight update; ``state`` holds omega, best_omega, error_sum,
+    last_error, best_gap, tau, sigma.  Returns the new state."""
+    keys = ["omega", "best_omega", "error_sum", "last_error", "best_gap", "tau", "sigma"]
+    s = np.array([state[k] for k in keys], dtype=np.float64)
+    f = _fn_lib(L).oracle_pid_update
+    f.restype = None
+    f(ctypes.c_double(primal_distance), ctypes.c_double(dual_distance), ctypes.c_double(rel_dual),
+      ctypes.c_double(rel_primal), ctypes.c_double(step_size), _ptr(s))
+    return dict(zip(keys, s.tolist()))
+
+
+def restart_decision(k: int, k_r: int, fpe: float, fpe0: float, last_trial: float, L=None) -> bool:
+    """The Stable3 restart decision at a major check."""
+    f = _fn_lib(L).oracle_restart_decision
+    f.restype = ctypes.c_int
+    return bool(f(ctypes.c_longlong(k), ctypes.c_longlong(k_r), ctypes.c_double(fpe), ctypes.c_double(fpe0),
+                  ctypes.c_double(last_trial)))
+
+
+def fpe(ptr, idx, val, ncols: int, x, xbar, y, ybar, omega: float, eta: float, L=None) -> float:
+    """Fixed-point error of the step (x, y) -> (xbar, ybar) under the CSR matrix."""
+    ptr, idx, val = _p(ptr, np.int32), _p(idx, np.int32), _p(val, np.float64)
+    x, xbar, y, ybar = (_p(v, np.float64) for v in (x, xbar, y, ybar))
+    f = _fn_lib(L).oracle_fpe
+    f.restype = ctypes.c_double
+    return float(f(ctypes.c_int(len(ptr) - 1), ctypes.c_int(ncols), _ptr(ptr), _ptr(idx), _ptr(val), _ptr(x),
+                   _ptr(xbar), _ptr(y), _ptr(ybar), ctypes.c_double(omega), ctypes.c_double(eta)))

@@ -18,6 +18,13 @@
 //   * power iteration .................... sigma_max(A~)^2 against a dense SVD (numpy)
 //   * KKT / termination record ........... closed forms at a constructed optimum (x0,y0,s0):
 //                                          r_p = r_d = gap = 0; objective c.x0 == b.y0
+//   * PID primal weight, restart decision, fixed-point error, beta/gamma
+//                                        .. hand-worked values (tests/test_oracle_pins.py):
+//                                          a PID sequence through the fall-back branch, a
+//                                          decision table at every threshold boundary, the
+//                                          PDHG-metric norm from an explicit dense matrix, a
+//                                          diagonal LP with E/L/G/ranged/free rows; each check
+//                                          fails on an oracle build with a planted mistake
 //   * end-to-end solve ................... closed-form optima of the generated LPs,
 //                                          afiro -464.7531, good-max 17, max_offset 0,
 //                                          HiGHS (scipy) optima committed as numbers,
@@ -460,6 +467,47 @@ void pid_update(double primal_distance,
   }
 }
 
+// should_cupdlpx_restart (pdlp_restart_strategy.cu:688-755) for one LP, with the
+// artificial criterion of should_do_artificial_restart (:1397-1419).  k counts all
+// iterations, k_r those since the last restart; last_trial is the FPE of the
+// previous major check.
+bool restart_decision(int64_t k, int64_t k_r, double fpe, double fpe0, double last_trial,
+                      const Hyper& H)
+{
+  if (k == H.major_iteration) return true;  // forced at the first major
+  if (k < H.major_iteration) return false;
+  if (fpe <= H.sufficient_reduction * fpe0) return true;
+  if (fpe <= H.necessary_reduction * fpe0 && fpe > last_trial) return true;
+  return (double)k_r >= H.artificial_threshold * (double)k;
+}
+
+// compute_fixed_error (pdlp.cu:1748-1855): dx = x_bar - x, dy = y_bar - y, the
+// interaction (A~^T y_bar - A~^T y) . dx of compute_interaction_and_movement
+// (adaptive_step_size_strategy.cu:351-517), and fixed_error_computation
+// (pdlp.cu:1401-1420): sqrt(omega |dx|^2 + |dy|^2 / omega + 2 eta interaction).
+// aty holds A~^T y; tmp_n (n) receives A~^T y_bar.
+double fixed_point_error(const Csr& AT, const std::vector<double>& x, const std::vector<double>& xbar,
+                         const std::vector<double>& y, const std::vector<double>& ybar,
+                         const std::vector<double>& aty, double omega, double eta,
+                         std::vector<double>& tmp_n)
+{
+  spmv(AT, ybar.data(), tmp_n.data());
+  double inter = 0.0, nx = 0.0, ny = 0.0;
+  for (size_t j = 0; j < x.size(); ++j) {
+    const double dx = xbar[j] - x[j];
+    const double t  = tmp_n[j] - aty[j];
+    inter += t * dx;
+    nx += dx * dx;
+  }
+  for (size_t i = 0; i < y.size(); ++i) {
+    const double dy = ybar[i] - y[i];
+    ny += dy * dy;
+  }
+  const double movement = nx * omega + ny / omega;
+  const double ci       = 2.0 * inter * eta;
+  return std::sqrt(movement + ci);
+}
+
 // Parameter and info slots shared with oracle/oracle.py.
 enum ParamSlot {
   P_ABS_PRIMAL = 0, P_REL_PRIMAL, P_ABS_DUAL, P_REL_DUAL, P_ABS_GAP, P_REL_GAP,
@@ -663,6 +711,54 @@ void oracle_kkt(int m, int n, const int* rowptr, const int* colidx, const double
   out[7] = std::sqrt(cn);
 }
 
+// One PID primal-weight update (pid_update above) on an explicit state with the
+// Stable3 gains.  state (in/out, 7 doubles): omega, best_omega, error_sum,
+// last_error, best_residual_gap, tau, sigma.
+void oracle_pid_update(double primal_distance, double dual_distance, double rel_dual,
+                       double rel_primal, double step_size, double* state)
+{
+  const Hyper H = hyper_for_mode(0);
+  Pid pid;
+  pid.err_sum  = state[2];
+  pid.last_err = state[3];
+  pid.best_gap = state[4];
+  double omega = state[0], best = state[1], tau = state[5], sigma = state[6];
+  pid_update(primal_distance, dual_distance, rel_dual, rel_primal, step_size, H, pid, omega, best,
+             tau, sigma);
+  state[0] = omega;
+  state[1] = best;
+  state[2] = pid.err_sum;
+  state[3] = pid.last_err;
+  state[4] = pid.best_gap;
+  state[5] = tau;
+  state[6] = sigma;
+}
+
+// The Stable3 restart decision at a major check (restart_decision above): 1 = restart.
+int oracle_restart_decision(long long k, long long k_r, double fpe, double fpe0, double last_trial)
+{
+  return restart_decision(k, k_r, fpe, fpe0, last_trial, hyper_for_mode(0)) ? 1 : 0;
+}
+
+// Fixed-point error of the step (x, y) -> (x_bar, y_bar) under the CSR matrix A
+// (in a solve, the scaled matrix), primal weight omega and step size eta.
+double oracle_fpe(int m, int n, const int* rowptr, const int* colidx, const double* vals,
+                  const double* x, const double* xbar, const double* y, const double* ybar,
+                  double omega, double eta)
+{
+  Csr A;
+  A.nrows = m;
+  A.ncols = n;
+  A.ptr.assign(rowptr, rowptr + m + 1);
+  A.idx.assign(colidx, colidx + rowptr[m]);
+  A.val.assign(vals, vals + rowptr[m]);
+  const Csr AT = transpose(A);
+  std::vector<double> xv(x, x + n), xb(xbar, xbar + n), yv(y, y + m), yb(ybar, ybar + m);
+  std::vector<double> aty(n), tmp(n);
+  spmv(AT, yv.data(), aty.data());
+  return fixed_point_error(AT, xv, xb, yv, yb, aty, omega, eta, tmp);
+}
+
 // Full Stable3 solve (run_pdlp -> pdlp_solver_t::run_solver, pdlp.cu:2013-2458).
 // Returns the termination status.  x_out (n), y_out (m), rc_out (n) receive the
 // unscaled returned iterate (maximisation: y and reduced costs negated, solve.cu:583-588).
@@ -797,15 +893,7 @@ int oracle_solve(int m, int n, const int* rowptr, const int* colidx, const doubl
       scale_dual(S, bor, yhat);
       scale_slack(S, bor, slack);
       if (is_major) {
-        // should_cupdlpx_restart (pdlp_restart_strategy.cu:688-755)
-        bool do_restart = false;
-        if (k == H.major_iteration) {
-          do_restart = true;
-        } else if (k > H.major_iteration) {
-          if (fpe <= H.sufficient_reduction * fpe0) do_restart = true;
-          else if (fpe <= H.necessary_reduction * fpe0 && fpe > last_trial) do_restart = true;
-          else if ((double)k_r >= H.artificial_threshold * (double)k) do_restart = true;  // :1397-1419
-        }
+        const bool do_restart = restart_decision(k, k_r, fpe, fpe0, last_trial, H);
         last_trial = fpe;
         if (do_restart) {
           // cupdlpx_restart (pdlp_restart_strategy.cu:867-985)
@@ -859,22 +947,7 @@ int oracle_solve(int m, int n, const int* rowptr, const int* colidx, const doubl
       if (should_major) yhat[i] = nd;
     }
     if (((k + 1) % H.major_iteration) == 0 || restarted) {
-      // compute_fixed_error (pdlp.cu:1748-1855), interaction via A^T (adaptive_step_size_strategy.cu:351-517)
-      spmv(Q.AT, ybar.data(), tmp_n.data());
-      double inter = 0.0, nx = 0.0, ny = 0.0;
-      for (int j = 0; j < n; ++j) {
-        const double dx = xbar[j] - x[j];
-        const double t  = tmp_n[j] - aty[j];
-        inter += t * dx;
-        nx += dx * dx;
-      }
-      for (int i = 0; i < m; ++i) {
-        const double dy = ybar[i] - y[i];
-        ny += dy * dy;
-      }
-      const double movement = nx * omega + ny / omega;
-      const double ci       = 2.0 * inter * eta;
-      fpe                   = std::sqrt(movement + ci);  // fixed_error_computation (pdlp.cu:1401-1420)
+      fpe = fixed_point_error(Q.AT, x, xbar, y, ybar, aty, omega, eta, tmp_n);
       if (restarted) fpe0 = fpe;
     }
     // halpern_update (pdlp.cu:2505-2563) with reflection coefficient 1.
