@@ -79,6 +79,60 @@ def pmc_traffic(kernel: str):
     return entry.get("hbm_bytes_per_launch") if entry else None
 
 
+def highs_dual_simplex(lp, time_limit: float) -> dict:
+    """HiGHS dual simplex (SciPy linprog method 'highs-ds', one thread, FP64) on the same
+    host: the stand-in for the reference CPU dual simplex, which does not build here."""
+    import numpy as np
+    import scipy
+    from scipy.optimize import linprog
+
+    A = lp.to_scipy()
+    eq = lp.con_lb == lp.con_ub
+    up = np.isfinite(lp.con_ub) & ~eq
+    lo = np.isfinite(lp.con_lb) & ~eq
+    A_ub = scipy.sparse.vstack([A[up], -A[lo]]).tocsr()
+    b_ub = np.concatenate([lp.con_ub[up], -lp.con_lb[lo]])
+    t = time.time()
+    res = linprog(lp.c, A_ub=A_ub if A_ub.shape[0] else None, b_ub=b_ub if A_ub.shape[0] else None,
+                  A_eq=A[eq] if eq.any() else None, b_eq=lp.con_lb[eq] if eq.any() else None,
+                  bounds=np.stack([lp.var_lb, lp.var_ub], axis=1), method="highs-ds",
+                  options={"time_limit": float(time_limit)})
+    out = {"solver": f"HiGHS dual simplex (SciPy {scipy.__version__} linprog highs-ds), 1 thread",
+           "host_cpus": os.cpu_count(), "time_limit_s": time_limit, "time_s": round(time.time() - t, 3),
+           "status": int(res.status), "message": str(res.message)[:80]}
+    if res.status == 0:
+        out["objective"] = float(res.fun)
+    return out
+
+
+def config0_extras(c_api, G, time_limit: float, simplex_time_limit: float) -> dict:
+    """Plain FP64 PDLP on the small random LP (m = 2 000, n = 5 000, 8 nonzeros per column,
+    0 <= x <= 10, equality rows, built-in optimum, seed 0), to 1e-4 and 1e-8, beside the
+    HiGHS dual simplex on the same host.  FP32-mixed is timed too."""
+    lp = G.random_lp(m=2000, n=5000, seed=0)
+    obj = lp.obj_opt
+    out = {"workload": "random LP m=2000 n=5000 nnz=%d, 8 nnz/col, 0<=x<=10, Ax=b, seed 0" % lp.nnz,
+           "closed_form_objective": obj}
+    for tol in (1e-4, 1e-8):
+        for p in ("fp64", "fp32_mixed"):
+            r = c_api.solve(lp, tol=tol, precision=p, time_limit=time_limit)
+            out[f"{p}@{tol:g}"] = {"status": c_api.STATUS.get(r.status, r.status), "iterations": r.iterations,
+                                   "time_s": round(r.solve_time, 4), "setup_s": round(r.info["setup_time"], 4),
+                                   "iters_per_s": round(r.iterations / max(r.info["loop_time"], 1e-12), 1),
+                                   "obj_rel_err": abs(r.primal_objective - obj) / (1.0 + abs(obj))}
+            print(f"[bench] config0 {p} tol {tol:g}: {out[f'{p}@{tol:g}']}", file=sys.stderr, flush=True)
+    if simplex_time_limit > 0:
+        try:
+            cs = highs_dual_simplex(lp, simplex_time_limit)
+            if "objective" in cs:
+                cs["obj_rel_err"] = abs(cs["objective"] - obj) / (1.0 + abs(obj))
+            out["cpu_simplex"] = cs
+            print(f"[bench] config0 cpu simplex: {cs}", file=sys.stderr, flush=True)
+        except Exception as e:  # scipy absent on the host
+            out["cpu_simplex"] = {"error": repr(e)}
+    return out
+
+
 def transport_extras(c_api, G, time_limit: float, cpu_simplex: bool) -> dict:
     """Transportation LP (20 000 sources x 50 000 sinks, 50 arcs each: n = 1 M, m = 70 k):
     FP32-mixed with FP64 refinement against plain FP64, to 1e-4 and 1e-8, objective error
@@ -198,7 +252,9 @@ def main() -> int:
     ap.add_argument("--no-cpu-simplex", action="store_true")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--batch-sequential-sample", type=int, default=64)
-    ap.add_argument("--only", choices=("transport", "batch", "unrolled"),
+    ap.add_argument("--simplex-time-limit", type=float, default=30.0,
+                    help="time limit (s) of the HiGHS dual simplex beside the small random LP (0: skip)")
+    ap.add_argument("--only", choices=("config0", "transport", "batch", "unrolled"),
                     help="run just this extra workload and print its result (for profiling)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
@@ -216,7 +272,9 @@ def main() -> int:
         return 1
     if args.copy_calibration_mb:
         copy_calibration(args.copy_calibration_mb)
-    extra_fns = {"transport": lambda: transport_extras(c_api, G, 120.0, not args.no_cpu_simplex),
+    extra_fns = {"config0": lambda: config0_extras(c_api, G, 120.0,
+                                                   0.0 if args.no_cpu_simplex else args.simplex_time_limit),
+                 "transport": lambda: transport_extras(c_api, G, 120.0, not args.no_cpu_simplex),
                  "batch": lambda: batch_extras(c_api, G, args.batch_count, args.batch_sequential_sample, 120.0),
                  "unrolled": lambda: unrolled_extras(G, 2000, 64)}
     if args.only:

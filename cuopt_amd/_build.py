@@ -1,4 +1,5 @@
-"""Compiles libcuopt_amd.so (HIP for gfx950 + C++ host code) in-tree with hipcc."""
+"""Compiles libcuopt_amd.so (HIP for gfx950 + C++ host code) in-tree with hipcc, and the
+plain C caller of its C ABI that the tests run (tests/c_driver)."""
 
 import os
 import subprocess
@@ -56,3 +57,22 @@ def build_library(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(link)}\n{r.stdout.decode(errors='replace')}")
     return LIB
+
+
+REPO = os.path.dirname(HERE)
+C_DRIVER_SRC = os.path.join(REPO, "tests", "c_driver", "c_api_driver.c")
+C_DRIVER = os.path.join(REPO, "tests", "c_driver", "build", "c_api_driver")
+
+
+def c_driver_command(src: str, out: str) -> list:
+    """cc line of a C99 program against include/cuopt_c.h, linked to libcuopt_amd.so."""
+    return ["gcc", "-std=c99", "-O1", "-Wall", "-Wextra", "-Werror", "-I" + os.path.join(REPO, "include"), src,
+            "-o", out, "-L" + HERE, "-lcuopt_amd", "-Wl,-rpath," + HERE, "-lm"]
+
+
+def build_c_driver() -> str:
+    os.makedirs(os.path.dirname(C_DRIVER), exist_ok=True)
+    r = subprocess.run(c_driver_command(C_DRIVER_SRC, C_DRIVER), stdout=subprocess.PIPE, stderr=subprocess.STDOUT)
+    if r.returncode != 0:
+        raise RuntimeError("C driver build failed:\n" + r.stdout.decode(errors="replace"))
+    return C_DRIVER

@@ -10,6 +10,7 @@
 // dual-simplex / barrier methods, are rejected with CUOPT_INVALID_ARGUMENT;
 // every PDLP solver mode runs the Stable3 method; every presolve value but Off
 // runs this engine's own presolve (presolve.hpp).
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <exception>
@@ -604,16 +605,19 @@ cuopt_int_t cuOptSolve(cuOptOptimizationProblem problem, cuOptSolverSettings set
   } else {
     try {
       R = solve_lp(lp, conf);
+      // a problem settled before the PDHG loop (presolve, closed form) took no steps
+      const double steps = R.info[CUOPT_AMD_INFO_STEPS];
+      const int iters    = std::isfinite(steps) ? (int)steps : 0;
       if (conf.log_to_console) {
         std::printf("Status: %s   Objective: %.8e  Iterations: %d  Time: %.3fs\n", status_string(R.status),
-                    R.primal_objective, (int)R.info[CUOPT_AMD_INFO_STEPS], R.solve_time);
+                    R.primal_objective, iters, R.solve_time);
         std::fflush(stdout);
       }
       // log_file: the same line, appended to the file (solve.cu:1136, logger.cpp:149-172)
       if (!conf.log_file.empty()) {
         if (FILE* f = std::fopen(conf.log_file.c_str(), "a")) {
           std::fprintf(f, "Status: %s   Objective: %.8e  Iterations: %d  Time: %.3fs\n", status_string(R.status),
-                       R.primal_objective, (int)R.info[CUOPT_AMD_INFO_STEPS], R.solve_time);
+                       R.primal_objective, iters, R.solve_time);
           std::fclose(f);
         }
       }

@@ -49,6 +49,7 @@ struct PdlpSettings {
   bool infeasibility_detection = false;
   bool strict_infeasibility    = false;
   bool per_constraint_residual = false;
+  int per_constraint_absolute  = 0;  // 1: |r_i| - rel |w_i| in place of the signed form
   bool save_best_primal_so_far = false;
   bool first_primal_feasible   = false;
   bool log_to_console          = true;

@@ -309,8 +309,11 @@ LpHost read_mps(const std::string& path)
   lp.sense = row_type;
   lp.rhs.assign(lp.m, 0.0);
   for (const auto& kv : rhs) lp.rhs[kv.first] = kv.second;
-  lp.ranged = !range.empty();
-  if (lp.ranged) {
+  // Both row forms, as the reference parser's data model carries them
+  // (mps_parser.cpp:348-412): sense / rhs and the [lower, upper] pairs with RANGES
+  // applied.  The pairs are the ones the solver reads.
+  lp.ranged = true;
+  {
     lp.lc.resize(lp.m);
     lp.uc.resize(lp.m);
     for (int i = 0; i < lp.m; ++i) {

@@ -61,15 +61,17 @@ __global__ __launch_bounds__(1024) void reduce_partials_kernel(const double* __r
   }
 }
 
-// per_constraint_residual (convergence_information.cu:390-416, 453-470, utils.cuh:578-590):
-// the largest |r_i| - rel * w_i of a residual vector, with w_i the larger finite
-// bound magnitude of row i (cb given, combine_finite_abs_bounds) or |c_i|, and
-// 0 as the neutral element.  One workgroup; the order of the max is fixed.
+// per_constraint_residual (convergence_information.cu:390-416, 453-470, utils.cuh:576-590):
+// the largest r_i - rel * w_i of a signed residual vector, with w_i the larger
+// finite bound magnitude of row i (cb given, combine_finite_abs_bounds) or the
+// signed c_i, and 0 as the neutral element.  absolute != 0 takes |r_i| and |c_i|
+// instead (an extension).  One workgroup; the order of the max is fixed.
 __global__ __launch_bounds__(1024) void linf_rel_kernel(const double* __restrict__ r,
                                                         const double2* __restrict__ cb,
                                                         const double* __restrict__ c,
                                                         int n,
                                                         double rel,
+                                                        int absolute,
                                                         double* __restrict__ out)
 {
   __shared__ double lds[1024];
@@ -81,9 +83,10 @@ __global__ __launch_bounds__(1024) void linf_rel_kernel(const double* __restrict
       if (isfinite(b.y)) w = fmax(w, fabs(b.y));
       if (isfinite(b.x)) w = fmax(w, fabs(b.x));
     } else {
-      w = fabs(c[i]);
+      w = absolute ? fabs(c[i]) : c[i];
     }
-    v = fmax(v, fabs(r[i]) - rel * w);
+    const double ri = absolute ? fabs(r[i]) : r[i];
+    v = fmax(v, ri - rel * w);
   }
   lds[threadIdx.x] = v;
   __syncthreads();
@@ -1237,11 +1240,12 @@ void PdlpSolver<V>::evaluate_kkt()
                stream_);
   launch_reduce(red_.get(), AT_.plan.red_blocks(), 4, scal_.get() + S_KKT_RD2, stream_);
   if (per) {
+    const int ab = st_.per_constraint_absolute;
     hipLaunchKernelGGL(linf_rel_kernel, dim3(1), dim3(1024), 0, stream_, rp_res_.get(), cb_orig_.get(), nullptr, m_,
-                       st_.rel_primal, scal_.get() + S_LINF_P);
+                       st_.rel_primal, ab, scal_.get() + S_LINF_P);
     CUOPT_HIP_CHECK(hipGetLastError());
     hipLaunchKernelGGL(linf_rel_kernel, dim3(1), dim3(1024), 0, stream_, rd_res_.get(), nullptr, c_orig_.get(), n_,
-                       st_.rel_dual, scal_.get() + S_LINF_D);
+                       st_.rel_dual, ab, scal_.get() + S_LINF_D);
     CUOPT_HIP_CHECK(hipGetLastError());
   }
   read_scalars();

@@ -71,22 +71,22 @@ SolverSettingsRegistry::SolverSettingsRegistry()
   // Float parameters (solver_settings.cu:61-79).
   floats_ = {
     {CUOPT_TIME_LIMIT, &p.time_limit, 0.0, inf, inf},
-    {"work_limit", sf(inf), 0.0, inf, inf},
+    {CUOPT_WORK_LIMIT, sf(inf), 0.0, inf, inf},
     {CUOPT_ABSOLUTE_DUAL_TOLERANCE, &p.abs_dual, 0.0, 1e-1, 1e-4},
     {CUOPT_RELATIVE_DUAL_TOLERANCE, &p.rel_dual, 0.0, 1e-1, 1e-4},
     {CUOPT_ABSOLUTE_PRIMAL_TOLERANCE, &p.abs_primal, 0.0, 1e-1, 1e-4},
     {CUOPT_RELATIVE_PRIMAL_TOLERANCE, &p.rel_primal, 0.0, 1e-1, 1e-4},
     {CUOPT_ABSOLUTE_GAP_TOLERANCE, &p.abs_gap, 0.0, 1e-1, 1e-4},
     {CUOPT_RELATIVE_GAP_TOLERANCE, &p.rel_gap, 0.0, 1e-1, 1e-4},
-    {"mip_absolute_tolerance", sf(1e-4), 0.0, 1e-1, 1e-4},
-    {"mip_relative_tolerance", sf(1e-4), 0.0, 1e-1, 1e-4},
-    {"mip_integrality_tolerance", sf(1e-5), 0.0, 1e-1, 1e-5},
-    {"mip_absolute_gap", sf(1e-10), 0.0, inf, 1e-10},
-    {"mip_relative_gap", sf(1e-4), 0.0, 1e-1, 1e-4},
+    {CUOPT_MIP_ABSOLUTE_TOLERANCE, sf(1e-4), 0.0, 1e-1, 1e-4},
+    {CUOPT_MIP_RELATIVE_TOLERANCE, sf(1e-4), 0.0, 1e-1, 1e-4},
+    {CUOPT_MIP_INTEGRALITY_TOLERANCE, sf(1e-5), 0.0, 1e-1, 1e-5},
+    {CUOPT_MIP_ABSOLUTE_GAP, sf(1e-10), 0.0, inf, 1e-10},
+    {CUOPT_MIP_RELATIVE_GAP, sf(1e-4), 0.0, 1e-1, 1e-4},
     {CUOPT_PRIMAL_INFEASIBLE_TOLERANCE, &p.primal_infeasible_tol, 0.0, 1e-1, 1e-10},
     {CUOPT_DUAL_INFEASIBLE_TOLERANCE, &p.dual_infeasible_tol, 0.0, 1e-1, 1e-10},
-    {"mip_cut_change_threshold", sf(1e-3), 0.0, inf, 1e-3},
-    {"mip_cut_min_orthogonality", sf(0.5), 0.0, 1.0, 0.5},
+    {CUOPT_MIP_CUT_CHANGE_THRESHOLD, sf(1e-3), 0.0, inf, 1e-3},
+    {CUOPT_MIP_CUT_MIN_ORTHOGONALITY, sf(0.5), 0.0, 1.0, 0.5},
   };
   // Integer parameters (solver_settings.cu:81-107), plus the precision switch.
   ints_ = {
@@ -95,26 +95,27 @@ SolverSettingsRegistry::SolverSettingsRegistry()
      CUOPT_PDLP_SOLVER_MODE_STABLE3},
     {CUOPT_METHOD, &p.method, CUOPT_METHOD_CONCURRENT, CUOPT_METHOD_BARRIER, CUOPT_METHOD_CONCURRENT},
     {CUOPT_NUM_CPU_THREADS, &p.num_cpu_threads, -1, imax, -1},
-    {"augmented", si(-1), -1, 1, -1},
-    {"folding", si(-1), -1, 1, -1},
-    {"dualize", si(-1), -1, 1, -1},
-    {"ordering", si(-1), -1, 1, -1},
-    {"barrier_dual_initial_point", si(-1), -1, 1, -1},
-    {"mip_cut_passes", si(10), -1, imax, 10},
-    {"mip_mixed_integer_rounding_cuts", si(-1), -1, 1, -1},
-    {"mip_mixed_integer_gomory_cuts", si(-1), -1, 1, -1},
-    {"mip_knapsack_cuts", si(-1), -1, 1, -1},
-    {"mip_strong_chvatal_gomory_cuts", si(-1), -1, 1, -1},
-    {"mip_reduced_cost_strengthening", si(-1), -1, imax, -1},
+    {CUOPT_AUGMENTED, si(-1), -1, 1, -1},
+    {CUOPT_FOLDING, si(-1), -1, 1, -1},
+    {CUOPT_DUALIZE, si(-1), -1, 1, -1},
+    {CUOPT_ORDERING, si(-1), -1, 1, -1},
+    {CUOPT_BARRIER_DUAL_INITIAL_POINT, si(-1), -1, 1, -1},
+    {CUOPT_MIP_CUT_PASSES, si(10), -1, imax, 10},
+    {CUOPT_MIP_MIXED_INTEGER_ROUNDING_CUTS, si(-1), -1, 1, -1},
+    {CUOPT_MIP_MIXED_INTEGER_GOMORY_CUTS, si(-1), -1, 1, -1},
+    {CUOPT_MIP_KNAPSACK_CUTS, si(-1), -1, 1, -1},
+    {CUOPT_MIP_STRONG_CHVATAL_GOMORY_CUTS, si(-1), -1, 1, -1},
+    {CUOPT_MIP_REDUCED_COST_STRENGTHENING, si(-1), -1, imax, -1},
     {CUOPT_NUM_GPUS, &p.num_gpus, 1, 2, 1},
-    {"mip_batch_pdlp_strong_branching", si(0), 0, 1, 0},
+    {CUOPT_MIP_BATCH_PDLP_STRONG_BRANCHING, si(0), 0, 1, 0},
     {CUOPT_PRESOLVE, &p.presolve, CUOPT_PRESOLVE_DEFAULT, CUOPT_PRESOLVE_PSLP, CUOPT_PRESOLVE_DEFAULT},
-    {"mip_determinism_mode", si(0), 0, 1, 0},
+    {CUOPT_MIP_DETERMINISM_MODE, si(0), 0, 1, 0},
     {CUOPT_RANDOM_SEED, &p.random_seed, -1, imax, -1},
-    {"mip_reliability_branching", si(-1), -1, imax, -1},
+    {CUOPT_MIP_RELIABILITY_BRANCHING, si(-1), -1, imax, -1},
     {CUOPT_AMD_PDLP_PRECISION, &p.precision, CUOPT_AMD_PRECISION_FP64, CUOPT_AMD_PRECISION_FP32_MIXED,
      CUOPT_AMD_PRECISION_FP64},
     {CUOPT_AMD_PDLP_REFINEMENT, &p.refinement, 0, 1, 1},
+    {CUOPT_AMD_PER_CONSTRAINT_ABSOLUTE, &p.per_constraint_absolute, 0, 1, 0},
   };
   // Bool parameters (solver_settings.cu:109-123).
   bools_ = {
@@ -123,12 +124,12 @@ SolverSettingsRegistry::SolverSettingsRegistry()
     {CUOPT_PER_CONSTRAINT_RESIDUAL, &p.per_constraint_residual, false},
     {CUOPT_SAVE_BEST_PRIMAL_SO_FAR, &p.save_best_primal_so_far, false},
     {CUOPT_FIRST_PRIMAL_FEASIBLE, &p.first_primal_feasible, false},
-    {"mip_scaling", sb(true), true},
-    {"mip_heuristics_only", sb(false), false},
+    {CUOPT_MIP_SCALING, sb(true), true},
+    {CUOPT_MIP_HEURISTICS_ONLY, sb(false), false},
     {CUOPT_LOG_TO_CONSOLE, &p.log_to_console, true},
     {CUOPT_CROSSOVER, &p.crossover, false},
-    {"eliminate_dense_columns", sb(true), true},
-    {"cudss_deterministic", sb(false), false},
+    {CUOPT_ELIMINATE_DENSE_COLUMNS, sb(true), true},
+    {CUOPT_CUDSS_DETERMINISTIC, sb(false), false},
     {CUOPT_DUAL_POSTSOLVE, &p.dual_postsolve, true},
   };
   // String parameters (solver_settings.cu:125-132).

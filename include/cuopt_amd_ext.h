@@ -34,6 +34,15 @@ extern "C" {
  * FP64 solves. */
 #define CUOPT_AMD_PDLP_REFINEMENT "pdlp_refinement"
 
+/* Integer parameter (0/1, default 0) changing the per_constraint_residual test
+ * (no reference counterpart).  0 keeps the reference's signed form
+ * (relative_residual_t, utils.cuh:576-590): max(0, max_i r_i - rel * w_i) with the
+ * signed residual r_i and w_i the row's larger finite bound magnitude, and
+ * max(0, max_j r_j - rel * c_j) with the signed dual residual and signed c_j, so a
+ * residual below the bound side passes.  1 takes |r_i| - rel * w_i and
+ * |r_j| - rel * |c_j|, which bounds every residual in magnitude. */
+#define CUOPT_AMD_PER_CONSTRAINT_ABSOLUTE "pdlp_per_constraint_absolute"
+
 /* Slots of cuOptAmdGetTerminationInfo.  Slots 0-9 mirror
  * additional_termination_information_t (pdlp/solver_solution.hpp:53-93). */
 #define CUOPT_AMD_INFO_STEPS               0  /* number_of_steps_taken */

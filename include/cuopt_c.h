@@ -27,7 +27,15 @@ extern "C" {
 #include <math.h>
 #endif
 
-/* ---- constants (constants.h:22-143) ------------------------------------- */
+/* ---- constants (constants.h:17-143) ------------------------------------- */
+#define CUOPT_INSTANTIATE_FLOAT  0
+#define CUOPT_INSTANTIATE_DOUBLE 1
+#define CUOPT_INSTANTIATE_INT32  1
+#define CUOPT_INSTANTIATE_INT64  0
+
+/* Parameter names (constants.h:22-76).  The PDLP engine acts on the LP ones;
+ * the MIP, barrier and work-limit names are accepted with the reference's
+ * ranges and defaults and have no effect on an LP solve. */
 #define CUOPT_ABSOLUTE_DUAL_TOLERANCE     "absolute_dual_tolerance"
 #define CUOPT_RELATIVE_DUAL_TOLERANCE     "relative_dual_tolerance"
 #define CUOPT_ABSOLUTE_PRIMAL_TOLERANCE   "absolute_primal_tolerance"
@@ -40,6 +48,7 @@ extern "C" {
 #define CUOPT_DUAL_INFEASIBLE_TOLERANCE   "dual_infeasible_tolerance"
 #define CUOPT_ITERATION_LIMIT             "iteration_limit"
 #define CUOPT_TIME_LIMIT                  "time_limit"
+#define CUOPT_WORK_LIMIT                  "work_limit"
 #define CUOPT_PDLP_SOLVER_MODE            "pdlp_solver_mode"
 #define CUOPT_METHOD                      "method"
 #define CUOPT_PER_CONSTRAINT_RESIDUAL     "per_constraint_residual"
@@ -48,13 +57,43 @@ extern "C" {
 #define CUOPT_LOG_FILE                    "log_file"
 #define CUOPT_LOG_TO_CONSOLE              "log_to_console"
 #define CUOPT_CROSSOVER                   "crossover"
+#define CUOPT_FOLDING                     "folding"
+#define CUOPT_AUGMENTED                   "augmented"
+#define CUOPT_DUALIZE                     "dualize"
+#define CUOPT_ORDERING                    "ordering"
+#define CUOPT_BARRIER_DUAL_INITIAL_POINT  "barrier_dual_initial_point"
+#define CUOPT_ELIMINATE_DENSE_COLUMNS     "eliminate_dense_columns"
+#define CUOPT_CUDSS_DETERMINISTIC         "cudss_deterministic"
 #define CUOPT_PRESOLVE                    "presolve"
 #define CUOPT_DUAL_POSTSOLVE              "dual_postsolve"
+#define CUOPT_MIP_DETERMINISM_MODE            "mip_determinism_mode"
+#define CUOPT_MIP_ABSOLUTE_TOLERANCE          "mip_absolute_tolerance"
+#define CUOPT_MIP_RELATIVE_TOLERANCE          "mip_relative_tolerance"
+#define CUOPT_MIP_INTEGRALITY_TOLERANCE       "mip_integrality_tolerance"
+#define CUOPT_MIP_ABSOLUTE_GAP                "mip_absolute_gap"
+#define CUOPT_MIP_RELATIVE_GAP                "mip_relative_gap"
+#define CUOPT_MIP_HEURISTICS_ONLY             "mip_heuristics_only"
+#define CUOPT_MIP_SCALING                     "mip_scaling"
+#define CUOPT_MIP_PRESOLVE                    "mip_presolve"
+#define CUOPT_MIP_RELIABILITY_BRANCHING       "mip_reliability_branching"
+#define CUOPT_MIP_CUT_PASSES                  "mip_cut_passes"
+#define CUOPT_MIP_MIXED_INTEGER_ROUNDING_CUTS "mip_mixed_integer_rounding_cuts"
+#define CUOPT_MIP_MIXED_INTEGER_GOMORY_CUTS   "mip_mixed_integer_gomory_cuts"
+#define CUOPT_MIP_KNAPSACK_CUTS               "mip_knapsack_cuts"
+#define CUOPT_MIP_STRONG_CHVATAL_GOMORY_CUTS  "mip_strong_chvatal_gomory_cuts"
+#define CUOPT_MIP_REDUCED_COST_STRENGTHENING  "mip_reduced_cost_strengthening"
+#define CUOPT_MIP_CUT_CHANGE_THRESHOLD        "mip_cut_change_threshold"
+#define CUOPT_MIP_CUT_MIN_ORTHOGONALITY       "mip_cut_min_orthogonality"
+#define CUOPT_MIP_BATCH_PDLP_STRONG_BRANCHING "mip_batch_pdlp_strong_branching"
 #define CUOPT_SOLUTION_FILE               "solution_file"
 #define CUOPT_NUM_CPU_THREADS             "num_cpu_threads"
 #define CUOPT_NUM_GPUS                    "num_gpus"
 #define CUOPT_USER_PROBLEM_FILE           "user_problem_file"
 #define CUOPT_RANDOM_SEED                 "random_seed"
+
+/* MIP determinism mode values (constants.h:78-80) */
+#define CUOPT_MODE_OPPORTUNISTIC 0
+#define CUOPT_MODE_DETERMINISTIC 1
 
 #define CUOPT_TERIMINATION_STATUS_NO_TERMINATION   0
 #define CUOPT_TERIMINATION_STATUS_OPTIMAL          1

@@ -19,7 +19,7 @@ _lib = None
 
 # Slots shared with pdlp_oracle.cpp.
 P_ABS_PRIMAL, P_REL_PRIMAL, P_ABS_DUAL, P_REL_DUAL, P_ABS_GAP, P_REL_GAP = range(6)
-P_ITER_LIMIT, P_TIME_LIMIT, P_THREADS, P_MODE, P_NPARAMS = 6, 7, 8, 9, 10
+P_ITER_LIMIT, P_TIME_LIMIT, P_THREADS, P_MODE, P_PER_CONSTRAINT, P_NPARAMS = 6, 7, 8, 9, 10, 11
 INFO_NAMES = [
     "status", "iterations", "l2_primal_residual", "l2_relative_primal_residual",
     "l2_dual_residual", "l2_relative_dual_residual", "primal_objective", "dual_objective",
@@ -89,15 +89,18 @@ def setup(lp, mode: int = 0, L=None) -> dict:
     return d
 
 
-def kkt(lp, x, y, slack) -> dict:
+def kkt(lp, x, y, slack, rel_primal: float = 0.0, rel_dual: float = 0.0, L=None) -> dict:
+    """KKT record of (x, y, slack) on the unscaled problem; linf_primal / linf_dual are
+    the per_constraint_residual maxima for the given relative tolerances."""
     ptr, idx, val = _p(lp.row_offsets, np.int32), _p(lp.col_indices, np.int32), _p(lp.values, np.float64)
     c, lc, uc = _p(lp.c, np.float64), _p(lp.con_lb, np.float64), _p(lp.con_ub, np.float64)
     x, y, slack = _p(x, np.float64), _p(y, np.float64), _p(slack, np.float64)
-    out = np.zeros(8)
-    lib().oracle_kkt(ctypes.c_int(lp.m), ctypes.c_int(lp.n), _ptr(ptr), _ptr(idx), _ptr(val), _ptr(c), _ptr(lc),
+    out = np.zeros(10)
+    _fn_lib(L).oracle_kkt(ctypes.c_int(lp.m), ctypes.c_int(lp.n), _ptr(ptr), _ptr(idx), _ptr(val), _ptr(c), _ptr(lc),
                      _ptr(uc), ctypes.c_int(int(lp.maximize)), ctypes.c_double(lp.offset), _ptr(x), _ptr(y),
-                     _ptr(slack), _ptr(out))
-    return dict(zip(["l2_primal", "l2_dual", "pobj", "dobj", "gap", "absobj", "b_norm", "c_norm"], out.tolist()))
+                     _ptr(slack), ctypes.c_double(rel_primal), ctypes.c_double(rel_dual), _ptr(out))
+    return dict(zip(["l2_primal", "l2_dual", "pobj", "dobj", "gap", "absobj", "b_norm", "c_norm", "linf_primal",
+                     "linf_dual"], out.tolist()))
 
 
 def _arrays(lp):
@@ -109,11 +112,15 @@ def _arrays(lp):
 
 
 def solve(lp, tol: float = 1e-4, iteration_limit: int = 2**31 - 1, time_limit: float = float("inf"),
-          threads: int = 1, x_init=None, y_init=None, trace_cap: int = 0) -> dict:
-    """Stable3 PDLP solve; returns dict with x, y, reduced_cost, info fields and optional trace."""
+          threads: int = 1, x_init=None, y_init=None, trace_cap: int = 0, per_constraint_residual: bool = False,
+          tolerances=None) -> dict:
+    """Stable3 PDLP solve; returns dict with x, y, reduced_cost, info fields and optional trace.
+    ``tolerances`` (optional): abs_primal, rel_primal, abs_dual, rel_dual, abs_gap, rel_gap in
+    place of ``tol`` for all six."""
     a = _arrays(lp)
     params = np.zeros(P_NPARAMS)
-    params[:6] = tol
+    params[:6] = tol if tolerances is None else tolerances
+    params[P_PER_CONSTRAINT] = 1.0 if per_constraint_residual else 0.0
     params[P_ITER_LIMIT] = iteration_limit
     params[P_TIME_LIMIT] = time_limit
     params[P_THREADS] = threads

@@ -360,38 +360,61 @@ double power_iteration(const Problem& Q, int* iters_out)
 // One evaluation of convergence_information_t::compute_convergence_information
 // in reflected mode (convergence_information.cu:326-483, 546-559, 574-606,
 // 649-690, 700-810) on the unscaled problem for (x, y, slack).
+// linf_primal / linf_dual: the per_constraint_residual maxima (see below).
 struct Kkt {
   double l2_primal = 0, l2_dual = 0, pobj = 0, dobj = 0, gap = 0, absobj = 0;
+  double linf_primal = 0, linf_dual = 0;
 };
 
+// combine_finite_abs_bounds (cpp/src/pdlp/utils.cuh:180-188): the larger finite
+// bound magnitude of a row, 0 when both bounds are infinite.
+inline double combined_abs_bound(double lo, double up)
+{
+  double v = 0.0;
+  if (std::isfinite(up)) v = std::max(v, std::fabs(up));
+  if (std::isfinite(lo)) v = std::max(v, std::fabs(lo));
+  return v;
+}
+
+// rel_primal / rel_dual: the relative primal / dual tolerances, used for the
+// per_constraint_residual maxima max(0, max_i (r_i - rel * w_i)) of
+// convergence_information.cu:390-416 (primal: w = combined_abs_bound) and :453-470
+// (dual: w = c_j), with the signed residuals and signed c_j of relative_residual_t
+// (utils.cuh:576-590) and 0 as the neutral element of the maximum.
 Kkt convergence_information(const Problem& P,
                             const std::vector<double>& x,
                             const std::vector<double>& y,
                             const std::vector<double>& slack,
                             std::vector<double>& tmp_m,
-                            std::vector<double>& tmp_n)
+                            std::vector<double>& tmp_n,
+                            double rel_primal = 0.0,
+                            double rel_dual   = 0.0)
 {
   Kkt K;
   spmv(P.A, x.data(), tmp_m.data());
-  double rp2 = 0.0, psum = 0.0;
+  double rp2 = 0.0, psum = 0.0, linf_p = 0.0;
   for (int i = 0; i < P.m; ++i) {
     const double Ax  = tmp_m[i];
     const double cl  = std::max(P.lc[i], std::min(Ax, P.uc[i]));
     const double r   = Ax - cl;
     rp2 += r * r;
+    linf_p = std::max(linf_p, r - rel_primal * combined_abs_bound(P.lc[i], P.uc[i]));
     psum += std::max(y[i], 0.0) * fin0(P.lc[i]) + std::min(y[i], 0.0) * fin0(P.uc[i]);
   }
+  K.linf_primal = linf_p;
   double pobj = dot(x.data(), P.c.data(), P.n);
   if (P.obj_sf != 1.0 || P.obj_off != 0.0) pobj = P.obj_sf * (pobj + P.obj_off);
   K.l2_primal = std::sqrt(rp2);
   spmv(P.AT, y.data(), tmp_n.data());
-  double rd2 = 0.0;
+  double rd2 = 0.0, linf_d = 0.0;
   for (int j = 0; j < P.n; ++j) {
     const double g = P.c[j] - tmp_n[j];
     const double r = g - slack[j];
     rd2 += r * r;
+    linf_d = std::max(linf_d, r - rel_dual * P.c[j]);
   }
-  K.l2_dual  = std::sqrt(rd2);
+  K.l2_dual   = std::sqrt(rd2);
+  K.linf_dual = linf_d;
   double dobj = dot(slack.data(), x.data(), P.n) + psum;
   if (P.obj_sf != 1.0 || P.obj_off != 0.0) dobj = P.obj_sf * (dobj + P.obj_off);
   K.pobj   = pobj;
@@ -401,13 +424,16 @@ Kkt convergence_information(const Problem& P,
   return K;
 }
 
-// check_termination_criteria_kernel, l2 branch (termination_strategy.cu:222-375).
-int termination_status(const Kkt& K, const double* tol, double b_norm, double c_norm)
+// check_termination_criteria_kernel (termination_strategy.cu:222-375): the l2
+// branch, or with per_constraint the L-inf branch (:317-332), whose maxima already
+// hold the relative terms.
+int termination_status(const Kkt& K, const double* tol, double b_norm, double c_norm, bool per_constraint = false)
 {
   // tol: abs_primal, rel_primal, abs_dual, rel_dual, abs_gap, rel_gap
   const bool optimal_gap     = K.gap <= tol[4] + tol[5] * K.absobj;
-  const bool primal_feasible = K.l2_primal <= tol[0] + tol[1] * b_norm;
-  if (K.l2_dual <= tol[2] + tol[3] * c_norm && primal_feasible && optimal_gap) return Optimal;
+  const bool primal_feasible = per_constraint ? K.linf_primal <= tol[0] : K.l2_primal <= tol[0] + tol[1] * b_norm;
+  const bool dual_feasible   = per_constraint ? K.linf_dual <= tol[2] : K.l2_dual <= tol[2] + tol[3] * c_norm;
+  if (dual_feasible && primal_feasible && optimal_gap) return Optimal;
   if (primal_feasible) return PrimalFeasible;
   return NoTermination;
 }
@@ -511,7 +537,7 @@ double fixed_point_error(const Csr& AT, const std::vector<double>& x, const std:
 // Parameter and info slots shared with oracle/oracle.py.
 enum ParamSlot {
   P_ABS_PRIMAL = 0, P_REL_PRIMAL, P_ABS_DUAL, P_REL_DUAL, P_ABS_GAP, P_REL_GAP,
-  P_ITER_LIMIT, P_TIME_LIMIT, P_THREADS, P_MODE, P_NPARAMS
+  P_ITER_LIMIT, P_TIME_LIMIT, P_THREADS, P_MODE, P_PER_CONSTRAINT, P_NPARAMS
 };
 enum InfoSlot {
   I_STATUS = 0, I_ITERS, I_L2_PRIMAL, I_REL_PRIMAL, I_L2_DUAL, I_REL_DUAL, I_POBJ, I_DOBJ,
@@ -690,15 +716,20 @@ int oracle_setup(int m, int n, const int* rowptr, const int* colidx, const doubl
 }
 
 // KKT record on the unscaled problem for a given (x, y, slack):
-// out = l2_primal, l2_dual, pobj, dobj, gap, absobj, b_norm, c_norm.
+// out = l2_primal, l2_dual, pobj, dobj, gap, absobj, b_norm, c_norm, linf_primal, linf_dual.
+// out[8], out[9] = the per_constraint_residual maxima for the relative
+// tolerances rel_primal, rel_dual.
 void oracle_kkt(int m, int n, const int* rowptr, const int* colidx, const double* vals,
                 const double* c, const double* lc, const double* uc, int maximize, double offset,
-                const double* x, const double* y, const double* slack, double* out)
+                const double* x, const double* y, const double* slack, double rel_primal,
+                double rel_dual, double* out)
 {
   Problem P;
   build_problem(m, n, rowptr, colidx, vals, c, lc, uc, nullptr, nullptr, maximize, offset, P);
   std::vector<double> xv(x, x + n), yv(y, y + m), sv(slack, slack + n), tm(m), tn(n);
-  Kkt K   = convergence_information(P, xv, yv, sv, tm, tn);
+  Kkt K   = convergence_information(P, xv, yv, sv, tm, tn, rel_primal, rel_dual);
+  out[8]  = K.linf_primal;
+  out[9]  = K.linf_dual;
   out[0]  = K.l2_primal;
   out[1]  = K.l2_dual;
   out[2]  = K.pobj;
@@ -778,6 +809,7 @@ int oracle_solve(int m, int n, const int* rowptr, const int* colidx, const doubl
   g_threads = std::max(1, (int)params[P_THREADS]);
   const double tol[6] = {params[P_ABS_PRIMAL], params[P_REL_PRIMAL], params[P_ABS_DUAL],
                          params[P_REL_DUAL],   params[P_ABS_GAP],    params[P_REL_GAP]};
+  const bool per_constraint     = params[P_PER_CONSTRAINT] != 0.0;
   const int64_t iteration_limit = (int64_t)params[P_ITER_LIMIT];
   const double time_limit       = params[P_TIME_LIMIT];
 
@@ -875,8 +907,8 @@ int oracle_solve(int m, int n, const int* rowptr, const int* colidx, const doubl
       unscale_primal(S, bor, xhat);
       unscale_dual(S, bor, yhat);
       unscale_slack(S, bor, slack);
-      K = convergence_information(P, xhat, yhat, slack, tmp_m, tmp_n);
-      const int st = termination_status(K, tol, b_norm, c_norm);
+      K = convergence_information(P, xhat, yhat, slack, tmp_m, tmp_n, tol[1], tol[3]);
+      const int st = termination_status(K, tol, b_norm, c_norm, per_constraint);
       double* rec  = nullptr;
       if (trace && trace_len && *trace_len < trace_cap) {
         rec = trace + (size_t)(*trace_len) * kTraceWidth;

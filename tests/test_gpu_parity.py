@@ -193,6 +193,26 @@ def test_solve_fp64_matches_oracle(tol):
     assert np.allclose(tr[:, 1:3], ot[:, 1:3], rtol=1e-6, atol=1e-12)
 
 
+@pytest.mark.parametrize("tol", [1e-4, 1e-8])
+def test_small_random_lp_2000x5000_fp64_matches_oracle(tol):
+    # The small random LP (m = 2 000, n = 5 000, 8 nonzeros per column, 0 <= x <= 10,
+    # equality rows, seed 0) in plain FP64: the oracle's iteration count and restart
+    # decisions, its objective to 1e-9, and the built-in optimum c.x0 = b.y0.
+    lp = G.random_lp(m=2000, n=5000, seed=0)
+    print(f"small random LP tol {tol:g}", flush=True)
+    r = c_api.solve(lp, tol=tol, precision="fp64", time_limit=TL, iteration_limit=100000)
+    o = O.solve(lp, tol=tol, threads=8, trace_cap=100000)
+    assert r.error == 0 and r.status == 1 and o["status"] == 1
+    assert r.iterations == o["iterations"]
+    assert rel(r.primal_objective, o["primal_objective"]) < 1e-9
+    assert np.array_equal(r.trace[:, 0], o["trace"][:, 0]) and np.array_equal(r.trace[:, 6], o["trace"][:, 6])
+    assert np.max(np.abs(r.x - o["x"])) < 1e-7 and np.max(np.abs(r.y - o["y"])) < 1e-7
+    assert rel(r.primal_objective, lp.obj_opt) < 10 * tol
+    # FP32-mixed (refined below 1e-4) meets the same test and optimum
+    f = c_api.solve(lp, tol=tol, precision="fp32_mixed", time_limit=TL, iteration_limit=100000)
+    assert f.status == 1 and rel(f.primal_objective, lp.obj_opt) < 10 * tol
+
+
 def test_skewed_fp64_matches_oracle_and_closed_form():
     lp = G.skewed_lp(m=20000, n=40000, seed=5)
     r = c_api.solve(lp, tol=1e-6, precision="fp64", time_limit=TL, iteration_limit=200000)
@@ -283,11 +303,36 @@ def _residuals(lp, r):
     return viol, w
 
 
-def test_per_constraint_residual():
-    # termination_strategy.cu:327-345: with per_constraint_residual the residual tests are
-    # max_i |r_p,i| - rel w_i <= abs (w_i the larger finite bound magnitude of row i) and
-    # max_j |r_d,j| - rel |c_j| <= abs.  The returned point meets both (checked in numpy);
-    # the L2-terminated point of the same LP does not, so the option changes the rule.
+def test_per_constraint_residual_signed_matches_oracle():
+    # convergence_information.cu:390-470, utils.cuh:576-590, termination_strategy.cu:317-332:
+    # the reference's signed rule, max(0, max_i r_i - rel w_i) <= abs and
+    # max(0, max_j r_d,j - rel c_j) <= abs.  FP64 GPU solve against the oracle with the same
+    # rule: the same stopping iteration; the returned point passes the rule in numpy.
+    import scipy.sparse as sp
+
+    lp = G.transportation_lp(sources=200, sinks=500, arcs=50, seed=3)  # costs > 0
+    tol = 1e-4
+    per = _solve_with(lp, tol, per_constraint_residual=True)
+    o = O.solve(lp, tol=tol, per_constraint_residual=True, iteration_limit=200000)
+    plain = _solve_with(lp, tol)
+    assert per.status == o["status"] == 1
+    assert per.iterations == o["iterations"] != plain.iterations
+    assert rel(per.primal_objective, o["primal_objective"]) < 1e-9
+    A = sp.csr_matrix((lp.values, lp.col_indices, lp.row_offsets), shape=(lp.m, lp.n))
+    viol, w = _residuals(lp, per)
+    ax = A @ per.x
+    r = ax - np.clip(ax, lp.con_lb, lp.con_ub)
+    rd = lp.c - A.T @ per.y - per.reduced_cost
+    assert max(0.0, np.max(r - tol * w)) <= tol * (1 + 1e-6)
+    assert max(0.0, np.max(rd - tol * lp.c)) <= tol * (1 + 1e-6)
+
+
+def test_per_constraint_residual_absolute_extension():
+    # pdlp_per_constraint_absolute = 1 (an extension): max_i |r_p,i| - rel w_i <= abs and
+    # max_j |r_d,j| - rel |c_j| <= abs, with w_i the larger finite bound magnitude of row i.
+    # The returned point meets both (checked in numpy); the L2-terminated point of the
+    # same LP does not, so the option changes the rule.  (This LP has costs below
+    # -abs / rel, under which the signed rule's dual test cannot pass.)
     import scipy.sparse as sp
 
     lp = G.random_lp(m=3000, n=7500, seed=41)
@@ -300,7 +345,7 @@ def test_per_constraint_residual():
         return np.max(viol - tol * w, initial=0.0), np.max(np.abs(rd) - tol * np.abs(lp.c), initial=0.0)
 
     plain = _solve_with(lp, tol)
-    per = _solve_with(lp, tol, per_constraint_residual=True)
+    per = _solve_with(lp, tol, per_constraint_residual=True, pdlp_per_constraint_absolute=1)
     assert plain.status == per.status == 1
     p_inf, d_inf = linf(per)
     assert p_inf <= tol * (1 + 1e-6) and d_inf <= tol * (1 + 1e-6)

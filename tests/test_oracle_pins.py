@@ -234,6 +234,89 @@ def test_bound_objective_rescaling_hand_worked():
     assert good, flags
 
 
+# ---------------------------------------------------------------- per-constraint residuals
+
+def _per_constraint_lp():
+    # A = I (3 x 3).  Rows: G with l = 2 (w = 2), L with u = 3 (w = 3), ranged [-1, 4]
+    # (w = max(|-1|, |4|) = 4).  c = (1, -1, 2).
+    n = 3
+    return G.LP(m=n, n=n, row_offsets=np.arange(n + 1, dtype=np.int32), col_indices=np.arange(n, dtype=np.int32),
+                values=np.ones(n), c=np.array([1.0, -1.0, 2.0]), con_lb=np.array([2.0, -INF, -1.0]),
+                con_ub=np.array([INF, 3.0, 4.0]), var_lb=np.zeros(n), var_ub=np.full(n, 10.0))
+
+
+def check_per_constraint(L=None):
+    """convergence_information.cu:390-470 with relative_residual_t (utils.cuh:576-590):
+    max(0, max_i r_i - rel w_i) over the signed residuals, signed c_j on the dual side."""
+    lp = _per_constraint_lp()
+    rel = 0.1
+    y0 = np.zeros(3)
+
+    def k(x, s):
+        return O.kkt(lp, np.array(x), y0, np.array(s), rel_primal=rel, rel_dual=rel, L=L)
+
+    ok = []
+    # x = (1, 3, 4): Ax - clamp = (-1, 0, 0); r - 0.1 w = (-1.2, -0.3, -0.4): every term is
+    # negative, so the maximum is the neutral 0 -- the row below its lower bound passes.
+    ok.append(k([1.0, 3.0, 4.0], [0, 0, 0])["linf_primal"] == 0.0)
+    # x = (2, 3, 4.5): r = (0, 0, 0.5); r - 0.1 w = (-0.2, -0.3, 0.1): 0.1 (ranged row, w = 4).
+    ok.append(math.isclose(k([2.0, 3.0, 4.5], [0, 0, 0])["linf_primal"], 0.1, rel_tol=1e-12))
+    # x = (1, 5, 4.5): r = (-1, 2, 0.5) -> (-1.2, 1.7, 0.1): 1.7.
+    ok.append(math.isclose(k([1.0, 5.0, 4.5], [0, 0, 0])["linf_primal"], 1.7, rel_tol=1e-12))
+    # Dual, y = 0: r_d = c - s.  s = (1, 0, 2): r_d = (0, -1, 0); r_d - 0.1 c = (-0.1, -0.9, -0.2): 0.
+    ok.append(k([2.0, 3.0, 4.0], [1.0, 0.0, 2.0])["linf_dual"] == 0.0)
+    # s = (1, -1.05, 2): r_d = (0, 0.05, 0); r_d - 0.1 c = (-0.1, 0.15, -0.2): 0.15 (signed c = -1).
+    ok.append(math.isclose(k([2.0, 3.0, 4.0], [1.0, -1.05, 2.0])["linf_dual"], 0.15, rel_tol=1e-12))
+    return all(ok), ok
+
+
+def test_per_constraint_residual_hand_worked():
+    good, flags = check_per_constraint()
+    assert good, flags
+
+
+def _signed_maxima(lp, x, y, s, tol):
+    import scipy.sparse as sp
+
+    A = sp.csr_matrix((lp.values, lp.col_indices, lp.row_offsets), shape=(lp.m, lp.n))
+    ax = A @ x
+    r = ax - np.clip(ax, lp.con_lb, lp.con_ub)
+    w = np.maximum(np.where(np.isfinite(lp.con_ub), np.abs(lp.con_ub), 0.0),
+                   np.where(np.isfinite(lp.con_lb), np.abs(lp.con_lb), 0.0))
+    rd = lp.c - A.T @ y - s
+    return max(0.0, np.max(r - tol * w)), max(0.0, np.max(rd - tol * lp.c))
+
+
+def test_per_constraint_solve_stops_at_the_first_check_passing_the_signed_rule():
+    # termination_strategy.cu:317-332: with per_constraint_residual the solve stops at the
+    # first check whose signed L-inf maxima (and the gap test) pass.  Transportation costs
+    # are positive, so the dual side -rel c_j loosens every column's test.
+    lp = G.transportation_lp(sources=200, sinks=500, arcs=50, seed=3)
+    tol = 1e-4
+    per = O.solve(lp, tol=tol, per_constraint_residual=True, iteration_limit=100000, trace_cap=100000)
+    l2 = O.solve(lp, tol=tol, iteration_limit=100000)
+    assert per["status"] == l2["status"] == 1
+    p_inf, d_inf = _signed_maxima(lp, per["x"], per["y"], per["reduced_cost"], tol)
+    assert p_inf <= tol and d_inf <= tol
+    assert per["iterations"] != l2["iterations"]
+
+
+def test_per_constraint_signed_rule_cannot_pass_a_cost_below_minus_abs_over_rel():
+    # With the signed c_j of relative_residual_t a column with c_j < -abs / rel needs
+    # r_j <= abs + rel c_j < 0 ... while r_j -> 0 at the optimum: the dual test cannot
+    # pass, so such an LP runs to its limit even though the l2 solve converges.  This is
+    # the reference's rule, kept for result identity.
+    lp = G.random_lp(m=200, n=500, seed=3)
+    tol = 1e-4
+    assert lp.c.min() < -1.0
+    l2 = O.solve(lp, tol=tol, iteration_limit=5000)
+    per = O.solve(lp, tol=tol, per_constraint_residual=True, iteration_limit=3000)
+    assert l2["status"] == 1 and l2["iterations"] < 3000
+    assert per["status"] == 4 and per["iterations"] == 3000
+    _, d_inf = _signed_maxima(lp, per["x"], per["y"], per["reduced_cost"], tol)
+    assert d_inf >= tol * (-lp.c.min()) - 1e-6
+
+
 # ---------------------------------------------------------------- planted mistakes
 
 # (name, check, original text, mutated text)
@@ -271,6 +354,16 @@ MUTANTS = [
      "S.beta  = 1.0 / (std::sqrt(sum_bounds_sq(P.lc, P.uc)) + 1.0);"),
     ("gamma_unscaled_objective", check_beta_gamma,
      "cs += Q.c[j] * Q.c[j] * 1.0;", "cs += P.c[j] * P.c[j] * 1.0;"),
+    ("per_constraint_abs_primal", check_per_constraint,
+     "linf_p = std::max(linf_p, r - rel_primal", "linf_p = std::max(linf_p, std::fabs(r) - rel_primal"),
+    ("per_constraint_abs_dual", check_per_constraint,
+     "linf_d = std::max(linf_d, r - rel_dual * P.c[j]);", "linf_d = std::max(linf_d, std::fabs(r) - rel_dual * P.c[j]);"),
+    ("per_constraint_abs_c", check_per_constraint,
+     "linf_d = std::max(linf_d, r - rel_dual * P.c[j]);", "linf_d = std::max(linf_d, r - rel_dual * std::fabs(P.c[j]));"),
+    ("per_constraint_neutral_minus_inf", check_per_constraint,
+     "double rp2 = 0.0, psum = 0.0, linf_p = 0.0;", "double rp2 = 0.0, psum = 0.0, linf_p = -kInf;"),
+    ("per_constraint_lower_bound_only", check_per_constraint,
+     "  if (std::isfinite(lo)) v = std::max(v, std::fabs(lo));", "  if (std::isfinite(lo)) v = std::fabs(lo);"),
 ]
 
 
@@ -307,7 +400,7 @@ def test_unmutated_source_builds_and_passes(tmp_path):
     subprocess.run(["g++", "-O1", "-std=c++17", "-fPIC", "-shared", "-fopenmp", "-ffp-contract=off", "-o", str(so),
                     ORACLE_SRC], check=True)
     L = ctypes.CDLL(str(so))
-    for check in (check_pid, check_restart, check_fpe, check_beta_gamma):
+    for check in (check_pid, check_restart, check_fpe, check_beta_gamma, check_per_constraint):
         assert check(L)[0], check.__name__
 
 
