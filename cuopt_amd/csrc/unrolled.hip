@@ -22,6 +22,8 @@
 // gathers B contiguous values of the operand (one 256-byte FP32 line for
 // B = 64) while the row's value / index stream is read once for the whole batch.
 #include <cmath>
+#include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -293,6 +295,29 @@ __global__ __launch_bounds__(kUB) void unr_spmm(const int* __restrict__ ptr, con
 
 inline int ew_blocks(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>((n + kUB - 1) / kUB, 4096)); }
 
+// Checkpoint slots addressed by a device-side segment counter, so that one
+// captured segment graph serves every segment: store copies src into slot
+// *seg of base, load copies slot *seg of base into dst.
+template <typename V>
+__global__ __launch_bounds__(kUB) void unr_ckpt_store(size_t len, const V* __restrict__ src, V* __restrict__ base,
+                                                      const int* __restrict__ seg)
+{
+  V* dst = base + (size_t)(*seg) * len;
+  for (size_t i = blockIdx.x * (size_t)kUB + threadIdx.x; i < len; i += (size_t)gridDim.x * kUB) dst[i] = src[i];
+}
+template <typename V>
+__global__ __launch_bounds__(kUB) void unr_ckpt_load(size_t len, const V* __restrict__ base,
+                                                     const int* __restrict__ seg, V* __restrict__ dst)
+{
+  const V* src = base + (size_t)(*seg) * len;
+  for (size_t i = blockIdx.x * (size_t)kUB + threadIdx.x; i < len; i += (size_t)gridDim.x * kUB) dst[i] = src[i];
+}
+// seg <- set ? value : seg + value (one lane; a vector store)
+__global__ void unr_seg_update(int* seg, int value, int set)
+{
+  if (threadIdx.x == 0 && blockIdx.x == 0) *seg = set ? value : *seg + value;
+}
+
 struct UnrolledImpl {
   int m = 0, n = 0, precision = 1;
   double lam = 0.0, eta = 1.0;
@@ -306,7 +331,58 @@ struct UnrolledImpl {
   int B = 0, K = 0, every = 1, ldb = 0;
   DevBuf<unsigned char> work;  // typed views below
   size_t ckpt_off = 0, seg_off = 0;
+  DevBuf<int> seg;  // checkpoint segment counter read by the segment graphs
+  // Captured segment graphs: a full forward segment starting from either ping-pong
+  // buffer, and a full backward segment (checkpoint load, recompute, reverse sweep)
+  // with an earlier segment before it.  Valid while graph_key matches.
+  hipGraphExec_t fwd_graph[2] = {nullptr, nullptr};
+  hipGraphExec_t bwd_graph    = nullptr;
+  std::vector<long long> graph_key;
+  hipStream_t cap = nullptr;  // private capture stream (default priority)
+  void drop_graphs()
+  {
+    for (hipGraphExec_t* g : {&fwd_graph[0], &fwd_graph[1], &bwd_graph}) {
+      if (*g) (void)hipGraphExecDestroy(*g);
+      *g = nullptr;
+    }
+    graph_key.clear();
+  }
+  ~UnrolledImpl()
+  {
+    drop_graphs();
+    if (cap) (void)hipStreamDestroy(cap);
+  }
 };
+
+// Graph replay of full segments unless CUOPT_AMD_GRAPHS=0 (direct launches of
+// the same kernels, for tests).
+inline bool unrolled_graphs_enabled()
+{
+  const char* g = std::getenv("CUOPT_AMD_GRAPHS");
+  return !(g && std::atoi(g) == 0);
+}
+
+// Records what f launches on the handle's capture stream and instantiates it.
+template <class F>
+hipGraphExec_t capture_graph(UnrolledImpl& U, F f)
+{
+  if (!U.cap) CUOPT_HIP_CHECK(hipStreamCreateWithFlags(&U.cap, hipStreamNonBlocking));
+  hipGraph_t g;
+  CUOPT_HIP_CHECK(hipStreamBeginCapture(U.cap, hipStreamCaptureModeThreadLocal));
+  try {
+    f(U.cap);
+  } catch (...) {
+    (void)hipStreamEndCapture(U.cap, &g);
+    if (g) (void)hipGraphDestroy(g);
+    throw;
+  }
+  CUOPT_HIP_CHECK(hipStreamEndCapture(U.cap, &g));
+  hipGraphExec_t e = nullptr;
+  const hipError_t r = hipGraphInstantiate(&e, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  CUOPT_HIP_CHECK(r);
+  return e;
+}
 
 template <typename V>
 struct Views {
@@ -382,20 +458,41 @@ void forward_impl(UnrolledImpl& U, const double* c, const double* lc, const doub
   Work<V> W      = carve<V>(U, nseg, true);
   Views<V> M     = views<V>(U);
   const size_t nB = (size_t)U.n * U.B, mB = (size_t)U.m * U.B, lcn = U.ldb ? mB : (size_t)U.m;
+  if (!U.seg.get()) U.seg.resize(1);
+  // The captured graphs hold the work buffer's addresses and the shapes.
+  const std::vector<long long> key{(long long)(uintptr_t)U.work.get(), U.B, U.every, U.ldb, (long long)sizeof(V)};
+  if (key != U.graph_key) {
+    U.drop_graphs();
+    U.graph_key = key;
+  }
   hipLaunchKernelGGL(unr_convert_in<V>, dim3(ew_blocks(nB)), dim3(kUB), 0, s, nB, c, W.c);
   hipLaunchKernelGGL(unr_convert_in<V>, dim3(ew_blocks(lcn)), dim3(kUB), 0, s, lcn, lc, W.lc);
   hipLaunchKernelGGL(unr_convert_in<V>, dim3(ew_blocks(lcn)), dim3(kUB), 0, s, lcn, uc, W.uc);
   hipLaunchKernelGGL(unr_init_x<V>, dim3(ew_blocks(nB)), dim3(kUB), 0, s, U.n, U.B, M.lb, M.ub, W.x[0]);
   hipLaunchKernelGGL(unr_zero<V>, dim3(ew_blocks(mB)), dim3(kUB), 0, s, mB, W.y[0]);
+  hipLaunchKernelGGL(unr_seg_update, dim3(1), dim3(kWave), 0, s, U.seg.get(), 0, 1);
+  // One segment from buffer p: checkpoint (x, y) into slot *seg, advance the
+  // counter, then L steps.  Full segments replay a captured copy of this.
+  auto segment = [&](int p, int L, hipStream_t q) {
+    hipLaunchKernelGGL(unr_ckpt_store<V>, dim3(ew_blocks(nB)), dim3(kUB), 0, q, nB, (const V*)W.x[p], W.ck_x,
+                       (const int*)U.seg.get());
+    hipLaunchKernelGGL(unr_ckpt_store<V>, dim3(ew_blocks(mB)), dim3(kUB), 0, q, mB, (const V*)W.y[p], W.ck_y,
+                       (const int*)U.seg.get());
+    hipLaunchKernelGGL(unr_seg_update, dim3(1), dim3(kWave), 0, q, U.seg.get(), 1, 0);
+    int c = p;
+    for (int t = 0; t < L; ++t, c ^= 1) step<V>(M, U, W, W.x[c], W.y[c], W.x[c ^ 1], W.y[c ^ 1], q);
+  };
+  const bool graphs = unrolled_graphs_enabled() && nseg > 1;
   int cur = 0;
-  for (int k = 0; k < U.K; ++k) {
-    if (k % U.every == 0) {
-      const int sg = k / U.every;
-      CUOPT_HIP_CHECK(hipMemcpyAsync(W.ck_x + (size_t)sg * nB, W.x[cur], nB * sizeof(V), hipMemcpyDeviceToDevice, s));
-      CUOPT_HIP_CHECK(hipMemcpyAsync(W.ck_y + (size_t)sg * mB, W.y[cur], mB * sizeof(V), hipMemcpyDeviceToDevice, s));
+  for (int sg = 0; sg < nseg; ++sg) {
+    const int L = std::min(U.K, (sg + 1) * U.every) - sg * U.every;
+    if (graphs && L == U.every) {
+      if (!U.fwd_graph[cur]) U.fwd_graph[cur] = capture_graph(U, [&](hipStream_t q) { segment(cur, L, q); });
+      CUOPT_HIP_CHECK(hipGraphLaunch(U.fwd_graph[cur], s));
+    } else {
+      segment(cur, L, s);
     }
-    step<V>(M, U, W, W.x[cur], W.y[cur], W.x[cur ^ 1], W.y[cur ^ 1], s);
-    cur ^= 1;
+    cur ^= (L & 1);
   }
   CUOPT_HIP_CHECK(hipGetLastError());
   hipLaunchKernelGGL(unr_convert_out<V>, dim3(ew_blocks(nB)), dim3(kUB), 0, s, nB, W.x[cur], 1.0, x_out);
@@ -419,31 +516,46 @@ void backward_impl(UnrolledImpl& U, const double* gx, const double* gy, double* 
   hipLaunchKernelGGL(unr_convert_in<V>, dim3(ew_blocks(nB)), dim3(kUB), 0, s, nB, gx, W.ax);
   // y-gradient: start value of ay (zero when absent), staged in gw's slot of size m*B? use gt as ay first.
   if (gy) hipLaunchKernelGGL(unr_convert_in<V>, dim3(ew_blocks(mB)), dim3(kUB), 0, s, mB, gy, W.gt);
-  bool started = false;
-  for (int sg = nseg - 1; sg >= 0; --sg) {
-    const int k0 = sg * U.every, k1 = std::min(U.K, k0 + U.every), L = k1 - k0;
-    // recompute states x_{k0..k1}, y_{k0..k1} into the segment store (slot t holds step k0 + t)
-    CUOPT_HIP_CHECK(hipMemcpyAsync(W.sx, W.ck_x + (size_t)sg * nB, nB * sizeof(V), hipMemcpyDeviceToDevice, s));
-    CUOPT_HIP_CHECK(hipMemcpyAsync(W.sy, W.ck_y + (size_t)sg * mB, mB * sizeof(V), hipMemcpyDeviceToDevice, s));
+  // The segments run last to first; the counter names the checkpoint slot.
+  hipLaunchKernelGGL(unr_seg_update, dim3(1), dim3(kWave), 0, s, U.seg.get(), nseg - 1, 1);
+  // One segment of L steps: load its checkpoint, recompute states x_{k0..k0+L},
+  // y_{k0..k0+L} into the segment store (slot t holds step k0 + t), run the
+  // reverse sweep, step the counter back.  last: the segment holding the final
+  // step (gt starts from the upstream y-gradient); first: segment 0, whose first
+  // step starts from the constant y = 0 and propagates into no earlier step.
+  auto segment = [&](int L, bool last, bool first, hipStream_t q) {
+    hipLaunchKernelGGL(unr_ckpt_load<V>, dim3(ew_blocks(nB)), dim3(kUB), 0, q, nB, (const V*)W.ck_x,
+                       (const int*)U.seg.get(), W.sx);
+    hipLaunchKernelGGL(unr_ckpt_load<V>, dim3(ew_blocks(mB)), dim3(kUB), 0, q, mB, (const V*)W.ck_y,
+                       (const int*)U.seg.get(), W.sy);
     for (int t = 0; t < L; ++t)
       step<V>(M, U, W, W.sx + (size_t)t * nB, W.sy + (size_t)t * mB, W.sx + (size_t)(t + 1) * nB,
-              W.sy + (size_t)(t + 1) * mB, s);
+              W.sy + (size_t)(t + 1) * mB, q);
     for (int t = L - 1; t >= 0; --t) {
       const V* xplus = W.sx + (size_t)(t + 1) * nB;
       const V* yplus = W.sy + (size_t)(t + 1) * mB;
-      if (!started) {  // last step overall: gt from the upstream y-gradient
-        hipLaunchKernelGGL(unr_adj_start<V>, dim3(ew_blocks(mB)), dim3(kUB), 0, s, U.m, U.B,
+      if (last && t == L - 1)
+        hipLaunchKernelGGL(unr_adj_start<V>, dim3(ew_blocks(mB)), dim3(kUB), 0, q, U.m, U.B,
                            gy ? (const V*)W.gt : (const V*)nullptr, yplus, (const V*)W.lc, (const V*)W.uc, U.ldb,
                            sigma, W.gt, W.glc, W.guc);
-        started = true;
-      }
-      hipLaunchKernelGGL(unr_adj_primal<V>, spmm_grid(U.n, U.B), dim3(kUB), 0, s, M.tptr, M.tidx, M.tval, U.n, U.B,
+      hipLaunchKernelGGL(unr_adj_primal<V>, spmm_grid(U.n, U.B), dim3(kUB), 0, q, M.tptr, M.tidx, M.tval, U.n, U.B,
                          (const V*)W.gt, xplus, M.lb, M.ub, inv_prox, W.ax, W.gw, W.gc);
-      // the first step starts from the constant y = 0: no earlier step to propagate into
-      if (sg > 0 || t > 0)
-        hipLaunchKernelGGL(unr_adj_dual<V>, spmm_grid(U.m, U.B), dim3(kUB), 0, s, M.aptr, M.aidx, M.aval, U.m, U.B,
+      if (!first || t > 0)
+        hipLaunchKernelGGL(unr_adj_dual<V>, spmm_grid(U.m, U.B), dim3(kUB), 0, q, M.aptr, M.aidx, M.aval, U.m, U.B,
                            (const V*)W.gw, tau, sigma, (const V*)(W.sy + (size_t)t * mB), (const V*)W.lc,
                            (const V*)W.uc, U.ldb, W.gt, W.glc, W.guc);
+    }
+    hipLaunchKernelGGL(unr_seg_update, dim3(1), dim3(kWave), 0, q, U.seg.get(), -1, 0);
+  };
+  const bool graphs = unrolled_graphs_enabled();
+  for (int sg = nseg - 1; sg >= 0; --sg) {
+    const int L = std::min(U.K, (sg + 1) * U.every) - sg * U.every;
+    const bool last = sg == nseg - 1, first = sg == 0;
+    if (graphs && !last && !first) {  // middle segments are always full
+      if (!U.bwd_graph) U.bwd_graph = capture_graph(U, [&](hipStream_t q) { segment(L, false, false, q); });
+      CUOPT_HIP_CHECK(hipGraphLaunch(U.bwd_graph, s));
+    } else {
+      segment(L, last, first, s);
     }
   }
   CUOPT_HIP_CHECK(hipGetLastError());

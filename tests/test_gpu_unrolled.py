@@ -160,3 +160,34 @@ def test_full_size_fp32_map_and_vjp_track_fp64():
     assert abs(l32 - l64) <= 1e-4 * abs(l64)
     assert np.linalg.norm(gc32 - gc64) <= 1e-2 * np.linalg.norm(gc64)
     assert np.linalg.norm(gb32 - gb64) <= 1e-2 * np.linalg.norm(gb64)
+
+
+def test_segment_graph_replays_match_direct_launches(monkeypatch):
+    # Full checkpoint segments replay captured graphs (forward: one per starting
+    # buffer; backward: one middle segment).  Three forward/backward calls with
+    # different objectives, torch work and synchronisations between them, give
+    # bitwise the results of launching the same kernels directly.
+    lp, A = problem(2, m=300, n=750)
+    B, K, every = 16, 200, 9  # 22 full segments of odd length and a 2-step tail
+    op = UnrolledPDHG(lp, ridge=1e-3, steps=K, checkpoint_every=every, precision="fp32_mixed")
+    out = {}
+    for flag in ("0", "1"):
+        monkeypatch.setenv("CUOPT_AMD_GRAPHS", flag)
+        runs = []
+        for rep in range(3):
+            print(f"unrolled graphs={flag} call {rep}", flush=True)
+            rng = np.random.default_rng(10 + rep)
+            ct = torch.tensor(lp.c[None, :] + 0.1 * rng.normal(size=(B, lp.n)), device=DEV, requires_grad=True)
+            bt = torch.tensor(lp.con_lb, device=DEV, requires_grad=True)
+            x, y = op(ct, bt)
+            w = torch.tensor(rng.normal(size=(B, lp.n)), device=DEV)
+            ((x * w).sum() + (y ** 2).sum()).backward()
+            junk = torch.ones(4096, device=DEV).cumsum(0)  # eager work between calls
+            torch.cuda.synchronize()
+            assert float(junk[-1]) == 4096.0
+            runs.append((x.detach().cpu().numpy(), y.detach().cpu().numpy(), ct.grad.cpu().numpy(),
+                         bt.grad.cpu().numpy()))
+        out[flag] = runs
+    for a, b in zip(out["0"], out["1"]):
+        for u, v in zip(a, b):
+            assert np.array_equal(u, v)
