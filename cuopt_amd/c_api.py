@@ -331,11 +331,17 @@ def make_settings(tol: float = 1e-4, precision: str = "fp64", iteration_limit: i
     return s
 
 
+def read_lp(path: str):
+    """An MPS file read by cuOptReadProblem, as a generators.LP rebuilt from the C getters."""
+    return Problem.read_mps(path).lp
+
+
 def solve(lp, tol: float = 1e-4, precision: str = "fp64", iteration_limit: int | None = None,
           time_limit: float | None = None, x_init=None, y_init=None, method: int = CUOPT_METHOD_PDLP,
           log: bool = False, refinement: bool | None = None, presolve: bool = False) -> Result:
-    """Solve ``lp`` (a generators.LP) through the C ABI with all six tolerances set to ``tol``."""
-    prob = Problem(lp)
+    """Solve ``lp`` (a generators.LP, or a :class:`Problem` such as one read by
+    ``Problem.read_mps``) through the C ABI with all six tolerances set to ``tol``."""
+    prob = lp if isinstance(lp, Problem) else Problem(lp)
     s = make_settings(tol, precision, iteration_limit, time_limit, method, log, refinement, presolve)
     if x_init is not None:
         s.set_initial_primal(x_init)

@@ -16,8 +16,8 @@ from scipy.optimize import linprog
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
 
+from cuopt_amd import c_api  # noqa: E402
 from cuopt_amd import generators as G  # noqa: E402
-from cuopt_amd.mps import read_mps  # noqa: E402
 
 CASES = {
     "transport_small": ("transportation_lp", dict(sources=200, sinks=500, arcs=50, seed=3)),
@@ -53,7 +53,7 @@ def main():
         out["cases"][name] = {"generator": fn, "kwargs": kw, "objective": highs(lp)}
         print(name, out["cases"][name]["objective"])
     for f in MPS:
-        lp = read_mps(os.path.join(HERE, f))
+        lp = c_api.read_lp(os.path.join(HERE, f))
         out["cases"][f] = {"mps": f, "objective": highs(lp)}
         print(f, out["cases"][f]["objective"])
     with open(os.path.join(HERE, "highs_optima.json"), "w") as fh:

@@ -15,7 +15,6 @@ import pytest
 
 from cuopt_amd import c_api
 from cuopt_amd import generators as G
-from cuopt_amd.mps import read_mps
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
@@ -133,7 +132,7 @@ def test_crossing_bounds_report_infeasible_without_gpu_work():
 
 
 def test_no_constraints():
-    lp = read_mps(os.path.join(GOLDEN, "lp-model-no-constraints.mps"))
+    lp = c_api.read_lp(os.path.join(GOLDEN, "lp-model-no-constraints.mps"))
     r = c_api.solve(lp, method=c_api.CUOPT_METHOD_PDLP)
     assert r.status == 6  # PDLP cannot run without constraints (solve.cu:563-568)
     r = c_api.solve(lp, method=c_api.CUOPT_METHOD_CONCURRENT)
@@ -206,7 +205,7 @@ def assert_same_lp(a, b):
 @pytest.mark.parametrize("fname", GOLDEN_MPS)
 def test_read_mps_matches_python_reader(fname):
     path = os.path.join(GOLDEN, fname)
-    assert_same_lp(c_api.Problem.read_mps(path).lp, read_mps(path))
+    assert_same_lp(c_api.Problem.read_mps(path).lp, c_api.read_lp(path))
 
 
 @pytest.mark.parametrize("fname", GOLDEN_MPS)

@@ -18,7 +18,6 @@ import pytest
 
 from cuopt_amd import c_api
 from cuopt_amd import generators as G
-from cuopt_amd.mps import read_mps
 from oracle import oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -243,8 +242,10 @@ def test_fp32_mixed_skewed():
 @pytest.mark.parametrize("fname", ["afiro_original.mps", "good-max.mps", "max_offset.mps",
                                    "lp_model_with_var_bounds.mps"])
 def test_fixtures_fp64(highs, fname):
-    lp = read_mps(os.path.join(GOLDEN, fname))
-    r = c_api.solve(lp, tol=1e-8, time_limit=TL, iteration_limit=1000000)
+    # the problem handle cuOptReadProblem returns is the one solved (the drop-in read path)
+    prob = c_api.Problem.read_mps(os.path.join(GOLDEN, fname))
+    lp = prob.lp
+    r = c_api.solve(prob, tol=1e-8, time_limit=TL, iteration_limit=1000000)
     o = O.solve(lp, tol=1e-8)
     assert r.status == 1
     assert r.iterations == o["iterations"]
@@ -497,8 +498,9 @@ def test_fp32_recentring_rescues_stalling_lps():
 
 
 def test_iteration_limit_semantics():
-    lp = read_mps(os.path.join(GOLDEN, "afiro_original.mps"))
-    r = c_api.solve(lp, tol=0.0, iteration_limit=10, time_limit=TL)
+    prob = c_api.Problem.read_mps(os.path.join(GOLDEN, "afiro_original.mps"))
+    lp = prob.lp
+    r = c_api.solve(prob, tol=0.0, iteration_limit=10, time_limit=TL)
     o = O.solve(lp, tol=0.0, iteration_limit=10)
     assert r.status == 4 and r.iterations == 10
     assert np.any(r.x != 0.0)
@@ -553,8 +555,8 @@ def test_graph_replays_match_direct_launches(monkeypatch):
 # iteration counts within a few checks) and the closed-form optima.
 def test_batch_fp64_matches_single_solves(highs):
     lps = [G.random_lp(m=m, n=int(2.5 * m), seed=100 + m) for m in (50, 73, 120, 200, 333, 400)]
-    lps.append(read_mps(os.path.join(GOLDEN, "afiro_original.mps")))
-    lps.append(read_mps(os.path.join(GOLDEN, "good-max.mps")))
+    lps.append(c_api.read_lp(os.path.join(GOLDEN, "afiro_original.mps")))
+    lps.append(c_api.read_lp(os.path.join(GOLDEN, "good-max.mps")))
     lps.append(G.skewed_lp(m=2000, n=4000, seed=5))
     rs = c_api.batch_solve(lps, tol=1e-8, precision="fp64", time_limit=TL, iteration_limit=500000)
     for lp, r in zip(lps, rs):
@@ -593,7 +595,7 @@ def test_batch_edge_cases_keep_order():
     crossing = G.random_lp(m=60, n=150, seed=4)
     crossing.var_lb = crossing.var_lb.copy()
     crossing.var_lb[0] = 11.0  # above its upper bound of 10
-    empty = read_mps(os.path.join(GOLDEN, "lp-model-no-constraints.mps"))
+    empty = c_api.read_lp(os.path.join(GOLDEN, "lp-model-no-constraints.mps"))
     rs = c_api.batch_solve([good, crossing, empty, good], tol=1e-6, time_limit=TL, iteration_limit=100000)
     assert [r.status for r in rs] == [1, 2, 6, 1]
     assert np.array_equal(rs[0].x, rs[3].x)

@@ -13,7 +13,6 @@ import pytest
 from cuopt_amd import c_api
 from cuopt_amd import generators as G
 from cuopt_amd import linear_programming as LPM
-from cuopt_amd.mps import read_mps
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 GOLDEN = os.path.join(ROOT, "tests", "golden")
@@ -168,10 +167,11 @@ def test_name_count_must_match():
 
 
 @pytest.mark.parametrize("fname", ["afiro_original.mps", "good-mps-fixed-ranges.mps", "lp_model_with_var_bounds.mps"])
-def test_parse_mps_matches_python_reader_and_keeps_names(fname):
+def test_parse_mps_matches_the_c_problem_and_keeps_names(fname):
+    # the data model carries what cuOptReadProblem's getters report
     path = os.path.join(GOLDEN, fname)
     dm = LPM.ParseMps(path)
-    ref = read_mps(path)
+    ref = c_api.read_lp(path)
     assert np.array_equal(dm.get_constraint_matrix_offsets(), ref.row_offsets)
     assert np.array_equal(dm.get_constraint_matrix_values(), ref.values)
     assert np.array_equal(dm.get_objective_coefficients(), ref.c)

@@ -12,8 +12,8 @@ import os
 import numpy as np
 import pytest
 
+from cuopt_amd import c_api
 from cuopt_amd import generators as G
-from cuopt_amd.mps import read_mps
 from oracle import oracle as O
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
@@ -51,7 +51,7 @@ def test_spmv_matches_scipy():
 
 
 def test_range_rows_map_to_closed_form_bounds():
-    lp = read_mps(os.path.join(GOLDEN, "good-mps-fixed-ranges.mps"))
+    lp = c_api.read_lp(os.path.join(GOLDEN, "good-mps-fixed-ranges.mps"))
     # Bounds stated in the fixture's own header comment.
     assert np.allclose(lp.con_lb, [1.2, 1.5, 7.9, 3.5, 3.9, 4.9])
     assert np.allclose(lp.con_ub, [5.4, 4.9, 9.5, 6.9, 3.9, 4.9])
@@ -60,7 +60,7 @@ def test_range_rows_map_to_closed_form_bounds():
 def test_methodical1_initial_step_and_weight_known_answer():
     # Reference KAT: cpp/tests/linear_programming/pdlp_test.cu:243-287 (afiro, Methodical1,
     # tolerance 1e-4): initial step size 1.4893, initial primal weight 0.0141652.
-    lp = read_mps(os.path.join(GOLDEN, "afiro_original.mps"))
+    lp = c_api.read_lp(os.path.join(GOLDEN, "afiro_original.mps"))
     s = O.setup(lp, mode=1)
     assert abs(s["eta"] - 1.4893) <= 1e-4
     assert abs(s["omega"] - 0.0141652) <= 1e-4
@@ -169,7 +169,7 @@ def test_solve_matches_highs(highs, name):
     ("lp_model_with_var_bounds.mps", 1e-8, 1e-6),
 ])
 def test_mps_fixtures(highs, fname, tol, bound):
-    lp = read_mps(os.path.join(GOLDEN, fname))
+    lp = c_api.read_lp(os.path.join(GOLDEN, fname))
     r = O.solve(lp, tol=tol)
     assert r["status"] == 1
     assert rel(r["primal_objective"], highs[fname]["objective"]) < bound
@@ -178,7 +178,7 @@ def test_mps_fixtures(highs, fname, tol, bound):
 
 
 def test_maximize_negates_duals():
-    lp = read_mps(os.path.join(GOLDEN, "good-max.mps"))
+    lp = c_api.read_lp(os.path.join(GOLDEN, "good-max.mps"))
     r = O.solve(lp, tol=1e-8)
     # max 2x1+3x2+x3 s.t. x1+2x2+x3 <= 10: the constraint dual of the max problem is >= 0
     # in the maximisation convention after the sign flip of solve.cu:583-588.
@@ -188,7 +188,7 @@ def test_maximize_negates_duals():
 
 def test_iteration_limit_returns_current_iterate():
     # pdlp_test.cu:124-147: iteration limit 10 with zero tolerance -> IterationLimit, x != 0.
-    lp = read_mps(os.path.join(GOLDEN, "afiro_original.mps"))
+    lp = c_api.read_lp(os.path.join(GOLDEN, "afiro_original.mps"))
     r = O.solve(lp, tol=0.0, iteration_limit=10)
     assert r["status"] == 4
     assert r["iterations"] == 10
@@ -196,7 +196,7 @@ def test_iteration_limit_returns_current_iterate():
 
 
 def test_no_constraints_is_numerical_error_for_pdlp():
-    lp = read_mps(os.path.join(GOLDEN, "lp-model-no-constraints.mps"))
+    lp = c_api.read_lp(os.path.join(GOLDEN, "lp-model-no-constraints.mps"))
     r = O.solve(lp)
     assert r["status"] == 6  # run_pdlp_solver, solve.cu:563-568
 

@@ -94,7 +94,6 @@ def test_file_parameters_without_gpu(tmp_path):
     # problem is written as MPS before the solve and reads back as the same LP; the
     # .sol file has the reference's layout (solution_writer.cu:16-40), without
     # variable lines when the problem has no names
-    from cuopt_amd.mps import read_mps
 
     lp = fully_reducible()
     prob = c_api.Problem(lp)
@@ -103,7 +102,7 @@ def test_file_parameters_without_gpu(tmp_path):
     assert st.set("solution_file", str(tmp_path / "out.sol")) == 0
     r = c_api.solve_problem(prob, st)
     assert r.status == 1
-    back = read_mps(str(tmp_path / "user.mps"))
+    back = c_api.read_lp(str(tmp_path / "user.mps"))
     assert np.allclose(back.c, lp.c) and np.allclose(back.values, lp.values)
     assert np.allclose(back.var_lb, lp.var_lb) and np.allclose(back.var_ub, lp.var_ub)
     lines = (tmp_path / "out.sol").read_text().splitlines()
