@@ -103,6 +103,8 @@ def lib():
         L.cuOptAmdGetTerminationInfo.argtypes = [vp, vp, i32]
         L.cuOptAmdGetTrace.argtypes = [vp, vp, i32, pi32]
         L.cuOptAmdSpmv.argtypes = [i32, i32, vp, vp, vp, vp, vp, i32]
+        L.cuOptAmdSpmvNoAlloc.argtypes = [i32, i32, vp, vp, vp, vp, vp, i32]
+        L.cuOptAmdSpmvScratchBytes.argtypes = [ctypes.POINTER(ctypes.c_size_t)]
         L.cuOptAmdDeviceCount.restype = i32
         L.cuOptAmdBenchmarkSteps.argtypes = [vp, vp, i32, i32, vp, i32]
         L.cuOptAmdBatchSolve.argtypes = [i32, vp, vp, vp]
@@ -407,6 +409,24 @@ def spmv(row_offsets, col_indices, values, x, ncols: int, precision: str = "fp64
     _check(lib().cuOptAmdSpmv(m, ncols, _p(ptr), _p(idx), _p(val), _p(x), _p(y), PRECISION[precision]),
            "cuOptAmdSpmv")
     return y[:m]
+
+
+def spmv_noalloc(row_offsets, col_indices, values, x, ncols: int, precision: str = "fp64") -> np.ndarray:
+    """y = A x through cuOptAmdSpmvNoAlloc: the same kernels as :func:`spmv`, with every
+    device buffer in the library's fixed per-GPU array (no device allocation)."""
+    ptr, idx, val = _a(row_offsets, np.int32), _a(col_indices, np.int32), _a(values, np.float64)
+    x = _a(x, np.float64)
+    m = len(ptr) - 1
+    y = np.zeros(max(m, 1))
+    _check(lib().cuOptAmdSpmvNoAlloc(m, ncols, _p(ptr), _p(idx), _p(val), _p(x), _p(y), PRECISION[precision]),
+           "cuOptAmdSpmvNoAlloc")
+    return y[:m]
+
+
+def spmv_scratch_bytes() -> int:
+    b = ctypes.c_size_t(0)
+    _check(lib().cuOptAmdSpmvScratchBytes(ctypes.byref(b)), "cuOptAmdSpmvScratchBytes")
+    return int(b.value)
 
 
 BENCH_NAMES = ["window_ms", "primal_kernel_ms", "dual_kernel_ms", "iterations", "setup_s", "checks", "status",

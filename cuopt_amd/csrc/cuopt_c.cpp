@@ -919,6 +919,43 @@ cuopt_int_t cuOptAmdSpmv(cuopt_int_t m,
   return CUOPT_SUCCESS;
 }
 
+cuopt_int_t cuOptAmdSpmvNoAlloc(cuopt_int_t m,
+                                cuopt_int_t n,
+                                const cuopt_int_t* ptr,
+                                const cuopt_int_t* idx,
+                                const cuopt_float_t* val,
+                                const cuopt_float_t* x,
+                                cuopt_float_t* y,
+                                cuopt_int_t precision)
+{
+  if (m < 0 || n < 0 || !ptr || !idx || !val || !x || !y) return CUOPT_INVALID_ARGUMENT;
+  if (precision != CUOPT_AMD_PRECISION_FP64 && precision != CUOPT_AMD_PRECISION_FP32_MIXED)
+    return CUOPT_INVALID_ARGUMENT;
+  if (ptr[0] != 0) return CUOPT_INVALID_ARGUMENT;
+  for (int i = 0; i < m; ++i)
+    if (ptr[i + 1] < ptr[i]) return CUOPT_INVALID_ARGUMENT;
+  for (int k = 0; k < ptr[m]; ++k)
+    if (idx[k] < 0 || idx[k] >= n) return CUOPT_INVALID_ARGUMENT;
+  try {
+    tiled_spmv_host_noalloc(m, n, ptr, idx, val, x, y, precision);
+  } catch (const validation_error& e) {
+    std::fprintf(stderr, "%s\n", e.what());
+    return CUOPT_INVALID_ARGUMENT;
+  } catch (const std::bad_alloc&) {
+    return CUOPT_OUT_OF_MEMORY;
+  } catch (...) {
+    return CUOPT_RUNTIME_ERROR;
+  }
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptAmdSpmvScratchBytes(size_t* bytes)
+{
+  if (!bytes) return CUOPT_INVALID_ARGUMENT;
+  *bytes = kSpmvScratchBytes;
+  return CUOPT_SUCCESS;
+}
+
 cuopt_int_t cuOptAmdBenchmarkSteps(cuOptOptimizationProblem problem,
                                    cuOptSolverSettings settings,
                                    cuopt_int_t warmup,

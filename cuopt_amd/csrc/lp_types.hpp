@@ -97,6 +97,14 @@ void constraint_bounds(const LpHost& lp, std::vector<double>& lc, std::vector<do
 void tiled_spmv_host(int m, int n, const int* rowptr, const int* colidx, const double* vals,
                      const double* x, double* y, int precision);
 
+// Device scratch of tiled_spmv_host_noalloc, a fixed array per GPU.
+constexpr size_t kSpmvScratchBytes = size_t(64) << 20;
+
+// The same SpMV with every device buffer in that fixed array: no device
+// allocation.  Throws validation_error when the staged data exceed it.
+void tiled_spmv_host_noalloc(int m, int n, const int* rowptr, const int* colidx, const double* vals,
+                             const double* x, double* y, int precision);
+
 int device_count();
 
 }  // namespace cuopt_amd

@@ -19,6 +19,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <random>
 #include <string>
 #include <type_traits>
@@ -686,25 +687,34 @@ void build_plan(MatStruct& M, hipStream_t s, const int* colidx = nullptr)
   M.plan.nlong = (int)p.longrows.size();
 }
 
+// The tiled SpMV launch (plus the long-row merge) over raw device pointers:
+// ptr / idx the CSR structure, plan the tile plan, long_part one partial per tile.
+template <class Epi>
+void launch_tiled_raw(int tile, const int* ptr, const int* idx, const typename Epi::Mat* vals, PlanDev plan,
+                      int nlong, uint64_t* long_part, double* red, Epi epi, hipStream_t s)
+{
+  if (plan.ntiles == 0) return;
+  if (tile == 512)
+    hipLaunchKernelGGL((tiled_spmv_kernel<Epi, 512>), dim3(plan.ntiles), dim3(kBlock), 0, s, ptr, idx, vals, plan,
+                       long_part, red, epi);
+  else if (tile == 1024)
+    hipLaunchKernelGGL((tiled_spmv_kernel<Epi, 1024>), dim3(plan.ntiles), dim3(kBlock), 0, s, ptr, idx, vals, plan,
+                       long_part, red, epi);
+  else
+    hipLaunchKernelGGL((tiled_spmv_kernel<Epi, 2048>), dim3(plan.ntiles), dim3(kBlock), 0, s, ptr, idx, vals, plan,
+                       long_part, red, epi);
+  CUOPT_HIP_CHECK(hipGetLastError());
+  if (nlong > 0) {
+    hipLaunchKernelGGL(long_rows_kernel<Epi>, dim3(nlong), dim3(kBlock), 0, s, plan, long_part, red, epi);
+    CUOPT_HIP_CHECK(hipGetLastError());
+  }
+}
+
 template <class Epi>
 void launch_tiled(const MatStruct& M, const typename Epi::Mat* vals, double* red, Epi epi, hipStream_t s)
 {
-  if (M.plan.ntiles == 0) return;
-  if (M.tile == 512)
-    hipLaunchKernelGGL((tiled_spmv_kernel<Epi, 512>), dim3(M.plan.ntiles), dim3(kBlock), 0, s, M.ptr.get(),
-                       M.idx.get(), vals, M.plan.dev(), M.plan.long_part.get(), red, epi);
-  else if (M.tile == 1024)
-    hipLaunchKernelGGL((tiled_spmv_kernel<Epi, 1024>), dim3(M.plan.ntiles), dim3(kBlock), 0, s, M.ptr.get(),
-                       M.idx.get(), vals, M.plan.dev(), M.plan.long_part.get(), red, epi);
-  else
-    hipLaunchKernelGGL((tiled_spmv_kernel<Epi, 2048>), dim3(M.plan.ntiles), dim3(kBlock), 0, s, M.ptr.get(),
-                       M.idx.get(), vals, M.plan.dev(), M.plan.long_part.get(), red, epi);
-  CUOPT_HIP_CHECK(hipGetLastError());
-  if (M.plan.nlong > 0) {
-    hipLaunchKernelGGL(long_rows_kernel<Epi>, dim3(M.plan.nlong), dim3(kBlock), 0, s, M.plan.dev(),
-                       M.plan.long_part.get(), red, epi);
-    CUOPT_HIP_CHECK(hipGetLastError());
-  }
+  launch_tiled_raw(M.tile, M.ptr.get(), M.idx.get(), vals, M.plan.dev(), M.plan.nlong, M.plan.long_part.get(), red,
+                   epi, s);
 }
 
 void launch_reduce(const double* part, int nblocks, int nred, double* out, hipStream_t s,
@@ -2022,6 +2032,103 @@ void tiled_spmv_host(int m, int n, const int* rowptr, const int* colidx, const d
     spmv_impl<float>(m, n, rowptr, colidx, vals, x, y);
   else
     spmv_impl<double>(m, n, rowptr, colidx, vals, x, y);
+}
+
+// ---- the SpMV entry point that allocates no device memory -----------------------
+//
+// Everything the tiled SpMV keeps on the device besides nothing of the caller's
+// (the staged CSR arrays, x, y, the tile plan and the long-row partials) lives in
+// this fixed array, one copy per GPU, held until the process ends.  Host code
+// reaches it only through the address hipGetSymbolAddress returns for the current
+// device; the array's name itself is not a device address in host code.
+static __device__ __align__(256) unsigned char scratch[kSpmvScratchBytes];
+
+namespace {
+
+std::mutex g_scratch_mu;
+std::vector<unsigned char*> g_scratch_ptr;  // per device
+
+unsigned char* spmv_scratch_for_current_device()
+{
+  int dev = 0;
+  CUOPT_HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if ((int)g_scratch_ptr.size() <= dev) g_scratch_ptr.resize(dev + 1, nullptr);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    CUOPT_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  return g_scratch_ptr[dev];
+}
+
+template <typename V>
+void spmv_noalloc_impl(int m, int n, const int* rowptr, const int* colidx, const double* vals, const double* x,
+                       double* y)
+{
+  const int nnz  = rowptr[m];
+  const int tile = 1024;
+  TilePlan p = plan_tiles(rowptr, m, tile - kVecPad, kMaxRowsPerTile, kBlock, colidx, n);
+  // Layout of the staged data, each region 256-byte aligned.
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) & ~(size_t)255;
+    return o;
+  };
+  const size_t ntiles = p.tiles.size(), nlong = p.longrows.size();
+  const size_t o_ptr = take(sizeof(int) * ((size_t)m + 1)), o_idx = take(sizeof(int) * ((size_t)nnz + kArrayPad));
+  const size_t o_val = take(sizeof(V) * ((size_t)nnz + kArrayPad)), o_x = take(sizeof(V) * std::max(n, 1));
+  const size_t o_y = take(sizeof(double) * std::max(m, 1)), o_tiles = take(sizeof(Tile) * std::max<size_t>(ntiles, 1));
+  const size_t o_meta = take(sizeof(int) * std::max<size_t>(ntiles, 1));
+  const size_t o_long = take(sizeof(LongRow) * std::max<size_t>(nlong, 1));
+  const size_t o_part = take(sizeof(uint64_t) * std::max<size_t>(ntiles, 1));
+  if (off > kSpmvScratchBytes)
+    throw validation_error("cuOptAmdSpmvNoAlloc: the problem needs " + std::to_string(off) +
+                           " bytes of device scratch, more than its fixed " + std::to_string(kSpmvScratchBytes));
+  unsigned char* base = spmv_scratch_for_current_device();
+  int* d_ptr = reinterpret_cast<int*>(base + o_ptr);
+  int* d_idx = reinterpret_cast<int*>(base + o_idx);
+  V* d_val   = reinterpret_cast<V*>(base + o_val);
+  V* d_x     = reinterpret_cast<V*>(base + o_x);
+  double* d_y       = reinterpret_cast<double*>(base + o_y);
+  Tile* d_tiles     = reinterpret_cast<Tile*>(base + o_tiles);
+  int* d_meta       = reinterpret_cast<int*>(base + o_meta);
+  LongRow* d_long   = reinterpret_cast<LongRow*>(base + o_long);
+  uint64_t* d_part  = reinterpret_cast<uint64_t*>(base + o_part);
+  // The per-thread default stream: nothing is created.  The array keeps the last
+  // call's contents, so the padding the tiles read past the end is cleared here.
+  hipStream_t s = hipStreamPerThread;
+  std::vector<V> hv(vals, vals + nnz), hx(x, x + n);
+  CUOPT_HIP_CHECK(hipMemcpyAsync(d_ptr, rowptr, sizeof(int) * ((size_t)m + 1), hipMemcpyHostToDevice, s));
+  CUOPT_HIP_CHECK(hipMemsetAsync(d_idx + nnz, 0, sizeof(int) * kArrayPad, s));
+  CUOPT_HIP_CHECK(hipMemsetAsync(d_val + nnz, 0, sizeof(V) * kArrayPad, s));
+  if (nnz) {
+    CUOPT_HIP_CHECK(hipMemcpyAsync(d_idx, colidx, sizeof(int) * (size_t)nnz, hipMemcpyHostToDevice, s));
+    CUOPT_HIP_CHECK(hipMemcpyAsync(d_val, hv.data(), sizeof(V) * (size_t)nnz, hipMemcpyHostToDevice, s));
+  }
+  if (n) CUOPT_HIP_CHECK(hipMemcpyAsync(d_x, hx.data(), sizeof(V) * (size_t)n, hipMemcpyHostToDevice, s));
+  if (ntiles) {
+    CUOPT_HIP_CHECK(hipMemcpyAsync(d_tiles, p.tiles.data(), sizeof(Tile) * ntiles, hipMemcpyHostToDevice, s));
+    CUOPT_HIP_CHECK(hipMemcpyAsync(d_meta, p.meta.data(), sizeof(int) * ntiles, hipMemcpyHostToDevice, s));
+  }
+  if (nlong)
+    CUOPT_HIP_CHECK(hipMemcpyAsync(d_long, p.longrows.data(), sizeof(LongRow) * nlong, hipMemcpyHostToDevice, s));
+  launch_tiled_raw(tile, d_ptr, d_idx, (const V*)d_val, PlanDev{d_tiles, d_meta, d_long, (int)ntiles}, (int)nlong,
+                   d_part, nullptr, PlainEpi<V>{d_x, d_y}, s);
+  if (m) CUOPT_HIP_CHECK(hipMemcpyAsync(y, d_y, sizeof(double) * (size_t)m, hipMemcpyDeviceToHost, s));
+  CUOPT_HIP_CHECK(hipStreamSynchronize(s));
+}
+
+}  // namespace
+
+void tiled_spmv_host_noalloc(int m, int n, const int* rowptr, const int* colidx, const double* vals, const double* x,
+                             double* y, int precision)
+{
+  if (precision == CUOPT_AMD_PRECISION_FP32_MIXED)
+    spmv_noalloc_impl<float>(m, n, rowptr, colidx, vals, x, y);
+  else
+    spmv_noalloc_impl<double>(m, n, rowptr, colidx, vals, x, y);
 }
 
 int device_count()

@@ -7,6 +7,8 @@
 #ifndef CUOPT_AMD_EXT_H
 #define CUOPT_AMD_EXT_H
 
+#include <stddef.h>
+
 #include "cuopt_c.h"
 
 #ifdef __cplusplus
@@ -96,6 +98,28 @@ cuopt_int_t cuOptAmdSpmv(cuopt_int_t num_rows,
                          const cuopt_float_t* x,
                          cuopt_float_t* y,
                          cuopt_int_t precision);
+
+/* cuOptAmdSpmv without device allocation (no counterpart in the reference): for
+ * callers that cannot allocate device memory (tight loops, code that must not call
+ * hipMalloc).  Same arguments and result as cuOptAmdSpmv, bitwise; every device
+ * buffer of the call (the staged CSR arrays, x, y, the tile plan and the long-row
+ * partials) lives in one fixed device array per GPU of cuOptAmdSpmvScratchBytes
+ * (64 MiB), held until the process ends.  A problem whose staged data exceed it
+ * is refused with CUOPT_INVALID_ARGUMENT: (m + 1) * 4 + (nnz + 16) * (4 + s) +
+ * n * s + m * 8 bytes plus at most 44 bytes per 1 020 nonzeros, s = 8 (FP64) or
+ * 4 (FP32-mixed), each part rounded up to 256 bytes.  Runs on the calling
+ * thread's default stream and returns after y is written.  Calls that overlap on
+ * one GPU share the array and race. */
+cuopt_int_t cuOptAmdSpmvNoAlloc(cuopt_int_t num_rows,
+                                cuopt_int_t num_cols,
+                                const cuopt_int_t* row_offsets,
+                                const cuopt_int_t* column_indices,
+                                const cuopt_float_t* values,
+                                const cuopt_float_t* x,
+                                cuopt_float_t* y,
+                                cuopt_int_t precision);
+/* Size in bytes of the fixed device array of cuOptAmdSpmvNoAlloc. */
+cuopt_int_t cuOptAmdSpmvScratchBytes(size_t* bytes);
 
 /* Benchmark of the PDHG loop (no reference counterpart).  Runs the full solve
  * setup and `warmup` PDHG iterations, then times `steps` iterations between two
