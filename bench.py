@@ -67,6 +67,36 @@ def copy_calibration(mib: int) -> None:
     print(f"[bench] copy calibration: fill of {mib} MiB, then copy of {mib} MiB", file=sys.stderr, flush=True)
 
 
+def stream_bandwidth(mib: int = 1024, reps: int = 10) -> dict:
+    """Achievable HBM bandwidth, STREAM style: torch's copy (b = a) and triad
+    (c = a + s b) kernels over `mib` MiB arrays, timed with HIP events; the best
+    of `reps` runs, counting the bytes each kernel must move (copy 2x, triad 3x)."""
+    import torch
+
+    n = (mib << 20) // 4
+    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    b = torch.ones(n, device="cuda", dtype=torch.float32)
+    c = torch.empty(n, device="cuda", dtype=torch.float32)
+    out = {}
+    for name, fn, moved in (("copy", lambda: c.copy_(a), 2), ("triad", lambda: torch.add(a, b, alpha=0.5, out=c), 3)):
+        fn()
+        best = None
+        for _ in range(reps):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            e1.synchronize()
+            ms = e0.elapsed_time(e1)
+            best = ms if best is None else min(best, ms)
+        out[name + "_GBps"] = round(moved * n * 4 / (best * 1e-3) / 1e9, 1)
+    del a, b, c
+    torch.cuda.empty_cache()
+    out["array_MiB"] = mib
+    print(f"[bench] stream: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def pmc_traffic(kernel: str):
     """HBM bytes per launch of `kernel` from the rocprofv3 --pmc summary committed in
     profiles/pmc_traffic.json (FETCH_SIZE and WRITE_SIZE passes scaled by a plain
@@ -190,9 +220,18 @@ def batch_extras(c_api, G, count: int, sample: int, time_limit: float) -> dict:
         errs = [abs(r.primal_objective - lp.obj_opt) / (1 + abs(lp.obj_opt)) for r, lp in zip(rs, lps)
                 if r.status == 1]
         its = sorted(r.iterations for r in rs)
+        # Roofline of the persistent launch: every LP's algorithmic bytes per PDHG iteration
+        # (B_iter at its own sizes) times its iterations, over the launch's event time.
+        sv = 4 if prec == "fp32_mixed" else 8
+        moved = sum(kernel_bytes(lp.m, lp.n, lp.nnz, sv)["iteration"] * r.iterations for lp, r in zip(lps, rs))
+        kern = rs[0].info["loop_time"]
         out[prec] = {"wall_s": round(wall, 4), "solve_time_s": round(rs[0].solve_time, 4), "solved": solved,
                      "lps_per_s": round(solved / wall, 1), "max_obj_rel_err": max(errs) if errs else None,
-                     "iterations_p50_max": [its[len(its) // 2], its[-1]], "total_iterations": int(sum(its))}
+                     "iterations_p50_max": [its[len(its) // 2], its[-1]], "total_iterations": int(sum(its)),
+                     "kernel_s": round(kern, 4),
+                     "roofline": {"bound": "hbm", "achieved": round(moved / kern / 1e9, 1), "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": round(moved / kern / 1e9 / HBM_PEAK_GBS, 4),
+                                  "algorithmic_bytes": moved}}
         print(f"[bench] batch {prec}: {out[prec]}", file=sys.stderr, flush=True)
     t = time.time()
     seq = [c_api.solve(lp, tol=1e-4, precision="fp32_mixed", time_limit=time_limit) for lp in lps[:sample]]
@@ -200,6 +239,17 @@ def batch_extras(c_api, G, count: int, sample: int, time_limit: float) -> dict:
     out["sequential_sample"] = sample
     out["sequential_lps_per_s"] = round(sum(r.status == 1 for r in seq) / seq_wall, 2)
     out["speedup_vs_sequential"] = round(out["fp32_mixed"]["lps_per_s"] / max(out["sequential_lps_per_s"], 1e-9), 1)
+    # The reference's batch loop (cython_solve.cu:286-352): at most four single-LP solves at a
+    # time, each on its own stream (one solver per host thread; the C calls release the GIL).
+    from concurrent.futures import ThreadPoolExecutor
+
+    t = time.time()
+    with ThreadPoolExecutor(max_workers=4) as ex:
+        four = list(ex.map(lambda lp: c_api.solve(lp, tol=1e-4, precision="fp32_mixed", time_limit=time_limit),
+                           lps[:sample]))
+    four_wall = time.time() - t
+    out["four_stream_lps_per_s"] = round(sum(r.status == 1 for r in four) / four_wall, 2)
+    out["speedup_vs_four_stream"] = round(out["fp32_mixed"]["lps_per_s"] / max(out["four_stream_lps_per_s"], 1e-9), 1)
     print(f"[bench] batch: {out}", file=sys.stderr, flush=True)
     return out
 
@@ -338,6 +388,14 @@ def main() -> int:
         "iteration_GBps": round(kb["iteration"] / ((w["primal_kernel_ms"] + w["dual_kernel_ms"]) * 1e-3) / 1e9, 1),
     }
 
+    stream = None
+    try:
+        stream = stream_bandwidth()
+        roofline["achievable_GBps"] = stream["copy_GBps"]
+        roofline["frac_of_achievable"] = round(achieved / stream["copy_GBps"], 4)
+    except Exception as e:  # never hides the headline line
+        print(f"[bench] stream failed: {e!r}", file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
@@ -362,9 +420,10 @@ def main() -> int:
     other = {p: {"iters_per_s": round(1000.0 * b["iterations"] / b["window_ms"], 1),
                  "primal_kernel_us": round(b["primal_kernel_ms"] * 1e3, 2),
                  "dual_kernel_us": round(b["dual_kernel_ms"] * 1e3, 2)} for p, b in windows.items()}
-    speedup = obj_diff = None
+    speedup = loop_ratio = obj_diff = None
     if "fp32_mixed" in solves and "fp64" in solves:
         speedup = round(solves["fp64"]["time_to_tol_s"] / solves["fp32_mixed"]["time_to_tol_s"], 3)
+        loop_ratio = round(solves["fp64"]["loop_s"] / solves["fp32_mixed"]["loop_s"], 3)
         o64 = solves["fp64"]["objective"]
         obj_diff = abs(solves["fp32_mixed"]["objective"] - o64) / (1.0 + abs(o64))
     out = {
@@ -374,8 +433,9 @@ def main() -> int:
         "config": {"workload": "skewed LP m=500000 n=1000000 nnz=%d, Zipf(1.1) rows, 1+Poisson(4) per column, "
                                "half = / half <=, seed 0" % lp.nnz, "precision": head, "parallelism": "1 GPU"},
         "windows": other, "time_to_tolerance": solves, "fp64_over_fp32_time_ratio": speedup,
+        "fp64_over_fp32_loop_ratio": loop_ratio,
         "fp32_vs_fp64_objective_rel_diff": obj_diff,
-        "roofline": roofline, "cpu_baseline": cpu, "other_workloads": extras,
+        "roofline": roofline, "stream": stream, "cpu_baseline": cpu, "other_workloads": extras,
     }
     print(json.dumps(out))
     return 0

@@ -100,7 +100,12 @@ struct BatchArgs {
   V* kink;
   double tol_ap, tol_rp, tol_ad, tol_rd, tol_ag, tol_rg;
   long long iter_limit;
-  unsigned long long tick_limit;  // wall-clock ticks from kernel start (0 = none)
+  // Per-LP time limit in wall-clock ticks (0 = none), counted from the moment the
+  // LP's workgroup starts: each LP gets the whole limit from its own dispatch, as
+  // each of the reference's per-LP solve_lp calls starts its own timer.  With more
+  // LPs than resident workgroups, later LPs start later, so the launch can take
+  // longer than the limit (at most about the number of dispatch waves times it).
+  unsigned long long tick_limit;
   const int* blk;                 // row-block starts of every LP's A and A^T (plan_row_blocks)
   int lds_bytes;                  // dynamic LDS of the launch: the stage, then the gathered vectors
 };
@@ -1171,10 +1176,24 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
     if (cap < kStageBytes) throw std::runtime_error("batch solve: the device has too little LDS per workgroup");
     args.lds_bytes = (int)std::min<long long>(kStageBytes + need, cap);
   }
+  // HIP events around the persistent launch: its device time is the batch's loop time.
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  struct EventGuard {
+    hipEvent_t* e;
+    ~EventGuard()
+    {
+      for (int i = 0; i < 2; ++i)
+        if (e[i]) (void)hipEventDestroy(e[i]);
+    }
+  } ev_guard{ev};
+  CUOPT_HIP_CHECK(hipEventCreate(&ev[0]));
+  CUOPT_HIP_CHECK(hipEventCreate(&ev[1]));
+  CUOPT_HIP_CHECK(hipEventRecord(ev[0], s));
   if (nb > 0) {
     hipLaunchKernelGGL(batch_pdlp_kernel<V>, dim3(nb), dim3(kBT), (unsigned)args.lds_bytes, s, args);
     CUOPT_HIP_CHECK(hipGetLastError());
   }
+  CUOPT_HIP_CHECK(hipEventRecord(ev[1], s));
   std::vector<BatchOut> h_out(std::max<size_t>(lps.size(), 1));
   std::vector<double> h_xu(N), h_yu(M), h_su(N);
   out.download(h_out.data(), h_out.size(), s);
@@ -1183,6 +1202,8 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
   su.download(h_su.data(), N, s);
   CUOPT_HIP_CHECK(hipStreamSynchronize(s));
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  float kernel_ms   = 0.f;
+  CUOPT_HIP_CHECK(hipEventElapsedTime(&kernel_ms, ev[0], ev[1]));
   for (int a : active) {
     const LpHost& lp   = *lps[a];
     const BatchDesc& d = desc[a];
@@ -1209,6 +1230,7 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
     r.info[CUOPT_AMD_INFO_GAP]          = o.gap;
     r.info[CUOPT_AMD_INFO_REL_GAP]      = o.gap / (1.0 + o.absobj);
     r.info[CUOPT_AMD_INFO_SOLVE_TIME]   = secs;
+    r.info[CUOPT_AMD_INFO_LOOP_TIME]    = 1e-3 * kernel_ms;  // the whole persistent launch
     r.info[CUOPT_AMD_INFO_ETA0]         = o.eta;
     r.info[CUOPT_AMD_INFO_OMEGA0]       = 1.0;
     r.info[CUOPT_AMD_INFO_OMEGA]        = o.omega;

@@ -58,7 +58,8 @@ extern "C" {
 #define CUOPT_AMD_INFO_REL_GAP             8
 #define CUOPT_AMD_INFO_SOLVE_TIME          9  /* seconds, from solve entry */
 #define CUOPT_AMD_INFO_SETUP_TIME          10 /* seconds spent before the first PDHG step */
-#define CUOPT_AMD_INFO_LOOP_TIME           11 /* seconds in the PDHG loop (incl. checks) */
+#define CUOPT_AMD_INFO_LOOP_TIME           11 /* seconds in the PDHG loop (incl. checks); batch solves: the
+                                                 persistent launch's device time (HIP events) */
 #define CUOPT_AMD_INFO_ETA0                12 /* initial step size */
 #define CUOPT_AMD_INFO_OMEGA0              13 /* initial primal weight */
 #define CUOPT_AMD_INFO_OMEGA               14 /* final primal weight */

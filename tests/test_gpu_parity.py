@@ -660,3 +660,21 @@ def test_batch_iteration_limit():
     lps = [G.random_lp(m=100, n=250, seed=s) for s in range(4)]
     rs = c_api.batch_solve(lps, tol=0.0, time_limit=TL, iteration_limit=50)
     assert all(r.status == 4 and r.iterations == 50 for r in rs)
+
+
+def test_batch_time_limit_counts_per_lp_from_its_dispatch():
+    # time_limit applies to each LP from the start of its own workgroup (as each of the
+    # reference's per-LP solve_lp calls starts its own timer).  A batch four times the
+    # resident capacity (two workgroups per CU) of LPs that cannot converge (tolerance 0)
+    # ends every LP with TimeLimit, and the call takes about one limit per dispatch wave.
+    import torch
+
+    cus = torch.cuda.get_device_properties(0).multi_processor_count
+    count, limit = 8 * cus, 0.05
+    waves = -(-count // (2 * cus))
+    lps = [G.random_lp(m=50, n=125, seed=s) for s in range(count)]
+    print(f"batch time limit: {count} LPs, {waves} waves, limit {limit}s", flush=True)
+    rs = c_api.batch_solve(lps, tol=0.0, time_limit=limit, iteration_limit=10**9)
+    assert all(r.status == c_api.CUOPT_TERIMINATION_STATUS_TIME_LIMIT for r in rs)
+    kernel = rs[0].info["loop_time"]
+    assert limit <= kernel <= (waves + 1) * limit * 1.5 + 0.5, (kernel, waves)
