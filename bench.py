@@ -312,6 +312,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-simplex", action="store_true")
     ap.add_argument("--batch-count", type=int, default=4096)
     ap.add_argument("--batch-sequential-sample", type=int, default=64)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="run the headline workload without the SpMV autotuner (pdlp_spmv_autotune)")
     ap.add_argument("--simplex-time-limit", type=float, default=30.0,
                     help="time limit (s) of the HiGHS dual simplex beside the small random LP (0: skip)")
     ap.add_argument("--only", choices=("config0", "transport", "batch", "unrolled"),
@@ -346,9 +348,20 @@ def main() -> int:
     print(f"[bench] generated skewed LP m={lp.m} n={lp.n} nnz={lp.nnz} in {gen_s:.1f}s", file=sys.stderr, flush=True)
 
     precs = args.precisions.split(",")
+    tune = {} if args.no_autotune else {"pdlp_spmv_autotune": 1}
+    tuning = {}
+    if tune:
+        # The autotuner's first run on this matrix (timed in setup, then cached for the process).
+        for p in precs:
+            r = c_api.solve(lp, tol=0.0, precision=p, iteration_limit=0, time_limit=args.solve_time_limit,
+                            params=tune)
+            tuning[p] = {"tune_time_s": round(r.info["tune_time"], 4),
+                         "A": {"tile": int(r.info["tile_a"]), "lanes_shift": int(r.info["lanes_bias_a"])},
+                         "A_T": {"tile": int(r.info["tile_at"]), "lanes_shift": int(r.info["lanes_bias_at"])}}
+            print(f"[bench] spmv autotune {p}: {tuning[p]}", file=sys.stderr, flush=True)
     windows = {}
     for p in precs:
-        b = c_api.benchmark_steps(lp, args.warmup, args.steps, precision=p, tol=0.0)
+        b = c_api.benchmark_steps(lp, args.warmup, args.steps, precision=p, tol=0.0, params=tune)
         windows[p] = b
         print(f"[bench] {p}: {int(b['iterations'])} steps in {b['window_ms']:.1f} ms "
               f"(primal {b['primal_kernel_ms'] * 1e3:.1f} us, dual {b['dual_kernel_ms'] * 1e3:.1f} us, "
@@ -357,7 +370,7 @@ def main() -> int:
     solves = {}
     if not args.no_solves:
         for p in precs:
-            r = c_api.solve(lp, tol=args.tol, precision=p, time_limit=args.solve_time_limit)
+            r = c_api.solve(lp, tol=args.tol, precision=p, time_limit=args.solve_time_limit, params=tune)
             err = abs(r.primal_objective - lp.obj_opt) / (1.0 + abs(lp.obj_opt))
             solves[p] = {
                 "status": c_api.STATUS.get(r.status, r.status), "iterations": r.iterations,
@@ -434,7 +447,7 @@ def main() -> int:
                                "half = / half <=, seed 0" % lp.nnz, "precision": head, "parallelism": "1 GPU"},
         "windows": other, "time_to_tolerance": solves, "fp64_over_fp32_time_ratio": speedup,
         "fp64_over_fp32_loop_ratio": loop_ratio,
-        "fp32_vs_fp64_objective_rel_diff": obj_diff,
+        "fp32_vs_fp64_objective_rel_diff": obj_diff, "spmv_autotune": tuning or None,
         "roofline": roofline, "stream": stream, "cpu_baseline": cpu, "other_workloads": extras,
     }
     print(json.dumps(out))

@@ -34,7 +34,7 @@ INFO_NAMES = [
     "l2_relative_dual_residual", "primal_objective", "dual_objective", "gap", "relative_gap",
     "solve_time", "setup_time", "loop_time", "eta0", "omega0", "omega", "restarts", "power_iterations",
     "sigma_max_sq", "beta", "gamma", "precision", "checks", "structure_time", "scaling_time", "power_time",
-    "refine_rounds",
+    "refine_rounds", "tile_a", "lanes_bias_a", "tile_at", "lanes_bias_at", "tune_time",
 ]
 TRACE_WIDTH = 10
 PRECISION = {"fp64": 0, "fp32_mixed": 1, "mixed": 1, "fp32": 1}
@@ -340,11 +340,15 @@ def read_lp(path: str):
 
 def solve(lp, tol: float = 1e-4, precision: str = "fp64", iteration_limit: int | None = None,
           time_limit: float | None = None, x_init=None, y_init=None, method: int = CUOPT_METHOD_PDLP,
-          log: bool = False, refinement: bool | None = None, presolve: bool = False) -> Result:
+          log: bool = False, refinement: bool | None = None, presolve: bool = False,
+          params: dict | None = None) -> Result:
     """Solve ``lp`` (a generators.LP, or a :class:`Problem` such as one read by
-    ``Problem.read_mps``) through the C ABI with all six tolerances set to ``tol``."""
+    ``Problem.read_mps``) through the C ABI with all six tolerances set to ``tol``;
+    ``params`` sets further parameters by name."""
     prob = lp if isinstance(lp, Problem) else Problem(lp)
     s = make_settings(tol, precision, iteration_limit, time_limit, method, log, refinement, presolve)
+    for name, v in (params or {}).items():
+        _check(s.set(name, v), name)
     if x_init is not None:
         s.set_initial_primal(x_init)
     if y_init is not None:
@@ -433,10 +437,13 @@ BENCH_NAMES = ["window_ms", "primal_kernel_ms", "dual_kernel_ms", "iterations", 
                "precision", "event_iterations"]
 
 
-def benchmark_steps(lp, warmup: int, steps: int, precision: str = "fp32_mixed", tol: float = 0.0) -> dict:
+def benchmark_steps(lp, warmup: int, steps: int, precision: str = "fp32_mixed", tol: float = 0.0,
+                    params: dict | None = None) -> dict:
     """Times exactly ``steps`` PDHG iterations after ``warmup`` (cuOptAmdBenchmarkSteps)."""
     prob = Problem(lp)
     s = SolverSettings()
+    for name, v in (params or {}).items():
+        _check(s.set(name, v), name)
     for name in ("absolute_primal_tolerance", "relative_primal_tolerance", "absolute_dual_tolerance",
                  "relative_dual_tolerance", "absolute_gap_tolerance", "relative_gap_tolerance"):
         _check(s.set(name, float(tol)), name)

@@ -624,6 +624,12 @@ __global__ __launch_bounds__(kBlock) void transpose_gather_kernel(int nnz, const
   }
 }
 
+// idx[k] <- label[idx[k]] (column relabelling).
+__global__ __launch_bounds__(kBlock) void relabel_kernel(int nnz, const int* __restrict__ label, int* __restrict__ idx)
+{
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < nnz; k += gridDim.x * kBlock) idx[k] = label[idx[k]];
+}
+
 // tptr[j] = first position of column j among the sorted column keys (j in [0, n]).
 __global__ __launch_bounds__(kBlock) void column_ptr_kernel(int n, int nnz, const int* __restrict__ cols,
                                                             int* __restrict__ tptr)
@@ -669,22 +675,73 @@ void device_transpose(int m, int n, int nnz, const int* ptr, const int* idx, con
   CUOPT_HIP_CHECK(hipStreamSynchronize(s));  // temporaries are freed on return
 }
 
+// Column order for the gathers of A x-bar: columns are numbered in the order in
+// which rows of at most `short_len` nonzeros first touch them (rows in index
+// order, columns in row order), then the untouched columns in index order.  The
+// nonzeros of consecutive short rows, which one workgroup reduces together, then
+// gather from nearby addresses of x-bar; rows longer than a tile gather over
+// sorted columns either way.  perm[k] is the original column of new column k.
+// Returns false (perm untouched) when the order is the identity.
+bool first_touch_order(int m, int n, const int* rowptr, const int* colidx, int short_len, std::vector<int>& perm)
+{
+  std::vector<int> label(n, -1);
+  perm.assign(n, 0);
+  int next = 0;
+  for (int i = 0; i < m; ++i) {
+    if (rowptr[i + 1] - rowptr[i] > short_len) continue;
+    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
+      const int j = colidx[k];
+      if (label[j] < 0) {
+        label[j]     = next;
+        perm[next++] = j;
+      }
+    }
+  }
+  for (int j = 0; j < n; ++j)
+    if (label[j] < 0) {
+      label[j]     = next;
+      perm[next++] = j;
+    }
+  for (int k = 0; k < n; ++k)
+    if (perm[k] != k) return true;
+  return false;
+}
+
 // colidx (host, sorted within rows) enables the XCD-aware tile order
 // (tile_plan.hpp); CUOPT_AMD_XCD_ORDER=0 turns it off for comparisons.
-void build_plan(MatStruct& M, hipStream_t s, const int* colidx = nullptr)
+void upload_plan(PlanBufs& P, const TilePlan& p, hipStream_t s)
+{
+  P.ntiles = (int)p.tiles.size();
+  P.tiles.resize(p.tiles.size());
+  P.tiles.upload(p.tiles, s);
+  P.meta.resize(p.meta.size());
+  P.meta.upload(p.meta, s);
+  P.longrows.resize(std::max<size_t>(1, p.longrows.size()));
+  if (!p.longrows.empty()) P.longrows.upload(p.longrows, s);
+  P.long_part.resize(std::max<size_t>(1, p.tiles.size()));
+  P.nlong = (int)p.longrows.size();
+}
+
+// Lanes per row of every row-run tile shifted by bias powers of two (clamped to
+// 1..64 lanes); chunk tiles are unchanged.
+void shift_lanes(TilePlan& p, int bias)
+{
+  for (int& v : p.meta)
+    if (v >= 0) v = std::min(6, std::max(0, v + bias));
+}
+
+TilePlan make_plan(const MatStruct& M, int tile, int bias, const int* colidx)
 {
   if (const char* e = std::getenv("CUOPT_AMD_XCD_ORDER"))
     if (std::atoi(e) == 0) colidx = nullptr;
-  TilePlan p = plan_tiles(M.h_ptr.data(), M.nrows, M.tile - kVecPad, kMaxRowsPerTile, kBlock, colidx, M.ncols);
-  M.plan.ntiles = (int)p.tiles.size();
-  M.plan.tiles.resize(p.tiles.size());
-  M.plan.tiles.upload(p.tiles, s);
-  M.plan.meta.resize(p.meta.size());
-  M.plan.meta.upload(p.meta, s);
-  M.plan.longrows.resize(std::max<size_t>(1, p.longrows.size()));
-  if (!p.longrows.empty()) M.plan.longrows.upload(p.longrows, s);
-  M.plan.long_part.resize(std::max<size_t>(1, p.tiles.size()));
-  M.plan.nlong = (int)p.longrows.size();
+  TilePlan p = plan_tiles(M.h_ptr.data(), M.nrows, tile - kVecPad, kMaxRowsPerTile, kBlock, colidx, M.ncols);
+  if (bias) shift_lanes(p, bias);
+  return p;
+}
+
+void build_plan(MatStruct& M, hipStream_t s, const int* colidx = nullptr)
+{
+  upload_plan(M.plan, make_plan(M, M.tile, 0, colidx), s);
 }
 
 // The tiled SpMV launch (plus the long-row merge) over raw device pointers:
@@ -722,6 +779,100 @@ void launch_reduce(const double* part, int nblocks, int nred, double* out, hipSt
 {
   hipLaunchKernelGGL(reduce_partials_kernel, dim3(1), dim3(1024), 0, s, part, nblocks, nred, out, skip);
   CUOPT_HIP_CHECK(hipGetLastError());
+}
+
+// ---- SpMV autotuner ----------------------------------------------------------
+//
+// Candidates for one matrix: tile capacities of 512, 1 024 and 2 048 nonzeros,
+// each with the planner's lanes per row G and with G / 2 and 2 G.  Each is timed
+// the way the reference's batch-size handler times its candidates
+// (optimal_batch_size_handler.cu:196-435): one untimed SpMV, then five between
+// two HIP events; the least time wins (ties keep the earlier candidate).  The
+// timed SpMV is the solver's tiled kernel with a plain y = A x epilogue over the
+// working-precision values.  Choices are cached per matrix for the process.
+
+struct SpmvChoice {
+  int tile = 1024, bias = 0;
+};
+
+std::mutex g_tune_mu;
+std::map<std::vector<uint64_t>, SpmvChoice> g_tune_cache;
+
+uint64_t fnv1a(const int* p, size_t n)
+{
+  uint64_t h = 1469598103934665603ULL;
+  for (size_t i = 0; i < n; ++i) {
+    h ^= (uint32_t)p[i];
+    h *= 1099511628211ULL;
+  }
+  return h;
+}
+
+// Picks (and installs in M) the fastest candidate; returns whether it timed anything.
+template <typename V>
+bool autotune_spmv(MatStruct& M, const V* vals, const int* colidx, hipStream_t s, SpmvChoice& out)
+{
+  const std::vector<uint64_t> key{(uint64_t)M.nrows, (uint64_t)M.ncols, (uint64_t)M.nnz, sizeof(V),
+                                  fnv1a(M.h_ptr.data(), M.h_ptr.size())};
+  {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      out    = it->second;
+      M.tile = out.tile;
+      upload_plan(M.plan, make_plan(M, out.tile, out.bias, colidx), s);
+      return false;
+    }
+  }
+  DevBuf<V> x(std::max(M.ncols, 1));
+  DevBuf<double> y(std::max(M.nrows, 1));
+  x.zero(s);
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  struct EventGuard {
+    hipEvent_t* e;
+    ~EventGuard()
+    {
+      for (int i = 0; i < 2; ++i)
+        if (e[i]) (void)hipEventDestroy(e[i]);
+    }
+  } guard{ev};
+  CUOPT_HIP_CHECK(hipEventCreate(&ev[0]));
+  CUOPT_HIP_CHECK(hipEventCreate(&ev[1]));
+  float best_ms = -1.f;
+  PlanBufs best_plan;
+  for (int tile : {512, 1024, 2048}) {
+    const TilePlan base = make_plan(M, tile, 0, colidx);
+    for (int bias : {0, -1, 1}) {
+      TilePlan p = base;
+      if (bias) {
+        shift_lanes(p, bias);
+        if (p.meta == base.meta) continue;  // no row-run tile changes: the same candidate
+      }
+      PlanBufs P;
+      upload_plan(P, p, s);
+      auto run = [&]() {
+        launch_tiled_raw(tile, M.ptr.get(), M.idx.get(), vals, P.dev(), P.nlong, P.long_part.get(), nullptr,
+                         PlainEpi<V>{x.get(), y.get()}, s);
+      };
+      run();
+      CUOPT_HIP_CHECK(hipEventRecord(ev[0], s));
+      for (int r = 0; r < 5; ++r) run();
+      CUOPT_HIP_CHECK(hipEventRecord(ev[1], s));
+      CUOPT_HIP_CHECK(hipEventSynchronize(ev[1]));
+      float ms = 0.f;
+      CUOPT_HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+      if (best_ms < 0.f || ms < best_ms) {
+        best_ms   = ms;
+        out       = SpmvChoice{tile, bias};
+        best_plan = std::move(P);
+      }
+    }
+  }
+  M.tile = out.tile;
+  M.plan = std::move(best_plan);
+  std::lock_guard<std::mutex> lock(g_tune_mu);
+  g_tune_cache[key] = out;
+  return true;
 }
 
 // conditional_major (cpp/src/pdlp/utils.cuh:62-81)
@@ -858,6 +1009,28 @@ class PdlpSolver {
   MatStruct A_, AT_;
   DevBuf<double> a_orig_, at_orig_, a_s_, at_s_;
   DevBuf<V> a_w_, at_w_;
+  // Host column indices of A sorted within rows (the tile planner's XCD order).
+  std::vector<int> a_sorted_idx_;
+  const int* a_idx_host_ = nullptr;
+  // Column relabelling (empty: columns as given): col_perm_[k] is the original
+  // column of internal column k.  Per-column inputs are gathered through it at
+  // upload and the returned x and reduced costs scattered back.
+  std::vector<int> col_perm_;
+  template <typename T>
+  std::vector<T> to_internal(const std::vector<T>& v) const
+  {
+    if (col_perm_.empty()) return v;
+    std::vector<T> w(v.size());
+    for (size_t k = 0; k < col_perm_.size(); ++k) w[k] = v[col_perm_[k]];
+    return w;
+  }
+  void to_original(std::vector<double>& v) const
+  {
+    if (col_perm_.empty() || v.size() != col_perm_.size()) return;
+    std::vector<double> w(v.size());
+    for (size_t k = 0; k < col_perm_.size(); ++k) w[col_perm_[k]] = v[k];
+    v.swap(w);
+  }
 
   DevBuf<double> Dr_, Dc_, rowv_, colv_;
   DevBuf<double> c_orig_, lb_orig_, ub_orig_, lc_orig_, uc_orig_;
@@ -932,8 +1105,9 @@ void PdlpSolver<V>::build_structures()
     }
   }
   if (!sorted_idx.empty()) {
-    idx_src = sorted_idx.data();
-    val_src = sorted_val.data();
+    a_sorted_idx_ = std::move(sorted_idx);
+    idx_src       = a_sorted_idx_.data();
+    val_src       = sorted_val.data();
   }
   A_.nrows = m_;
   A_.ncols = n_;
@@ -957,14 +1131,41 @@ void PdlpSolver<V>::build_structures()
   A_.ptr.upload(A_.h_ptr, stream_);
   A_.idx.upload(idx_src, nnz, stream_);
   a_orig_.upload(val_src, nnz, stream_);
+  // Column order: as given, or first-touch (first_touch_order); automatic for
+  // matrices of at least 2^20 nonzeros.
+  const int order = st_.column_order;
+  col_perm_.clear();
+  if (order == 1 || (order < 0 && nnz >= (1 << 20))) {
+    std::vector<int> perm;
+    if (first_touch_order(m_, n_, lp_.rowptr.data(), idx_src, A_.tile - kVecPad, perm)) col_perm_ = std::move(perm);
+  }
+  if (!col_perm_.empty()) {
+    std::vector<int> label(n_);
+    for (int k = 0; k < n_; ++k) label[col_perm_[k]] = k;
+    DevBuf<int> d_label(n_);
+    d_label.upload(label, stream_);
+    hipLaunchKernelGGL(relabel_kernel, dim3(grid_for(nnz)), dim3(kBlock), 0, stream_, nnz, d_label.get(),
+                       A_.idx.get());
+    CUOPT_HIP_CHECK(hipGetLastError());
+  }
   // A^T on the device (compute_transpose_of_problem, problem.cu:472-489): a
   // stable radix sort of the nonzeros by column keeps each column's entries in
   // row order, as the host counting sort of the reference does.
   device_transpose(m_, n_, nnz, A_.ptr.get(), A_.idx.get(), a_orig_.get(), AT_.ptr.get(), AT_.idx.get(),
                    at_orig_.get(), A_.rowid.get(), AT_.rowid.get(), stream_);
+  if (!col_perm_.empty()) {
+    // Rows of A sorted by the new labels: the transpose of A^T (whose rows are the
+    // relabelled columns, each in row order) lists every row's entries by column.
+    device_transpose(n_, m_, nnz, AT_.ptr.get(), AT_.idx.get(), at_orig_.get(), A_.ptr.get(), A_.idx.get(),
+                     a_orig_.get(), AT_.rowid.get(), A_.rowid.get(), stream_);
+    a_sorted_idx_.resize(nnz);
+    A_.idx.download(a_sorted_idx_.data(), nnz, stream_);
+    idx_src = a_sorted_idx_.data();
+  }
   AT_.h_ptr.resize(n_ + 1);
   AT_.ptr.download(AT_.h_ptr.data(), n_ + 1, stream_);
   CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));  // host sources above are temporaries
+  a_idx_host_ = idx_src;
   build_plan(A_, stream_, idx_src);
   build_plan(AT_, stream_);
 }
@@ -1019,12 +1220,13 @@ void PdlpSolver<V>::setup_scaling()
   ub_orig_.resize(n_);
   lc_orig_.resize(m_);
   uc_orig_.resize(m_);
-  std::vector<double> cmin(lp_.c.begin(), lp_.c.end());
+  std::vector<double> cmin = to_internal(lp_.c);
   if (lp_.maximize)
     for (double& v : cmin) v = -v;  // convert_to_maximization_problem (problem_helpers.cuh:132-151)
+  const std::vector<double> lbi = to_internal(lb_), ubi = to_internal(ub_);
   c_orig_.upload(cmin, stream_);
-  lb_orig_.upload(lb_, stream_);
-  ub_orig_.upload(ub_, stream_);
+  lb_orig_.upload(lbi, stream_);
+  ub_orig_.upload(ubi, stream_);
   lc_orig_.upload(lc_, stream_);
   uc_orig_.upload(uc_, stream_);
   cs_.resize(n_);
@@ -1206,9 +1408,11 @@ void PdlpSolver<V>::init_iterates()
   su_.resize(n_);
   step_sc_.resize(1);
   DevBuf<double> xi, yi;
+  std::vector<double> x_init;
   if (!st_.initial_primal.empty()) {
+    x_init = to_internal(st_.initial_primal);
     xi.resize(n_);
-    xi.upload(st_.initial_primal.data(), n_, stream_);
+    xi.upload(x_init.data(), n_, stream_);
   }
   if (!st_.initial_dual.empty()) {
     yi.resize(m_);
@@ -1379,6 +1583,8 @@ void PdlpSolver<V>::finish(SolveResult& R, int status)
   yu_.download(R.y.data(), m_, stream_);
   su_.download(R.rc.data(), n_, stream_);
   CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
+  to_original(R.x);
+  to_original(R.rc);
   if (lp_.maximize) {  // solve.cu:583-588
     for (double& v : R.y) v = -v;
     for (double& v : R.rc) v = -v;
@@ -1435,6 +1641,21 @@ SolveResult PdlpSolver<V>::run()
     launch_ew(A_.nnz, nullptr, ConvertF<V>{a_s_.get(), a_w_.get()}, stream_);
     launch_ew(A_.nnz, nullptr, ConvertF<V>{at_s_.get(), at_w_.get()}, stream_);
   }
+  R.info[CUOPT_AMD_INFO_TUNE_TIME] = 0.0;
+  if (st_.spmv_autotune) {
+    const double t = elapsed();
+    SpmvChoice ca, cat;
+    const bool timed = autotune_spmv<V>(A_, a_w_.get(), a_idx_host_, stream_, ca);
+    const bool timed_t = autotune_spmv<V>(AT_, at_w_.get(), nullptr, stream_, cat);
+    CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
+    if (timed || timed_t) R.info[CUOPT_AMD_INFO_TUNE_TIME] = elapsed() - t;
+    R.info[CUOPT_AMD_INFO_LANES_BIAS_A]  = ca.bias;
+    R.info[CUOPT_AMD_INFO_LANES_BIAS_AT] = cat.bias;
+  } else {
+    R.info[CUOPT_AMD_INFO_LANES_BIAS_A] = R.info[CUOPT_AMD_INFO_LANES_BIAS_AT] = 0.0;
+  }
+  R.info[CUOPT_AMD_INFO_TILE_A]  = A_.tile;
+  R.info[CUOPT_AMD_INFO_TILE_AT] = AT_.tile;
   // Stable3: initial primal weight 1 (bound_objective_rescaling branch, pdlp.cu:2799-2807).
   omega_ = best_omega_ = 1.0;
   tau_   = eta_ / omega_;

@@ -71,7 +71,67 @@ Presolved presolve(const LpHost& lp, const std::vector<double>& lc0, const std::
       }
     return !cq.empty();
   };
-  while (!rq.empty() || !cq.empty() || recount_columns()) {
+  // Redundant rows: a row whose activity range over the current column bounds lies
+  // inside [lc, uc] can never bind, so it is dropped (its dual stays 0).  Dual
+  // fixing: a column whose cost does not rise when it decreases, and whose every
+  // entry only loosens its row when it decreases (a > 0 in a row without a lower
+  // bound, a < 0 in a row without an upper bound), takes its lower bound; the
+  // mirror case takes its upper bound.  Its reduced cost c - A^T y then has the
+  // sign of its bound by the signs of those rows' duals, so postsolve needs nothing
+  // more than the recomputed c - A^T y.
+  auto redundant_rows = [&]() {
+    bool changed = false;
+    for (int i = 0; i < m; ++i) {
+      if (!row_on[i] || rcount[i] <= 1) continue;
+      double lo = 0.0, hi = 0.0;
+      for (int k = lp.rowptr[i]; k < lp.rowptr[i + 1]; ++k) {
+        const int j    = lp.colidx[k];
+        const double a = lp.vals[k];
+        if (!col_on[j] || a == 0.0) continue;
+        lo += a > 0.0 ? a * lb[j] : a * ub[j];
+        hi += a > 0.0 ? a * ub[j] : a * lb[j];
+      }
+      if (!(lo >= lc[i] && hi <= uc[i])) continue;  // NaN (inf - inf) never qualifies
+      row_on[i] = 0;
+      ++P.removed_rows;
+      changed = true;
+      for (int k = lp.rowptr[i]; k < lp.rowptr[i + 1]; ++k) {
+        const int j = lp.colidx[k];
+        if (col_on[j] && --ccount[j] == 0) cq.push_back(j);
+      }
+    }
+    return changed;
+  };
+  auto dual_fixing = [&]() {
+    bool changed = false;
+    for (int j = 0; j < n; ++j) {
+      if (!col_on[j] || ccount[j] == 0 || lb[j] == ub[j]) continue;
+      bool down_ok = true, up_ok = true;  // decreasing / increasing x_j loosens every row
+      for (int k = cptr[j]; k < cptr[j + 1] && (down_ok || up_ok); ++k) {
+        const int i = crow[k];
+        const double a = cval[k];
+        if (!row_on[i] || a == 0.0) continue;
+        const bool no_lo = !std::isfinite(lc[i]) && lc[i] < 0, no_up = !std::isfinite(uc[i]) && uc[i] > 0;
+        if (a > 0.0) {
+          down_ok = down_ok && no_lo;
+          up_ok   = up_ok && no_up;
+        } else {
+          down_ok = down_ok && no_up;
+          up_ok   = up_ok && no_lo;
+        }
+      }
+      const double cj = cost(j);
+      double v        = 0.0;
+      if (down_ok && cj >= 0.0 && std::isfinite(lb[j])) v = lb[j];
+      else if (up_ok && cj <= 0.0 && std::isfinite(ub[j])) v = ub[j];
+      else continue;  // (toward an infinite bound the cost need not be bounded: left to the solver)
+      lb[j] = ub[j] = v;
+      cq.push_back(j);
+      changed = true;
+    }
+    return changed;
+  };
+  while (!rq.empty() || !cq.empty() || recount_columns() || redundant_rows() || dual_fixing()) {
     while (!rq.empty()) {
       const int i = rq.back();
       rq.pop_back();

@@ -10,7 +10,12 @@
 //   - a singleton row a x_j in [lc, uc]: turned into bounds on x_j; in postsolve
 //     its y takes the reduced cost of x_j when x_j sits on a bound it supplied;
 //   - an empty column: set to the bound its cost prefers (unbounded if that
-//     bound is infinite and the cost nonzero), reduced cost = cost.
+//     bound is infinite and the cost nonzero), reduced cost = cost;
+//   - a redundant row (activity range over the column bounds inside [lc, uc]):
+//     dropped; y = 0;
+//   - dual fixing: a column that can move toward one of its finite bounds without
+//     raising the cost or tightening any row is fixed there; its reduced cost
+//     c - A^T y has the right sign by the signs of its rows' duals.
 // Afterwards a row whose activity range over the column bounds misses its
 // [lc, uc] (beyond 1e-9 relative) reports the problem infeasible.
 // Everything runs in FP64 in the user's objective sense; the reduced problem

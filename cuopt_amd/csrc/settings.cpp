@@ -116,6 +116,8 @@ SolverSettingsRegistry::SolverSettingsRegistry()
      CUOPT_AMD_PRECISION_FP64},
     {CUOPT_AMD_PDLP_REFINEMENT, &p.refinement, 0, 1, 1},
     {CUOPT_AMD_PER_CONSTRAINT_ABSOLUTE, &p.per_constraint_absolute, 0, 1, 0},
+    {CUOPT_AMD_SPMV_AUTOTUNE, &p.spmv_autotune, 0, 1, 0},
+    {CUOPT_AMD_COLUMN_ORDER, &p.column_order, -1, 1, -1},
   };
   // Bool parameters (solver_settings.cu:109-123).
   bools_ = {

@@ -36,6 +36,28 @@ extern "C" {
  * FP64 solves. */
 #define CUOPT_AMD_PDLP_REFINEMENT "pdlp_refinement"
 
+/* Integer parameter (0/1, default 0) enabling the SpMV autotuner (no reference
+ * counterpart; the reference's only tuner times SpMM batch sizes,
+ * optimal_batch_size_handler.cu:196-435).  At setup each of A and A^T is tried
+ * with tile capacities of 512, 1 024 and 2 048 nonzeros and with the planner's
+ * lanes per row G, G / 2 and 2 G (the row binning); each candidate's SpMV is
+ * timed once untimed and then five times with HIP events on the real matrix, and
+ * the fastest is used.  The choice is cached per matrix (sizes, precision and a
+ * hash of the row offsets) for the life of the process.  Candidates sum a row in
+ * different orders, so with the tuner on, results can differ from run to run in
+ * the last bits; off (the default), every solve uses capacity 1 024 and the
+ * planner's G and is bitwise reproducible. */
+#define CUOPT_AMD_SPMV_AUTOTUNE "pdlp_spmv_autotune"
+
+/* Integer parameter (-1, 0 or 1, default -1) choosing the solver's internal column
+ * order (no reference counterpart).  1 numbers the columns in the order in which
+ * rows of at most one tile (1 020 nonzeros) first touch them, rows in index
+ * order, so that the rows one workgroup reduces together gather x-bar from nearby
+ * addresses; 0 keeps the given order; -1 (the default) uses 1 for matrices of at
+ * least 2^20 nonzeros.  The LP is the same LP: inputs and outputs stay in the
+ * caller's column order; only the order of the sums in a row changes. */
+#define CUOPT_AMD_COLUMN_ORDER "pdlp_column_order"
+
 /* Integer parameter (0/1, default 0) changing the per_constraint_residual test
  * (no reference counterpart).  0 keeps the reference's signed form
  * (relative_residual_t, utils.cuh:576-590): max(0, max_i r_i - rel * w_i) with the
@@ -74,7 +96,12 @@ extern "C" {
 #define CUOPT_AMD_INFO_SCALING_TIME        23 /* setup: Ruiz, Pock-Chambolle, bound/objective rescaling (s) */
 #define CUOPT_AMD_INFO_POWER_TIME          24 /* setup: power iteration for the step size (s) */
 #define CUOPT_AMD_INFO_REFINE_ROUNDS       25 /* FP64 recentrings of a FP32-mixed solve */
-#define CUOPT_AMD_INFO_COUNT               26
+#define CUOPT_AMD_INFO_TILE_A              26 /* SpMV tile capacity over A (nonzeros) */
+#define CUOPT_AMD_INFO_LANES_BIAS_A        27 /* lanes-per-row shift over A (-1, 0, +1: G / 2, G, 2 G) */
+#define CUOPT_AMD_INFO_TILE_AT             28 /* the same over A^T */
+#define CUOPT_AMD_INFO_LANES_BIAS_AT       29
+#define CUOPT_AMD_INFO_TUNE_TIME           30 /* setup: SpMV autotuning (s; 0 when off or cached) */
+#define CUOPT_AMD_INFO_COUNT               31
 
 /* Copies up to n slots of the last solve's statistics into info. */
 cuopt_int_t cuOptAmdGetTerminationInfo(cuOptSolution solution, cuopt_float_t* info, cuopt_int_t n);

@@ -678,3 +678,59 @@ def test_batch_time_limit_counts_per_lp_from_its_dispatch():
     assert all(r.status == c_api.CUOPT_TERIMINATION_STATUS_TIME_LIMIT for r in rs)
     kernel = rs[0].info["loop_time"]
     assert limit <= kernel <= (waves + 1) * limit * 1.5 + 0.5, (kernel, waves)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
+def test_spmv_autotune_picks_a_candidate_caches_it_and_solves(precision):
+    # pdlp_spmv_autotune = 1: a tile capacity in {512, 1024, 2048} and a lanes shift in
+    # {-1, 0, 1} for each of A and A^T, timed at setup on the first solve of a matrix and
+    # cached for the next; the tuned solve reaches the optimum (only the summation order
+    # of row sums differs from the untuned solve).
+    lp = G.skewed_lp(m=20000, n=40000, seed=9)
+    tune = {"pdlp_spmv_autotune": 1}
+    first = c_api.solve(lp, tol=1e-6, precision=precision, time_limit=TL, iteration_limit=200000, params=tune)
+    again = c_api.solve(lp, tol=1e-6, precision=precision, time_limit=TL, iteration_limit=200000, params=tune)
+    plain = c_api.solve(lp, tol=1e-6, precision=precision, time_limit=TL, iteration_limit=200000)
+    for k in ("tile_a", "tile_at"):
+        assert first.info[k] in (512, 1024, 2048) and again.info[k] == first.info[k]
+    for k in ("lanes_bias_a", "lanes_bias_at"):
+        assert first.info[k] in (-1, 0, 1) and again.info[k] == first.info[k]
+    assert first.info["tune_time"] > 0 and again.info["tune_time"] == 0
+    assert plain.info["tile_a"] == plain.info["tile_at"] == 1024 and plain.info["tune_time"] == 0
+    assert first.status == again.status == plain.status == 1
+    assert np.array_equal(first.x, again.x)  # the same plan: bitwise the same solve
+    # both stop within the 1e-6 relative KKT test, on either side of the optimum
+    assert rel(first.primal_objective, plain.primal_objective) < 1e-5
+    assert rel(first.primal_objective, lp.obj_opt) < 1e-5
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
+def test_column_relabelling_is_the_same_lp(precision):
+    # pdlp_column_order = 1 numbers the columns internally by first touch of the short
+    # rows; inputs (c, bounds, the initial point) and outputs (x, reduced costs) stay in
+    # the caller's order.  The relabelled solve reaches the same optimum as the plain one,
+    # and a start at the built-in optimum is still a fixed point (x returned unchanged).
+    import scipy.sparse as sp
+
+    lp = G.skewed_lp(m=20000, n=40000, seed=11)
+    A = sp.csr_matrix((lp.values, lp.col_indices, lp.row_offsets), shape=(lp.m, lp.n))
+    plain = c_api.solve(lp, tol=1e-8, precision=precision, time_limit=TL, iteration_limit=400000,
+                        params={"pdlp_column_order": 0})
+    relab = c_api.solve(lp, tol=1e-8, precision=precision, time_limit=TL, iteration_limit=400000,
+                        params={"pdlp_column_order": 1})
+    assert plain.status == relab.status == 1
+    assert rel(relab.primal_objective, plain.primal_objective) < 1e-7
+    assert rel(relab.primal_objective, lp.obj_opt) < 1e-6
+    ax = A @ relab.x
+    b = np.where(np.isfinite(lp.con_ub), lp.con_ub, 0.0)
+    assert np.linalg.norm(ax - np.clip(ax, lp.con_lb, lp.con_ub)) <= 1e-6 * (1 + np.linalg.norm(b))
+    assert np.all(relab.x >= lp.var_lb - 1e-9) and np.all(relab.x <= lp.var_ub + 1e-9)
+    # reduced costs in the caller's order: c - A^T y minus the dual residual
+    assert np.linalg.norm(lp.c - A.T @ relab.y - relab.reduced_cost) <= 1e-6 * (1 + np.linalg.norm(lp.c))
+    fixed = c_api.solve(lp, tol=1e-8, precision=precision, x_init=lp.x_opt, y_init=lp.y_opt, time_limit=TL,
+                        iteration_limit=10000) if precision == "fp64" else None
+    if fixed is not None:
+        fx = c_api.solve(lp, tol=1e-8, precision=precision, x_init=lp.x_opt, y_init=lp.y_opt, time_limit=TL,
+                         iteration_limit=10000, params={"pdlp_column_order": 1})
+        assert fx.status == fixed.status == 1 and fx.iterations == fixed.iterations == 10
+        assert np.allclose(fx.x, lp.x_opt, atol=1e-8)

@@ -325,3 +325,65 @@ def test_presolve_infeasibility_is_sound_against_highs():
         assert ref.status == 2, seed
         detected += 1
     assert detected > 100
+
+
+def test_redundant_row_and_dual_fixing_hand_worked():
+    # min x0 - x1 + 2 x2, x in [0, 10]^4:
+    #   r0: x0 + x1 + x2 <= 50   redundant: its activity is at most 30
+    #   r1: x0 - x2 <= 4         x0 only loosens it downwards, cost 1 > 0: x0 = 0 (dual fixing);
+    #                            then -x2 <= 4 is a singleton row, x2 empty at cost 2: x2 = 0
+    #   r2: x1 + x3 >= 1         x1 only loosens it upwards, cost -1 < 0: x1 = 10; x3 (cost 0) too
+    # Optimum -10 at (0, 10, 0, 10); every row slack, so y = 0 and the reduced costs are c.
+    A = np.array([[1.0, 1.0, 1.0, 0.0], [1.0, 0.0, -1.0, 0.0], [0.0, 1.0, 0.0, 1.0]])
+    lp = lp_from(A, [1.0, -1.0, 2.0, 0.0], [-INF, -INF, 1.0], [50.0, 4.0, INF], [0.0] * 4, [10.0] * 4)
+    r = c_api.solve(lp, tol=1e-9, presolve=True)
+    assert r.error == 0 and r.status == 1
+    np.testing.assert_array_equal(r.x, [0.0, 10.0, 0.0, 10.0])
+    np.testing.assert_array_equal(r.y, [0.0, 0.0, 0.0])
+    np.testing.assert_array_equal(r.reduced_cost, lp.c)
+    assert r.primal_objective == -10.0
+    assert kkt_ok(lp, r)
+
+
+def _one_sided_case(seed):
+    """Small LPs of one-sided rows (<= or >=) over box bounds, with costs of either
+    sign: dual fixing and redundant rows often settle them completely."""
+    rng = np.random.default_rng(seed)
+    n, m = int(rng.integers(2, 7)), int(rng.integers(1, 5))
+    lb = rng.uniform(-3, 0, n)
+    ub = lb + rng.uniform(0.5, 4, n)
+    A = np.where(rng.random((m, n)) < 0.5, rng.choice([-1.0, 1.0], (m, n)) * rng.uniform(0.5, 2, (m, n)), 0.0)
+    amin = np.sum(np.where(A > 0, A * lb, A * ub), axis=1)
+    amax = np.sum(np.where(A > 0, A * ub, A * lb), axis=1)
+    t = rng.uniform(amin - 1, amax + 1)
+    le = rng.random(m) < 0.5
+    lc = np.where(le, -INF, t)
+    uc = np.where(le, t, INF)
+    return lp_from(A, rng.normal(size=n), lc, uc, lb, ub, maximize=bool(rng.random() < 0.3))
+
+
+def test_presolve_dual_fixing_and_redundant_rows_against_highs():
+    # LPs that presolve settles without the GPU are checked against HiGHS: status,
+    # objective, and optimality of the postsolved (x, y, reduced costs) in numpy.
+    from scipy.optimize import linprog
+
+    settled = 0
+    for seed in range(400):
+        lp = _one_sided_case(seed)
+        r = c_api.solve(lp, tol=1e-9, presolve=True)
+        if r.error != 0:  # left a reduced problem for the GPU
+            continue
+        A = sp.csr_matrix((lp.values, lp.col_indices, lp.row_offsets), shape=(lp.m, lp.n)).toarray()
+        sgn = -1.0 if lp.maximize else 1.0
+        fin_u, fin_l = np.isfinite(lp.con_ub), np.isfinite(lp.con_lb)
+        A_ub = np.vstack([A[fin_u], -A[fin_l]])
+        b_ub = np.concatenate([lp.con_ub[fin_u], -lp.con_lb[fin_l]])
+        ref = linprog(sgn * lp.c, A_ub=A_ub, b_ub=b_ub, bounds=list(zip(lp.var_lb, lp.var_ub)), method="highs")
+        if ref.status == 2:
+            assert r.status == 2, seed
+        else:
+            assert ref.status == 0 and r.status == 1, (seed, ref.status, r.status)
+            assert abs(r.primal_objective - sgn * ref.fun) <= 1e-9 * (1 + abs(ref.fun)), seed
+            assert kkt_ok(lp, r, tol=1e-9), seed
+        settled += 1
+    assert settled > 120
