@@ -13,6 +13,7 @@
 // Iterations between two termination checks are issued back to back on one
 // stream with no host synchronisation; the host reads one small record per check.
 #include <algorithm>
+#include <climits>
 #include <cstdio>
 #include <chrono>
 #include <cmath>
@@ -612,6 +613,11 @@ __global__ __launch_bounds__(kBlock) void iota_kernel(int n, int* __restrict__ v
   for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) v[k] = k;
 }
 
+__global__ __launch_bounds__(kBlock) void fill_int_kernel(int n, int value, int* __restrict__ v)
+{
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) v[k] = value;
+}
+
 __global__ __launch_bounds__(kBlock) void transpose_gather_kernel(int nnz, const int* __restrict__ perm,
                                                                   const int* __restrict__ rowid_a,
                                                                   const double* __restrict__ val,
@@ -622,6 +628,41 @@ __global__ __launch_bounds__(kBlock) void transpose_gather_kernel(int nnz, const
     tidx[d]     = rowid_a[k];
     tval[d]     = val[k];
   }
+}
+
+// key[j] <- the least position k of a nonzero of column j in a row of at most
+// short_len nonzeros (one lane per row; key starts at INT_MAX).
+__global__ __launch_bounds__(kBlock) void first_touch_kernel(const int* __restrict__ ptr, const int* __restrict__ idx,
+                                                             int m, int short_len, int* __restrict__ key)
+{
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < m; i += gridDim.x * kBlock) {
+    const int b = ptr[i], e = ptr[i + 1];
+    if (e - b > short_len) continue;
+    for (int k = b; k < e; ++k) atomicMin(&key[idx[k]], k);
+  }
+}
+
+// label[perm[k]] = k; *moved = 1 when some perm[k] != k.
+__global__ __launch_bounds__(kBlock) void label_kernel(int n, const int* __restrict__ perm, int* __restrict__ label,
+                                                       int* __restrict__ moved)
+{
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) {
+    const int j = perm[k];
+    label[j]    = k;
+    if (j != k) atomicOr(moved, 1);
+  }
+}
+
+// dst[k] = src[perm[k]] (to the internal column order) and dst[perm[k]] = src[k] (back).
+__global__ __launch_bounds__(kBlock) void gather_kernel(int n, const int* __restrict__ perm,
+                                                        const double* __restrict__ src, double* __restrict__ dst)
+{
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) dst[k] = src[perm[k]];
+}
+__global__ __launch_bounds__(kBlock) void scatter_kernel(int n, const int* __restrict__ perm,
+                                                         const double* __restrict__ src, double* __restrict__ dst)
+{
+  for (int k = blockIdx.x * kBlock + threadIdx.x; k < n; k += gridDim.x * kBlock) dst[perm[k]] = src[k];
 }
 
 // idx[k] <- label[idx[k]] (column relabelling).
@@ -680,31 +721,31 @@ void device_transpose(int m, int n, int nnz, const int* ptr, const int* idx, con
 // order, columns in row order), then the untouched columns in index order.  The
 // nonzeros of consecutive short rows, which one workgroup reduces together, then
 // gather from nearby addresses of x-bar; rows longer than a tile gather over
-// sorted columns either way.  perm[k] is the original column of new column k.
-// Returns false (perm untouched) when the order is the identity.
-bool first_touch_order(int m, int n, const int* rowptr, const int* colidx, int short_len, std::vector<int>& perm)
+// sorted columns either way.  On the device: the first position touching each
+// column by an atomic minimum, then a stable radix sort of the columns by it.
+// label (n) receives each original column's new number; perm (n) the original
+// column of each new number.  Returns false when the order is the identity.
+bool first_touch_order(int m, int n, const int* ptr, const int* idx, int short_len, int* label, int* perm,
+                       hipStream_t s)
 {
-  std::vector<int> label(n, -1);
-  perm.assign(n, 0);
-  int next = 0;
-  for (int i = 0; i < m; ++i) {
-    if (rowptr[i + 1] - rowptr[i] > short_len) continue;
-    for (int k = rowptr[i]; k < rowptr[i + 1]; ++k) {
-      const int j = colidx[k];
-      if (label[j] < 0) {
-        label[j]     = next;
-        perm[next++] = j;
-      }
-    }
-  }
-  for (int j = 0; j < n; ++j)
-    if (label[j] < 0) {
-      label[j]     = next;
-      perm[next++] = j;
-    }
-  for (int k = 0; k < n; ++k)
-    if (perm[k] != k) return true;
-  return false;
+  DevBuf<int> key(n), key_sorted(n), cols(n), moved(1);
+  hipLaunchKernelGGL(fill_int_kernel, dim3(grid_for(n)), dim3(kBlock), 0, s, n, INT_MAX, key.get());
+  hipLaunchKernelGGL(first_touch_kernel, dim3(grid_for(m)), dim3(kBlock), 0, s, ptr, idx, m, short_len, key.get());
+  hipLaunchKernelGGL(iota_kernel, dim3(grid_for(n)), dim3(kBlock), 0, s, n, cols.get());
+  CUOPT_HIP_CHECK(hipGetLastError());
+  size_t temp_bytes = 0;
+  CUOPT_HIP_CHECK(rocprim::radix_sort_pairs(nullptr, temp_bytes, key.get(), key_sorted.get(), cols.get(), perm,
+                                            (size_t)n, 0, 31, s));
+  DevBuf<unsigned char> temp(std::max<size_t>(temp_bytes, 1));
+  CUOPT_HIP_CHECK(rocprim::radix_sort_pairs(temp.get(), temp_bytes, key.get(), key_sorted.get(), cols.get(), perm,
+                                            (size_t)n, 0, 31, s));
+  hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(kBlock), 0, s, 1, 0, moved.get());
+  hipLaunchKernelGGL(label_kernel, dim3(grid_for(n)), dim3(kBlock), 0, s, n, perm, label, moved.get());
+  CUOPT_HIP_CHECK(hipGetLastError());
+  int h_moved = 0;
+  moved.download(&h_moved, 1, s);
+  CUOPT_HIP_CHECK(hipStreamSynchronize(s));  // temporaries are freed on return
+  return h_moved != 0;
 }
 
 // colidx (host, sorted within rows) enables the XCD-aware tile order
@@ -1012,24 +1053,38 @@ class PdlpSolver {
   // Host column indices of A sorted within rows (the tile planner's XCD order).
   std::vector<int> a_sorted_idx_;
   const int* a_idx_host_ = nullptr;
-  // Column relabelling (empty: columns as given): col_perm_[k] is the original
-  // column of internal column k.  Per-column inputs are gathered through it at
-  // upload and the returned x and reduced costs scattered back.
-  std::vector<int> col_perm_;
-  template <typename T>
-  std::vector<T> to_internal(const std::vector<T>& v) const
+  // Column relabelling (unallocated: columns as given): col_perm_[k] is the
+  // original column of internal column k.  Per-column inputs are gathered through
+  // it on the device at upload and the returned x and reduced costs scattered back.
+  DevBuf<int> col_perm_;
+  // dst (n, internal order) <- host vector v in the caller's column order.
+  void upload_columns(const double* v, DevBuf<double>& dst)
   {
-    if (col_perm_.empty()) return v;
-    std::vector<T> w(v.size());
-    for (size_t k = 0; k < col_perm_.size(); ++k) w[k] = v[col_perm_[k]];
-    return w;
+    if (!col_perm_.get()) {
+      dst.upload(v, n_, stream_);
+      return;
+    }
+    DevBuf<double> tmp(n_);
+    tmp.upload(v, n_, stream_);
+    hipLaunchKernelGGL(gather_kernel, dim3(grid_for(n_)), dim3(kBlock), 0, stream_, n_, col_perm_.get(),
+                       tmp.get(), dst.get());
+    CUOPT_HIP_CHECK(hipGetLastError());
+    CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));  // tmp is freed on return
   }
-  void to_original(std::vector<double>& v) const
+  // host v (caller's column order) <- src (n, internal order); synchronous.
+  void download_columns(const DevBuf<double>& src, double* v)
   {
-    if (col_perm_.empty() || v.size() != col_perm_.size()) return;
-    std::vector<double> w(v.size());
-    for (size_t k = 0; k < col_perm_.size(); ++k) w[col_perm_[k]] = v[k];
-    v.swap(w);
+    if (!col_perm_.get()) {
+      src.download(v, n_, stream_);
+      CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
+      return;
+    }
+    DevBuf<double> tmp(n_);
+    hipLaunchKernelGGL(scatter_kernel, dim3(grid_for(n_)), dim3(kBlock), 0, stream_, n_, col_perm_.get(),
+                       src.get(), tmp.get());
+    CUOPT_HIP_CHECK(hipGetLastError());
+    tmp.download(v, n_, stream_);
+    CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
   }
 
   DevBuf<double> Dr_, Dc_, rowv_, colv_;
@@ -1134,26 +1189,23 @@ void PdlpSolver<V>::build_structures()
   // Column order: as given, or first-touch (first_touch_order); automatic for
   // matrices of at least 2^20 nonzeros.
   const int order = st_.column_order;
-  col_perm_.clear();
-  if (order == 1 || (order < 0 && nnz >= (1 << 20))) {
-    std::vector<int> perm;
-    if (first_touch_order(m_, n_, lp_.rowptr.data(), idx_src, A_.tile - kVecPad, perm)) col_perm_ = std::move(perm);
-  }
-  if (!col_perm_.empty()) {
-    std::vector<int> label(n_);
-    for (int k = 0; k < n_; ++k) label[col_perm_[k]] = k;
-    DevBuf<int> d_label(n_);
-    d_label.upload(label, stream_);
-    hipLaunchKernelGGL(relabel_kernel, dim3(grid_for(nnz)), dim3(kBlock), 0, stream_, nnz, d_label.get(),
-                       A_.idx.get());
-    CUOPT_HIP_CHECK(hipGetLastError());
+  col_perm_.release();
+  if (n_ > 0 && (order == 1 || (order < 0 && nnz >= (1 << 20)))) {
+    DevBuf<int> label(n_), perm(n_);
+    if (first_touch_order(m_, n_, A_.ptr.get(), A_.idx.get(), A_.tile - kVecPad, label.get(), perm.get(), stream_)) {
+      hipLaunchKernelGGL(relabel_kernel, dim3(grid_for(nnz)), dim3(kBlock), 0, stream_, nnz, label.get(),
+                         A_.idx.get());
+      CUOPT_HIP_CHECK(hipGetLastError());
+      CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));  // label is freed on leaving this scope
+      col_perm_ = std::move(perm);
+    }
   }
   // A^T on the device (compute_transpose_of_problem, problem.cu:472-489): a
   // stable radix sort of the nonzeros by column keeps each column's entries in
   // row order, as the host counting sort of the reference does.
   device_transpose(m_, n_, nnz, A_.ptr.get(), A_.idx.get(), a_orig_.get(), AT_.ptr.get(), AT_.idx.get(),
                    at_orig_.get(), A_.rowid.get(), AT_.rowid.get(), stream_);
-  if (!col_perm_.empty()) {
+  if (col_perm_.get()) {
     // Rows of A sorted by the new labels: the transpose of A^T (whose rows are the
     // relabelled columns, each in row order) lists every row's entries by column.
     device_transpose(n_, m_, nnz, AT_.ptr.get(), AT_.idx.get(), at_orig_.get(), A_.ptr.get(), A_.idx.get(),
@@ -1220,13 +1272,12 @@ void PdlpSolver<V>::setup_scaling()
   ub_orig_.resize(n_);
   lc_orig_.resize(m_);
   uc_orig_.resize(m_);
-  std::vector<double> cmin = to_internal(lp_.c);
+  std::vector<double> cmin(lp_.c.begin(), lp_.c.end());
   if (lp_.maximize)
     for (double& v : cmin) v = -v;  // convert_to_maximization_problem (problem_helpers.cuh:132-151)
-  const std::vector<double> lbi = to_internal(lb_), ubi = to_internal(ub_);
-  c_orig_.upload(cmin, stream_);
-  lb_orig_.upload(lbi, stream_);
-  ub_orig_.upload(ubi, stream_);
+  upload_columns(cmin.data(), c_orig_);
+  upload_columns(lb_.data(), lb_orig_);
+  upload_columns(ub_.data(), ub_orig_);
   lc_orig_.upload(lc_, stream_);
   uc_orig_.upload(uc_, stream_);
   cs_.resize(n_);
@@ -1408,11 +1459,9 @@ void PdlpSolver<V>::init_iterates()
   su_.resize(n_);
   step_sc_.resize(1);
   DevBuf<double> xi, yi;
-  std::vector<double> x_init;
   if (!st_.initial_primal.empty()) {
-    x_init = to_internal(st_.initial_primal);
     xi.resize(n_);
-    xi.upload(x_init.data(), n_, stream_);
+    upload_columns(st_.initial_primal.data(), xi);
   }
   if (!st_.initial_dual.empty()) {
     yi.resize(m_);
@@ -1579,12 +1628,9 @@ void PdlpSolver<V>::finish(SolveResult& R, int status)
   R.x.resize(n_);
   R.y.resize(m_);
   R.rc.resize(n_);
-  xu_.download(R.x.data(), n_, stream_);
   yu_.download(R.y.data(), m_, stream_);
-  su_.download(R.rc.data(), n_, stream_);
-  CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
-  to_original(R.x);
-  to_original(R.rc);
+  download_columns(xu_, R.x.data());
+  download_columns(su_, R.rc.data());
   if (lp_.maximize) {  // solve.cu:583-588
     for (double& v : R.y) v = -v;
     for (double& v : R.rc) v = -v;
