@@ -52,6 +52,7 @@ struct PdlpSettings {
   int per_constraint_absolute  = 0;  // 1: |r_i| - rel |w_i| in place of the signed form
   int spmv_autotune            = 0;  // 1: time tile / lanes candidates at setup
   int column_order             = -1; // -1 automatic, 0 as given, 1 first-touch relabelling
+  int stop_on_nan              = 0;  // 1: a NaN residual ends a single-LP solve (NumericalError)
   bool save_best_primal_so_far = false;
   bool first_primal_feasible   = false;
   bool log_to_console          = true;

@@ -150,9 +150,13 @@ LpHost read_mps(const std::string& path)
       case ROWS: {
         if (t.size() < 2) throw mps_parse_error("bad ROWS line: " + line);
         const std::string ty = upper(t[0]);
-        if (ty == "N") {  // the first N row (or the one OBJNAME names) is the objective
+        if (ty == "N") {
+          // The first N row is the objective unless OBJNAME named one; every later N
+          // row is ignored, the one OBJNAME names included, so that row's COLUMNS
+          // entries are dropped and only its RHS (the offset) is read
+          // (mps_parser.cpp:851-856, 995, 1095).
           if (obj_row.empty()) obj_row = t[1];
-          else if (t[1] != obj_row) ignored_rows.insert(t[1]);
+          else ignored_rows.insert(t[1]);
           break;
         }
         if (ty != "E" && ty != "L" && ty != "G") throw mps_parse_error("bad row type in line: " + line);
@@ -197,14 +201,14 @@ LpHost read_mps(const std::string& path)
         // Optional set name.  RHS: there is none when the first field is a row
         // or the objective (mps_parser.cpp:1052-1056); RANGES: an odd field count.
         size_t first = (t.size() % 2 == 1) ? 1 : 0;
-        if (sec == RHS) first = (t[0] == obj_row || row_id.count(t[0]) || ignored_rows.count(t[0])) ? 0 : 1;
+        if (sec == RHS) first = (t[0] == obj_row || row_id.count(t[0])) ? 0 : 1;
         if (t.size() - first == 1 || (t.size() - first) % 2 != 0)
           throw mps_parse_error("bad RHS/RANGES line: " + line);
         for (size_t p = first; p + 1 < t.size(); p += 2) {
           const double v = number(t[p + 1], line);
-          if (sec == RHS && ignored_rows.count(t[p])) {
-            continue;
-          } else if (sec == RHS && t[p] == obj_row) {
+          // RHS of the objective row is the negated offset; any other row must be a
+          // constraint, an ignored N row included (mps_parser.cpp:1095-1107)
+          if (sec == RHS && t[p] == obj_row) {
             offset = -v;
           } else if (sec == RHS) {
             rhs[row_of(t[p], line)] = v;

@@ -1898,9 +1898,10 @@ int PdlpSolver<V>::pdhg_loop(SolveResult& R, const Tolerances& tol, bool refine)
     // first_primal_feasible (pdlp.cu:928-983): a primal feasible, not optimal iterate ends the solve
     if (k > 1 && st_.first_primal_feasible && !bench && primal_feasible(tol))
       return done(CUOPT_TERIMINATION_STATUS_PRIMAL_FEASIBLE);
-    // Divergence guard: a NaN residual can never satisfy the test again, so stop
-    // with a numerical error instead of running into the limits.
-    if (std::isnan(l2p_) || std::isnan(l2d_) || std::isnan(gap_))
+    // pdlp_stop_on_nan (an extension): a NaN residual can never satisfy the test
+    // again, so stop with a numerical error.  Off, as in the reference's Stable3
+    // loop (pdlp.cu:854-1180), the solve runs on to its limits.
+    if (st_.stop_on_nan && (std::isnan(l2p_) || std::isnan(l2d_) || std::isnan(gap_)))
       return done(CUOPT_TERIMINATION_STATUS_NUMERICAL_ERROR);
     if (st_.save_best_primal_so_far && !bench) record_best(tol);  // pdlp.cu:1171-1175
     if (elapsed() >= st_.time_limit) {

@@ -58,6 +58,14 @@ extern "C" {
  * caller's column order; only the order of the sums in a row changes. */
 #define CUOPT_AMD_COLUMN_ORDER "pdlp_column_order"
 
+/* Integer parameter (0/1, default 0; no reference counterpart).  1 ends a
+ * single-LP solve with CUOPT_TERIMINATION_STATUS_NUMERICAL_ERROR at the first
+ * termination check whose residuals or gap are NaN.  0 keeps the reference's
+ * Stable3 behaviour: such a solve runs on to its iteration or time limit.
+ * cuOptAmdBatchSolve always stops an LP at a NaN check, since its persistent
+ * launch must end. */
+#define CUOPT_AMD_STOP_ON_NAN "pdlp_stop_on_nan"
+
 /* Integer parameter (0/1, default 0) changing the per_constraint_residual test
  * (no reference counterpart).  0 keeps the reference's signed form
  * (relative_residual_t, utils.cuh:576-590): max(0, max_i r_i - rel * w_i) with the

@@ -135,14 +135,24 @@ def test_objective_row_choice(name, objective):
     assert list(dm.get_row_names()) == ["ROW1", "ROW2"]  # further N rows are not constraints
 
 
-def test_objname_selects_its_row_coefficients(tmp_path):
-    # The row OBJNAME names is the objective, with its COLUMNS entries; the
-    # other N row's entries are dropped.
+def test_objname_row_keeps_only_its_offset(tmp_path):
+    # mps_parser.cpp:851-856, 995, 1095: once OBJNAME has named the objective, every N
+    # row of the ROWS section is filed as ignored, the named one included, so COLUMNS
+    # entries of either N row are dropped (a zero objective); an RHS entry on the named
+    # row is still read as the negated offset, and one on the other N row is an unknown
+    # row name.  Kept as the reference parses it, for identical problems.
     src = open(os.path.join(MPS, "good-mps-fixed-two-objectives-objname.mps")).read()
     src = src.replace("    VAR2      COST      0.1", "    VAR2      COST      0.1\n    VAR2      COST6679327  7.5")
     f = tmp_path / "objname.mps"
-    f.write_text(src)
-    assert ParseMps(str(f)).get_objective_coefficients().tolist() == [0.0, 7.5]
+    f.write_text(src.replace("    RHS1      ROW1      5.4", "    RHS1      COST6679327  2.5\n    RHS1      ROW1      5.4"))
+    dm = ParseMps(str(f))
+    assert dm.get_objective_name() == "COST6679327"
+    assert dm.get_objective_coefficients().tolist() == [0.0, 0.0]
+    assert dm.get_objective_offset() == -2.5
+    g = tmp_path / "other.mps"
+    g.write_text(src.replace("    RHS1      ROW1      5.4", "    RHS1      COST      2.5\n    RHS1      ROW1      5.4"))
+    with pytest.raises(Exception):
+        ParseMps(str(g))
 
 
 BAD = ([f"bad-mps-{i}.mps" for i in (1, 2, 3, 4, 5, 6, 7, 9, 10, 11, 12, 13, 14, 15)]
