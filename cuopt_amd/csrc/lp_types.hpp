@@ -53,6 +53,7 @@ struct PdlpSettings {
   int spmv_autotune            = 0;  // 1: time tile / lanes candidates at setup
   int column_order             = -1; // -1 automatic, 0 as given, 1 first-touch relabelling
   int stop_on_nan              = 0;  // 1: a NaN residual ends a single-LP solve (NumericalError)
+  int long_row_merge           = -1; // rows longer than a tile: 0 merge launch, 1 last chunk merges, -1 auto
   bool save_best_primal_so_far = false;
   bool first_primal_feasible   = false;
   bool log_to_console          = true;

@@ -569,6 +569,9 @@ struct PlanBufs {
   DevBuf<int> meta;
   DevBuf<LongRow> longrows;
   DevBuf<uint64_t> long_part;
+  DevBuf<int> long_cnt;  // chunks of each long row done in the running launch (0 between launches)
+  int max_chunks = 0;    // most chunks of one long row
+  bool fused     = false;  // long rows merged by their last chunk inside the tiled launch
   int ntiles = 0, nlong = 0;
   // workgroups that write reduction partials: the tiles, then one per long row
   int red_blocks() const { return ntiles + nlong; }
@@ -750,7 +753,22 @@ bool first_touch_order(int m, int n, const int* ptr, const int* idx, int short_l
 
 // colidx (host, sorted within rows) enables the XCD-aware tile order
 // (tile_plan.hpp); CUOPT_AMD_XCD_ORDER=0 turns it off for comparisons.
-void upload_plan(PlanBufs& P, const TilePlan& p, hipStream_t s)
+// Long rows (more than a tile) are merged either by a launch of their own after
+// the tiled one, or inside the tiled launch by their last chunk.  Setting
+// pdlp_long_row_merge: 0 / 1 force one form; -1 (default) merges inside the launch
+// when no row has more than kFusedMaxChunks chunks, since every chunk of a row
+// counts itself on one device-scope counter and a few hundred such atomics on one
+// address take longer than the merge launch they replace (measured at 500 k x 1 M,
+// 465 chunks: FP32 dual step 34.9 -> 50.4 us; at 20 k x 40 k, 22 chunks, the
+// solve loop 0.19 -> 0.16 s).  With -1 the SpMV tuner times both forms.
+constexpr int kFusedMaxChunks = 64;
+
+void set_merge(PlanBufs& P, int mode)
+{
+  P.fused = P.nlong > 0 && (mode == 1 || (mode < 0 && P.max_chunks <= kFusedMaxChunks));
+}
+
+void upload_plan(PlanBufs& P, const TilePlan& p, hipStream_t s, int merge_mode = -1)
 {
   P.ntiles = (int)p.tiles.size();
   P.tiles.resize(p.tiles.size());
@@ -761,6 +779,11 @@ void upload_plan(PlanBufs& P, const TilePlan& p, hipStream_t s)
   if (!p.longrows.empty()) P.longrows.upload(p.longrows, s);
   P.long_part.resize(std::max<size_t>(1, p.tiles.size()));
   P.nlong = (int)p.longrows.size();
+  P.long_cnt.resize(std::max<size_t>(1, p.longrows.size()));
+  P.long_cnt.zero(s);
+  P.max_chunks = 0;
+  for (const LongRow& lr : p.longrows) P.max_chunks = std::max(P.max_chunks, lr.nchunks);
+  set_merge(P, merge_mode);
 }
 
 // Lanes per row of every row-run tile shifted by bias powers of two (clamped to
@@ -780,29 +803,32 @@ TilePlan make_plan(const MatStruct& M, int tile, int bias, const int* colidx)
   return p;
 }
 
-void build_plan(MatStruct& M, hipStream_t s, const int* colidx = nullptr)
+void build_plan(MatStruct& M, hipStream_t s, const int* colidx = nullptr, int merge_mode = -1)
 {
-  upload_plan(M.plan, make_plan(M, M.tile, 0, colidx), s);
+  upload_plan(M.plan, make_plan(M, M.tile, 0, colidx), s, merge_mode);
 }
 
 // The tiled SpMV launch (plus the long-row merge) over raw device pointers:
 // ptr / idx the CSR structure, plan the tile plan, long_part one partial per tile.
+// long_cnt (one zeroed counter per long row) merges each long row in the launch's
+// last chunk to finish; null runs the separate long_rows_kernel.
 template <class Epi>
 void launch_tiled_raw(int tile, const int* ptr, const int* idx, const typename Epi::Mat* vals, PlanDev plan,
-                      int nlong, uint64_t* long_part, double* red, Epi epi, hipStream_t s)
+                      int nlong, uint64_t* long_part, double* red, Epi epi, hipStream_t s, int* long_cnt = nullptr)
 {
   if (plan.ntiles == 0) return;
+  if (nlong == 0) long_cnt = nullptr;
   if (tile == 512)
     hipLaunchKernelGGL((tiled_spmv_kernel<Epi, 512>), dim3(plan.ntiles), dim3(kBlock), 0, s, ptr, idx, vals, plan,
-                       long_part, red, epi);
+                       long_part, red, epi, long_cnt);
   else if (tile == 1024)
     hipLaunchKernelGGL((tiled_spmv_kernel<Epi, 1024>), dim3(plan.ntiles), dim3(kBlock), 0, s, ptr, idx, vals, plan,
-                       long_part, red, epi);
+                       long_part, red, epi, long_cnt);
   else
     hipLaunchKernelGGL((tiled_spmv_kernel<Epi, 2048>), dim3(plan.ntiles), dim3(kBlock), 0, s, ptr, idx, vals, plan,
-                       long_part, red, epi);
+                       long_part, red, epi, long_cnt);
   CUOPT_HIP_CHECK(hipGetLastError());
-  if (nlong > 0) {
+  if (nlong > 0 && !long_cnt) {
     hipLaunchKernelGGL(long_rows_kernel<Epi>, dim3(nlong), dim3(kBlock), 0, s, plan, long_part, red, epi);
     CUOPT_HIP_CHECK(hipGetLastError());
   }
@@ -812,7 +838,7 @@ template <class Epi>
 void launch_tiled(const MatStruct& M, const typename Epi::Mat* vals, double* red, Epi epi, hipStream_t s)
 {
   launch_tiled_raw(M.tile, M.ptr.get(), M.idx.get(), vals, M.plan.dev(), M.plan.nlong, M.plan.long_part.get(), red,
-                   epi, s);
+                   epi, s, M.plan.fused ? M.plan.long_cnt.get() : nullptr);
 }
 
 void launch_reduce(const double* part, int nblocks, int nred, double* out, hipStream_t s,
@@ -834,6 +860,7 @@ void launch_reduce(const double* part, int nblocks, int nred, double* out, hipSt
 
 struct SpmvChoice {
   int tile = 1024, bias = 0;
+  int merge = -1;  // long rows: 1 merged in the tiled launch, 0 in their own launch, -1 none
 };
 
 std::mutex g_tune_mu;
@@ -850,11 +877,12 @@ uint64_t fnv1a(const int* p, size_t n)
 }
 
 // Picks (and installs in M) the fastest candidate; returns whether it timed anything.
+// merge_mode as pdlp_long_row_merge: -1 times both long-row merges, 0 / 1 only that one.
 template <typename V>
-bool autotune_spmv(MatStruct& M, const V* vals, const int* colidx, hipStream_t s, SpmvChoice& out)
+bool autotune_spmv(MatStruct& M, const V* vals, const int* colidx, hipStream_t s, int merge_mode, SpmvChoice& out)
 {
   const std::vector<uint64_t> key{(uint64_t)M.nrows, (uint64_t)M.ncols, (uint64_t)M.nnz, sizeof(V),
-                                  fnv1a(M.h_ptr.data(), M.h_ptr.size())};
+                                  fnv1a(M.h_ptr.data(), M.h_ptr.size()), (uint64_t)(merge_mode + 1)};
   {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     auto it = g_tune_cache.find(key);
@@ -862,6 +890,7 @@ bool autotune_spmv(MatStruct& M, const V* vals, const int* colidx, hipStream_t s
       out    = it->second;
       M.tile = out.tile;
       upload_plan(M.plan, make_plan(M, out.tile, out.bias, colidx), s);
+      M.plan.fused = out.merge == 1;
       return false;
     }
   }
@@ -880,7 +909,7 @@ bool autotune_spmv(MatStruct& M, const V* vals, const int* colidx, hipStream_t s
   CUOPT_HIP_CHECK(hipEventCreate(&ev[0]));
   CUOPT_HIP_CHECK(hipEventCreate(&ev[1]));
   float best_ms = -1.f;
-  PlanBufs best_plan;
+  TilePlan best_tp;
   for (int tile : {512, 1024, 2048}) {
     const TilePlan base = make_plan(M, tile, 0, colidx);
     for (int bias : {0, -1, 1}) {
@@ -891,26 +920,32 @@ bool autotune_spmv(MatStruct& M, const V* vals, const int* colidx, hipStream_t s
       }
       PlanBufs P;
       upload_plan(P, p, s);
-      auto run = [&]() {
-        launch_tiled_raw(tile, M.ptr.get(), M.idx.get(), vals, P.dev(), P.nlong, P.long_part.get(), nullptr,
-                         PlainEpi<V>{x.get(), y.get()}, s);
-      };
-      run();
-      CUOPT_HIP_CHECK(hipEventRecord(ev[0], s));
-      for (int r = 0; r < 5; ++r) run();
-      CUOPT_HIP_CHECK(hipEventRecord(ev[1], s));
-      CUOPT_HIP_CHECK(hipEventSynchronize(ev[1]));
-      float ms = 0.f;
-      CUOPT_HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
-      if (best_ms < 0.f || ms < best_ms) {
-        best_ms   = ms;
-        out       = SpmvChoice{tile, bias};
-        best_plan = std::move(P);
+      // long rows merged by their own launch (0) and / or by their last chunk (1)
+      const int f0 = P.nlong > 0 && merge_mode == 1 ? 1 : 0;
+      const int f1 = P.nlong > 0 && merge_mode != 0 ? 1 : 0;
+      for (int fused = f0; fused <= f1; ++fused) {
+        auto run = [&]() {
+          launch_tiled_raw(tile, M.ptr.get(), M.idx.get(), vals, P.dev(), P.nlong, P.long_part.get(), nullptr,
+                           PlainEpi<V>{x.get(), y.get()}, s, fused ? P.long_cnt.get() : nullptr);
+        };
+        run();
+        CUOPT_HIP_CHECK(hipEventRecord(ev[0], s));
+        for (int r = 0; r < 5; ++r) run();
+        CUOPT_HIP_CHECK(hipEventRecord(ev[1], s));
+        CUOPT_HIP_CHECK(hipEventSynchronize(ev[1]));
+        float ms = 0.f;
+        CUOPT_HIP_CHECK(hipEventElapsedTime(&ms, ev[0], ev[1]));
+        if (best_ms < 0.f || ms < best_ms) {
+          best_ms = ms;
+          out     = SpmvChoice{tile, bias, P.nlong > 0 ? fused : -1};
+          best_tp = p;
+        }
       }
     }
   }
   M.tile = out.tile;
-  M.plan = std::move(best_plan);
+  upload_plan(M.plan, best_tp, s);
+  M.plan.fused = out.merge == 1;
   std::lock_guard<std::mutex> lock(g_tune_mu);
   g_tune_cache[key] = out;
   return true;
@@ -1218,8 +1253,8 @@ void PdlpSolver<V>::build_structures()
   AT_.ptr.download(AT_.h_ptr.data(), n_ + 1, stream_);
   CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));  // host sources above are temporaries
   a_idx_host_ = idx_src;
-  build_plan(A_, stream_, idx_src);
-  build_plan(AT_, stream_);
+  build_plan(A_, stream_, idx_src, st_.long_row_merge);
+  build_plan(AT_, stream_, nullptr, st_.long_row_merge);
 }
 
 template <typename V>
@@ -1691,8 +1726,8 @@ SolveResult PdlpSolver<V>::run()
   if (st_.spmv_autotune) {
     const double t = elapsed();
     SpmvChoice ca, cat;
-    const bool timed = autotune_spmv<V>(A_, a_w_.get(), a_idx_host_, stream_, ca);
-    const bool timed_t = autotune_spmv<V>(AT_, at_w_.get(), nullptr, stream_, cat);
+    const bool timed   = autotune_spmv<V>(A_, a_w_.get(), a_idx_host_, stream_, st_.long_row_merge, ca);
+    const bool timed_t = autotune_spmv<V>(AT_, at_w_.get(), nullptr, stream_, st_.long_row_merge, cat);
     CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
     if (timed || timed_t) R.info[CUOPT_AMD_INFO_TUNE_TIME] = elapsed() - t;
     R.info[CUOPT_AMD_INFO_LANES_BIAS_A]  = ca.bias;
@@ -1700,8 +1735,10 @@ SolveResult PdlpSolver<V>::run()
   } else {
     R.info[CUOPT_AMD_INFO_LANES_BIAS_A] = R.info[CUOPT_AMD_INFO_LANES_BIAS_AT] = 0.0;
   }
-  R.info[CUOPT_AMD_INFO_TILE_A]  = A_.tile;
-  R.info[CUOPT_AMD_INFO_TILE_AT] = AT_.tile;
+  R.info[CUOPT_AMD_INFO_TILE_A]   = A_.tile;
+  R.info[CUOPT_AMD_INFO_TILE_AT]  = AT_.tile;
+  R.info[CUOPT_AMD_INFO_MERGE_A]  = A_.plan.nlong > 0 ? (A_.plan.fused ? 1 : 0) : -1;
+  R.info[CUOPT_AMD_INFO_MERGE_AT] = AT_.plan.nlong > 0 ? (AT_.plan.fused ? 1 : 0) : -1;
   // Stable3: initial primal weight 1 (bound_objective_rescaling branch, pdlp.cu:2799-2807).
   omega_ = best_omega_ = 1.0;
   tau_   = eta_ / omega_;

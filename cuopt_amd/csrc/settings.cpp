@@ -119,6 +119,7 @@ SolverSettingsRegistry::SolverSettingsRegistry()
     {CUOPT_AMD_SPMV_AUTOTUNE, &p.spmv_autotune, 0, 1, 0},
     {CUOPT_AMD_COLUMN_ORDER, &p.column_order, -1, 1, -1},
     {CUOPT_AMD_STOP_ON_NAN, &p.stop_on_nan, 0, 1, 0},
+    {CUOPT_AMD_LONG_ROW_MERGE, &p.long_row_merge, -1, 1, -1},
   };
   // Bool parameters (solver_settings.cu:109-123).
   bools_ = {

@@ -204,6 +204,10 @@ __device__ inline Acc block_merge(Acc a, uint64_t* lds)
 //   __device__ typename Acc::Elem product(Mat v, int col, int k) const;
 //   __device__ Pre load(int r) const;                   issued before the products
 //   __device__ void row(int r, const Acc& sum, const Pre& p, double (&red)[kMaxRed]) const;
+template <class Epi, bool ATOMIC>
+__device__ inline void merge_long_row(const PlanDev& plan, int lid, uint64_t* long_part, double* red_part,
+                                      const Epi& epi, uint64_t* acc_lds, double (*red_lds)[kMaxRed]);
+
 template <class Epi, int TILE>
 __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restrict__ rowptr,
                                                             const int* __restrict__ colidx,
@@ -211,7 +215,8 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
                                                             PlanDev plan,
                                                             uint64_t* __restrict__ long_part,
                                                             double* __restrict__ red_part,
-                                                            Epi epi)
+                                                            Epi epi,
+                                                            int* __restrict__ long_cnt)
 {
   using Acc  = typename Epi::Acc;
   using Elem = typename Acc::Elem;
@@ -219,6 +224,7 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
   __shared__ Elem prod[TILE];
   __shared__ double red_lds[kBlock / kWave][kMaxRed];
   __shared__ uint64_t acc_lds[kBlock / kWave];
+  __shared__ int last_chunk;
 
   if (epi.skip()) return;
   const int t    = blockIdx.x;
@@ -304,12 +310,35 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
       if (sub == 0) epi.row(r, a, epi.load(r), red);
     }
   } else {
-    // Chunk of a long row: only the workgroup partial is stored here; the
-    // row's epilogue runs in long_rows_kernel after this launch.
+    // Chunk of a long row: the workgroup partial goes to the chunk's slot.
+    // Without long_cnt the row's epilogue runs in long_rows_kernel after this
+    // launch.  With it, the chunk that finishes last runs it here: every chunk
+    // publishes its partial and then counts itself with device-scope atomics
+    // (read-modify-writes, performed where all XCDs see them, each waited for
+    // before the next is issued); the last one reads the partials back the same
+    // way, so no stale L2 copy of another XCD's partial can be read.
     Acc a = Acc::zero();
     for (int k = tid; k < nnz; k += kBlock) a.add(prod[k]);
     a = block_merge(a, acc_lds);
-    if (tid == 0) long_part[tl.r1] = a.pack();  // the chunk's slot
+    const int lid = -(mt + 1);
+    if (!long_cnt) {
+      if (tid == 0) long_part[tl.r1] = a.pack();  // the chunk's slot
+    } else {
+      if (tid == 0) {
+        (void)__hip_atomic_exchange(&long_part[tl.r1], a.pack(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __builtin_amdgcn_s_waitcnt(0);
+        const int before = __hip_atomic_fetch_add(&long_cnt[lid], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        last_chunk       = before == plan.longrows[lid].nchunks - 1;
+      }
+      __syncthreads();
+    }
+    if (long_cnt && last_chunk) {
+      if constexpr (Epi::NRED > 0) block_sum_store<Epi::NRED>(red, red_part + (size_t)t * Epi::NRED, red_lds);
+      __syncthreads();
+      merge_long_row<Epi, true>(plan, lid, long_part, red_part, epi, acc_lds, red_lds);
+      if (tid == 0) (void)__hip_atomic_exchange(&long_cnt[lid], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return;
+    }
   }
   if constexpr (Epi::NRED > 0) block_sum_store<Epi::NRED>(red, red_part + (size_t)t * Epi::NRED, red_lds);
 }
@@ -319,28 +348,40 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
 // epilogue.  Runs after tiled_spmv_kernel on the same stream, so the kernel
 // boundary makes the partials visible across XCDs without in-kernel fences.
 // Reduction partials go to red_part[(ntiles + b) * NRED ...].
-template <class Epi>
-__global__ __launch_bounds__(kBlock) void long_rows_kernel(PlanDev plan,
-                                                           const uint64_t* __restrict__ long_part,
-                                                           double* __restrict__ red_part,
-                                                           Epi epi)
+template <class Epi, bool ATOMIC>
+__device__ inline void merge_long_row(const PlanDev& plan, int lid, uint64_t* long_part, double* red_part,
+                                      const Epi& epi, uint64_t* acc_lds, double (*red_lds)[kMaxRed])
 {
-  using Acc = typename Epi::Acc;
-  __shared__ double red_lds[kBlock / kWave][kMaxRed];
-  __shared__ uint64_t acc_lds[kBlock / kWave];
-  if (epi.skip()) return;
-  const LongRow lr = plan.longrows[blockIdx.x];
+  using Acc        = typename Epi::Acc;
+  const LongRow lr = plan.longrows[lid];
   typename Epi::Pre pre{};
   if (threadIdx.x == 0) pre = epi.load(lr.row);  // in flight during the merge
   Acc tot = Acc::zero();
-  for (int ch = threadIdx.x; ch < lr.nchunks; ch += kBlock) tot.merge(Acc::unpack(long_part[lr.first + ch]));
+  for (int ch = threadIdx.x; ch < lr.nchunks; ch += kBlock) {
+    uint64_t* slot = long_part + lr.first + ch;
+    const uint64_t v =
+      ATOMIC ? __hip_atomic_fetch_add(slot, (uint64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *slot;
+    tot.merge(Acc::unpack(v));
+  }
   tot = block_merge(tot, acc_lds);
   double red[kMaxRed];
 #pragma unroll
   for (int k = 0; k < kMaxRed; ++k) red[k] = 0.0;
   if (threadIdx.x == 0) epi.row(lr.row, tot, pre, red);
   if constexpr (Epi::NRED > 0)
-    block_sum_store<Epi::NRED>(red, red_part + (size_t)(plan.ntiles + blockIdx.x) * Epi::NRED, red_lds);
+    block_sum_store<Epi::NRED>(red, red_part + (size_t)(plan.ntiles + lid) * Epi::NRED, red_lds);
+}
+
+template <class Epi>
+__global__ __launch_bounds__(kBlock) void long_rows_kernel(PlanDev plan,
+                                                           uint64_t* __restrict__ long_part,
+                                                           double* __restrict__ red_part,
+                                                           Epi epi)
+{
+  __shared__ double red_lds[kBlock / kWave][kMaxRed];
+  __shared__ uint64_t acc_lds[kBlock / kWave];
+  if (epi.skip()) return;
+  merge_long_row<Epi, false>(plan, blockIdx.x, long_part, red_part, epi, acc_lds, red_lds);
 }
 
 // out[k] = sum over blocks b (in order of a fixed tree) of part[b * nred + k].

@@ -40,7 +40,9 @@ extern "C" {
  * counterpart; the reference's only tuner times SpMM batch sizes,
  * optimal_batch_size_handler.cu:196-435).  At setup each of A and A^T is tried
  * with tile capacities of 512, 1 024 and 2 048 nonzeros and with the planner's
- * lanes per row G, G / 2 and 2 G (the row binning); each candidate's SpMV is
+ * lanes per row G, G / 2 and 2 G (the row binning), and, when rows longer than a
+ * tile exist, their merge inside the SpMV launch or in a launch of its own; each
+ * candidate's SpMV is
  * timed once untimed and then five times with HIP events on the real matrix, and
  * the fastest is used.  The choice is cached per matrix (sizes, precision and a
  * hash of the row offsets) for the life of the process.  Candidates sum a row in
@@ -65,6 +67,14 @@ extern "C" {
  * cuOptAmdBatchSolve always stops an LP at a NaN check, since its persistent
  * launch must end. */
 #define CUOPT_AMD_STOP_ON_NAN "pdlp_stop_on_nan"
+
+/* Integer parameter (-1/0/1, default -1; no reference counterpart).  Rows longer
+ * than an SpMV tile are summed in chunks.  0 merges the chunks in a launch of its
+ * own after the SpMV; 1 lets the last chunk of each row merge them inside the SpMV
+ * launch (device-scope atomics on a per-row counter); -1 picks 1 when no row has
+ * more than 64 chunks, and with pdlp_spmv_autotune times both.  The results are
+ * bitwise the same either way. */
+#define CUOPT_AMD_LONG_ROW_MERGE "pdlp_long_row_merge"
 
 /* Integer parameter (0/1, default 0) changing the per_constraint_residual test
  * (no reference counterpart).  0 keeps the reference's signed form
@@ -109,7 +119,10 @@ extern "C" {
 #define CUOPT_AMD_INFO_TILE_AT             28 /* the same over A^T */
 #define CUOPT_AMD_INFO_LANES_BIAS_AT       29
 #define CUOPT_AMD_INFO_TUNE_TIME           30 /* setup: SpMV autotuning (s; 0 when off or cached) */
-#define CUOPT_AMD_INFO_COUNT               31
+#define CUOPT_AMD_INFO_MERGE_A             31 /* rows of A longer than a tile: 1 merged by their last chunk
+                                                 in the SpMV launch, 0 by a launch of their own, -1 none */
+#define CUOPT_AMD_INFO_MERGE_AT            32 /* the same over A^T */
+#define CUOPT_AMD_INFO_COUNT               33
 
 /* Copies up to n slots of the last solve's statistics into info. */
 cuopt_int_t cuOptAmdGetTerminationInfo(cuOptSolution solution, cuopt_float_t* info, cuopt_int_t n);

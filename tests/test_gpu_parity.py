@@ -734,3 +734,24 @@ def test_column_relabelling_is_the_same_lp(precision):
                          iteration_limit=10000, params={"pdlp_column_order": 1})
         assert fx.status == fixed.status == 1 and fx.iterations == fixed.iterations == 10
         assert np.allclose(fx.x, lp.x_opt, atol=1e-8)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
+@pytest.mark.parametrize("shape", [(20000, 40000, 5), (100000, 200000, 3)])
+def test_long_row_merge_forms_are_bitwise_the_same(precision, shape):
+    # pdlp_long_row_merge: rows longer than an SpMV tile are summed in chunks, merged
+    # either by a launch of their own (0) or by the row's last chunk inside the SpMV
+    # launch through device-scope atomics (1).  The merge tree is the same, so the
+    # whole solve (graph-captured iteration blocks included) is bitwise the same.
+    # The larger LP has rows of more than 64 chunks, where -1 keeps the merge launch.
+    m, n, seed = shape
+    lp = G.skewed_lp(m=m, n=n, seed=seed)
+    runs = {mode: c_api.solve(lp, tol=1e-6, precision=precision, time_limit=TL, iteration_limit=3000,
+                              params={"pdlp_column_order": 1, "pdlp_long_row_merge": mode})
+            for mode in (0, 1, -1)}
+    assert runs[0].info["merge_a"] == 0 and runs[1].info["merge_a"] == 1
+    assert runs[-1].info["merge_a"] == (1 if m == 20000 else 0)
+    for mode in (1, -1):
+        assert runs[mode].iterations == runs[0].iterations
+        for f in ("x", "y", "reduced_cost"):
+            assert np.array_equal(getattr(runs[mode], f), getattr(runs[0], f)), (mode, f)
