@@ -779,7 +779,7 @@ void upload_plan(PlanBufs& P, const TilePlan& p, hipStream_t s, int merge_mode =
   if (!p.longrows.empty()) P.longrows.upload(p.longrows, s);
   P.long_part.resize(std::max<size_t>(1, p.tiles.size()));
   P.nlong = (int)p.longrows.size();
-  P.long_cnt.resize(std::max<size_t>(1, p.longrows.size()));
+  P.long_cnt.resize(std::max<size_t>(1, p.longrows.size()) * kCntStride);
   P.long_cnt.zero(s);
   P.max_chunks = 0;
   for (const LongRow& lr : p.longrows) P.max_chunks = std::max(P.max_chunks, lr.nchunks);
@@ -829,7 +829,7 @@ void launch_tiled_raw(int tile, const int* ptr, const int* idx, const typename E
                        long_part, red, epi, long_cnt);
   CUOPT_HIP_CHECK(hipGetLastError());
   if (nlong > 0 && !long_cnt) {
-    hipLaunchKernelGGL(long_rows_kernel<Epi>, dim3(nlong), dim3(kBlock), 0, s, plan, long_part, red, epi);
+    hipLaunchKernelGGL(long_rows_kernel<Epi>, dim3(nlong), dim3(kWave), 0, s, plan, long_part, red, epi);
     CUOPT_HIP_CHECK(hipGetLastError());
   }
 }

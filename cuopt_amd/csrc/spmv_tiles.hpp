@@ -206,7 +206,11 @@ __device__ inline Acc block_merge(Acc a, uint64_t* lds)
 //   __device__ void row(int r, const Acc& sum, const Pre& p, double (&red)[kMaxRed]) const;
 template <class Epi, bool ATOMIC>
 __device__ inline void merge_long_row(const PlanDev& plan, int lid, uint64_t* long_part, double* red_part,
-                                      const Epi& epi, uint64_t* acc_lds, double (*red_lds)[kMaxRed]);
+                                      const Epi& epi);
+
+// Ints between the chunk counters of consecutive long rows: each counter has a
+// 256-byte line of its own, so the atomics of different rows do not queue on one line.
+constexpr int kCntStride = 64;
 
 template <class Epi, int TILE>
 __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restrict__ rowptr,
@@ -224,7 +228,6 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
   __shared__ Elem prod[TILE];
   __shared__ double red_lds[kBlock / kWave][kMaxRed];
   __shared__ uint64_t acc_lds[kBlock / kWave];
-  __shared__ int last_chunk;
 
   if (epi.skip()) return;
   const int t    = blockIdx.x;
@@ -310,78 +313,96 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
       if (sub == 0) epi.row(r, a, epi.load(r), red);
     }
   } else {
-    // Chunk of a long row: the workgroup partial goes to the chunk's slot.
-    // Without long_cnt the row's epilogue runs in long_rows_kernel after this
-    // launch.  With it, the chunk that finishes last runs it here: every chunk
-    // publishes its partial and then counts itself with device-scope atomics
-    // (read-modify-writes, performed where all XCDs see them, each waited for
-    // before the next is issued); the last one reads the partials back the same
-    // way, so no stale L2 copy of another XCD's partial can be read.
+    // Chunk of a long row: the workgroup partial goes to the chunk's slot, and
+    // wave 0 alone finishes the chunk (the other waves retire).  Without long_cnt
+    // the row's epilogue runs in long_rows_kernel after this launch.  With it,
+    // the chunk that finishes last runs it here: every chunk publishes its
+    // partial and then counts itself with device-scope atomics (read-modify-
+    // writes, performed where all XCDs see them, the second issued after the
+    // first completed); the last one reads the partials back the same way, so
+    // no stale L2 copy of another XCD's partial can be read.
     Acc a = Acc::zero();
     for (int k = tid; k < nnz; k += kBlock) a.add(prod[k]);
     a = block_merge(a, acc_lds);
+    if (tid >= kWave) return;
     const int lid = -(mt + 1);
+    if constexpr (Epi::NRED > 0)  // a chunk runs no row epilogue: its reduction slot is 0
+      if (tid == 0)
+#pragma unroll
+        for (int k = 0; k < Epi::NRED; ++k) red_part[(size_t)t * Epi::NRED + k] = 0.0;
     if (!long_cnt) {
       if (tid == 0) long_part[tl.r1] = a.pack();  // the chunk's slot
-    } else {
-      if (tid == 0) {
-        (void)__hip_atomic_exchange(&long_part[tl.r1], a.pack(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __builtin_amdgcn_s_waitcnt(0);
-        const int before = __hip_atomic_fetch_add(&long_cnt[lid], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        last_chunk       = before == plan.longrows[lid].nchunks - 1;
-      }
-      __syncthreads();
-    }
-    if (long_cnt && last_chunk) {
-      if constexpr (Epi::NRED > 0) block_sum_store<Epi::NRED>(red, red_part + (size_t)t * Epi::NRED, red_lds);
-      __syncthreads();
-      merge_long_row<Epi, true>(plan, lid, long_part, red_part, epi, acc_lds, red_lds);
-      if (tid == 0) (void)__hip_atomic_exchange(&long_cnt[lid], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return;
     }
+    int before = 0;
+    if (tid == 0) {
+      (void)__hip_atomic_exchange(&long_part[tl.r1], a.pack(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __builtin_amdgcn_s_waitcnt(0);
+      before = __hip_atomic_fetch_add(&long_cnt[lid * kCntStride], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    before = __shfl(before, 0, kWave);
+    if (before != plan.longrows[lid].nchunks - 1) return;
+    merge_long_row<Epi, true>(plan, lid, long_part, red_part, epi);
+    if (tid == 0) (void)__hip_atomic_exchange(&long_cnt[lid * kCntStride], 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return;
   }
   if constexpr (Epi::NRED > 0) block_sum_store<Epi::NRED>(red, red_part + (size_t)t * Epi::NRED, red_lds);
 }
 
-// Epilogue of the rows longer than a tile: workgroup b merges the chunk
-// partials of long row b (all lanes load, fixed-tree merge) and runs the row
-// epilogue.  Runs after tiled_spmv_kernel on the same stream, so the kernel
-// boundary makes the partials visible across XCDs without in-kernel fences.
-// Reduction partials go to red_part[(ntiles + b) * NRED ...].
+// Epilogue of long row lid, run by one whole wave: lane l merges chunk partials
+// l, l + 64, l + 128, ... in that order (eight loads in flight), a butterfly
+// merges the lanes, and lane 0 runs the row epilogue and writes the row's
+// reduction slot red_part[(ntiles + lid) * NRED ...].  ATOMIC reads the partials
+// with device-scope atomics (the in-launch merge); without it they are plain
+// loads, made visible by the kernel boundary before long_rows_kernel.
 template <class Epi, bool ATOMIC>
 __device__ inline void merge_long_row(const PlanDev& plan, int lid, uint64_t* long_part, double* red_part,
-                                      const Epi& epi, uint64_t* acc_lds, double (*red_lds)[kMaxRed])
+                                      const Epi& epi)
 {
   using Acc        = typename Epi::Acc;
+  constexpr int kU = 8;
+  const int lane   = threadIdx.x & (kWave - 1);
   const LongRow lr = plan.longrows[lid];
   typename Epi::Pre pre{};
-  if (threadIdx.x == 0) pre = epi.load(lr.row);  // in flight during the merge
+  if (lane == 0) pre = epi.load(lr.row);  // in flight during the merge
   Acc tot = Acc::zero();
-  for (int ch = threadIdx.x; ch < lr.nchunks; ch += kBlock) {
-    uint64_t* slot = long_part + lr.first + ch;
-    const uint64_t v =
-      ATOMIC ? __hip_atomic_fetch_add(slot, (uint64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *slot;
-    tot.merge(Acc::unpack(v));
-  }
-  tot = block_merge(tot, acc_lds);
-  double red[kMaxRed];
+  for (int base = lane; base < lr.nchunks; base += kU * kWave) {
+    uint64_t v[kU];
 #pragma unroll
-  for (int k = 0; k < kMaxRed; ++k) red[k] = 0.0;
-  if (threadIdx.x == 0) epi.row(lr.row, tot, pre, red);
-  if constexpr (Epi::NRED > 0)
-    block_sum_store<Epi::NRED>(red, red_part + (size_t)(plan.ntiles + lid) * Epi::NRED, red_lds);
+    for (int u = 0; u < kU; ++u) {
+      const int ch = base + u * kWave;
+      v[u]         = 0;
+      if (ch < lr.nchunks) {
+        uint64_t* slot = long_part + lr.first + ch;
+        v[u] = ATOMIC ? __hip_atomic_fetch_add(slot, (uint64_t)0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : *slot;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+      if (base + u * kWave < lr.nchunks) tot.merge(Acc::unpack(v[u]));
+  }
+#pragma unroll
+  for (int off = kWave / 2; off > 0; off >>= 1) tot.merge(tot.shfl(off));
+  if (lane == 0) {
+    double red[kMaxRed];
+#pragma unroll
+    for (int k = 0; k < kMaxRed; ++k) red[k] = 0.0;
+    epi.row(lr.row, tot, pre, red);
+    if constexpr (Epi::NRED > 0)
+#pragma unroll
+      for (int k = 0; k < Epi::NRED; ++k) red_part[(size_t)(plan.ntiles + lid) * Epi::NRED + k] = red[k];
+  }
 }
 
+// One wave per long row, after tiled_spmv_kernel on the same stream.
 template <class Epi>
-__global__ __launch_bounds__(kBlock) void long_rows_kernel(PlanDev plan,
-                                                           uint64_t* __restrict__ long_part,
-                                                           double* __restrict__ red_part,
-                                                           Epi epi)
+__global__ __launch_bounds__(kWave) void long_rows_kernel(PlanDev plan,
+                                                          uint64_t* __restrict__ long_part,
+                                                          double* __restrict__ red_part,
+                                                          Epi epi)
 {
-  __shared__ double red_lds[kBlock / kWave][kMaxRed];
-  __shared__ uint64_t acc_lds[kBlock / kWave];
   if (epi.skip()) return;
-  merge_long_row<Epi, false>(plan, blockIdx.x, long_part, red_part, epi, acc_lds, red_lds);
+  merge_long_row<Epi, false>(plan, blockIdx.x, long_part, red_part, epi);
 }
 
 // out[k] = sum over blocks b (in order of a fixed tree) of part[b * nred + k].
