@@ -358,9 +358,10 @@ def solve(lp, tol: float = 1e-4, precision: str = "fp64", iteration_limit: int |
 
 def batch_solve(lps, tol: float = 1e-4, precision: str = "fp64", iteration_limit: int | None = None,
                 time_limit: float | None = None, **params) -> list[Result]:
-    """cuOptAmdBatchSolve over ``lps``: one persistent launch for the whole list.  Further
-    keyword arguments are settings by parameter name."""
-    probs = [Problem(lp) for lp in lps]
+    """cuOptAmdBatchSolve over ``lps`` (LP objects or already built ``Problem`` handles): one
+    persistent launch for the whole list.  Further keyword arguments are settings by parameter
+    name."""
+    probs = [lp if isinstance(lp, Problem) else Problem(lp) for lp in lps]
     s = make_settings(tol, precision, iteration_limit, time_limit)
     for name, v in params.items():
         _check(s.set(name, v), name)

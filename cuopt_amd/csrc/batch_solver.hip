@@ -3,8 +3,9 @@
 // The heterogeneous batch of the reference (call_batch_solve, cython_solve.cu:286-352,
 // which runs independent solve_lp calls on a few streams) becomes a single kernel
 // launch: workgroup g owns LP order[g] for its whole life -- CSR transpose,
-// Ruiz / Pock-Chambolle scaling, bound / objective rescaling, the power
-// iteration for the step size, and the Stable3 loop with FP64 termination
+// Ruiz / Pock-Chambolle scaling, bound / objective rescaling, the step size
+// (the reference's power iteration in FP64; in FP32-mixed, Lanczos as the
+// single-LP solver's), and the Stable3 loop with FP64 termination
 // checks, cuPDLPx restarts and the PID primal weight -- so the batch needs no
 // host synchronisation and no per-iteration launch at all.  LPs are handed out
 // largest first, so the hardware dispatcher's in-order assignment of
@@ -25,6 +26,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <numeric>
 #include <random>
 #include <thread>
@@ -106,9 +108,14 @@ struct BatchArgs {
   // LPs than resident workgroups, later LPs start later, so the launch can take
   // longer than the limit (at most about the number of dispatch waves times it).
   unsigned long long tick_limit;
+  double* lz;                     // FP32-mixed: Lanczos coefficients, kLzStride per LP (by LP index)
   const int* blk;                 // row-block starts of every LP's A and A^T (plan_row_blocks)
   int lds_bytes;                  // dynamic LDS of the launch: the stage, then the gathered vectors
 };
+
+// Lanczos steps at most (and their coefficient slots per LP), as in the single-LP solver.
+constexpr int kLzMax    = 240;
+constexpr int kLzStride = 2 * kLzMax + 2;
 
 constexpr int kMajorB = 200;
 constexpr double kArtificialB = 0.36, kSufficientB = 0.2, kNecessaryB = 0.8;
@@ -421,6 +428,72 @@ struct BPowerAEpi {
   }
 };
 
+// Lanczos on A~ A~^T (FP32-mixed): A~^T q, then A~ (A~^T q), FP64 values formed
+// on the fly in the operand order of the scaled copies.
+struct BLzAtEpi {
+  using Mat = double;
+  using Acc = AccSum64;
+  struct Pre {};
+  const double* Dr;
+  const double* Dc;
+  const double* q;
+  double* atq;
+  __device__ double product(double v, int col, int r) const { return ((v * Dc[r]) * Dr[col]) * q[col]; }
+  __device__ Pre load(int) const { return Pre{}; }
+  __device__ void row(int r, const Acc& s, const Pre&, double (&)[kMaxRed]) const { atq[r] = s.value(); }
+};
+struct BLzAEpi {
+  using Mat = double;
+  using Acc = AccSum64;
+  struct Pre {};
+  const double* Dr;
+  const double* Dc;
+  const double* atq;
+  double* z;
+  __device__ double product(double v, int col, int r) const { return ((v * Dr[r]) * Dc[col]) * atq[col]; }
+  __device__ Pre load(int) const { return Pre{}; }
+  __device__ void row(int r, const Acc& s, const Pre&, double (&)[kMaxRed]) const { z[r] = s.value(); }
+};
+
+// Largest eigenvalue of the symmetric tridiagonal matrix of k Lanczos steps,
+// ab[2i] = alpha_i (diagonal) and ab[2i+1] = beta_i^2 (squared off-diagonal),
+// by Sturm counts: each round every lane counts the eigenvalues below one of kBT
+// points spread evenly over the bracket (Gershgorin to start), and the bracket
+// becomes the gap between the last point with fewer than k below and the first
+// with all k below, kBT + 1 times narrower.  Stops at the single-LP solver's
+// bisection tolerance (relative width 1e-15).  Whole workgroup; slot is one LDS int.
+__device__ inline double wg_tridiag_max_eig(const double* ab, int k, int* slot)
+{
+  double lo = 0.0, hi = 0.0;
+  for (int i = 0; i < k; ++i) {
+    const double r = (i > 0 ? sqrt(ab[2 * i - 1]) : 0.0) + (i + 1 < k ? sqrt(ab[2 * i + 1]) : 0.0);
+    lo = fmin(lo, ab[2 * i] - r);
+    hi = fmax(hi, ab[2 * i] + r);
+  }
+  for (int round = 0; round < 64 && hi - lo > 1e-15 * fmax(fabs(hi), 1e-300); ++round) {
+    const double step = (hi - lo) / (double)(kBT + 1);
+    const double x    = lo + step * (double)(threadIdx.x + 1);
+    int c             = 0;
+    double d          = 1.0;
+    for (int i = 0; i < k; ++i) {
+      d = ab[2 * i] - x - (i > 0 ? ab[2 * i - 1] / d : 0.0);
+      if (d == 0.0) d = -1e-300;
+      if (d < 0.0) ++c;
+    }
+    if (threadIdx.x == 0) *slot = kBT;
+    __syncthreads();
+    if (c >= k) atomicMin(slot, (int)threadIdx.x);
+    __syncthreads();
+    const int f = *slot;
+    __syncthreads();
+    const double nhi = f < kBT ? lo + step * (double)(f + 1) : hi;
+    const double nlo = f > 0 ? lo + step * (double)f : lo;
+    lo               = uniform_d(nlo);
+    hi               = uniform_d(nhi);
+  }
+  return hi;
+}
+
 // y = A x (FP64), row index passed to product() as in the other batch passes.
 struct BPlainEpi {
   using Mat = double;
@@ -618,36 +691,104 @@ __global__ __launch_bounds__(kBT, kWgPerCu) void batch_pdlp_kernel(BatchArgs<V> 
     cb_w[i]    = bb;
     cb_orig[i] = make_double2(lc[i], uc[i]);
   }
-  // ---- power iteration (compute_initial_step_size, pdlp.cu:2622-2738)
-  for (int i = tid; i < m; i += kBT) pz[i] = A.z0[i];
-  zero_red(red);
-  for (int i = tid; i < m; i += kBT) red[0] += A.z0[i] * A.z0[i];
-  wg_sum<1>(red, lds);
-  double nrm = sqrt(red[0]), s2 = 0.0;
+  double s2  = 0.0;
   int piters = 0;
-  double* zo = pz;
-  double* zn = pz2;
-  while (piters < 5000) {
-    zero_red(red);
-    rows_pass(tptr, tidx, torig, d.lgt, tblk, st, BPowerAtEpi{Dr, Dc, zo, nrm, patq}, red);
-    __syncthreads();
-    rows_pass(aptr, aidx, aorig, d.lga, ablk, st, BPowerAEpi{Dr, Dc, patq, zo, nrm, zn}, red);
-    wg_sum<1>(red, lds);
-    s2 = red[0];
+  if constexpr (std::is_same<V, float>::value) {
+    // ---- FP32-mixed: Lanczos on A~ A~^T, as the single-LP solver's
+    // lanczos_sigma_max_sq (hashed start, at most 240 steps, stop when the Ritz
+    // value moves by at most 1e-6 relative or beta^2 vanishes).  Scratch vectors
+    // not yet in use: q, q_prev in pz / pz2, w in rowv (scaling is done), A~ A~^T q
+    // in yu (first written by the first termination check).
+    double* q  = pz;
+    double* qp = pz2;
+    double* w  = rowv;
+    double* zb = yu;
+    double* ab = A.lz + (size_t)A.order[blockIdx.x] * kLzStride;
     zero_red(red);
     for (int i = tid; i < m; i += kBT) {
-      const double q = zo[i] / nrm;
-      const double r = q * -s2 + zn[i];
-      red[0] += r * r;
-      red[1] += zn[i] * zn[i];
+      const double v = hashed_start(i);
+      q[i]           = v;
+      qp[i]          = 0.0;
+      red[0] += v * v;
     }
-    wg_sum<2>(red, lds);
-    ++piters;
-    if (sqrt(red[0]) < 1e-4) break;
-    nrm = sqrt(red[1]);
-    double* t = zo;
-    zo        = zn;
-    zn        = t;
+    wg_sum<1>(red, lds);
+    const double inv = 1.0 / sqrt(red[0]);
+    for (int i = tid; i < m; i += kBT) q[i] = q[i] * inv;
+    __syncthreads();
+    const int kmax = min(m, kLzMax);
+    double theta = 0.0, prev = -1.0;
+    while (piters < kmax) {
+      const int j = piters;
+      zero_red(red);
+      rows_pass(tptr, tidx, torig, d.lgt, tblk, st, BLzAtEpi{Dr, Dc, q, patq}, red);
+      __syncthreads();
+      rows_pass(aptr, aidx, aorig, d.lga, ablk, st, BLzAEpi{Dr, Dc, patq, zb}, red);
+      const double bp = j > 0 ? sqrt(ab[2 * j - 1]) : 0.0;
+      zero_red(red);
+      for (int i = tid; i < m; i += kBT) {
+        const double v = zb[i] - bp * qp[i];
+        w[i]           = v;
+        red[0] += q[i] * v;
+      }
+      wg_sum<1>(red, lds);
+      const double alpha = red[0];
+      zero_red(red);
+      for (int i = tid; i < m; i += kBT) {
+        const double v = w[i] - alpha * q[i];
+        w[i]           = v;
+        red[0] += v * v;
+      }
+      wg_sum<1>(red, lds);
+      const double b2 = red[0];
+      if (tid == 0) {
+        ab[2 * j]     = alpha;
+        ab[2 * j + 1] = b2;
+      }
+      __syncthreads();
+      piters = j + 1;
+      theta  = wg_tridiag_max_eig(ab, j + 1, scan_lds);
+      const bool stop = (j > 2 && fabs(theta - prev) <= 1e-6 * theta) || !(b2 > 1e-26 * theta * theta);
+      prev            = theta;
+      if (stop) break;
+      const double sb = sqrt(b2);
+      for (int i = tid; i < m; i += kBT) {
+        qp[i] = q[i];
+        q[i]  = w[i] / sb;
+      }
+      __syncthreads();
+    }
+    s2 = theta;
+  } else {
+    // ---- power iteration (compute_initial_step_size, pdlp.cu:2622-2738)
+    for (int i = tid; i < m; i += kBT) pz[i] = A.z0[i];
+    zero_red(red);
+    for (int i = tid; i < m; i += kBT) red[0] += A.z0[i] * A.z0[i];
+    wg_sum<1>(red, lds);
+    double nrm = sqrt(red[0]);
+    double* zo = pz;
+    double* zn = pz2;
+    while (piters < 5000) {
+      zero_red(red);
+      rows_pass(tptr, tidx, torig, d.lgt, tblk, st, BPowerAtEpi{Dr, Dc, zo, nrm, patq}, red);
+      __syncthreads();
+      rows_pass(aptr, aidx, aorig, d.lga, ablk, st, BPowerAEpi{Dr, Dc, patq, zo, nrm, zn}, red);
+      wg_sum<1>(red, lds);
+      s2 = red[0];
+      zero_red(red);
+      for (int i = tid; i < m; i += kBT) {
+        const double q = zo[i] / nrm;
+        const double r = q * -s2 + zn[i];
+        red[0] += r * r;
+        red[1] += zn[i] * zn[i];
+      }
+      wg_sum<2>(red, lds);
+      ++piters;
+      if (sqrt(red[0]) < 1e-4) break;
+      nrm = sqrt(red[1]);
+      double* t = zo;
+      zo        = zn;
+      zn        = t;
+    }
   }
   const double eta = 0.998 / sqrt(s2);
 
@@ -979,16 +1120,19 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
     max_m = std::max(max_m, lp.m);
   }
   // host arrays (rows sorted by column, as problem_t keeps them)
-  std::vector<int> h_ptr(tot_ap), h_idx(tot_nnz);
-  std::vector<double> h_val(tot_nnz), h_c(tot_n), h_lb(tot_n), h_ub(tot_n), h_lc(tot_m), h_uc(tot_m);
+  // Left uninitialised (every element is written below): zero-filling half a
+  // gigabyte on one thread took longer than the threaded copy itself.
+  std::unique_ptr<int[]> h_ptr(new int[tot_ap]), h_idx(new int[tot_nnz]);
+  std::unique_ptr<double[]> h_val(new double[tot_nnz]), h_c(new double[tot_n]), h_lb(new double[tot_n]),
+    h_ub(new double[tot_n]), h_lc(new double[tot_m]), h_uc(new double[tot_m]);
   auto fill = [&](int t, int nt) {
     for (int q = t; q < nb; q += nt) {
       const int a      = active[q];
       const LpHost& lp = *lps[a];
       const BatchDesc& d = desc[a];
-      std::copy(lp.rowptr.begin(), lp.rowptr.begin() + lp.m + 1, h_ptr.begin() + d.aptr0);
-      std::copy(lp.colidx.begin(), lp.colidx.begin() + d.nnz, h_idx.begin() + d.nz0);
-      std::copy(lp.vals.begin(), lp.vals.begin() + d.nnz, h_val.begin() + d.nz0);
+      std::copy(lp.rowptr.begin(), lp.rowptr.begin() + lp.m + 1, h_ptr.get() + d.aptr0);
+      std::copy(lp.colidx.begin(), lp.colidx.begin() + d.nnz, h_idx.get() + d.nz0);
+      std::copy(lp.vals.begin(), lp.vals.begin() + d.nnz, h_val.get() + d.nz0);
       for (int i = 0; i < lp.m; ++i) {
         const int b = lp.rowptr[i], e = lp.rowptr[i + 1];
         bool sorted = true;
@@ -1007,8 +1151,8 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
         h_lb[d.col0 + j] = lp.lb[j];
         h_ub[d.col0 + j] = lp.ub[j];
       }
-      std::copy(lcs[a].begin(), lcs[a].end(), h_lc.begin() + d.row0);
-      std::copy(ucs[a].begin(), ucs[a].end(), h_uc.begin() + d.row0);
+      std::copy(lcs[a].begin(), lcs[a].end(), h_lc.get() + d.row0);
+      std::copy(ucs[a].begin(), ucs[a].end(), h_uc.get() + d.row0);
     }
   };
   {
@@ -1081,17 +1225,18 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
   DevBuf<double> xstar(refine ? N : 1), ystar(refine ? M : 1), ax64(refine ? M : 1), aty64(refine ? N : 1);
   DevBuf<V> kink(refine ? M : 1);
   DevBuf<BatchOut> out(std::max<size_t>(lps.size(), 1));
+  DevBuf<double> lz(std::is_same<V, float>::value ? std::max<size_t>(lps.size(), 1) * kLzStride : 1);
   d_desc.upload(desc, s);
   if (!h_blk.empty()) d_blk.upload(h_blk, s);
   d_order.upload(order, s);
-  a_ptr.upload(h_ptr, s);
-  a_idx.upload(h_idx, s);
-  a_orig.upload(h_val, s);
-  c.upload(h_c, s);
-  lb.upload(h_lb, s);
-  ub.upload(h_ub, s);
-  lc.upload(h_lc, s);
-  uc.upload(h_uc, s);
+  a_ptr.upload(h_ptr.get(), tot_ap, s);
+  a_idx.upload(h_idx.get(), tot_nnz, s);
+  a_orig.upload(h_val.get(), tot_nnz, s);
+  c.upload(h_c.get(), tot_n, s);
+  lb.upload(h_lb.get(), tot_n, s);
+  ub.upload(h_ub.get(), tot_n, s);
+  lc.upload(h_lc.get(), tot_m, s);
+  uc.upload(h_uc.get(), tot_m, s);
   dz0.upload(z0, s);
 
   BatchArgs<V> args{};
@@ -1141,6 +1286,7 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
   args.patq = patq.get();
   args.z0 = dz0.get();
   args.out = out.get();
+  args.lz  = lz.get();
   args.tol_ap = st.abs_primal;
   args.tol_rp = st.rel_primal;
   args.tol_ad = st.abs_dual;

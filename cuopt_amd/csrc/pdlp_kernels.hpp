@@ -39,6 +39,18 @@ __device__ inline T mx(T a, T b)
 }
 __device__ inline double fin0(double v) { return isfinite(v) ? v : 0.0; }
 
+// Deterministic Lanczos start vector in [-1, 1): an integer hash of the index.
+__device__ inline double hashed_start(int i)
+{
+  uint32_t h = (uint32_t)i * 2654435761u + 0x9e3779b9u;
+  h ^= h >> 16;
+  h *= 0x85ebca6bu;
+  h ^= h >> 13;
+  h *= 0xc2b2ae35u;
+  h ^= h >> 16;
+  return (double)h * (2.0 / 4294967296.0) - 1.0;
+}
+
 template <typename V>
 struct AccFor;
 template <>

@@ -216,16 +216,7 @@ struct HashStartF {  // deterministic start vector in [-1, 1) (integer hash of t
   static constexpr int NRED = 0;
   double* v;
   __device__ bool skip() const { return false; }
-  __device__ void operator()(int i, double (&)[kMaxRed]) const
-  {
-    uint32_t h = (uint32_t)i * 2654435761u + 0x9e3779b9u;
-    h ^= h >> 16;
-    h *= 0x85ebca6bu;
-    h ^= h >> 13;
-    h *= 0xc2b2ae35u;
-    h ^= h >> 16;
-    v[i] = (double)h * (2.0 / 4294967296.0) - 1.0;
-  }
+  __device__ void operator()(int i, double (&)[kMaxRed]) const { v[i] = hashed_start(i); }
 };
 struct ScaleByF {  // v *= s
   static constexpr int NRED = 0;

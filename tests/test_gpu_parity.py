@@ -581,6 +581,29 @@ def test_batch_fp32_mixed_config0_style():
     assert max(r.iterations for r in rs) < 20000  # no stalled member (recentring)
 
 
+def test_batch_fp32_step_size_by_lanczos():
+    # FP32-mixed batch members estimate sigma_max(A~)^2 as FP32-mixed single-LP solves
+    # do: Lanczos on A~ A~^T from the same hashed start, with the Ritz value found by
+    # Sturm counts.  Only the order of the row sums differs, so the estimate agrees
+    # with the single-LP one far inside the 0.2 % margin of eta = 0.998 / sigma_max,
+    # and both agree with a dense SVD of the oracle's scaled matrix.
+    import scipy.sparse as sp
+
+    lps = [G.random_lp(m=m, n=int(2.5 * m), seed=200 + m) for m in (50, 120, 400)]
+    lps.append(G.skewed_lp(m=2000, n=4000, seed=5))
+    rs = c_api.batch_solve(lps, tol=1e-4, precision="fp32_mixed", time_limit=TL, iteration_limit=0)
+    for lp, r in zip(lps, rs):
+        s = c_api.solve(lp, tol=1e-4, precision="fp32_mixed", time_limit=TL, iteration_limit=0)
+        assert abs(r.info["sigma_max_sq"] - s.info["sigma_max_sq"]) <= 1e-10 * s.info["sigma_max_sq"], lp.name
+        assert abs(r.info["power_iterations"] - s.info["power_iterations"]) <= 1, lp.name
+        assert r.info["power_iterations"] < 100
+        o = O.setup(lp)
+        At = sp.csr_matrix((o["scaled_values"], lp.col_indices, lp.row_offsets), shape=(lp.m, lp.n)).toarray()
+        smax2 = np.linalg.svd(At, compute_uv=False)[0] ** 2
+        assert abs(r.info["sigma_max_sq"] - smax2) <= 1e-5 * smax2, lp.name
+        assert abs(r.info["eta0"] - 0.998 / np.sqrt(r.info["sigma_max_sq"])) <= 1e-14 * r.info["eta0"]
+
+
 def test_batch_fp32_recentring_to_1e8():
     lps = G.batch_lps(count=4096, seed=0)
     pick = [lps[i] for i in (1672, 2723, 3846, 0, 1, 2)]
