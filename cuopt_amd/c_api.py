@@ -35,6 +35,7 @@ INFO_NAMES = [
     "solve_time", "setup_time", "loop_time", "eta0", "omega0", "omega", "restarts", "power_iterations",
     "sigma_max_sq", "beta", "gamma", "precision", "checks", "structure_time", "scaling_time", "power_time",
     "refine_rounds", "tile_a", "lanes_bias_a", "tile_at", "lanes_bias_at", "tune_time", "merge_a", "merge_at",
+    "lp_time", "lp_start",
 ]
 TRACE_WIDTH = 10
 PRECISION = {"fp64": 0, "fp32_mixed": 1, "mixed": 1, "fp32": 1}

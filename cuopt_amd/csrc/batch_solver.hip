@@ -61,6 +61,7 @@ struct BatchOut {
   double l2p, l2d, pobj, dobj, gap, absobj, b_norm, c_norm;
   double eta, omega, sigma_max_sq, beta, gamma;
   int status, iters, restarts, checks, power_iters, recentrings, pad[2];
+  unsigned long long t0, t1;  // wall clock of the workgroup's start and end (device ticks)
 };
 
 template <typename V>
@@ -191,14 +192,17 @@ __device__ inline void zero_red(double (&r)[kMaxRed])
 // fixed shuffle tree, so the sums do not depend on the blocking.
 constexpr int kStage      = 2560;                // entries per stage buffer
 constexpr int kBlockNnz   = kStage - 8;          // nonzeros per staged block (room for the copy's slack)
-constexpr int kStageBytes = 2 * kStage * 12;     // two buffers of values (8 bytes) and indices (4 bytes)
+constexpr int kStageBytes = 2 * kStage * 12;     // two buffers of 8-byte values and 4-byte indices
 constexpr int kNnzPad     = 4;                   // extra elements after each nonzero array (16 bytes or more)
 
-struct Stage {               // LDS of one workgroup
-  int* idx;                  // 2 x kStage column indices (buffer b at idx + b * kStage)
-  unsigned char* vals;       // 2 x kStage values of 8 bytes (FP64 or FP32 matrices), same order
-  uint64_t* merge;           // kBT / kWave accumulator slots
-  bool dbl;                  // two buffers; else one (block b's copy is waited for before b)
+// The stage occupies the start of the workgroup's dynamic LDS.  A pass over
+// values of VB bytes (8 for the FP64 matrices, 4 for FP32 working ones) lays
+// out its buffers by VB: values of buffer b at base + b * kStage * VB, then the
+// indices; it double-buffers when dbl8 / dbl4 says the two buffers fit.
+struct Stage {
+  unsigned char* base;
+  uint64_t* merge;  // kBT / kWave accumulator slots
+  bool dbl8, dbl4;  // two buffers for passes over 8- / 4-byte values
 };
 
 struct RowBlocks {
@@ -285,19 +289,22 @@ __device__ inline void rows_pass(const int* __restrict__ ptr,
                                  const Epi& epi,
                                  double (&red)[kMaxRed])
 {
-  using Acc       = typename Epi::Acc;
-  using Mat       = typename Epi::Mat;
-  constexpr int W = (int)(sizeof(Mat) / 4);  // dwords per value
+  using Acc        = typename Epi::Acc;
+  using Mat        = typename Epi::Mat;
+  constexpr int W  = (int)(sizeof(Mat) / 4);  // dwords per value
+  constexpr int VB = 4 * W;                   // bytes per staged value
   const int G = 1 << lg, sub = threadIdx.x & (G - 1), grp = threadIdx.x >> lg;
   const int nb = blocks.nblk;
   if (nb <= 0) return;
+  const bool dbl       = W == 1 ? st.dbl4 : st.dbl8;
+  int* const sidx_base = reinterpret_cast<int*>(st.base + (dbl ? 2 : 1) * kStage * VB);
   // Buffer addresses are computed, not indexed from an array: a runtime-indexed
   // pointer array would live in scratch, and its load would wait for the copy
   // in flight.
   auto stage = [&](int za, int zb, int buf) {
     copy_to_lds(reinterpret_cast<const int*>(vals), (long long)W * za, (long long)W * zb,
-                reinterpret_cast<int*>(st.vals + buf * (kStage * 8)));
-    copy_to_lds(idx, za, zb, st.idx + buf * kStage);
+                reinterpret_cast<int*>(st.base + buf * (kStage * VB)));
+    copy_to_lds(idx, za, zb, sidx_base + buf * kStage);
   };
   // (r0, r1) are block bi's rows, (r1, r2) block bi + 1's, z the matching
   // nonzero offsets; r3 / z3 (block bi + 2's end) are loaded during block bi.
@@ -308,13 +315,13 @@ __device__ inline void rows_pass(const int* __restrict__ ptr,
     r2 = blocks.start[2];
     z2 = ptr[r2];
   }
-  if (st.dbl && z1 - z0 <= kBlockNnz) stage(z0, z1, 0);
+  if (dbl && z1 - z0 <= kBlockNnz) stage(z0, z1, 0);
   // This lane group's row of the current block: its ptr entries.  The loads
   // are clamped into range instead of guarded, so that no branch waits for them
   // (a wait for a load also waits for every copy issued before it).
   int pb = ptr[min(r0 + grp, r1)], pe = ptr[min(r0 + grp + 1, r1)];
   for (int bi = 0; bi < nb; ++bi) {
-    const int cur = st.dbl ? bi & 1 : 0;
+    const int cur = dbl ? bi & 1 : 0;
     // Every wave's copy of block bi has landed (the barrier waits for each
     // wave's memory operations), and block bi - 1's reads of the other buffer
     // are done, so it may be refilled.
@@ -322,8 +329,8 @@ __device__ inline void rows_pass(const int* __restrict__ ptr,
     const int b = pb - z0, e = pe - z0;
     const int r3          = bi + 3 <= nb ? blocks.start[bi + 3] : r2;
     const bool stage_next = bi + 1 < nb && z2 - z1 <= kBlockNnz;
-    if (st.dbl && stage_next) stage(z1, z2, cur ^ 1);
-    if (!st.dbl && z1 - z0 <= kBlockNnz) {  // one buffer: copy this block now and wait for it
+    if (dbl && stage_next) stage(z1, z2, cur ^ 1);
+    if (!dbl && z1 - z0 <= kBlockNnz) {  // one buffer: copy this block now and wait for it
       stage(z0, z1, 0);
       __syncthreads();
     }
@@ -339,8 +346,8 @@ __device__ inline void rows_pass(const int* __restrict__ ptr,
       const auto pre    = epi.load(r);
       // where the copy put nonzero z0 (values are 8-byte aligned when W = 2,
       // so their dword offset is even)
-      const int* sidx  = st.idx + cur * kStage + dword_mis(idx + z0);
-      const Mat* svals = reinterpret_cast<const Mat*>(st.vals + cur * (kStage * 8)) +
+      const int* sidx  = sidx_base + cur * kStage + dword_mis(idx + z0);
+      const Mat* svals = reinterpret_cast<const Mat*>(st.base + cur * (kStage * VB)) +
                          dword_mis(reinterpret_cast<const int*>(vals) + (long long)W * z0) / W;
       Acc a = Acc::zero();
       int k = b + sub;
@@ -554,17 +561,20 @@ __global__ __launch_bounds__(kBT, kWgPerCu) void batch_pdlp_kernel(BatchArgs<V> 
   // x-bar (dual step) -- live in LDS when the LP fits: each gather is then an
   // LDS read instead of a round trip to L2 / MALL.  Nothing reads them after
   // the kernel (the solution is the unscaled xu / yu / su).
-  // A double-buffered stage when the vectors fit beside it (or do not fit at
-  // all); a single buffer when that lets them fit: gathers from LDS outweigh
-  // the copy overlap.
+  // They sit at the end of the dynamic LDS when a single-buffered 8-byte stage
+  // still fits before them (gathers from LDS outweigh the copy overlap); each
+  // pass of the PDHG loop then double-buffers when its two buffers fit in what
+  // is left.  Until the loop starts the vectors are unused, so the setup passes
+  // may double-buffer over the whole of it.
+  const long long lds_bytes = A.lds_bytes;
   const long long vec_bytes = (long long)(2 * m + n) * (long long)sizeof(V);
-  const bool dbl = kStageBytes + vec_bytes <= (long long)A.lds_bytes ||
-                   kStageBytes / 2 + vec_bytes > (long long)A.lds_bytes;
-  const int stage_bytes = dbl ? kStageBytes : kStageBytes / 2;
-  const Stage st{reinterpret_cast<int*>(dyn_lds + (dbl ? 2 : 1) * kStage * 8), dyn_lds, merge_lds, dbl};
+  const long long vec_off   = (lds_bytes - vec_bytes) & ~15LL;
+  const bool vec_in_lds     = vec_off >= (long long)kStage * 12;
+  const long long region    = vec_in_lds ? vec_off : lds_bytes;  // the stage's LDS in the loop
+  Stage st{dyn_lds, merge_lds, 2LL * kStage * 12 <= lds_bytes, 2LL * kStage * 8 <= lds_bytes};
   const RowBlocks ablk{A.blk + d.ablk0, d.nablk}, tblk{A.blk + d.tblk0, d.ntblk};
-  if (stage_bytes + vec_bytes <= (long long)A.lds_bytes) {
-    V* base = reinterpret_cast<V*>(dyn_lds + stage_bytes);
+  if (vec_in_lds) {
+    V* base = reinterpret_cast<V*>(dyn_lds + vec_off);
     ys[0]   = base;
     ys[1]   = base + m;
     xbar    = base + 2 * m;
@@ -791,6 +801,10 @@ __global__ __launch_bounds__(kBT, kWgPerCu) void batch_pdlp_kernel(BatchArgs<V> 
     }
   }
   const double eta = 0.998 / sqrt(s2);
+
+  // ---- the loop's stage: beside the vectors
+  st.dbl8 = 2LL * kStage * 12 <= region;
+  st.dbl4 = 2LL * kStage * 8 <= region;
 
   // ---- initial iterates (zeros, projected)
   for (int j = tid; j < n; j += kBT) {
@@ -1079,6 +1093,8 @@ __global__ __launch_bounds__(kBT, kWgPerCu) void batch_pdlp_kernel(BatchArgs<V> 
     o.checks = checks;
     o.power_iters = piters;
     o.recentrings = recentrings;
+    o.t0 = t_start;
+    o.t1 = wall_clock64();
     A.out[A.order[blockIdx.x]] = o;
   }
 }
@@ -1309,18 +1325,17 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
     args.tick_limit    = ticks >= 1.8e19 ? 0ULL : (unsigned long long)ticks;
   }
   {
-    // LDS: the row-block stage, then the gathered vectors of the largest LP when
-    // they fit in the per-workgroup limit and a CU's LDS divided by kWgPerCu.
+    // LDS: the row-block stage and, at the end, each LP's gathered vectors when
+    // they fit; as much as the per-workgroup limit and a CU's LDS divided by
+    // kWgPerCu allow.
     int dev = 0, per_block = 0, per_cu = 0;
     CUOPT_HIP_CHECK(hipGetDevice(&dev));
     CUOPT_HIP_CHECK(hipDeviceGetAttribute(&per_block, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
     CUOPT_HIP_CHECK(hipDeviceGetAttribute(&per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev));
     const long long static_lds = 4096;  // the kernel's own __shared__ arrays, rounded up
     long long cap = std::min<long long>(per_block, per_cu / kWgPerCu) - static_lds;
-    long long need = 0;
-    for (int a : active) need = std::max<long long>(need, (2LL * desc[a].m + desc[a].n) * (long long)sizeof(V));
     if (cap < kStageBytes) throw std::runtime_error("batch solve: the device has too little LDS per workgroup");
-    args.lds_bytes = (int)std::min<long long>(kStageBytes + need, cap);
+    args.lds_bytes = (int)cap;  // two workgroups per CU either way (their register budget)
   }
   // HIP events around the persistent launch: its device time is the batch's loop time.
   hipEvent_t ev[2] = {nullptr, nullptr};
@@ -1350,6 +1365,15 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   float kernel_ms   = 0.f;
   CUOPT_HIP_CHECK(hipEventElapsedTime(&kernel_ms, ev[0], ev[1]));
+  double tick_hz = 1.0;
+  unsigned long long first_tick = ~0ULL;
+  {
+    int dev = 0, khz = 0;
+    CUOPT_HIP_CHECK(hipGetDevice(&dev));
+    CUOPT_HIP_CHECK(hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, dev));
+    tick_hz = std::max(1, khz) * 1e3;
+    for (int a : active) first_tick = std::min(first_tick, h_out[a].t0);
+  }
   for (int a : active) {
     const LpHost& lp   = *lps[a];
     const BatchDesc& d = desc[a];
@@ -1388,6 +1412,8 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
     r.info[CUOPT_AMD_INFO_PRECISION]    = std::is_same<V, double>::value ? 0 : 1;
     r.info[CUOPT_AMD_INFO_CHECKS]       = o.checks;
     r.info[CUOPT_AMD_INFO_REFINE_ROUNDS] = o.recentrings;
+    r.info[CUOPT_AMD_INFO_LP_TIME]      = (double)(o.t1 - o.t0) / tick_hz;
+    r.info[CUOPT_AMD_INFO_LP_START]     = (double)(o.t0 - first_tick) / tick_hz;
   }
   return R;
 }

@@ -122,7 +122,11 @@ extern "C" {
 #define CUOPT_AMD_INFO_MERGE_A             31 /* rows of A longer than a tile: 1 merged by their last chunk
                                                  in the SpMV launch, 0 by a launch of their own, -1 none */
 #define CUOPT_AMD_INFO_MERGE_AT            32 /* the same over A^T */
-#define CUOPT_AMD_INFO_COUNT               33
+#define CUOPT_AMD_INFO_LP_TIME             33 /* batch solves: this LP's device time, from its workgroup's
+                                                 start to its end (s); single solves: 0 */
+#define CUOPT_AMD_INFO_LP_START            34 /* batch solves: when this LP's workgroup started, after the
+                                                 first one of the launch (s); single solves: 0 */
+#define CUOPT_AMD_INFO_COUNT               35
 
 /* Copies up to n slots of the last solve's statistics into info. */
 cuopt_int_t cuOptAmdGetTerminationInfo(cuOptSolution solution, cuopt_float_t* info, cuopt_int_t n);
