@@ -16,6 +16,7 @@
 #include <climits>
 #include <cstdio>
 #include <chrono>
+#include <thread>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -648,6 +649,13 @@ __global__ __launch_bounds__(kBlock) void label_kernel(int n, const int* __restr
 }
 
 // dst[k] = src[perm[k]] (to the internal column order) and dst[perm[k]] = src[k] (back).
+// out[i] = src[pos[i]]: index entries at chosen positions.
+__global__ __launch_bounds__(kBlock) void gather_idx_kernel(int n, const int* __restrict__ pos,
+                                                            const int* __restrict__ src, int* __restrict__ out)
+{
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) out[i] = src[pos[i]];
+}
+
 __global__ __launch_bounds__(kBlock) void gather_kernel(int n, const int* __restrict__ perm,
                                                         const double* __restrict__ src, double* __restrict__ dst)
 {
@@ -785,18 +793,27 @@ void shift_lanes(TilePlan& p, int bias)
     if (v >= 0) v = std::min(6, std::max(0, v + bias));
 }
 
-TilePlan make_plan(const MatStruct& M, int tile, int bias, const int* colidx)
+// Where the planner finds the column indices of its XCD-aware order: the whole
+// host array, a sample holding the positions it reads, or neither (no reordering).
+struct ColSource {
+  const int* host         = nullptr;
+  const ColSample* sample = nullptr;
+};
+
+TilePlan make_plan(const MatStruct& M, int tile, int bias, ColSource cols)
 {
   if (const char* e = std::getenv("CUOPT_AMD_XCD_ORDER"))
-    if (std::atoi(e) == 0) colidx = nullptr;
-  TilePlan p = plan_tiles(M.h_ptr.data(), M.nrows, tile - kVecPad, kMaxRowsPerTile, kBlock, colidx, M.ncols);
+    if (std::atoi(e) == 0) cols = ColSource{};
+  TilePlan p = cols.sample
+                 ? plan_tiles(M.h_ptr.data(), M.nrows, tile - kVecPad, kMaxRowsPerTile, kBlock, *cols.sample, M.ncols)
+                 : plan_tiles(M.h_ptr.data(), M.nrows, tile - kVecPad, kMaxRowsPerTile, kBlock, cols.host, M.ncols);
   if (bias) shift_lanes(p, bias);
   return p;
 }
 
-void build_plan(MatStruct& M, hipStream_t s, const int* colidx = nullptr, int merge_mode = -1)
+void build_plan(MatStruct& M, hipStream_t s, ColSource cols = {}, int merge_mode = -1)
 {
-  upload_plan(M.plan, make_plan(M, M.tile, 0, colidx), s, merge_mode);
+  upload_plan(M.plan, make_plan(M, M.tile, 0, cols), s, merge_mode);
 }
 
 // The tiled SpMV launch (plus the long-row merge) over raw device pointers:
@@ -870,7 +887,7 @@ uint64_t fnv1a(const int* p, size_t n)
 // Picks (and installs in M) the fastest candidate; returns whether it timed anything.
 // merge_mode as pdlp_long_row_merge: -1 times both long-row merges, 0 / 1 only that one.
 template <typename V>
-bool autotune_spmv(MatStruct& M, const V* vals, const int* colidx, hipStream_t s, int merge_mode, SpmvChoice& out)
+bool autotune_spmv(MatStruct& M, const V* vals, ColSource cols, hipStream_t s, int merge_mode, SpmvChoice& out)
 {
   const std::vector<uint64_t> key{(uint64_t)M.nrows, (uint64_t)M.ncols, (uint64_t)M.nnz, sizeof(V),
                                   fnv1a(M.h_ptr.data(), M.h_ptr.size()), (uint64_t)(merge_mode + 1)};
@@ -880,7 +897,7 @@ bool autotune_spmv(MatStruct& M, const V* vals, const int* colidx, hipStream_t s
     if (it != g_tune_cache.end()) {
       out    = it->second;
       M.tile = out.tile;
-      upload_plan(M.plan, make_plan(M, out.tile, out.bias, colidx), s);
+      upload_plan(M.plan, make_plan(M, out.tile, out.bias, cols), s);
       M.plan.fused = out.merge == 1;
       return false;
     }
@@ -902,7 +919,7 @@ bool autotune_spmv(MatStruct& M, const V* vals, const int* colidx, hipStream_t s
   float best_ms = -1.f;
   TilePlan best_tp;
   for (int tile : {512, 1024, 2048}) {
-    const TilePlan base = make_plan(M, tile, 0, colidx);
+    const TilePlan base = make_plan(M, tile, 0, cols);
     for (int bias : {0, -1, 1}) {
       TilePlan p = base;
       if (bias) {
@@ -1078,7 +1095,8 @@ class PdlpSolver {
   DevBuf<V> a_w_, at_w_;
   // Host column indices of A sorted within rows (the tile planner's XCD order).
   std::vector<int> a_sorted_idx_;
-  const int* a_idx_host_ = nullptr;
+  ColSample a_cols_;     // A's columns at the planner's chunk midpoints (relabelled matrices)
+  ColSource a_src_;      // where plans of A read columns: the host indices or a_cols_
   // Column relabelling (unallocated: columns as given): col_perm_[k] is the
   // original column of internal column k.  Per-column inputs are gathered through
   // it on the device at upload and the returned x and reduced costs scattered back.
@@ -1168,11 +1186,31 @@ void PdlpSolver<V>::build_structures()
   const double* val_src = lp_.vals.data();
   std::vector<int> sorted_idx;
   std::vector<double> sorted_val;
-  for (int i = 0; i < m_; ++i) {
+  // The check runs on host threads over row ranges (it reads every index once);
+  // the rare unsorted rows are then sorted in order.
+  std::vector<int> unsorted;
+  {
+    const int nt = nnz >= (1 << 20) ? std::max(1, std::min(16, (int)std::thread::hardware_concurrency())) : 1;
+    std::vector<std::vector<int>> found(nt);
+    auto check = [&](int t) {
+      const int r0 = (int)((long long)m_ * t / nt), r1 = (int)((long long)m_ * (t + 1) / nt);
+      for (int i = r0; i < r1; ++i) {
+        const int b = lp_.rowptr[i], e = lp_.rowptr[i + 1];
+        for (int k = b + 1; k < e; ++k)
+          if (idx_src[k - 1] > idx_src[k]) {
+            found[t].push_back(i);
+            break;
+          }
+      }
+    };
+    std::vector<std::thread> th;
+    for (int t = 1; t < nt; ++t) th.emplace_back(check, t);
+    check(0);
+    for (auto& x : th) x.join();
+    for (const auto& f : found) unsorted.insert(unsorted.end(), f.begin(), f.end());
+  }
+  for (int i : unsorted) {
     const int b = lp_.rowptr[i], e = lp_.rowptr[i + 1];
-    bool sorted = true;
-    for (int k = b + 1; k < e && sorted; ++k) sorted = idx_src[k - 1] <= idx_src[k];
-    if (sorted) continue;
     if (sorted_idx.empty()) {
       sorted_idx.assign(lp_.colidx.begin(), lp_.colidx.begin() + nnz);
       sorted_val.assign(lp_.vals.begin(), lp_.vals.begin() + nnz);
@@ -1236,16 +1274,31 @@ void PdlpSolver<V>::build_structures()
     // relabelled columns, each in row order) lists every row's entries by column.
     device_transpose(n_, m_, nnz, AT_.ptr.get(), AT_.idx.get(), at_orig_.get(), A_.ptr.get(), A_.idx.get(),
                      a_orig_.get(), AT_.rowid.get(), A_.rowid.get(), stream_);
-    a_sorted_idx_.resize(nnz);
-    A_.idx.download(a_sorted_idx_.data(), nnz, stream_);
-    idx_src = a_sorted_idx_.data();
+    // The planner reads one column per chunk of a long row: gather those, for
+    // every tile capacity the solver or its tuner may plan with, instead of
+    // downloading all of A's relabelled indices.
+    std::vector<int> pos;
+    for (int t : {512, 1024, 2048, A_.tile}) chunk_midpoints(A_.h_ptr.data(), m_, t - kVecPad, pos);
+    std::sort(pos.begin(), pos.end());
+    pos.erase(std::unique(pos.begin(), pos.end()), pos.end());
+    a_cols_.col.assign(pos.size(), 0);
+    if (!pos.empty()) {
+      DevBuf<int> dpos(pos.size()), dcol(pos.size());
+      dpos.upload(pos, stream_);
+      hipLaunchKernelGGL(gather_idx_kernel, dim3(grid_for((int)pos.size())), dim3(kBlock), 0, stream_, (int)pos.size(),
+                         dpos.get(), A_.idx.get(), dcol.get());
+      CUOPT_HIP_CHECK(hipGetLastError());
+      dcol.download(a_cols_.col.data(), pos.size(), stream_);
+      CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));  // dpos / dcol are freed on leaving this scope
+    }
+    a_cols_.pos = std::move(pos);
   }
   AT_.h_ptr.resize(n_ + 1);
   AT_.ptr.download(AT_.h_ptr.data(), n_ + 1, stream_);
   CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));  // host sources above are temporaries
-  a_idx_host_ = idx_src;
-  build_plan(A_, stream_, idx_src, st_.long_row_merge);
-  build_plan(AT_, stream_, nullptr, st_.long_row_merge);
+  a_src_ = col_perm_.get() ? ColSource{nullptr, &a_cols_} : ColSource{idx_src, nullptr};
+  build_plan(A_, stream_, a_src_, st_.long_row_merge);
+  build_plan(AT_, stream_, ColSource{}, st_.long_row_merge);
 }
 
 template <typename V>
@@ -1717,8 +1770,8 @@ SolveResult PdlpSolver<V>::run()
   if (st_.spmv_autotune) {
     const double t = elapsed();
     SpmvChoice ca, cat;
-    const bool timed   = autotune_spmv<V>(A_, a_w_.get(), a_idx_host_, stream_, st_.long_row_merge, ca);
-    const bool timed_t = autotune_spmv<V>(AT_, at_w_.get(), nullptr, stream_, st_.long_row_merge, cat);
+    const bool timed   = autotune_spmv<V>(A_, a_w_.get(), a_src_, stream_, st_.long_row_merge, ca);
+    const bool timed_t = autotune_spmv<V>(AT_, at_w_.get(), ColSource{}, stream_, st_.long_row_merge, cat);
     CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
     if (timed || timed_t) R.info[CUOPT_AMD_INFO_TUNE_TIME] = elapsed() - t;
     R.info[CUOPT_AMD_INFO_LANES_BIAS_A]  = ca.bias;
@@ -2304,7 +2357,7 @@ static void spmv_impl(int m, int n, const int* rowptr, const int* colidx, const 
     M.idx.resize(M.nnz + kArrayPad);
     M.idx.zero(s);
     M.idx.upload(colidx, M.nnz, s);
-    build_plan(M, s, colidx);  // XCD-aware order (sums are order independent)
+    build_plan(M, s, ColSource{colidx, nullptr});  // XCD-aware order (sums are order independent)
     std::vector<V> hv(vals, vals + M.nnz), hx(x, x + n);
     DevBuf<V> dv(M.nnz + kArrayPad), dx(std::max(n, 1));
     dv.zero(s);

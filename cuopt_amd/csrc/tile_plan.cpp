@@ -2,6 +2,7 @@
 #include "tile_plan.hpp"
 
 #include <algorithm>
+#include <stdexcept>
 
 namespace cuopt_amd {
 
@@ -28,8 +29,11 @@ int pick_lanes_log2(int rows, int maxlen, int block)
 
 }  // namespace
 
-TilePlan plan_tiles(const int* rowptr, int nrows, int tile_nnz, int max_rows, int block, const int* colidx, int ncols,
-                    int xcds)
+// The planner; col_at(k) is the column of nonzero k (read only at chunk
+// midpoints, and only when aware).
+template <class ColAt>
+static TilePlan plan_impl(const int* rowptr, int nrows, int tile_nnz, int max_rows, int block, bool aware, ColAt col_at,
+                   int ncols, int xcds)
 {
   TilePlan p;
   std::vector<int> chunk_xcd;  // per chunk tile: XCD owning its columns (-1 for row-run tiles)
@@ -47,8 +51,8 @@ TilePlan plan_tiles(const int* rowptr, int nrows, int tile_nnz, int max_rows, in
         p.tiles.push_back(Tile{r, slots + c, z0, z1});
         p.meta.push_back(-(lid + 1));
         int x = -1;
-        if (colidx && ncols > 0 && xcds > 1) {
-          const long long mid = colidx[(z0 + z1 - 1) / 2];
+        if (aware) {
+          const long long mid = col_at((z0 + z1 - 1) / 2);
           x                   = (int)std::min<long long>(xcds - 1, mid * xcds / ncols);
         }
         chunk_xcd.push_back(x);
@@ -69,7 +73,7 @@ TilePlan plan_tiles(const int* rowptr, int nrows, int tile_nnz, int max_rows, in
     p.meta.push_back(pick_lanes_log2(r - r0, maxlen, block));
     chunk_xcd.push_back(-1);
   }
-  if (!(colidx && ncols > 0 && xcds > 1) || p.longrows.empty()) return p;
+  if (!aware || p.longrows.empty()) return p;
   // Interleave: position t prefers a chunk owned by XCD t % xcds.
   std::vector<std::vector<int>> queue(xcds);
   std::vector<int> free_tiles;
@@ -106,6 +110,34 @@ TilePlan plan_tiles(const int* rowptr, int nrows, int tile_nnz, int max_rows, in
     q.meta.push_back(p.meta[t]);
   }
   return q;
+}
+
+TilePlan plan_tiles(const int* rowptr, int nrows, int tile_nnz, int max_rows, int block, const int* colidx, int ncols,
+                    int xcds)
+{
+  return plan_impl(rowptr, nrows, tile_nnz, max_rows, block, colidx && ncols > 0 && xcds > 1,
+                   [colidx](int k) { return colidx[k]; }, ncols, xcds);
+}
+
+void chunk_midpoints(const int* rowptr, int nrows, int tile_nnz, std::vector<int>& out)
+{
+  for (int r = 0; r < nrows; ++r) {
+    const int b = rowptr[r], e = rowptr[r + 1];
+    if (e - b <= tile_nnz) continue;
+    for (int z0 = b; z0 < e; z0 += tile_nnz) out.push_back((z0 + std::min(z0 + tile_nnz, e) - 1) / 2);
+  }
+}
+
+TilePlan plan_tiles(const int* rowptr, int nrows, int tile_nnz, int max_rows, int block, const ColSample& cols,
+                    int ncols, int xcds)
+{
+  auto col_at = [&cols](int k) {
+    const auto it = std::lower_bound(cols.pos.begin(), cols.pos.end(), k);
+    if (it == cols.pos.end() || *it != k) throw std::logic_error("tile plan: column sample lacks a chunk midpoint");
+    return cols.col[it - cols.pos.begin()];
+  };
+  return plan_impl(rowptr, nrows, tile_nnz, max_rows, block, ncols > 0 && xcds > 1 && !cols.pos.empty(), col_at,
+                   ncols, xcds);
 }
 
 }  // namespace cuopt_amd

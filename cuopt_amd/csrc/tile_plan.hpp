@@ -49,4 +49,21 @@ struct TilePlan {
 TilePlan plan_tiles(const int* rowptr, int nrows, int tile_nnz, int max_rows, int block, const int* colidx = nullptr,
                     int ncols = 0, int xcds = 8);
 
+// The column indices the XCD-aware order reads -- the middle nonzero of every
+// chunk of a long row -- at chosen positions, for a matrix whose index array is
+// not on the host.  pos is ascending; col[i] is the column of nonzero pos[i].
+struct ColSample {
+  std::vector<int> pos;
+  std::vector<int> col;
+};
+
+// Appends the positions plan_tiles reads for tile_nnz (one per chunk of every row
+// longer than tile_nnz); sort and deduplicate before building a ColSample.
+void chunk_midpoints(const int* rowptr, int nrows, int tile_nnz, std::vector<int>& out);
+
+// plan_tiles with the columns taken from a sample that holds every position
+// chunk_midpoints(rowptr, nrows, tile_nnz) lists.
+TilePlan plan_tiles(const int* rowptr, int nrows, int tile_nnz, int max_rows, int block, const ColSample& cols,
+                    int ncols, int xcds = 8);
+
 }  // namespace cuopt_amd
