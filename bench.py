@@ -362,8 +362,10 @@ def main() -> int:
             r = c_api.solve(lp, tol=0.0, precision=p, iteration_limit=0, time_limit=args.solve_time_limit,
                             params=tune)
             tuning[p] = {"tune_time_s": round(r.info["tune_time"], 4),
-                         "A": {"tile": int(r.info["tile_a"]), "lanes_shift": int(r.info["lanes_bias_a"])},
-                         "A_T": {"tile": int(r.info["tile_at"]), "lanes_shift": int(r.info["lanes_bias_at"])}}
+                         "A": {"tile": int(r.info["tile_a"]), "lanes_shift": int(r.info["lanes_bias_a"]),
+                               "long_row_merge": int(r.info["merge_a"])},
+                         "A_T": {"tile": int(r.info["tile_at"]), "lanes_shift": int(r.info["lanes_bias_at"]),
+                                 "long_row_merge": int(r.info["merge_at"])}}
             print(f"[bench] spmv autotune {p}: {tuning[p]}", file=sys.stderr, flush=True)
     windows = {}
     for p in precs:

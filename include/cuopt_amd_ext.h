@@ -42,9 +42,10 @@ extern "C" {
  * with tile capacities of 512, 1 024 and 2 048 nonzeros and with the planner's
  * lanes per row G, G / 2 and 2 G (the row binning), and, when rows longer than a
  * tile exist, their merge inside the SpMV launch or in a launch of its own; each
- * candidate's SpMV is
- * timed once untimed and then five times with HIP events on the real matrix, and
- * the fastest is used.  The choice is cached per matrix (sizes, precision and a
+ * candidate's SpMV is run once untimed and then ten times between HIP events on
+ * the real matrix, and the fastest is used unless the untuned plan is within 3 %
+ * of it (then the untuned plan stays, so that timing noise does not change the
+ * choice from run to run).  The choice is cached per matrix (sizes, precision and a
  * hash of the row offsets) for the life of the process.  Candidates sum a row in
  * different orders, so with the tuner on, results can differ from run to run in
  * the last bits; off (the default), every solve uses capacity 1 024 and the
