@@ -46,6 +46,29 @@ def kernel_bytes(m: int, n: int, nnz: int, sv: int) -> dict:
     return {"primal": primal, "dual": dual, "iteration": primal + dual}
 
 
+def unrolled_step_bytes(m: int, n: int, nnz: int, batch: int, sv: int = 4) -> int:
+    """Algorithmic bytes of one unrolled PDHG step over `batch` objectives (batch-interleaved
+    vectors, each operand touched once):
+      primal (rows of A^T): offsets 4(n+1), indices and values nnz(4+sv), gather y (m B sv),
+        read x and c (2 n B sv), l and u (2 n sv), write x+ and x-bar (2 n B sv)
+      dual (rows of A): offsets 4(m+1), nnz(4+sv), gather x-bar (n B sv), read y (m B sv),
+        l_c and u_c (2 m sv), write y+ (m B sv)
+    """
+    primal = 4 * (n + 1) + nnz * (4 + sv) + m * batch * sv + 2 * n * batch * sv + 2 * n * sv + 2 * n * batch * sv
+    dual = 4 * (m + 1) + nnz * (4 + sv) + n * batch * sv + m * batch * sv + 2 * m * sv + m * batch * sv
+    return primal + dual
+
+
+def loop_roofline(m: int, n: int, nnz: int, sv: int, iterations: int, loop_s: float) -> dict:
+    """PDHG loop roofline of a single-LP solve: B_iter (kernel_bytes) times the iterations over
+    the loop time (termination checks and recentrings are in the time, not in the bytes)."""
+    moved = kernel_bytes(m, n, nnz, sv)["iteration"] * iterations
+    achieved = moved / max(loop_s, 1e-12) / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "bytes_per_iteration": kernel_bytes(m, n, nnz, sv)["iteration"],
+            "loop_s": round(loop_s, 4)}
+
+
 def copy_calibration(mib: int) -> None:
     """One device fill (writes `mib` MiB) and one device-to-device copy (reads and
     writes `mib` MiB), each a single runtime blit kernel: the reference points that
@@ -175,12 +198,16 @@ def transport_extras(c_api, G, time_limit: float, cpu_simplex: bool) -> dict:
     obj = case["objective"]
     out = {"workload": "transportation LP 20000x50000, 50 arcs/source, seed 0 (n=%d m=%d nnz=%d)" % (lp.n, lp.m, lp.nnz),
            "oracle_objective": obj}
+    for p in ("fp32_mixed", "fp64"):  # untimed: loads the kernels when this leg runs alone
+        c_api.solve(lp, tol=1e-4, precision=p, iteration_limit=0, time_limit=time_limit)
     for tol in (1e-4, 1e-8):
         for p in ("fp32_mixed", "fp64"):
             r = c_api.solve(lp, tol=tol, precision=p, time_limit=time_limit)
             out[f"{p}@{tol:g}"] = {"status": c_api.STATUS.get(r.status, r.status), "iterations": r.iterations,
                                    "time_s": round(r.solve_time, 4), "recentrings": int(r.info["refine_rounds"]),
-                                   "obj_rel_err": abs(r.primal_objective - obj) / (1.0 + abs(obj))}
+                                   "obj_rel_err": abs(r.primal_objective - obj) / (1.0 + abs(obj)),
+                                   "roofline": loop_roofline(lp.m, lp.n, lp.nnz, 4 if p == "fp32_mixed" else 8,
+                                                             r.iterations, r.info["loop_time"])}
             print(f"[bench] transport {p} tol {tol:g}: {out[f'{p}@{tol:g}']}", file=sys.stderr, flush=True)
     if cpu_simplex:
         try:
@@ -296,6 +323,13 @@ def unrolled_extras(G, steps: int, batch: int) -> dict:
            "forward_ms": round((t1 - t0) * 1e3, 2), "backward_ms": round((t2 - t1) * 1e3, 2),
            "forward_gpu_ms": round(ev[0].elapsed_time(ev[1]), 2), "backward_gpu_ms": round(ev[1].elapsed_time(ev[2]), 2),
            "forward_us_per_step": round((t1 - t0) * 1e6 / steps, 2), "eta": op.step_size}
+    # Roofline of the forward pass: every step's algorithmic bytes over the HIP-event time.
+    fwd_s = ev[0].elapsed_time(ev[1]) * 1e-3
+    moved = unrolled_step_bytes(lp.m, lp.n, lp.nnz, batch) * steps
+    achieved = moved / max(fwd_s, 1e-12) / 1e9
+    out["forward_roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                               "frac": round(achieved / HBM_PEAK_GBS, 4),
+                               "bytes_per_step": unrolled_step_bytes(lp.m, lp.n, lp.nnz, batch)}
     print(f"[bench] unrolled: {out}", file=sys.stderr, flush=True)
     return out
 
