@@ -291,6 +291,7 @@ __device__ inline void rows_pass(const int* __restrict__ ptr,
 {
   using Acc        = typename Epi::Acc;
   using Mat        = typename Epi::Mat;
+  using Pre        = typename Epi::Pre;
   constexpr int W  = (int)(sizeof(Mat) / 4);  // dwords per value
   constexpr int VB = 4 * W;                   // bytes per staged value
   const int G = 1 << lg, sub = threadIdx.x & (G - 1), grp = threadIdx.x >> lg;
@@ -316,10 +317,14 @@ __device__ inline void rows_pass(const int* __restrict__ ptr,
     z2 = ptr[r2];
   }
   if (dbl && z1 - z0 <= kBlockNnz) stage(z0, z1, 0);
-  // This lane group's row of the current block: its ptr entries.  The loads
-  // are clamped into range instead of guarded, so that no branch waits for them
-  // (a wait for a load also waits for every copy issued before it).
+  // This lane group's row of the current block: its ptr entries and its
+  // epilogue operands, both loaded one block ahead so that the barrier that
+  // waits for a block's copy also covers them.  The loads are clamped into range
+  // instead of guarded, so that no branch waits for them (a wait for a load also
+  // waits for every copy issued before it); a clamped row's operands go unused.
+  auto load_pre = [&](int r, int rend) { return epi.load(min(r, max(rend - 1, 0))); };
   int pb = ptr[min(r0 + grp, r1)], pe = ptr[min(r0 + grp + 1, r1)];
+  Pre pre = load_pre(r0 + grp, r1);
   for (int bi = 0; bi < nb; ++bi) {
     const int cur = dbl ? bi & 1 : 0;
     // Every wave's copy of block bi has landed (the barrier waits for each
@@ -335,15 +340,14 @@ __device__ inline void rows_pass(const int* __restrict__ ptr,
       __syncthreads();
     }
     const int npb = ptr[min(r1 + grp, r2)], npe = ptr[min(r1 + grp + 1, r2)];
-    if (z1 - z0 > kBlockNnz) {  // one row longer than a block: read it in place
-      const auto pre = epi.load(r0);
-      Acc a          = Acc::zero();
+    const Pre npre = load_pre(r1 + grp, r2);
+    if (z1 - z0 > kBlockNnz) {  // one row longer than a block: read it in place (lane 0's pre is row r0's)
+      Acc a = Acc::zero();
       for (int k = z0 + (int)threadIdx.x; k < z1; k += kBT) a.add(epi.product(vals[k], idx[k], r0));
       a = wg_merge(a, st.merge);
       if (threadIdx.x == 0) epi.row(r0, a, pre, red);
     } else if (r0 + grp < r1) {
       const int r       = r0 + grp;
-      const auto pre    = epi.load(r);
       // where the copy put nonzero z0 (values are 8-byte aligned when W = 2,
       // so their dword offset is even)
       const int* sidx  = sidx_base + cur * kStage + dword_mis(idx + z0);
@@ -372,8 +376,9 @@ __device__ inline void rows_pass(const int* __restrict__ ptr,
     z1 = z2;
     r2 = r3;
     z2 = z3;
-    pb = npb;
-    pe = npe;
+    pb  = npb;
+    pe  = npe;
+    pre = npre;
   }
   __syncthreads();  // the pass's stores are visible to the whole workgroup
 }
