@@ -383,6 +383,26 @@ __device__ inline void rows_pass(const int* __restrict__ ptr,
   __syncthreads();  // the pass's stores are visible to the whole workgroup
 }
 
+// A PDHG step epilogue whose gathered vector lives in LDS, read through an
+// LDS-typed pointer.  Through the generic pointer each gather is a flat load,
+// which the compiler waits for with both counters: one gather in flight at a
+// time, and the first wait of a block also waits for the next block's copy.
+// Typed LDS reads wait on the LDS counter only, so the products keep several
+// reads in flight and overlap the copy.
+template <class Base>
+struct LdsGather : Base {
+  using Acc = typename Base::Acc;
+  using Mat = typename Base::Mat;
+  const __attribute__((address_space(3))) Mat* g;
+  __device__ typename Acc::Elem product(Mat v, int col, int) const { return Acc::prod(v, g[col]); }
+};
+
+template <typename T>
+__device__ inline const __attribute__((address_space(3))) T* as_lds(const T* p)
+{
+  return (const __attribute__((address_space(3))) T*)p;
+}
+
 // Ruiz / Pock-Chambolle statistics of |(a * D_row-side) * D_col-side| in the
 // reference operand order; TRANSPOSED passes A^T (its rows are columns of A).
 template <bool TRANSPOSED, bool MAXNORM>
@@ -1025,15 +1045,19 @@ __global__ __launch_bounds__(kBT, kWgPerCu) void batch_pdlp_kernel(BatchArgs<V> 
         const int jj = (int)kr;
         zero_red(red);
 #define CUOPT_BPRIMAL(MAJ, FP)                                                                              \
-  rows_pass(tptr, tidx, tw, d.lgt, tblk, st,                                                                        \
-            PrimalStepEpi<V, MAJ, FP>{ys[ci], xs[ci], c_w, bnd_w, xanc, xbar, xs[nx], xhat, slack, aty, &sc, jj,    \
-                                      refine_active},                                                         \
-            red)
+  do {                                                                                                        \
+    const PrimalStepEpi<V, MAJ, FP> e{ys[ci], xs[ci], c_w, bnd_w, xanc, xbar, xs[nx], xhat, slack, aty, &sc, jj,   \
+                                      refine_active};                                                         \
+    if (vec_in_lds) rows_pass(tptr, tidx, tw, d.lgt, tblk, st, LdsGather<decltype(e)>{e, as_lds(ys[ci])}, red);  \
+    else rows_pass(tptr, tidx, tw, d.lgt, tblk, st, e, red);                                                     \
+  } while (0)
 #define CUOPT_BDUAL(MAJ, FP)                                                                                 \
-  rows_pass(aptr, aidx, aw, d.lga, ablk, st,                                                                         \
-            DualStepEpi<V, MAJ, FP>{xbar, ys[ci], cb_w, yanc, ys[nx], yhat, ybar, &sc, jj,                         \
-                                    refine_active ? (const V*)kink : nullptr},                                  \
-            red)
+  do {                                                                                                        \
+    const DualStepEpi<V, MAJ, FP> e{xbar, ys[ci], cb_w, yanc, ys[nx], yhat, ybar, &sc, jj,                         \
+                                    refine_active ? (const V*)kink : nullptr};                                  \
+    if (vec_in_lds) rows_pass(aptr, aidx, aw, d.lga, ablk, st, LdsGather<decltype(e)>{e, as_lds((const V*)xbar)}, red); \
+    else rows_pass(aptr, aidx, aw, d.lga, ablk, st, e, red);                                                     \
+  } while (0)
         if (should_major) {
           if (fpe_step) CUOPT_BPRIMAL(true, true);
           else CUOPT_BPRIMAL(true, false);
