@@ -35,7 +35,8 @@ Presolved presolve(const LpHost& lp, const std::vector<double>& lc0, const std::
   P.lb_from.assign(n, -1);
   P.ub_from.assign(n, -1);
   double offset = lp.offset;
-  auto cost = [&](int j) { return lp.maximize ? -lp.c[j] : lp.c[j]; };  // minimisation sense
+  std::vector<double> cc(lp.c.begin(), lp.c.begin() + n);  // costs (user sense) after substitutions
+  auto cost = [&](int j) { return lp.maximize ? -cc[j] : cc[j]; };  // minimisation sense
 
   std::vector<int> rq, cq;  // rows / columns to look at again
   for (int i = 0; i < m; ++i)
@@ -47,7 +48,7 @@ Presolved presolve(const LpHost& lp, const std::vector<double>& lc0, const std::
     col_on[j]      = 0;
     P.x_removed[j] = v;
     ++P.removed_cols;
-    offset += lp.c[j] * v;
+    offset += cc[j] * v;
     for (int k = cptr[j]; k < cptr[j + 1]; ++k) {
       const int i = crow[k];
       if (!row_on[i]) continue;
@@ -131,7 +132,76 @@ Presolved presolve(const LpHost& lp, const std::vector<double>& lc0, const std::
     }
     return changed;
   };
-  while (!rq.empty() || !cq.empty() || recount_columns() || redundant_rows() || dual_fixing()) {
+  // Free column singletons (see presolve.hpp).
+  auto free_column_singletons = [&]() {
+    bool changed = false;
+    for (int j = 0; j < n; ++j) {
+      if (!col_on[j] || ccount[j] != 1) continue;
+      int i    = -1;
+      double a = 0.0;
+      bool two = false;
+      for (int k = cptr[j]; k < cptr[j + 1]; ++k) {
+        if (!row_on[crow[k]]) continue;
+        if (i < 0 || crow[k] == i) {
+          i = crow[k];
+          a += cval[k];  // a repeated column index sums
+        } else {
+          two = true;
+        }
+      }
+      if (two || i < 0 || a == 0.0) continue;
+      double lo = 0.0, hi = 0.0;  // activity range of the row's other entries
+      for (int k = lp.rowptr[i]; k < lp.rowptr[i + 1]; ++k) {
+        const int jj   = lp.colidx[k];
+        const double v = lp.vals[k];
+        if (!col_on[jj] || jj == j || v == 0.0) continue;
+        lo += v > 0.0 ? v * lb[jj] : v * ub[jj];
+        hi += v > 0.0 ? v * ub[jj] : v * lb[jj];
+      }
+      bool freecol = !std::isfinite(lb[j]) && lb[j] < 0 && !std::isfinite(ub[j]) && ub[j] > 0;
+      if (!freecol) {  // implied free: x_j = (t - rest) / a over t in [lc, uc] stays in its bounds
+        const double xlo = a > 0.0 ? (lc[i] - hi) / a : (uc[i] - lo) / a;
+        const double xhi = a > 0.0 ? (uc[i] - lo) / a : (lc[i] - hi) / a;
+        freecol = xlo >= lb[j] && xhi <= ub[j];  // NaN (inf - inf) never qualifies
+      }
+      if (!freecol) continue;
+      const double cj   = cc[j];
+      const double cmin = lp.maximize ? -cj : cj;
+      double b          = lc[i];
+      if (lc[i] != uc[i]) {
+        if (cmin == 0.0) {
+          b = std::nan("");
+        } else {
+          // min (c_j / a)(t - rest): t at the lower bound when c_j / a > 0
+          b = cmin / a > 0.0 ? lc[i] : uc[i];
+          if (!std::isfinite(b)) continue;  // unbounded direction or infeasible rest: left to the solver
+        }
+      }
+      if (std::isfinite(b)) offset += cj * b / a;
+      for (int k = lp.rowptr[i]; k < lp.rowptr[i + 1]; ++k) {
+        const int jj = lp.colidx[k];
+        if (!col_on[jj] || jj == j) continue;
+        cc[jj] -= cj * lp.vals[k] / a;
+        if (--ccount[jj] == 0) cq.push_back(jj);
+      }
+      // The row's bounds in terms of the original row: fixed columns were
+      // subtracted from lc / uc when removed, and postsolve sums every entry.
+      double shift = 0.0;
+      for (int k = lp.rowptr[i]; k < lp.rowptr[i + 1]; ++k)
+        if (!col_on[lp.colidx[k]]) shift += lp.vals[k] * P.x_removed[lp.colidx[k]];
+      P.ops.push_back({'c', (int)P.col_singletons.size()});
+      P.col_singletons.push_back({i, j, a, cj, b + shift, lc[i] + shift, uc[i] + shift});
+      row_on[i] = 0;
+      col_on[j] = 0;
+      ccount[j] = 0;
+      ++P.removed_rows;
+      ++P.removed_cols;
+      changed = true;
+    }
+    return changed;
+  };
+  while (!rq.empty() || !cq.empty() || recount_columns() || redundant_rows() || dual_fixing() ||
+         free_column_singletons()) {
     while (!rq.empty()) {
       const int i = rq.back();
       rq.pop_back();
@@ -168,7 +238,8 @@ Presolved presolve(const LpHost& lp, const std::vector<double>& lc0, const std::
         continue;
       }
       const int s    = (int)P.singletons.size();
-      P.singletons.push_back({i, j, a});
+      P.ops.push_back({'s', s});
+      P.singletons.push_back({i, j, a, cc[j]});
       const double lo = a > 0 ? lc[i] / a : uc[i] / a;
       const double hi = a > 0 ? uc[i] / a : lc[i] / a;
       if (lo > lb[j]) {
@@ -265,7 +336,7 @@ Presolved presolve(const LpHost& lp, const std::vector<double>& lc0, const std::
   }
   R.m = (int)P.row_map.size();
   for (int j : P.col_map) {
-    R.c.push_back(lp.c[j]);
+    R.c.push_back(cc[j]);
     P.lb.push_back(lb[j]);
     P.ub.push_back(ub[j]);
   }
@@ -283,30 +354,51 @@ void postsolve(const Presolved& P, const LpHost& lp, SolveResult& R)
   std::vector<double> x(P.x_removed), y(m, 0.0);
   for (size_t j = 0; j < P.col_map.size() && j < R.x.size(); ++j) x[P.col_map[j]] = R.x[j];
   for (size_t i = 0; i < P.row_map.size() && i < R.y.size(); ++i) y[P.row_map[i]] = R.y[i];
-  // z = c - A^T y over the original rows (singleton rows still at y = 0)
-  auto reduced_costs = [&](std::vector<double>& z) {
-    z.assign(n, 0.0);
-    for (int j = 0; j < n; ++j) z[j] = lp.c[j];
-    for (int i = 0; i < m; ++i)
-      if (y[i] != 0.0)
-        for (int k = lp.rowptr[i]; k < lp.rowptr[i + 1]; ++k) z[lp.colidx[k]] -= lp.vals[k] * y[i];
+  // Column singletons' values, latest first: each depends only on columns still
+  // in the problem after it, whose values are known by then.
+  for (int q = (int)P.ops.size() - 1; q >= 0; --q) {
+    if (P.ops[q].first != 'c') continue;
+    const auto& C = P.col_singletons[P.ops[q].second];
+    double rest   = 0.0;
+    for (int k = lp.rowptr[C.row]; k < lp.rowptr[C.row + 1]; ++k)
+      if (lp.colidx[k] != C.col) rest += lp.vals[k] * x[lp.colidx[k]];
+    const double t = std::isnan(C.b) ? std::min(std::max(rest, C.lo), C.hi) : C.b;
+    x[C.col]       = (t - rest) / C.a;
+  }
+  // z = c - A^T y over the original rows (removed rows still at y = 0), then the
+  // removed rows' duals, latest first.
+  std::vector<double> z(lp.c.begin(), lp.c.begin() + n);
+  for (int i = 0; i < m; ++i)
+    if (y[i] != 0.0)
+      for (int k = lp.rowptr[i]; k < lp.rowptr[i + 1]; ++k) z[lp.colidx[k]] -= lp.vals[k] * y[i];
+  auto add_dual = [&](int row, double dy) {
+    y[row] += dy;
+    for (int k = lp.rowptr[row]; k < lp.rowptr[row + 1]; ++k) z[lp.colidx[k]] -= lp.vals[k] * dy;
   };
-  std::vector<double> z;
-  reduced_costs(z);
-  // A singleton row whose bound holds x_j takes x_j's reduced cost (latest first).
-  for (int s = (int)P.singletons.size() - 1; s >= 0; --s) {
-    const auto& S = P.singletons[s];
-    const int j   = S.col;
+  for (int q = (int)P.ops.size() - 1; q >= 0; --q) {
+    if (P.ops[q].first == 'c') {  // x_j's reduced cost c_j - a y_i is 0
+      const auto& C = P.col_singletons[P.ops[q].second];
+      add_dual(C.row, C.cj / C.a);
+      continue;
+    }
+    // A singleton row whose bound holds x_j takes x_j's reduced cost in the
+    // problem as it stood then: z_j less the cost moved onto x_j before it.
+    const int s      = P.ops[q].second;
+    const auto& S    = P.singletons[s];
+    const int j      = S.col;
     const bool at_lo = P.lb_from[j] == s, at_hi = P.ub_from[j] == s;
     if (!at_lo && !at_hi) continue;
-    const double zj = lp.maximize ? -z[j] : z[j];  // minimisation sense
+    const double zt = z[j] - (lp.c[j] - S.cj);
+    const double zj = lp.maximize ? -zt : zt;  // minimisation sense
     // at a lower bound the reduced cost is >= 0, at an upper bound <= 0
     const bool use = (at_lo && zj > 0.0) || (at_hi && zj < 0.0);
-    if (!use) continue;
-    const double dy = z[j] / S.a;
-    y[S.row] += dy;
-    for (int k = lp.rowptr[S.row]; k < lp.rowptr[S.row + 1]; ++k) z[lp.colidx[k]] -= lp.vals[k] * dy;
+    if (use) add_dual(S.row, zt / S.a);
   }
+  // the reduced costs of the final duals, recomputed in one pass
+  z.assign(lp.c.begin(), lp.c.begin() + n);
+  for (int i = 0; i < m; ++i)
+    if (y[i] != 0.0)
+      for (int k = lp.rowptr[i]; k < lp.rowptr[i + 1]; ++k) z[lp.colidx[k]] -= lp.vals[k] * y[i];
   double obj = lp.offset;
   for (int j = 0; j < n; ++j) obj += lp.c[j] * x[j];
   R.x  = std::move(x);

@@ -15,7 +15,15 @@
 //     dropped; y = 0;
 //   - dual fixing: a column that can move toward one of its finite bounds without
 //     raising the cost or tightening any row is fixed there; its reduced cost
-//     c - A^T y has the right sign by the signs of its rows' duals.
+//     c - A^T y has the right sign by the signs of its rows' duals;
+//   - a free column singleton: a column whose only entry a is in row i and whose
+//     bounds are free, or implied by row i over the other columns' bounds, is
+//     solved from the row and substituted: its cost moves onto the row's other
+//     columns (c_k -= c_j a_ik / a) and the offset, and the row goes.  An
+//     inequality row settles at the bound its cost prefers (left to the solver if
+//     that bound is infinite); with zero cost any point of the row will do.  In
+//     postsolve x_j = (b - sum a_ik x_k) / a and y_i = c_j / a, with c_j the cost
+//     when it was removed, so x_j's reduced cost is 0.
 // Afterwards a row whose activity range over the column bounds misses its
 // [lc, uc] (beyond 1e-9 relative) reports the problem infeasible.
 // Everything runs in FP64 in the user's objective sense; the reduced problem
@@ -30,6 +38,8 @@ namespace cuopt_amd {
 
 struct Presolved {
   enum Status { REDUCED, INFEASIBLE, UNBOUNDED } status = REDUCED;
+  // Costs below are in the user's objective sense, as they stood when the
+  // reduction was made (free column singletons move cost between columns).
   LpHost reduced;                   // ranged form; lc / uc below
   std::vector<double> lc, uc, lb, ub;  // bounds of the reduced problem
   std::vector<int> row_map;         // reduced row -> original row
@@ -38,8 +48,18 @@ struct Presolved {
   struct Singleton {
     int row, col;
     double a;
+    double cj;  // the column's cost at that time
   };
   std::vector<Singleton> singletons;  // in removal order
+  struct ColSingleton {
+    int row, col;
+    double a;
+    double cj;     // the column's cost at that time: the row's dual is cj / a
+    double b;      // the row's activity, original row (NaN: zero cost, any point of [lo, hi])
+    double lo, hi; // the row's bounds then, original row (columns fixed before it included)
+  };
+  std::vector<ColSingleton> col_singletons;  // in removal order
+  std::vector<std::pair<char, int>> ops;     // removal order: ('s', singleton) or ('c', column singleton)
   std::vector<int> lb_from, ub_from;  // original column -> singleton that set its bound (-1: none)
   int removed_rows = 0, removed_cols = 0;
 };

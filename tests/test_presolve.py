@@ -212,6 +212,40 @@ def test_presolve_matches_the_plain_solve(precision):
         assert kkt_ok(lp, b, tol=1e-5)
 
 
+def with_slack_columns(base, seed, frac=0.3):
+    """base plus a private free column (a signed slack with a random cost) in a random
+    share of its rows, which become ranged: presolve substitutes those rows away and the
+    GPU solves what is left."""
+    rng = np.random.default_rng(seed)
+    A0 = sp.csr_matrix((base.values, base.col_indices, base.row_offsets), shape=(base.m, base.n))
+    rows = np.flatnonzero(rng.random(base.m) < frac)
+    S = sp.csr_matrix((rng.choice([-1.0, 1.0], len(rows)) * rng.uniform(0.5, 2.0, len(rows)),
+                       (rows, np.arange(len(rows)))), shape=(base.m, len(rows)))
+    A = sp.hstack([A0, S]).tocsr()
+    lc, uc = base.con_lb.copy(), base.con_ub.copy()
+    lc[rows] -= rng.uniform(0.1, 1.0, len(rows))
+    uc[rows] += rng.uniform(0.1, 1.0, len(rows))
+    c = np.concatenate([base.c, rng.uniform(-0.05, 0.05, len(rows))])
+    lb = np.concatenate([base.var_lb, np.full(len(rows), -INF)])
+    ub = np.concatenate([base.var_ub, np.full(len(rows), INF)])
+    return lp_from(A, c, lc, uc, lb, ub)
+
+
+@gpu
+@pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
+def test_presolve_with_column_singletons_matches_the_plain_solve(precision):
+    # A 300 x 750 random LP whose rows partly carry free slack columns: presolve substitutes
+    # those rows away (each settles at the bound its slack's cost prefers) and the GPU
+    # solves the rest; the postsolved pair is optimal for the original LP and has the
+    # objective of the solve without presolve.
+    lp = with_slack_columns(G.random_lp(m=300, n=750, seed=4), seed=4)
+    a = c_api.solve(lp, tol=1e-8, precision=precision, time_limit=120.0, presolve=False)
+    b = c_api.solve(lp, tol=1e-8, precision=precision, time_limit=120.0, presolve=True)
+    assert a.status == b.status == 1
+    assert abs(a.primal_objective - b.primal_objective) <= 1e-6 * (1 + abs(a.primal_objective))
+    assert kkt_ok(lp, b, tol=1e-5)
+
+
 @gpu
 def test_presolve_on_skewed_lp_with_empty_rows():
     lp = G.skewed_lp(m=4000, n=8000, seed=3)  # many empty rows
@@ -387,3 +421,137 @@ def test_presolve_dual_fixing_and_redundant_rows_against_highs():
             assert kkt_ok(lp, r, tol=1e-9), seed
         settled += 1
     assert settled > 120
+
+
+def test_free_column_singleton_hand_worked():
+    # min x0 - 3 x1 + s, x0, x1 in [0, 10], s free:
+    #   r0: x0 + x1 - s = 2   s appears only here and is free: s = x0 + x1 - 2
+    # Substituted, the objective is 2 x0 - 2 x1 - 2 (cost 1 of s moved onto x0 and x1 as
+    # -1 * a_i / a = +1 each, offset 1 * 2 / -1 = -2), so x0 = 0, x1 = 10, s = 8, objective -22.
+    # The row's dual is c_s / a = -1, and z = c - A^T y = (2, -2, 0).
+    A = np.array([[1.0, 1.0, -1.0]])
+    lp = lp_from(A, [1.0, -3.0, 1.0], [2.0], [2.0], [0.0, 0.0, -INF], [10.0, 10.0, INF])
+    r = c_api.solve(lp, tol=1e-9, presolve=True)
+    assert r.error == 0 and r.status == 1
+    np.testing.assert_array_equal(r.x, [0.0, 10.0, 8.0])
+    np.testing.assert_array_equal(r.y, [-1.0])
+    np.testing.assert_array_equal(r.reduced_cost, [2.0, -2.0, 0.0])
+    assert r.primal_objective == -22.0
+    assert kkt_ok(lp, r)
+
+
+def test_implied_free_singleton_in_an_inequality_row_hand_worked():
+    # max 2 x0 + x1 - s with x0, x1 in [0, 1], s in [-5, 5] (implied free: from
+    # r0, s = x0 + x1 - t with t in [-1, 1] stays in [-1, 3]):
+    #   r0: -1 <= x0 + x1 - s <= 1
+    # Maximising -s pushes the row activity up to 1 (c_s / a = (-(-1)) / -1 < 0 in the
+    # minimisation sense takes the upper bound): s = x0 + x1 - 1 and the objective is
+    # x0 + 0 x1 + 1, so x0 = 1; x1 (cost 0) goes to its projection of 0 and s = 0.
+    A = np.array([[1.0, 1.0, -1.0]])
+    lp = lp_from(A, [2.0, 1.0, -1.0], [-1.0], [1.0], [0.0, 0.0, -5.0], [1.0, 1.0, 5.0], maximize=True)
+    r = c_api.solve(lp, tol=1e-9, presolve=True)
+    assert r.error == 0 and r.status == 1
+    np.testing.assert_array_equal(r.x, [1.0, 0.0, 0.0])
+    assert r.primal_objective == 2.0
+    np.testing.assert_array_equal(r.y, [1.0])  # c_s / a = -1 / -1
+    assert kkt_ok(lp, r)
+
+
+def _singleton_column_case(seed):
+    """Rows that each own a (implied) free column singleton -- the slack-like columns of
+    modelled LPs -- over box-bounded shared columns (some fixed), equality or ranged rows,
+    costs of either sign: presolve substitutes every row away and the shared columns end
+    empty."""
+    rng = np.random.default_rng(seed)
+    n, m = int(rng.integers(1, 6)), int(rng.integers(1, 5))
+    lb = rng.uniform(-3, 0, n)
+    ub = lb + rng.uniform(0.5, 4, n)
+    fixed = rng.random(n) < 0.25  # substituted before the rows are solved for their slacks
+    ub[fixed] = lb[fixed]
+    A = np.where(rng.random((m, n)) < 0.6, rng.normal(size=(m, n)), 0.0)
+    S = np.diag(rng.choice([-1.0, 1.0], m) * rng.uniform(0.5, 2.0, m))  # one private column per row
+    lc, uc, slb, sub = [], [], [], []
+    for i in range(m):
+        t = rng.uniform(-2, 2)
+        if rng.random() < 0.5:
+            lc.append(t)
+            uc.append(t)
+        else:
+            lc.append(t - rng.uniform(0.1, 2))
+            uc.append(t + rng.uniform(0.1, 2))
+        if rng.random() < 0.5:  # free
+            slb.append(-INF)
+            sub.append(INF)
+        else:  # wide enough to be implied by the row
+            slb.append(-1e3)
+            sub.append(1e3)
+    full = np.hstack([A, S])
+    c = rng.normal(size=n + m)
+    c[n:][rng.random(m) < 0.2] = 0.0
+    return lp_from(full, c, lc, uc, np.concatenate([lb, slb]), np.concatenate([ub, sub]),
+                   maximize=bool(rng.random() < 0.3))
+
+
+def test_free_column_singletons_against_highs():
+    # LPs that presolve settles with column-singleton substitutions (no GPU): status,
+    # objective and optimality of the postsolved (x, y, reduced costs) against HiGHS.
+    from scipy.optimize import linprog
+
+    settled = 0
+    for seed in range(400):
+        lp = _singleton_column_case(seed)
+        r = c_api.solve(lp, tol=1e-9, presolve=True)
+        if r.error != 0:  # left a reduced problem for the GPU
+            continue
+        A = sp.csr_matrix((lp.values, lp.col_indices, lp.row_offsets), shape=(lp.m, lp.n)).toarray()
+        sgn = -1.0 if lp.maximize else 1.0
+        eq = lp.con_lb == lp.con_ub
+        fin_u, fin_l = np.isfinite(lp.con_ub) & ~eq, np.isfinite(lp.con_lb) & ~eq
+        A_ub = np.vstack([A[fin_u], -A[fin_l]])
+        b_ub = np.concatenate([lp.con_ub[fin_u], -lp.con_lb[fin_l]])
+        ref = linprog(sgn * lp.c, A_ub=A_ub if len(b_ub) else None, b_ub=b_ub if len(b_ub) else None,
+                      A_eq=A[eq] if eq.any() else None, b_eq=lp.con_lb[eq] if eq.any() else None,
+                      bounds=list(zip(lp.var_lb, lp.var_ub)), method="highs")
+        if ref.status == 2:
+            assert r.status == 2, seed
+        else:
+            assert ref.status == 0 and r.status == 1, (seed, ref.status, r.status)
+            assert abs(r.primal_objective - sgn * ref.fun) <= 1e-9 * (1 + abs(ref.fun)), seed
+            assert kkt_ok(lp, r, tol=1e-9), seed
+        settled += 1
+    assert settled > 300
+
+
+def test_singleton_row_after_a_cost_transfer_hand_worked():
+    # min -2 x0 + 0.5 x1 + s, x0, x1 in [0, 10], s free:
+    #   r0: x0 + x1 - s = 2    s is a free column singleton: s = x0 + x1 - 2, its cost 1
+    #                          moves onto x0 and x1 (costs -1 and 1.5, offset -2)
+    #   r1: x0 + x1 >= 13      then x0 (cost -1, only loosens r1 upwards) is fixed at 10
+    #                          by dual fixing, r1 becomes the singleton x1 >= 3, and x1
+    #                          (cost 1.5) takes that bound: x = (10, 3, 11), objective -7.5.
+    # Duals: r1 holds x1 at the bound it supplied, so y_1 is x1's reduced cost in the problem
+    # as it stood then, 1.5 (its cost after the transfer), and y_0 = c_s / a = -1; z = (-2.5, 0, 0).
+    A = np.array([[1.0, 1.0, -1.0], [1.0, 1.0, 0.0]])
+    lp = lp_from(A, [-2.0, 0.5, 1.0], [2.0, 13.0], [2.0, INF], [0.0, 0.0, -INF], [10.0, 10.0, INF])
+    r = c_api.solve(lp, tol=1e-9, presolve=True)
+    assert r.error == 0 and r.status == 1
+    np.testing.assert_allclose(r.x, [10.0, 3.0, 11.0], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(r.y, [-1.0, 1.5], rtol=0, atol=1e-12)
+    np.testing.assert_allclose(r.reduced_cost, [-2.5, 0.0, 0.0], rtol=0, atol=1e-12)
+    assert abs(r.primal_objective - (-7.5)) < 1e-12
+    assert kkt_ok(lp, r)
+
+
+def test_column_singleton_row_with_a_fixed_column_hand_worked():
+    # The hand-worked free-column-singleton LP with a fixed column x3 = 3 added to its row
+    # (and 3 to the row's bounds): x3 is substituted first, so the row reads x0 + x1 - s = 2
+    # when s is solved from it; postsolve must not count x3 twice.  Same optimum as before:
+    # x = (0, 10, 8, 3), objective -22 + 0 * x3, y = -1.
+    A = np.array([[1.0, 1.0, -1.0, 1.0]])
+    lp = lp_from(A, [1.0, -3.0, 1.0, 0.0], [5.0], [5.0], [0.0, 0.0, -INF, 3.0], [10.0, 10.0, INF, 3.0])
+    r = c_api.solve(lp, tol=1e-9, presolve=True)
+    assert r.error == 0 and r.status == 1
+    np.testing.assert_array_equal(r.x, [0.0, 10.0, 8.0, 3.0])
+    np.testing.assert_array_equal(r.y, [-1.0])
+    assert r.primal_objective == -22.0
+    assert kkt_ok(lp, r)
