@@ -899,9 +899,11 @@ bool autotune_spmv(MatStruct& M, const V* vals, ColSource cols, hipStream_t s, i
     std::lock_guard<std::mutex> lock(g_tune_mu);
     auto it = g_tune_cache.find(key);
     if (it != g_tune_cache.end()) {
-      out    = it->second;
-      M.tile = out.tile;
-      upload_plan(M.plan, make_plan(M, out.tile, out.bias, cols), s);
+      out = it->second;
+      if (out.tile != M.tile || out.bias != 0) {  // else the plan in place is the cached one
+        M.tile = out.tile;
+        upload_plan(M.plan, make_plan(M, out.tile, out.bias, cols), s);
+      }
       M.plan.fused = out.merge == 1;
       return false;
     }
