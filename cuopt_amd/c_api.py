@@ -292,7 +292,8 @@ def _take_result(sol, lp) -> Result:
         L.cuOptGetTerminationStatus(sol, ctypes.byref(st))
         L.cuOptGetErrorStatus(sol, ctypes.byref(es))
         buf = ctypes.create_string_buffer(1024)
-        L.cuOptGetErrorString(sol, buf, 1024)
+        if es.value != 0:  # the message of a successful solve is empty
+            L.cuOptGetErrorString(sol, buf, 1024)
         x, y, z = np.zeros(max(lp.n, 1)), np.zeros(max(lp.m, 1)), np.zeros(max(lp.n, 1))
         L.cuOptGetPrimalSolution(sol, _p(x))
         L.cuOptGetDualSolution(sol, _p(y))
@@ -306,7 +307,8 @@ def _take_result(sol, lp) -> Result:
         n = ctypes.c_int32()
         L.cuOptAmdGetTrace(sol, None, 0, ctypes.byref(n))
         tr = np.zeros((max(n.value, 1), TRACE_WIDTH))
-        L.cuOptAmdGetTrace(sol, _p(tr), n.value, ctypes.byref(n))
+        if n.value > 0:
+            L.cuOptAmdGetTrace(sol, _p(tr), n.value, ctypes.byref(n))
         return Result(st.value, es.value, buf.value.decode(), x[: lp.n], y[: lp.m], z[: lp.n], po.value, do.value,
                       tm.value, dict(zip(INFO_NAMES, info.tolist())), tr[: n.value])
     finally:

@@ -29,6 +29,8 @@
 #include <memory>
 #include <numeric>
 #include <random>
+#include <exception>
+#include <mutex>
 #include <thread>
 #include <vector>
 
@@ -1128,6 +1130,29 @@ __global__ __launch_bounds__(kBT, kWgPerCu) void batch_pdlp_kernel(BatchArgs<V> 
   }
 }
 
+// f(q) for q in [0, count) on up to 16 host threads (q strided over the threads);
+// the first exception thrown by any call is rethrown after all have joined.
+template <class F>
+void host_parallel_for(int count, F f)
+{
+  const int nt = std::max(1, std::min({16, count, (int)std::thread::hardware_concurrency()}));
+  std::exception_ptr err;
+  std::mutex mu;
+  auto work = [&](int t) {
+    try {
+      for (int q = t; q < count; q += nt) f(q);
+    } catch (...) {
+      std::lock_guard<std::mutex> lock(mu);
+      if (!err) err = std::current_exception();
+    }
+  };
+  std::vector<std::thread> th;
+  for (int t = 1; t < nt; ++t) th.emplace_back(work, t);
+  work(0);
+  for (auto& x : th) x.join();
+  if (err) std::rethrow_exception(err);
+}
+
 template <typename V>
 std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
                                    const std::vector<int>& active,
@@ -1403,7 +1428,8 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
     tick_hz = std::max(1, khz) * 1e3;
     for (int a : active) first_tick = std::min(first_tick, h_out[a].t0);
   }
-  for (int a : active) {
+  host_parallel_for((int)active.size(), [&](int q) {
+    const int a        = active[q];
     const LpHost& lp   = *lps[a];
     const BatchDesc& d = desc[a];
     const BatchOut& o  = h_out[a];
@@ -1443,7 +1469,7 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
     r.info[CUOPT_AMD_INFO_REFINE_ROUNDS] = o.recentrings;
     r.info[CUOPT_AMD_INFO_LP_TIME]      = (double)(o.t1 - o.t0) / tick_hz;
     r.info[CUOPT_AMD_INFO_LP_START]     = (double)(o.t0 - first_tick) / tick_hz;
-  }
+  });
   return R;
 }
 
@@ -1455,7 +1481,8 @@ std::vector<SolveResult> batch_solve(const std::vector<const LpHost*>& lps, cons
   std::vector<SolveResult> R(lps.size());
   std::vector<int> active;
   std::vector<std::vector<double>> lcs(lps.size()), ucs(lps.size());
-  for (size_t a = 0; a < lps.size(); ++a) {
+  std::vector<char> on_device(lps.size(), 0);
+  host_parallel_for((int)lps.size(), [&](int a) {
     const LpHost& lp = *lps[a];
     constraint_bounds(lp, lcs[a], ucs[a]);
     bool crossing = false;
@@ -1467,10 +1494,12 @@ std::vector<SolveResult> batch_solve(const std::vector<const LpHost*>& lps, cons
       R[a].y.assign(lp.m, 0.0);
       R[a].rc.assign(lp.n, 0.0);
       R[a].solve_time = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-      continue;
+      return;
     }
-    active.push_back((int)a);
-  }
+    on_device[a] = 1;
+  });
+  for (size_t a = 0; a < lps.size(); ++a)
+    if (on_device[a]) active.push_back((int)a);
   // LPs answered on the host above keep their results if the device part fails.
   std::vector<SolveResult> keep = R;
   try {
