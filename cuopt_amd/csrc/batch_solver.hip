@@ -405,6 +405,23 @@ __device__ inline const __attribute__((address_space(3))) T* as_lds(const T* p)
   return (const __attribute__((address_space(3))) T*)p;
 }
 
+// The same with the gathered vector in global memory (an LP whose vectors do not
+// fit in LDS): typed global loads are waited for one counter at a time, in
+// order, so a lane keeps its four gathers in flight instead of one.
+template <class Base>
+struct GlobalGather : Base {
+  using Acc = typename Base::Acc;
+  using Mat = typename Base::Mat;
+  const __attribute__((address_space(1))) Mat* g;
+  __device__ typename Acc::Elem product(Mat v, int col, int) const { return Acc::prod(v, g[col]); }
+};
+
+template <typename T>
+__device__ inline const __attribute__((address_space(1))) T* as_global(const T* p)
+{
+  return (const __attribute__((address_space(1))) T*)p;
+}
+
 // Ruiz / Pock-Chambolle statistics of |(a * D_row-side) * D_col-side| in the
 // reference operand order; TRANSPOSED passes A^T (its rows are columns of A).
 template <bool TRANSPOSED, bool MAXNORM>
@@ -1051,14 +1068,14 @@ __global__ __launch_bounds__(kBT, kWgPerCu) void batch_pdlp_kernel(BatchArgs<V> 
     const PrimalStepEpi<V, MAJ, FP> e{ys[ci], xs[ci], c_w, bnd_w, xanc, xbar, xs[nx], xhat, slack, aty, &sc, jj,   \
                                       refine_active};                                                         \
     if (vec_in_lds) rows_pass(tptr, tidx, tw, d.lgt, tblk, st, LdsGather<decltype(e)>{e, as_lds(ys[ci])}, red);  \
-    else rows_pass(tptr, tidx, tw, d.lgt, tblk, st, e, red);                                                     \
+    else rows_pass(tptr, tidx, tw, d.lgt, tblk, st, GlobalGather<decltype(e)>{e, as_global(ys[ci])}, red);        \
   } while (0)
 #define CUOPT_BDUAL(MAJ, FP)                                                                                 \
   do {                                                                                                        \
     const DualStepEpi<V, MAJ, FP> e{xbar, ys[ci], cb_w, yanc, ys[nx], yhat, ybar, &sc, jj,                         \
                                     refine_active ? (const V*)kink : nullptr};                                  \
     if (vec_in_lds) rows_pass(aptr, aidx, aw, d.lga, ablk, st, LdsGather<decltype(e)>{e, as_lds((const V*)xbar)}, red); \
-    else rows_pass(aptr, aidx, aw, d.lga, ablk, st, e, red);                                                     \
+    else rows_pass(aptr, aidx, aw, d.lga, ablk, st, GlobalGather<decltype(e)>{e, as_global((const V*)xbar)}, red); \
   } while (0)
         if (should_major) {
           if (fpe_step) CUOPT_BPRIMAL(true, true);
