@@ -778,3 +778,43 @@ def test_long_row_merge_forms_are_bitwise_the_same(precision, shape):
         assert runs[mode].iterations == runs[0].iterations
         for f in ("x", "y", "reduced_cost"):
             assert np.array_equal(getattr(runs[mode], f), getattr(runs[0], f)), (mode, f)
+
+
+def _highs(lp):
+    """HiGHS (SciPy linprog) on a parsed LP: (status, objective with the offset, in the user's sense)."""
+    import scipy.sparse as sp
+    from scipy.optimize import linprog
+
+    A = sp.csr_matrix((lp.values, lp.col_indices, lp.row_offsets), shape=(lp.m, lp.n))
+    eq = lp.con_lb == lp.con_ub
+    up, lo = np.isfinite(lp.con_ub) & ~eq, np.isfinite(lp.con_lb) & ~eq
+    sg = -1.0 if lp.maximize else 1.0
+    A_ub = sp.vstack([A[up], -A[lo]]) if (up.any() or lo.any()) else None
+    b_ub = np.concatenate([lp.con_ub[up], -lp.con_lb[lo]]) if A_ub is not None else None
+    ref = linprog(sg * lp.c, A_ub=A_ub, b_ub=b_ub, A_eq=A[eq] if eq.any() else None,
+                  b_eq=lp.con_lb[eq] if eq.any() else None, bounds=list(zip(lp.var_lb, lp.var_ub)), method="highs")
+    return ref.status, (sg * ref.fun + lp.offset if ref.status == 0 else None)
+
+
+# The reference parser corpus's LP fixtures that HiGHS solves to an optimum
+# (datasets/linear_programming, read by mps_parser_test.cpp): solved through the
+# read handle in both precisions.
+PARSER_OPTIMAL = ["free-format-mps-1.mps", "good-mps-1.mps", "good-mps-1-clrf.mps", "good-mps-1-comments.mps",
+                  "good-mps-fixed-empty-name.mps", "good-mps-fixed-no-name.mps",
+                  "good-mps-fixed-two-objectives.mps", "good-mps-fixed-two-objectives-objname.mps",
+                  "good-mps-fixed-two-objectives-objname-next-line.mps", "good-mps-rhs-cost.mps",
+                  "good-mps-some-var-bounds.mps", "good-mps-upper-bound-inf-var.mps"]
+
+
+@pytest.mark.parametrize("fname", PARSER_OPTIMAL)
+def test_reference_parser_fixtures_solve_to_the_highs_optimum(fname):
+    prob = c_api.Problem.read_mps(os.path.join(GOLDEN, "mps", fname))
+    lp = prob.lp
+    status, obj = _highs(lp)
+    assert status == 0
+    for precision in ("fp64", "fp32_mixed"):
+        r = c_api.solve(prob, tol=1e-8, precision=precision, time_limit=TL, iteration_limit=200000)
+        assert r.status == 1, (fname, precision, r.status)
+        assert abs(r.primal_objective - obj) <= 1e-6 * (1 + abs(obj)), (fname, precision)
+        if precision == "fp64":
+            assert r.iterations == O.solve(lp, tol=1e-8)["iterations"], fname

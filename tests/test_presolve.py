@@ -555,3 +555,15 @@ def test_column_singleton_row_with_a_fixed_column_hand_worked():
     np.testing.assert_array_equal(r.y, [-1.0])
     assert r.primal_objective == -22.0
     assert kkt_ok(lp, r)
+
+
+@pytest.mark.parametrize("fname,status", [("good-mps-fixed-var.mps", 2), ("good-mps-free-var.mps", 3),
+                                          ("good-mps-lower-bound-inf-var.mps", 3)])
+def test_reference_parser_fixtures_settled_by_presolve(fname, status):
+    # Fixtures of the reference parser corpus that HiGHS finds infeasible (2) or unbounded
+    # (3): presolve reaches the same verdict with no GPU.
+    import os
+
+    lp = c_api.read_lp(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "mps", fname))
+    r = c_api.solve(lp, tol=1e-8, presolve=True, time_limit=5.0)
+    assert r.error == 0 and r.status == status
