@@ -820,6 +820,32 @@ void build_plan(MatStruct& M, hipStream_t s, ColSource cols = {}, int merge_mode
 // ptr / idx the CSR structure, plan the tile plan, long_part one partial per tile.
 // long_cnt (one zeroed counter per long row) merges each long row in the launch's
 // last chunk to finish; null runs the separate long_rows_kernel.
+// Workgroups of a tiled launch: as many as are resident at once (the occupancy
+// of this instantiation times the CUs), rounded down to a multiple of 8 so that
+// workgroup b -- and every tile b + k * grid it loops over -- runs on XCD b mod 8,
+// where the plan placed that tile.  A resident grid keeps each workgroup's waves
+// alive across tiles instead of retiring and dispatching one workgroup per tile:
+// measured at 500 k x 1 M, FP32 dual step 37.8 -> 35.7 us, FP64 44.6 -> 43.2 us
+// (twice the resident count: 36.9 us).  CUOPT_AMD_PERSISTENT=0 launches one
+// workgroup per tile.
+template <class Epi, int TILE>
+int tiled_grid(int ntiles)
+{
+  static const bool persistent = []() {
+    const char* e = std::getenv("CUOPT_AMD_PERSISTENT");
+    return !(e && std::atoi(e) == 0);
+  }();
+  if (!persistent) return ntiles;
+  static const int resident = []() {
+    int per_cu = 0, cus = 0, dev = 0;
+    CUOPT_HIP_CHECK(hipGetDevice(&dev));
+    CUOPT_HIP_CHECK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, tiled_spmv_kernel<Epi, TILE>, kBlock, 0));
+    CUOPT_HIP_CHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+    return std::max(8, (per_cu * cus) & ~7);
+  }();
+  return std::min(ntiles, resident);
+}
+
 template <class Epi>
 void launch_tiled_raw(int tile, const int* ptr, const int* idx, const typename Epi::Mat* vals, PlanDev plan,
                       int nlong, uint64_t* long_part, double* red, Epi epi, hipStream_t s, int* long_cnt = nullptr)
@@ -827,14 +853,14 @@ void launch_tiled_raw(int tile, const int* ptr, const int* idx, const typename E
   if (plan.ntiles == 0) return;
   if (nlong == 0) long_cnt = nullptr;
   if (tile == 512)
-    hipLaunchKernelGGL((tiled_spmv_kernel<Epi, 512>), dim3(plan.ntiles), dim3(kBlock), 0, s, ptr, idx, vals, plan,
-                       long_part, red, epi, long_cnt);
+    hipLaunchKernelGGL((tiled_spmv_kernel<Epi, 512>), dim3(tiled_grid<Epi, 512>(plan.ntiles)), dim3(kBlock), 0, s,
+                       ptr, idx, vals, plan, long_part, red, epi, long_cnt);
   else if (tile == 1024)
-    hipLaunchKernelGGL((tiled_spmv_kernel<Epi, 1024>), dim3(plan.ntiles), dim3(kBlock), 0, s, ptr, idx, vals, plan,
-                       long_part, red, epi, long_cnt);
+    hipLaunchKernelGGL((tiled_spmv_kernel<Epi, 1024>), dim3(tiled_grid<Epi, 1024>(plan.ntiles)), dim3(kBlock), 0, s,
+                       ptr, idx, vals, plan, long_part, red, epi, long_cnt);
   else
-    hipLaunchKernelGGL((tiled_spmv_kernel<Epi, 2048>), dim3(plan.ntiles), dim3(kBlock), 0, s, ptr, idx, vals, plan,
-                       long_part, red, epi, long_cnt);
+    hipLaunchKernelGGL((tiled_spmv_kernel<Epi, 2048>), dim3(tiled_grid<Epi, 2048>(plan.ntiles)), dim3(kBlock), 0, s,
+                       ptr, idx, vals, plan, long_part, red, epi, long_cnt);
   CUOPT_HIP_CHECK(hipGetLastError());
   if (nlong > 0 && !long_cnt) {
     hipLaunchKernelGGL(long_rows_kernel<Epi>, dim3(nlong), dim3(kWave), 0, s, plan, long_part, red, epi);

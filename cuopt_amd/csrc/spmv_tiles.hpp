@@ -213,24 +213,14 @@ __device__ inline void merge_long_row(const PlanDev& plan, int lid, uint64_t* lo
 constexpr int kCntStride = 64;
 
 template <class Epi, int TILE>
-__global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restrict__ rowptr,
-                                                            const int* __restrict__ colidx,
-                                                            const typename Epi::Mat* __restrict__ vals,
-                                                            PlanDev plan,
-                                                            uint64_t* __restrict__ long_part,
-                                                            double* __restrict__ red_part,
-                                                            Epi epi,
-                                                            int* __restrict__ long_cnt)
+__device__ inline void tile_body(int t, const int* __restrict__ rowptr, const int* __restrict__ colidx,
+                                 const typename Epi::Mat* __restrict__ vals, const PlanDev& plan,
+                                 uint64_t* __restrict__ long_part, double* __restrict__ red_part, const Epi& epi,
+                                 int* __restrict__ long_cnt, typename Epi::Acc::Elem* prod,
+                                 double (*red_lds)[kMaxRed], uint64_t* acc_lds)
 {
   using Acc  = typename Epi::Acc;
-  using Elem = typename Acc::Elem;
   using Mat  = typename Epi::Mat;
-  __shared__ Elem prod[TILE];
-  __shared__ double red_lds[kBlock / kWave][kMaxRed];
-  __shared__ uint64_t acc_lds[kBlock / kWave];
-
-  if (epi.skip()) return;
-  const int t    = blockIdx.x;
   const Tile tl  = plan.tiles[t];
   const int mt   = plan.meta[t];
   const int tid  = threadIdx.x;
@@ -347,6 +337,33 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
     return;
   }
   if constexpr (Epi::NRED > 0) block_sum_store<Epi::NRED>(red, red_part + (size_t)t * Epi::NRED, red_lds);
+}
+
+// Workgroup b processes tiles b, b + gridDim.x, ... (the launch sizes the grid
+// to the resident workgroups, a multiple of 8, so every tile of workgroup b runs
+// on XCD b mod 8 as the plan placed it).  A barrier before every later tile
+// orders the reuse of the LDS buffers: after a row-run tile without reductions,
+// or a long-row chunk that wave 0 finishes alone, other waves may reach the next
+// tile first.
+template <class Epi, int TILE>
+__global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restrict__ rowptr,
+                                                            const int* __restrict__ colidx,
+                                                            const typename Epi::Mat* __restrict__ vals,
+                                                            PlanDev plan,
+                                                            uint64_t* __restrict__ long_part,
+                                                            double* __restrict__ red_part,
+                                                            Epi epi,
+                                                            int* __restrict__ long_cnt)
+{
+  using Elem = typename Epi::Acc::Elem;
+  __shared__ Elem prod[TILE];
+  __shared__ double red_lds[kBlock / kWave][kMaxRed];
+  __shared__ uint64_t acc_lds[kBlock / kWave];
+  if (epi.skip()) return;
+  for (int t = blockIdx.x; t < plan.ntiles; t += gridDim.x) {
+    if (t != (int)blockIdx.x) __syncthreads();
+    tile_body<Epi, TILE>(t, rowptr, colidx, vals, plan, long_part, red_part, epi, long_cnt, prod, red_lds, acc_lds);
+  }
 }
 
 // Epilogue of long row lid, run by one whole wave: lane l merges chunk partials
