@@ -17,14 +17,16 @@
 // recomputed from its checkpoint before its reverse sweep.  The c and (l_c, u_c)
 // gradients are accumulated in FP64.
 //
-// Layout: batch-interleaved vectors, v[i * B + b].  SpMM kernels give one
-// wavefront to a matrix row and one lane to each batch member, so every nonzero
-// gathers B contiguous values of the operand (one 256-byte FP32 line for
-// B = 64) while the row's value / index stream is read once for the whole batch.
+// Layout: batch-interleaved vectors, v[i * B + b].  SpMM kernels give a
+// wavefront 1, 2 or 4 consecutive matrix rows and one lane to each batch member,
+// so every nonzero gathers B contiguous values of the operand (one 256-byte FP32
+// line for B = 64) while the rows' value / index stream is read once for the
+// whole batch.
 #include <cmath>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <type_traits>
 #include <vector>
 
 #include "batch_size.hpp"
@@ -36,103 +38,213 @@ namespace cuopt_amd {
 
 namespace {
 
-constexpr int kUB = 256;              // 4 wavefronts: 4 rows per workgroup
-constexpr int kRowsPerWg = kUB / kWave;
+constexpr int kUB = 256;              // 4 wavefronts per workgroup
+constexpr int kWavesPerWg = kUB / kWave;
 
-inline dim3 spmm_grid(int rows, int B) { return dim3((rows + kRowsPerWg - 1) / kRowsPerWg, (B + kWave - 1) / kWave); }
-
-// Row of this wavefront and batch member of this lane; false when out of range.
-__device__ inline bool row_lane(int rows, int B, int& r, int& b)
+// Rows per wavefront of the SpMM kernels: each wavefront takes R consecutive
+// rows, so that one load of their indices and values across its lanes, and up
+// to 16 operand gathers, are in flight at once.  Measured at 20 000 x 50 000 with
+// 400 k nonzeros and B = 64 (forward / backward pass of 2 000 steps): R = 1 for
+// both matrices 78.5 / 188.0 ms, 2 for both 75.2 / 180.3 ms, 4 for both 79.2 /
+// 185.7 ms.  The gathers, not the index loads, bound these kernels: a 256-byte
+// operand row per nonzero from a vector larger than one XCD's L2.  At least
+// 4 096 wavefronts are kept to fill the chip.  CUOPT_AMD_UNR_RPW_A / _T (read
+// when a handle is created) force R for A / A^T.
+inline int rows_per_wave(int rows, long long nnz)
 {
-  r = blockIdx.x * kRowsPerWg + (int)threadIdx.x / kWave;
-  b = blockIdx.y * kWave + ((int)threadIdx.x & (kWave - 1));
-  return r < rows && b < B;
+  if (rows <= 0) return 1;
+  const double avg = (double)nnz / rows;
+  int R = avg <= 4.0 ? 4 : avg <= 24.0 ? 2 : 1;
+  while (R > 1 && rows / R < 4096) R /= 2;
+  return R;
 }
 
-template <typename V>
-__device__ inline typename AccFor<V>::type row_dot(const int* __restrict__ ptr,
-                                                   const int* __restrict__ idx,
-                                                   const V* __restrict__ val,
-                                                   const V* __restrict__ vec,
-                                                   int r,
-                                                   int b,
-                                                   int B)
+inline dim3 rows_grid(int rows, int B, int R)
 {
-  using Acc   = typename AccFor<V>::type;
-  Acc a       = Acc::zero();
-  const int e = ptr[r + 1];
-  int k       = ptr[r];
-  // Eight nonzeros at a time: their (wave-uniform) indices and values, then
-  // their eight operand loads, are all in flight before the first product is
-  // added; the products are still added in increasing k.
-  for (; k + 8 <= e; k += 8) {
-    int c[8];
-    V v[8], x[8];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      c[q] = idx[k + q];
-      v[q] = val[k + q];
-    }
-#pragma unroll
-    for (int q = 0; q < 8; ++q) x[q] = vec[(size_t)c[q] * B + b];
-#pragma unroll
-    for (int q = 0; q < 8; ++q) a.add(Acc::prod(v[q], x[q]));
+  const int per_wg = kWavesPerWg * R;
+  return dim3((std::max(rows, 1) + per_wg - 1) / per_wg, (B + kWave - 1) / kWave);
+}
+
+// Calls f(std::integral_constant<int, R>) for the supported rows per wavefront.
+template <class F>
+void with_rows(int R, F&& f)
+{
+  switch (R) {
+    case 4: f(std::integral_constant<int, 4>{}); break;
+    case 2: f(std::integral_constant<int, 2>{}); break;
+    default: f(std::integral_constant<int, 1>{}); break;
   }
-  if (k + 4 <= e) {
-    int c[4];
-    V v[4], x[4];
+}
+
+// First row of this wavefront (wave-uniform), its row count and this lane's
+// batch member.  Lanes past B keep b = 0 (an in-bounds address) and bok false:
+// they take part in the wavefront's shared loads but store nothing.
+template <int R>
+__device__ inline bool rows_lane(int rows, int B, int& r0, int& nr, int& b, bool& bok)
+{
+  r0 = __builtin_amdgcn_readfirstlane((int)(blockIdx.x * kWavesPerWg + threadIdx.x / kWave) * R);
+  if (r0 >= rows) return false;
+  nr        = min(R, rows - r0);
+  const int bl = (int)(blockIdx.y * kWave + (threadIdx.x & (kWave - 1)));
+  bok       = bl < B;
+  b         = bok ? bl : 0;
+  return true;
+}
+
+__device__ inline int lane_value(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+__device__ inline float lane_value(float v, int l)
+{
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
+}
+__device__ inline double lane_value(double v, int l)
+{
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, l);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), l);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Dot products of rows r0 .. r0 + nr - 1 with the batch-interleaved operand
+// (lane b reads vec[col * B + b]) into out[0 .. nr - 1].  The rows' nonzeros are
+// one contiguous run [ptr[r0], ptr[r0 + nr]); the wavefront loads it 64 entries
+// at a time across its lanes, broadcasts each index and value with readlane,
+// and issues 16 (then 4, then 1) gathers before adding their products.  Each
+// row's products are added in increasing k, as a row-at-a-time loop would.
+template <typename V, int R>
+__device__ inline void rows_dot(const int* __restrict__ ptr, const int* __restrict__ idx,
+                                const V* __restrict__ val, const V* __restrict__ vec, int r0, int nr, int b, int B,
+                                typename AccFor<V>::type (&out)[R])
+{
+  using Acc = typename AccFor<V>::type;
+  constexpr int G = 16;
+  const int lane = (int)(threadIdx.x & (kWave - 1));
+  const int pl   = ptr[r0 + min(lane, nr)];  // lane i < nr + 1 holds ptr[r0 + i]
+  const int pend = lane_value(pl, nr);
+  int cur = 0, bound = lane_value(pl, 1);  // row being summed and its end
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      c[q] = idx[k + q];
-      v[q] = val[k + q];
+  for (int i = 0; i < R; ++i) out[i] = Acc::zero();
+  Acc a = Acc::zero();
+  // A select per row keeps out[] in registers (a store at a run-time index
+  // would put it in scratch memory).
+  auto close = [&]() {
+#pragma unroll
+    for (int i = 0; i < R; ++i) out[i] = i == cur ? a : out[i];
+  };
+  // Before the product at position k: close the rows that end at k (empty ones included).
+  auto advance = [&](int k) {
+    while (k == bound) {
+      close();
+      a = Acc::zero();
+      ++cur;
+      bound = lane_value(pl, min(cur + 1, nr));
     }
+  };
+  for (int base = lane_value(pl, 0); base < pend; base += kWave) {
+    const int cnt = min(kWave, pend - base);
+    int ci = 0;
+    V vi   = V(0);
+    if (lane < cnt) {
+      ci = idx[base + lane];
+      vi = val[base + lane];
+    }
+    int t = 0;
+    for (; t + G <= cnt; t += G) {
+      V x[G];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) x[q] = vec[(size_t)c[q] * B + b];
+      for (int q = 0; q < G; ++q) x[q] = vec[(size_t)lane_value(ci, t + q) * B + b];
 #pragma unroll
-    for (int q = 0; q < 4; ++q) a.add(Acc::prod(v[q], x[q]));
-    k += 4;
+      for (int q = 0; q < G; ++q) {
+        advance(base + t + q);
+        a.add(Acc::prod(lane_value(vi, t + q), x[q]));
+      }
+    }
+    for (; t + 4 <= cnt; t += 4) {
+      V x[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) x[q] = vec[(size_t)lane_value(ci, t + q) * B + b];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        advance(base + t + q);
+        a.add(Acc::prod(lane_value(vi, t + q), x[q]));
+      }
+    }
+    for (; t < cnt; ++t) {
+      const V x = vec[(size_t)lane_value(ci, t) * B + b];
+      advance(base + t);
+      a.add(Acc::prod(lane_value(vi, t), x));
+    }
   }
-  for (; k < e; ++k) a.add(Acc::prod(val[k], vec[(size_t)idx[k] * B + b]));
-  return a;
+  close();
 }
 
 // ---- forward ----------------------------------------------------------------
 
-template <typename V>
+template <typename V, int R>
 __global__ __launch_bounds__(kUB) void unr_primal(const int* __restrict__ tptr, const int* __restrict__ tidx,
                                                   const V* __restrict__ tval, int n, int B, const V* __restrict__ y,
                                                   const V* __restrict__ x, const V* __restrict__ c,
                                                   const V* __restrict__ lb, const V* __restrict__ ub, V tau,
                                                   V inv_prox, V* __restrict__ xn, V* __restrict__ xb)
 {
-  int j, b;
-  if (!row_lane(n, B, j, b)) return;
-  const size_t o = (size_t)j * B + b;
-  // The lane's own operands are loaded before the row's gathers, so their
-  // latency overlaps the gathers'.
-  const V xr = x[o], cr = c[o], lo = lb[j], hi = ub[j];
-  const auto aty = row_dot(tptr, tidx, tval, y, j, b, B);
-  const V v     = (xr - tau * aty.rsub(cr)) * inv_prox;
-  const V xnew  = mx(mn(v, hi), lo);
-  xn[o]         = xnew;
-  xb[o]         = V(2) * xnew - xr;
+  int j0, nr, b;
+  bool bok;
+  if (!rows_lane<R>(n, B, j0, nr, b, bok)) return;
+  // The rows' own operands are loaded before their gathers, so their latency
+  // overlaps the gathers'.
+  V xr[R], cr[R], lo[R], hi[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int j    = j0 + min(i, nr - 1);
+    const size_t o = (size_t)j * B + b;
+    xr[i] = x[o];
+    cr[i] = c[o];
+    lo[i] = lb[j];
+    hi[i] = ub[j];
+  }
+  typename AccFor<V>::type aty[R];
+  rows_dot<V, R>(tptr, tidx, tval, y, j0, nr, b, B, aty);
+  if (!bok) return;
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    if (i < nr) {
+      const size_t o = (size_t)(j0 + i) * B + b;
+      const V v      = (xr[i] - tau * aty[i].rsub(cr[i])) * inv_prox;
+      const V xnew   = mx(mn(v, hi[i]), lo[i]);
+      xn[o]          = xnew;
+      xb[o]          = V(2) * xnew - xr[i];
+    }
+  }
 }
 
-template <typename V>
+template <typename V, int R>
 __global__ __launch_bounds__(kUB) void unr_dual(const int* __restrict__ aptr, const int* __restrict__ aidx,
                                                 const V* __restrict__ aval, int m, int B, const V* __restrict__ xb,
                                                 const V* __restrict__ y, const V* __restrict__ lc,
                                                 const V* __restrict__ uc, int ldb, V sigma, V* __restrict__ yn)
 {
-  int i, b;
-  if (!row_lane(m, B, i, b)) return;
-  const size_t o = (size_t)i * B + b;
-  const size_t q = ldb ? (size_t)i * ldb + b : (size_t)i;  // per-batch bounds, or one per row
-  const V yo = y[o], lo = lc[q], hi = uc[q];  // loaded before the gathers
-  const auto ax  = row_dot(aptr, aidx, aval, xb, i, b, B);
-  const V t      = ax.rsub(yo / sigma);
-  const V proj   = mx(-hi, mn(t, -lo));
-  yn[o]          = (t - proj) * sigma;
+  int i0, nr, b;
+  bool bok;
+  if (!rows_lane<R>(m, B, i0, nr, b, bok)) return;
+  V yo[R], lo[R], hi[R];  // loaded before the gathers
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int i    = i0 + min(k, nr - 1);
+    const size_t q = ldb ? (size_t)i * ldb + b : (size_t)i;  // per-batch bounds, or one per row
+    yo[k] = y[(size_t)i * B + b];
+    lo[k] = lc[q];
+    hi[k] = uc[q];
+  }
+  typename AccFor<V>::type ax[R];
+  rows_dot<V, R>(aptr, aidx, aval, xb, i0, nr, b, B, ax);
+  if (!bok) return;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    if (k < nr) {
+      const V t    = ax[k].rsub(yo[k] / sigma);
+      const V proj = mx(-hi[k], mn(t, -lo[k]));
+      yn[(size_t)(i0 + k) * B + b] = (t - proj) * sigma;
+    }
+  }
 }
 
 // ---- backward ---------------------------------------------------------------
@@ -178,7 +290,7 @@ __global__ __launch_bounds__(kUB) void unr_adj_start(int m, int B, const V* ay, 
 
 // Rows j of A^T: gxb = -A^T gt; gw = inner(x+) (ax + 2 gxb) / (1 + tau lam);
 // ax <- gw - gxb (adjoint of x before the step); gc_acc += gw (FP64).
-template <typename V>
+template <typename V, int R>
 __global__ __launch_bounds__(kUB) void unr_adj_primal(const int* __restrict__ tptr, const int* __restrict__ tidx,
                                                       const V* __restrict__ tval, int n, int B,
                                                       const V* __restrict__ gt, const V* __restrict__ xplus,
@@ -186,24 +298,42 @@ __global__ __launch_bounds__(kUB) void unr_adj_primal(const int* __restrict__ tp
                                                       V* __restrict__ ax, V* __restrict__ gw,
                                                       double* __restrict__ gc_acc)
 {
-  int j, b;
-  if (!row_lane(n, B, j, b)) return;
-  const size_t o = (size_t)j * B + b;
-  // Element operands first (gt, which the row gathers, is none of them).
-  const V xp      = xplus[o], axo = ax[o];
-  const bool in   = xp > lb[j] && xp < ub[j];
-  const double gc = gc_acc[o];
-  const V gxb     = -row_dot(tptr, tidx, tval, gt, j, b, B).value();
-  const V w       = in ? (axo + V(2) * gxb) * inv_prox : V(0);
-  gw[o]           = w;
-  ax[o]           = w - gxb;
-  gc_acc[o]       = gc + (double)w;
+  int j0, nr, b;
+  bool bok;
+  if (!rows_lane<R>(n, B, j0, nr, b, bok)) return;
+  // Element operands first (gt, which the rows gather, is none of them).
+  V axo[R];
+  bool in[R];
+  double gc[R];
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    const int j    = j0 + min(i, nr - 1);
+    const size_t o = (size_t)j * B + b;
+    const V xp     = xplus[o];
+    axo[i]         = ax[o];
+    in[i]          = xp > lb[j] && xp < ub[j];
+    gc[i]          = gc_acc[o];
+  }
+  typename AccFor<V>::type s[R];
+  rows_dot<V, R>(tptr, tidx, tval, gt, j0, nr, b, B, s);
+  if (!bok) return;
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    if (i < nr) {
+      const size_t o = (size_t)(j0 + i) * B + b;
+      const V gxb    = -s[i].value();
+      const V w      = in[i] ? (axo[i] + V(2) * gxb) * inv_prox : V(0);
+      gw[o]          = w;
+      ax[o]          = w - gxb;
+      gc_acc[o]      = gc[i] + (double)w;
+    }
+  }
 }
 
 // Rows i of A: ay_prev = gt / sigma + tau A gw; then, if there is an earlier
 // step, its gt from ay_prev and the clamp states of its y+ (= y before this step).
 // gt is read and rewritten in place, element by element.
-template <typename V>
+template <typename V, int R>
 __global__ __launch_bounds__(kUB) void unr_adj_dual(const int* __restrict__ aptr, const int* __restrict__ aidx,
                                                     const V* __restrict__ aval, int m, int B,
                                                     const V* __restrict__ gw, V tau, V sigma,
@@ -211,19 +341,38 @@ __global__ __launch_bounds__(kUB) void unr_adj_dual(const int* __restrict__ aptr
                                                     const V* __restrict__ uc, int ldb, V* __restrict__ gt,
                                                     double* __restrict__ glc, double* __restrict__ guc)
 {
-  int i, b;
-  if (!row_lane(m, B, i, b)) return;
-  const size_t o = (size_t)i * B + b;
-  const size_t q = ldb ? (size_t)i * ldb + b : (size_t)i;  // per-batch bounds, or one per row
-  // Element operands first (gw, which the row gathers, is none of them).
-  const V gto = gt[o], yp = yprev[o], lo = lc[q], hi = uc[q];
-  const double glo = glc[o], guo = guc[o];
-  const V agw    = row_dot(aptr, aidx, aval, gw, i, b, B).value();
-  const V ay     = gto / sigma + tau * agw;  // adjoint of y before this step
-  double dl, du;
-  gt[o]  = dual_adjoint(ay, yp, lo, hi, sigma, dl, du);
-  glc[o] = glo + dl;
-  guc[o] = guo + du;
+  int i0, nr, b;
+  bool bok;
+  if (!rows_lane<R>(m, B, i0, nr, b, bok)) return;
+  // Element operands first (gw, which the rows gather, is none of them).
+  V gto[R], yp[R], lo[R], hi[R];
+  double glo[R], guo[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int i    = i0 + min(k, nr - 1);
+    const size_t o = (size_t)i * B + b;
+    const size_t q = ldb ? (size_t)i * ldb + b : (size_t)i;  // per-batch bounds, or one per row
+    gto[k] = gt[o];
+    yp[k]  = yprev[o];
+    lo[k]  = lc[q];
+    hi[k]  = uc[q];
+    glo[k] = glc[o];
+    guo[k] = guc[o];
+  }
+  typename AccFor<V>::type s[R];
+  rows_dot<V, R>(aptr, aidx, aval, gw, i0, nr, b, B, s);
+  if (!bok) return;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    if (k < nr) {
+      const size_t o = (size_t)(i0 + k) * B + b;
+      const V ay     = gto[k] / sigma + tau * s[k].value();  // adjoint of y before this step
+      double dl, du;
+      gt[o]  = dual_adjoint(ay, yp[k], lo[k], hi[k], sigma, dl, du);
+      glc[o] = glo[k] + dl;
+      guc[o] = guo[k] + du;
+    }
+  }
 }
 
 template <typename V>
@@ -283,14 +432,23 @@ __global__ __launch_bounds__(kUB) void unr_spmv64(const int* __restrict__ ptr, c
 }
 
 // Y = A X, batch-interleaved (the SpMM timed by the batch-size search).
-template <typename V>
+template <typename V, int R>
 __global__ __launch_bounds__(kUB) void unr_spmm(const int* __restrict__ ptr, const int* __restrict__ idx,
                                                 const V* __restrict__ val, int rows, int B, const V* __restrict__ x,
                                                 V* __restrict__ y)
 {
-  int r, b;
-  if (!row_lane(rows, B, r, b)) return;
-  y[(size_t)r * B + b] = row_dot(ptr, idx, val, x, r, b, B).value();
+  int r0, nr, b;
+  bool bok;
+  if (!rows_lane<R>(rows, B, r0, nr, b, bok)) return;
+  typename AccFor<V>::type s[R];
+  rows_dot<V, R>(ptr, idx, val, x, r0, nr, b, B, s);
+  if (!bok) return;
+#pragma unroll
+  for (int i = 0; i < R; ++i) {
+    if (i < nr) {
+      y[(size_t)(r0 + i) * B + b] = s[i].value();
+    }
+  }
 }
 
 inline int ew_blocks(size_t n) { return (int)std::max<size_t>(1, std::min<size_t>((n + kUB - 1) / kUB, 4096)); }
@@ -320,6 +478,7 @@ __global__ void unr_seg_update(int* seg, int value, int set)
 
 struct UnrolledImpl {
   int m = 0, n = 0, precision = 1;
+  int rpw_a = 1, rpw_t = 1;  // rows per wavefront of the SpMM kernels over A and A^T
   double lam = 0.0, eta = 1.0;
   // structure (device)
   DevBuf<int> aptr, aidx, tptr, tidx;
@@ -444,10 +603,16 @@ template <typename V>
 void step(const Views<V>& M, UnrolledImpl& U, const Work<V>& W, const V* x, const V* y, V* xn, V* yn, hipStream_t s)
 {
   const V tau = (V)U.eta, sigma = (V)U.eta, inv_prox = (V)(1.0 / (1.0 + U.eta * U.lam));
-  hipLaunchKernelGGL(unr_primal<V>, spmm_grid(U.n, U.B), dim3(kUB), 0, s, M.tptr, M.tidx, M.tval, U.n, U.B, y, x,
-                     W.c, M.lb, M.ub, tau, inv_prox, xn, W.xb);
-  hipLaunchKernelGGL(unr_dual<V>, spmm_grid(U.m, U.B), dim3(kUB), 0, s, M.aptr, M.aidx, M.aval, U.m, U.B, W.xb, y,
-                     W.lc, W.uc, U.ldb, sigma, yn);
+  with_rows(U.rpw_t, [&](auto RC) {
+    constexpr int R = decltype(RC)::value;
+    hipLaunchKernelGGL((unr_primal<V, R>), rows_grid(U.n, U.B, R), dim3(kUB), 0, s, M.tptr, M.tidx, M.tval, U.n,
+                       U.B, y, x, (const V*)W.c, M.lb, M.ub, tau, inv_prox, xn, W.xb);
+  });
+  with_rows(U.rpw_a, [&](auto RC) {
+    constexpr int R = decltype(RC)::value;
+    hipLaunchKernelGGL((unr_dual<V, R>), rows_grid(U.m, U.B, R), dim3(kUB), 0, s, M.aptr, M.aidx, M.aval, U.m, U.B,
+                       (const V*)W.xb, y, (const V*)W.lc, (const V*)W.uc, U.ldb, sigma, yn);
+  });
 }
 
 template <typename V>
@@ -538,12 +703,18 @@ void backward_impl(UnrolledImpl& U, const double* gx, const double* gy, double* 
         hipLaunchKernelGGL(unr_adj_start<V>, dim3(ew_blocks(mB)), dim3(kUB), 0, q, U.m, U.B,
                            gy ? (const V*)W.gt : (const V*)nullptr, yplus, (const V*)W.lc, (const V*)W.uc, U.ldb,
                            sigma, W.gt, W.glc, W.guc);
-      hipLaunchKernelGGL(unr_adj_primal<V>, spmm_grid(U.n, U.B), dim3(kUB), 0, q, M.tptr, M.tidx, M.tval, U.n, U.B,
-                         (const V*)W.gt, xplus, M.lb, M.ub, inv_prox, W.ax, W.gw, W.gc);
+      with_rows(U.rpw_t, [&](auto RC) {
+        constexpr int R = decltype(RC)::value;
+        hipLaunchKernelGGL((unr_adj_primal<V, R>), rows_grid(U.n, U.B, R), dim3(kUB), 0, q, M.tptr, M.tidx, M.tval,
+                           U.n, U.B, (const V*)W.gt, xplus, M.lb, M.ub, inv_prox, W.ax, W.gw, W.gc);
+      });
       if (!first || t > 0)
-        hipLaunchKernelGGL(unr_adj_dual<V>, spmm_grid(U.m, U.B), dim3(kUB), 0, q, M.aptr, M.aidx, M.aval, U.m, U.B,
-                           (const V*)W.gw, tau, sigma, (const V*)(W.sy + (size_t)t * mB), (const V*)W.lc,
-                           (const V*)W.uc, U.ldb, W.gt, W.glc, W.guc);
+        with_rows(U.rpw_a, [&](auto RC) {
+          constexpr int R = decltype(RC)::value;
+          hipLaunchKernelGGL((unr_adj_dual<V, R>), rows_grid(U.m, U.B, R), dim3(kUB), 0, q, M.aptr, M.aidx, M.aval,
+                             U.m, U.B, (const V*)W.gw, tau, sigma, (const V*)(W.sy + (size_t)t * mB),
+                             (const V*)W.lc, (const V*)W.uc, U.ldb, W.gt, W.glc, W.guc);
+        });
     }
     hipLaunchKernelGGL(unr_seg_update, dim3(1), dim3(kWave), 0, q, U.seg.get(), -1, 0);
   };
@@ -581,6 +752,10 @@ UnrolledHandle* unrolled_create(int m, int n, const int* rowptr, const int* coli
     U->lam = lam;
     U->precision = precision;
     const int nnz = rowptr[m];
+    U->rpw_a      = rows_per_wave(m, nnz);
+    U->rpw_t      = rows_per_wave(n, nnz);
+    if (const char* e = std::getenv("CUOPT_AMD_UNR_RPW_A")) U->rpw_a = std::atoi(e);  // 1, 2 or 4
+    if (const char* e = std::getenv("CUOPT_AMD_UNR_RPW_T")) U->rpw_t = std::atoi(e);
     // A^T by a stable counting sort
     std::vector<int> tp(n + 1, 0), ti(nnz);
     std::vector<double> tv(nnz);
@@ -721,11 +896,17 @@ int optimal_batch_size_impl(UnrolledImpl& U, int max_batch)
       hipLaunchKernelGGL(unr_zero<V>, dim3(ew_blocks(mB)), dim3(kUB), 0, s, mB, y.get());
       auto run = [&]() {
         if (U.m > 0)
-          hipLaunchKernelGGL(unr_spmm<V>, spmm_grid(U.m, B), dim3(kUB), 0, s, M.aptr, M.aidx, M.aval, U.m, B,
-                             (const V*)x.get(), y.get());
+          with_rows(U.rpw_a, [&](auto RC) {
+            constexpr int R = decltype(RC)::value;
+            hipLaunchKernelGGL((unr_spmm<V, R>), rows_grid(U.m, B, R), dim3(kUB), 0, s, M.aptr, M.aidx, M.aval, U.m,
+                               B, (const V*)x.get(), y.get());
+          });
         if (U.n > 0)
-          hipLaunchKernelGGL(unr_spmm<V>, spmm_grid(U.n, B), dim3(kUB), 0, s, M.tptr, M.tidx, M.tval, U.n, B,
-                             (const V*)y.get(), x.get());
+          with_rows(U.rpw_t, [&](auto RC) {
+            constexpr int R = decltype(RC)::value;
+            hipLaunchKernelGGL((unr_spmm<V, R>), rows_grid(U.n, B, R), dim3(kUB), 0, s, M.tptr, M.tidx, M.tval, U.n,
+                               B, (const V*)y.get(), x.get());
+          });
       };
       run();
       CUOPT_HIP_CHECK(hipEventRecord(e0, s));

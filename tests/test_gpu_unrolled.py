@@ -74,6 +74,67 @@ def test_ranged_rows_vjp_matches_oracle():
     assert relerr(ut.grad.cpu().numpy().T, guc) < 1e-9
 
 
+def ragged_lp(seed=3, m=150, n=400):
+    """Equality LP whose rows and columns have uneven lengths: empty rows and columns,
+    rows of 1-3 nonzeros, and rows and columns far longer than a wavefront's 64-entry
+    index load (so the SpMM kernels' row runs cross load and gather-group boundaries)."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    rows, cols = [], []
+    for i in range(m):
+        k = 0 if i % 17 == 5 else (int(rng.integers(90, 140)) if i % 23 == 7 else int(rng.integers(1, 4)))
+        live = np.flatnonzero(np.arange(n) % 31 != 11)  # every 31st column stays empty
+        rows += [i] * k
+        cols += list(rng.choice(live, size=k, replace=False))
+    heavy = 3  # one column that many rows share
+    for i in range(0, m, 2):
+        if i % 17 != 5:
+            rows.append(i)
+            cols.append(heavy)
+    A = sp.csr_matrix((rng.normal(size=len(rows)), (rows, cols)), shape=(m, n))
+    A.sum_duplicates()
+    A.sort_indices()
+    x0 = rng.uniform(0, 1, size=n)
+    b = A @ x0
+    lp = G.LP(m=m, n=n, row_offsets=A.indptr.astype(np.int32), col_indices=A.indices.astype(np.int32),
+              values=A.data.astype(np.float64), c=rng.normal(size=n), con_lb=b.copy(), con_ub=b.copy(),
+              var_lb=np.zeros(n), var_ub=np.full(n, 10.0))
+    return lp, A
+
+
+@pytest.mark.parametrize("rows_per_wave", [1, 2, 4])
+@pytest.mark.parametrize("B", [5, 70])
+def test_rows_per_wavefront_forms_match_oracle(monkeypatch, rows_per_wave, B):
+    # The SpMM kernels give each wavefront 1, 2 or 4 consecutive rows (chosen from the
+    # matrix's shape; forced here).  Every form adds each row's products in increasing
+    # order, so FP64 forward and VJP match the oracle to 1e-9 on a ragged matrix, with
+    # batch sizes below and above one wavefront.
+    monkeypatch.setenv("CUOPT_AMD_UNR_RPW_A", str(rows_per_wave))
+    monkeypatch.setenv("CUOPT_AMD_UNR_RPW_T", str(rows_per_wave))
+    lp, A = ragged_lp()
+    assert np.diff(lp.row_offsets).max() > 64 and np.diff(A.tocsc().indptr).max() > 64
+    assert (np.diff(lp.row_offsets) == 0).any() and (np.diff(A.tocsc().indptr) == 0).any()
+    K, every, lam = 40, 7, 1e-3
+    op = UnrolledPDHG(lp, ridge=lam, steps=K, checkpoint_every=every, precision="fp64")
+    eta = op.step_size
+    rng = np.random.default_rng(rows_per_wave * 100 + B)
+    c = lp.c[None, :] + 0.1 * rng.normal(size=(B, lp.n))
+    b = lp.con_lb.copy()
+    print(f"ragged unrolled R={rows_per_wave} B={B}", flush=True)
+    ct = torch.tensor(c, device=DEV, requires_grad=True)
+    bt = torch.tensor(b, device=DEV, requires_grad=True)
+    x, y = op(ct, bt)
+    xo, yo, xs, ys = U.forward(A, c.T.copy(), b, b, lp.var_lb, lp.var_ub, lam, eta, K, keep=True)
+    assert relerr(x.detach().cpu().numpy().T, xo) < 1e-9
+    assert relerr(y.detach().cpu().numpy().T, yo) < 1e-9
+    target = rng.uniform(0, 1, size=(B, lp.n))
+    loss = ((x - torch.tensor(target, device=DEV)) ** 2).sum() + (y ** 2).sum()
+    loss.backward()
+    gc, glc, guc = U.backward(A, b, b, lp.var_lb, lp.var_ub, lam, eta, xs, ys, 2 * (xo - target.T), 2 * yo)
+    assert relerr(ct.grad.cpu().numpy().T, gc) < 1e-9
+    assert relerr(bt.grad.cpu().numpy(), (glc + guc).sum(axis=1)) < 1e-9
+
+
 def test_vjp_matches_central_differences_fp64():
     # Sized so that no clamp of the K steps switches within +-h along the test
     # directions (the map is piecewise linear; at 300 x 750 x 200 steps some do,
