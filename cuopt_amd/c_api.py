@@ -9,6 +9,7 @@ from __future__ import annotations
 
 import ctypes
 import dataclasses
+import gc
 import os
 
 import numpy as np
@@ -109,6 +110,7 @@ def lib():
         L.cuOptAmdDeviceCount.restype = i32
         L.cuOptAmdBenchmarkSteps.argtypes = [vp, vp, i32, i32, vp, i32]
         L.cuOptAmdBatchSolve.argtypes = [i32, vp, vp, vp]
+        L.cuOptAmdGetSolutions.argtypes = [i32, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -284,33 +286,44 @@ def solve_problem(problem: Problem, settings: SolverSettings) -> Result:
     return _take_result(sol, problem.lp)
 
 
+_NO_TRACE = np.zeros((0, TRACE_WIDTH))
+
+
 def _take_result(sol, lp) -> Result:
-    """Reads every getter of a solution handle, then destroys it."""
+    """Reads every getter of a solution handle, then destroys it.  x, y, the reduced costs
+    and the termination info are views of one host buffer, so that one array address
+    serves the four getters that fill them."""
     L = lib()
     try:
         st, es = ctypes.c_int32(), ctypes.c_int32()
         L.cuOptGetTerminationStatus(sol, ctypes.byref(st))
         L.cuOptGetErrorStatus(sol, ctypes.byref(es))
-        buf = ctypes.create_string_buffer(1024)
+        msg = ""
         if es.value != 0:  # the message of a successful solve is empty
+            buf = ctypes.create_string_buffer(1024)
             L.cuOptGetErrorString(sol, buf, 1024)
-        x, y, z = np.zeros(max(lp.n, 1)), np.zeros(max(lp.m, 1)), np.zeros(max(lp.n, 1))
-        L.cuOptGetPrimalSolution(sol, _p(x))
-        L.cuOptGetDualSolution(sol, _p(y))
-        L.cuOptGetReducedCosts(sol, _p(z))
+            msg = buf.value.decode()
+        n, m, ni = lp.n, lp.m, len(INFO_NAMES)
+        out = np.zeros(2 * n + m + ni)  # a failed solve may leave arrays unwritten
+        base = out.__array_interface__["data"][0]
+        L.cuOptGetPrimalSolution(sol, base)
+        L.cuOptGetDualSolution(sol, base + 8 * n)
+        L.cuOptGetReducedCosts(sol, base + 8 * (n + m))
+        L.cuOptAmdGetTerminationInfo(sol, base + 8 * (2 * n + m), ni)
         po, do, tm = ctypes.c_double(), ctypes.c_double(), ctypes.c_double()
         L.cuOptGetObjectiveValue(sol, ctypes.byref(po))
         L.cuOptGetDualObjectiveValue(sol, ctypes.byref(do))
         L.cuOptGetSolveTime(sol, ctypes.byref(tm))
-        info = np.zeros(len(INFO_NAMES))
-        L.cuOptAmdGetTerminationInfo(sol, _p(info), len(INFO_NAMES))
-        n = ctypes.c_int32()
-        L.cuOptAmdGetTrace(sol, None, 0, ctypes.byref(n))
-        tr = np.zeros((max(n.value, 1), TRACE_WIDTH))
-        if n.value > 0:
-            L.cuOptAmdGetTrace(sol, _p(tr), n.value, ctypes.byref(n))
-        return Result(st.value, es.value, buf.value.decode(), x[: lp.n], y[: lp.m], z[: lp.n], po.value, do.value,
-                      tm.value, dict(zip(INFO_NAMES, info.tolist())), tr[: n.value])
+        nt = ctypes.c_int32()
+        L.cuOptAmdGetTrace(sol, None, 0, ctypes.byref(nt))
+        tr = _NO_TRACE
+        if nt.value > 0:
+            tr = np.zeros((nt.value, TRACE_WIDTH))
+            L.cuOptAmdGetTrace(sol, _p(tr), nt.value, ctypes.byref(nt))
+            tr = tr[: nt.value]
+        info = out[2 * n + m:]
+        return Result(st.value, es.value, msg, out[:n], out[n:n + m], out[n + m:2 * n + m], po.value, do.value,
+                      tm.value, dict(zip(INFO_NAMES, info.tolist())), tr)
     finally:
         L.cuOptDestroySolution(ctypes.byref(sol))
 
@@ -370,13 +383,58 @@ def batch_solve(lps, tol: float = 1e-4, precision: str = "fp64", iteration_limit
         _check(s.set(name, v), name)
     hs = (ctypes.c_void_p * max(len(probs), 1))(*[p.handle.value for p in probs])
     sols = (ctypes.c_void_p * max(len(probs), 1))()
-    lib().cuOptAmdBatchSolve(len(probs), hs, s.handle, sols)
-    out = []
-    for i, p in enumerate(probs):
-        if not sols[i]:
-            raise CuOptError(CUOPT_RUNTIME_ERROR, "cuOptAmdBatchSolve returned no solution")
-        out.append(_take_result(ctypes.c_void_p(sols[i]), p.lp))
-    return out
+    L = lib()
+    L.cuOptAmdBatchSolve(len(probs), hs, s.handle, sols)
+    return _take_batch(sols, probs)
+
+
+def _take_batch(sols, probs) -> list[Result]:
+    """Reads the solutions of a batch with one cuOptAmdGetSolutions call into one host
+    buffer (every result's x, y and reduced costs are views of it), then destroys them.
+    Batch solutions carry no per-check trace."""
+    L = lib()
+    count = len(probs)
+    try:
+        for i in range(count):
+            if not sols[i]:
+                raise CuOptError(CUOPT_RUNTIME_ERROR, "cuOptAmdBatchSolve returned no solution")
+        ni = len(INFO_NAMES)
+        ns = np.fromiter((p.lp.n for p in probs), np.int64, count)
+        ms = np.fromiter((p.lp.m for p in probs), np.int64, count)
+        offsets = np.zeros(count + 1, np.int64)
+        np.cumsum(2 * ns + ms + ni, out=offsets[1:])
+        store = np.zeros(max(int(offsets[-1]), 1))
+        status, err = np.zeros(count, np.int32), np.zeros(count, np.int32)
+        objs = np.zeros((count, 3))
+        _check(L.cuOptAmdGetSolutions(count, sols, _p(store), _p(offsets), _p(status), _p(err), _p(objs)),
+               "cuOptAmdGetSolutions")
+        st, er, ob = status.tolist(), err.tolist(), objs.tolist()
+        out = []
+        # Thousands of small results: the cyclic collector would rescan them all as they
+        # accumulate (none of these objects forms a cycle).
+        collect = gc.isenabled()
+        gc.disable()
+        try:
+            for i, (o, n, m) in enumerate(zip(offsets[:-1].tolist(), ns.tolist(), ms.tolist())):
+                msg = ""
+                if er[i] != 0:
+                    buf = ctypes.create_string_buffer(1024)
+                    L.cuOptGetErrorString(sols[i], buf, 1024)
+                    msg = buf.value.decode()
+                e = o + 2 * n + m
+                out.append(Result(st[i], er[i], msg, store[o:o + n], store[o + n:o + n + m], store[o + n + m:e],
+                                  ob[i][0], ob[i][1], ob[i][2], dict(zip(INFO_NAMES, store[e:e + ni].tolist())),
+                                  _NO_TRACE))
+        finally:
+            if collect:
+                gc.enable()
+        return out
+    finally:
+        for i in range(count):
+            if sols[i]:
+                h = ctypes.c_void_p(sols[i])
+                L.cuOptDestroySolution(ctypes.byref(h))
+                sols[i] = None
 
 
 _BATCH_COST = ctypes.CFUNCTYPE(ctypes.c_double, ctypes.c_int32, ctypes.c_void_p)

@@ -1295,24 +1295,64 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
     ~StreamGuard() { (void)hipStreamDestroy(s); }
   } guard{s};
   const size_t M = std::max(tot_m, 1), N = std::max(tot_n, 1), Z = std::max<long long>(tot_nnz, 1);
-  DevBuf<BatchDesc> d_desc(std::max<size_t>(desc.size(), 1));
-  DevBuf<int> d_blk(std::max<size_t>(h_blk.size(), 1));
+  DevArena arena;  // every working array below: one allocation
+  auto d_desc = arena.take<BatchDesc>(std::max<size_t>(desc.size(), 1));
+  auto d_blk = arena.take<int>(std::max<size_t>(h_blk.size(), 1));
   // The nonzero arrays read by row passes carry kNnzPad elements of padding:
   // a block's 16-byte copy may read up to 3 dwords past its last nonzero.
   const size_t Zp = Z + kNnzPad;
-  DevBuf<int> d_order(std::max(nb, 1)), a_ptr(std::max(tot_ap, 1)), a_idx(Zp), t_ptr(std::max(tot_tp, 1)),
-    t_idx(Zp), t_pos(Z), t_fill(N);
-  DevBuf<double> a_orig(Zp), t_orig(Zp), c(N), lb(N), ub(N), lc(M), uc(M), Dr(M), Dc(N), rowv(M), colv(N);
-  DevBuf<V> a_w(Zp), t_w(Zp), c_w(N), x0(N), x1(N), y0(M), y1(M), xa(N), ya(M), xbar(N), ybar(M), xhat(N), yhat(M),
-    slack(N), aty(N);
-  DevBuf<V2> bnd_w(N), cb_w(M);
-  DevBuf<double2> cb_orig(M);
-  DevBuf<double> xu(N), yu(M), su(N), pz(M), pz2(M), patq(N), dz0(z0.size());
+  auto d_order = arena.take<int>(std::max(nb, 1));
+  auto a_ptr = arena.take<int>(std::max(tot_ap, 1));
+  auto a_idx = arena.take<int>(Zp);
+  auto t_ptr = arena.take<int>(std::max(tot_tp, 1));
+  auto t_idx = arena.take<int>(Zp);
+  auto t_pos = arena.take<int>(Z);
+  auto t_fill = arena.take<int>(N);
+  auto a_orig = arena.take<double>(Zp);
+  auto t_orig = arena.take<double>(Zp);
+  auto c = arena.take<double>(N);
+  auto lb = arena.take<double>(N);
+  auto ub = arena.take<double>(N);
+  auto lc = arena.take<double>(M);
+  auto uc = arena.take<double>(M);
+  auto Dr = arena.take<double>(M);
+  auto Dc = arena.take<double>(N);
+  auto rowv = arena.take<double>(M);
+  auto colv = arena.take<double>(N);
+  auto a_w = arena.take<V>(Zp);
+  auto t_w = arena.take<V>(Zp);
+  auto c_w = arena.take<V>(N);
+  auto x0 = arena.take<V>(N);
+  auto x1 = arena.take<V>(N);
+  auto y0 = arena.take<V>(M);
+  auto y1 = arena.take<V>(M);
+  auto xa = arena.take<V>(N);
+  auto ya = arena.take<V>(M);
+  auto xbar = arena.take<V>(N);
+  auto ybar = arena.take<V>(M);
+  auto xhat = arena.take<V>(N);
+  auto yhat = arena.take<V>(M);
+  auto slack = arena.take<V>(N);
+  auto aty = arena.take<V>(N);
+  auto bnd_w = arena.take<V2>(N);
+  auto cb_w = arena.take<V2>(M);
+  auto cb_orig = arena.take<double2>(M);
+  auto xu = arena.take<double>(N);
+  auto yu = arena.take<double>(M);
+  auto su = arena.take<double>(N);
+  auto pz = arena.take<double>(M);
+  auto pz2 = arena.take<double>(M);
+  auto patq = arena.take<double>(N);
+  auto dz0 = arena.take<double>(z0.size());
   const bool refine = std::is_same<V, float>::value && st.refinement != 0;
-  DevBuf<double> xstar(refine ? N : 1), ystar(refine ? M : 1), ax64(refine ? M : 1), aty64(refine ? N : 1);
-  DevBuf<V> kink(refine ? M : 1);
-  DevBuf<BatchOut> out(std::max<size_t>(lps.size(), 1));
-  DevBuf<double> lz(std::is_same<V, float>::value ? std::max<size_t>(lps.size(), 1) * kLzStride : 1);
+  auto xstar = arena.take<double>(refine ? N : 1);
+  auto ystar = arena.take<double>(refine ? M : 1);
+  auto ax64 = arena.take<double>(refine ? M : 1);
+  auto aty64 = arena.take<double>(refine ? N : 1);
+  auto kink = arena.take<V>(refine ? M : 1);
+  auto out = arena.take<BatchOut>(std::max<size_t>(lps.size(), 1));
+  auto lz = arena.take<double>(std::is_same<V, float>::value ? std::max<size_t>(lps.size(), 1) * kLzStride : 1);
+  arena.commit();
   d_desc.upload(desc, s);
   if (!h_blk.empty()) d_blk.upload(h_blk, s);
   d_order.upload(order, s);
@@ -1427,11 +1467,12 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
   }
   CUOPT_HIP_CHECK(hipEventRecord(ev[1], s));
   std::vector<BatchOut> h_out(std::max<size_t>(lps.size(), 1));
-  std::vector<double> h_xu(N), h_yu(M), h_su(N);
+  // Uninitialised: the downloads below overwrite every element.
+  std::unique_ptr<double[]> h_xu(new double[N]), h_yu(new double[M]), h_su(new double[N]);
   out.download(h_out.data(), h_out.size(), s);
-  xu.download(h_xu.data(), N, s);
-  yu.download(h_yu.data(), M, s);
-  su.download(h_su.data(), N, s);
+  xu.download(h_xu.get(), N, s);
+  yu.download(h_yu.get(), M, s);
+  su.download(h_su.get(), N, s);
   CUOPT_HIP_CHECK(hipStreamSynchronize(s));
   const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   float kernel_ms   = 0.f;
@@ -1452,9 +1493,9 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
     const BatchOut& o  = h_out[a];
     SolveResult& r     = R[a];
     r.status = o.status;
-    r.x.assign(h_xu.begin() + d.col0, h_xu.begin() + d.col0 + lp.n);
-    r.y.assign(h_yu.begin() + d.row0, h_yu.begin() + d.row0 + lp.m);
-    r.rc.assign(h_su.begin() + d.col0, h_su.begin() + d.col0 + lp.n);
+    r.x.assign(h_xu.get() + d.col0, h_xu.get() + d.col0 + lp.n);
+    r.y.assign(h_yu.get() + d.row0, h_yu.get() + d.row0 + lp.m);
+    r.rc.assign(h_su.get() + d.col0, h_su.get() + d.col0 + lp.n);
     if (lp.maximize) {  // solve.cu:583-588
       for (double& v : r.y) v = -v;
       for (double& v : r.rc) v = -v;

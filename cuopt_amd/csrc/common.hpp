@@ -2,6 +2,7 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+#include <algorithm>
 
 #include <cstddef>
 #include <cstdio>
@@ -79,6 +80,49 @@ class DevBuf {
  private:
   T* p_     = nullptr;
   size_t n_ = 0;
+};
+
+// Device arrays carved from one allocation: one hipMalloc and one hipFree for a
+// whole set of working arrays (a batch solve uses about fifty).  take<T>(n)
+// reserves a 256-byte aligned part; commit() allocates them all; a part's get()
+// is valid after commit() and until the arena is destroyed.
+class DevArena {
+ public:
+  template <typename T>
+  class Part {
+   public:
+    T* get() const { return a_->mem_.get() ? reinterpret_cast<T*>(a_->mem_.get() + off_) : nullptr; }
+    size_t size() const { return n_; }
+    void upload(const T* h, size_t n, hipStream_t s)
+    {
+      if (n) CUOPT_HIP_CHECK(hipMemcpyAsync(get(), h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
+    void upload(const std::vector<T>& h, hipStream_t s) { upload(h.data(), h.size(), s); }
+    void download(T* h, size_t n, hipStream_t s) const
+    {
+      if (n) CUOPT_HIP_CHECK(hipMemcpyAsync(h, get(), n * sizeof(T), hipMemcpyDeviceToHost, s));
+    }
+
+   private:
+    friend class DevArena;
+    Part(const DevArena* a, size_t off, size_t n) : a_(a), off_(off), n_(n) {}
+    const DevArena* a_;
+    size_t off_, n_;
+  };
+  template <typename T>
+  Part<T> take(size_t n)
+  {
+    if (mem_.get()) throw std::logic_error("DevArena::take after commit");
+    const size_t off = bytes_;
+    bytes_ += (n * sizeof(T) + 255) & ~(size_t)255;
+    return Part<T>(this, off, n);
+  }
+  void commit() { mem_.resize(std::max<size_t>(bytes_, 1)); }
+  size_t bytes() const { return bytes_; }
+
+ private:
+  DevBuf<unsigned char> mem_;
+  size_t bytes_ = 0;
 };
 
 }  // namespace cuopt_amd

@@ -10,6 +10,7 @@
 // dual-simplex / barrier methods, are rejected with CUOPT_INVALID_ARGUMENT;
 // every PDLP solver mode runs the Stable3 method; every presolve value but Off
 // runs this engine's own presolve (presolve.hpp).
+#include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -17,6 +18,9 @@
 #include <limits>
 #include <new>
 #include <string>
+#include <thread>
+#include <vector>
+#include <algorithm>
 
 #include "../../include/cuopt_amd_ext.h"
 #include "common.hpp"
@@ -644,6 +648,7 @@ cuopt_int_t cuOptAmdBatchSolve(cuopt_int_t num_problems,
                                cuOptSolverSettings settings,
                                cuOptSolution* solutions)
 {
+  const auto t_call = std::chrono::steady_clock::now();
   if (num_problems < 0 || (num_problems > 0 && (!problems || !solutions)) || !settings) return CUOPT_INVALID_ARGUMENT;
   for (int i = 0; i < num_problems; ++i) {
     solutions[i] = nullptr;
@@ -698,11 +703,16 @@ cuopt_int_t cuOptAmdBatchSolve(cuopt_int_t num_problems,
       code = CUOPT_OUT_OF_MEMORY;
     for (int i : where) fail(sols[i]->res, code, msg);
   }
+  // Every solution reports the whole call's wall time: checks, the device solve,
+  // result assembly and the release of the working memory.
+  const double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_call).count();
   for (int i = 0; i < num_problems; ++i) {
     SolveResult& R = sols[i]->res;
     if (R.x.size() != (size_t)sols[i]->n) R.x.assign(sols[i]->n, 0.0);
     if (R.y.size() != (size_t)sols[i]->m) R.y.assign(sols[i]->m, 0.0);
     if (R.rc.size() != (size_t)sols[i]->n) R.rc.assign(sols[i]->n, 0.0);
+    R.solve_time                     = secs;
+    R.info[CUOPT_AMD_INFO_SOLVE_TIME] = secs;
     solutions[i] = sols[i];
   }
   return first_error;
@@ -889,6 +899,64 @@ cuopt_int_t cuOptAmdGetTrace(cuOptSolution solution, cuopt_float_t* rows, cuopt_
   *len                 = nrec;
   const int ncopy      = nrec < cap ? nrec : cap;
   if (ncopy > 0) std::memcpy(rows, R.trace.data(), (size_t)ncopy * CUOPT_AMD_TRACE_WIDTH * sizeof(double));
+  return CUOPT_SUCCESS;
+}
+
+cuopt_int_t cuOptAmdGetSolutions(cuopt_int_t num_solutions,
+                                 const cuOptSolution* solutions,
+                                 cuopt_float_t* store,
+                                 const int64_t* offsets,
+                                 cuopt_int_t* status,
+                                 cuopt_int_t* error_status,
+                                 cuopt_float_t* objectives)
+{
+  if (num_solutions < 0) return CUOPT_INVALID_ARGUMENT;
+  if (num_solutions == 0) return CUOPT_SUCCESS;
+  if (!solutions || !store || !offsets || !status || !error_status || !objectives) return CUOPT_INVALID_ARGUMENT;
+  for (int i = 0; i < num_solutions; ++i) {
+    if (!solutions[i] || offsets[i] < 0) return CUOPT_INVALID_ARGUMENT;
+    const SolutionHandle* h = as_solution(solutions[i]);
+    if (offsets[i + 1] - offsets[i] != 2 * (int64_t)h->n + h->m + CUOPT_AMD_INFO_COUNT) return CUOPT_INVALID_ARGUMENT;
+  }
+  auto one = [&](int i) {
+    const SolutionHandle* h = as_solution(solutions[i]);
+    const SolveResult& R    = h->res;
+    double* o               = store + offsets[i];
+    auto put = [&](const std::vector<double>& v, size_t len) {
+      if (v.size() == len) {
+        if (len) std::memcpy(o, v.data(), len * sizeof(double));
+      } else {
+        std::fill(o, o + len, 0.0);
+      }
+      o += len;
+    };
+    put(R.x, (size_t)h->n);
+    put(R.y, (size_t)h->m);
+    put(R.rc, (size_t)h->n);
+    std::memcpy(o, R.info, sizeof(R.info));
+    status[i]             = R.status;
+    error_status[i]       = R.error;
+    objectives[3 * i]     = R.primal_objective;
+    objectives[3 * i + 1] = R.dual_objective;
+    objectives[3 * i + 2] = R.solve_time;
+  };
+  // Threads only when there is enough to copy; a failed thread start copies serially.
+  const int64_t bytes = offsets[num_solutions] * (int64_t)sizeof(double);
+  const int nt        = bytes < (4 << 20) ? 1
+                                          : std::max(1, std::min({16, num_solutions,
+                                                                  (int)std::thread::hardware_concurrency()}));
+  std::vector<std::thread> th;
+  auto work = [&](int t) {
+    for (int i = t; i < num_solutions; i += nt) one(i);
+  };
+  int started = 1;
+  try {
+    for (int t = 1; t < nt; ++t, ++started) th.emplace_back(work, t);
+  } catch (...) {
+  }
+  work(0);
+  for (auto& x : th) x.join();
+  for (int t = started; t < nt; ++t) work(t);
   return CUOPT_SUCCESS;
 }
 

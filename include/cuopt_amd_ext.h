@@ -139,6 +139,26 @@ cuopt_int_t cuOptAmdGetTerminationInfo(cuOptSolution solution, cuopt_float_t* in
 #define CUOPT_AMD_TRACE_WIDTH 10
 cuopt_int_t cuOptAmdGetTrace(cuOptSolution solution, cuopt_float_t* rows, cuopt_int_t cap, cuopt_int_t* len);
 
+/* Reads num_solutions solutions in one call (no reference counterpart: for
+ * callers that read thousands of small solutions, such as a batch, where one
+ * call per getter costs more than the copies).  For solution i, with n variables
+ * and m constraints, x (n), y (m), the reduced costs (n) and the termination
+ * info (CUOPT_AMD_INFO_COUNT values, as cuOptAmdGetTerminationInfo) are written
+ * one after another from store + offsets[i], and must fill exactly
+ * [offsets[i], offsets[i + 1]) (offsets has num_solutions + 1 entries); arrays a
+ * failed solve did not produce are written as zeros.  status[i] and
+ * error_status[i] receive the termination and error statuses, and
+ * objectives[3 i .. 3 i + 2] the primal objective, dual objective and solve time.
+ * Every size is checked before anything is written; the copies run on up to 16
+ * host threads.  The handles stay valid (destroy each with cuOptDestroySolution). */
+cuopt_int_t cuOptAmdGetSolutions(cuopt_int_t num_solutions,
+                                 const cuOptSolution* solutions,
+                                 cuopt_float_t* store,
+                                 const int64_t* offsets,
+                                 cuopt_int_t* status,
+                                 cuopt_int_t* error_status,
+                                 cuopt_float_t* objectives);
+
 /* y = A x on the GPU with the solver's tiled row-block SpMV (the kernel that
  * replaces the cusparseSpMV calls of pdhg.cu:285-347).  Host arrays in and
  * out.  precision: CUOPT_AMD_PRECISION_FP64 (FP64 products and sums) or
