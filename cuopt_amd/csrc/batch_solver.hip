@@ -1365,6 +1365,18 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
   lc.upload(h_lc.get(), tot_m, s);
   uc.upload(h_uc.get(), tot_m, s);
   dz0.upload(z0, s);
+  // Marks the end of the uploads, so that the host staging can be released while
+  // the persistent launch runs.
+  hipEvent_t uploaded = nullptr;
+  struct UploadGuard {
+    hipEvent_t* e;
+    ~UploadGuard()
+    {
+      if (*e) (void)hipEventDestroy(*e);
+    }
+  } up_guard{&uploaded};
+  CUOPT_HIP_CHECK(hipEventCreateWithFlags(&uploaded, hipEventDisableTiming));
+  CUOPT_HIP_CHECK(hipEventRecord(uploaded, s));
 
   BatchArgs<V> args{};
   args.desc = d_desc.get();
@@ -1466,9 +1478,30 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
     CUOPT_HIP_CHECK(hipGetLastError());
   }
   CUOPT_HIP_CHECK(hipEventRecord(ev[1], s));
+  // Host work while the launch runs: release the input staging once its uploads
+  // are done, page in the download buffers and size every result's arrays, so
+  // that the downloads and the assembly after the launch only copy.
+  CUOPT_HIP_CHECK(hipEventSynchronize(uploaded));
+  for (auto* p : {&h_ptr, &h_idx}) p->reset();
+  for (auto* p : {&h_val, &h_c, &h_lb, &h_ub, &h_lc, &h_uc}) p->reset();
   std::vector<BatchOut> h_out(std::max<size_t>(lps.size(), 1));
-  // Uninitialised: the downloads below overwrite every element.
   std::unique_ptr<double[]> h_xu(new double[N]), h_yu(new double[M]), h_su(new double[N]);
+  host_parallel_for(16, [&](int t) {
+    auto touch = [&](double* p, size_t len) {
+      const size_t b = len * t / 16, e = len * (t + 1) / 16;
+      std::fill(p + b, p + e, 0.0);
+    };
+    touch(h_xu.get(), N);
+    touch(h_yu.get(), M);
+    touch(h_su.get(), N);
+  });
+  host_parallel_for((int)active.size(), [&](int q) {
+    const int a      = active[q];
+    const LpHost& lp = *lps[a];
+    R[a].x.resize(lp.n);
+    R[a].y.resize(lp.m);
+    R[a].rc.resize(lp.n);
+  });
   out.download(h_out.data(), h_out.size(), s);
   xu.download(h_xu.get(), N, s);
   yu.download(h_yu.get(), M, s);
@@ -1493,9 +1526,9 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
     const BatchOut& o  = h_out[a];
     SolveResult& r     = R[a];
     r.status = o.status;
-    r.x.assign(h_xu.get() + d.col0, h_xu.get() + d.col0 + lp.n);
-    r.y.assign(h_yu.get() + d.row0, h_yu.get() + d.row0 + lp.m);
-    r.rc.assign(h_su.get() + d.col0, h_su.get() + d.col0 + lp.n);
+    std::copy(h_xu.get() + d.col0, h_xu.get() + d.col0 + lp.n, r.x.begin());
+    std::copy(h_yu.get() + d.row0, h_yu.get() + d.row0 + lp.m, r.y.begin());
+    std::copy(h_su.get() + d.col0, h_su.get() + d.col0 + lp.n, r.rc.begin());
     if (lp.maximize) {  // solve.cu:583-588
       for (double& v : r.y) v = -v;
       for (double& v : r.rc) v = -v;
