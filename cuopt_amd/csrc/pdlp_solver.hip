@@ -1540,9 +1540,7 @@ void PdlpSolver<V>::power_iteration()
   CUOPT_HIP_CHECK(hipMemcpyAsync(st.get(), &h, sizeof(h), hipMemcpyHostToDevice, stream_));
   double* zo = za.get();
   double* zn = zb.get();
-  int launched = 0;
-  while (launched < 5000) {
-    const int batch = std::min(16, 5000 - launched);
+  auto launch_batch = [&](int batch) {
     for (int b = 0; b < batch; ++b) {
       launch_tiled(AT_, at_s_.get(), nullptr, PowerAtEpi{zo, st.get(), atq.get()}, stream_);
       launch_tiled(A_, a_s_.get(), red_.get(), PowerAEpi{atq.get(), zo, st.get(), zn}, stream_);
@@ -1552,6 +1550,42 @@ void PdlpSolver<V>::power_iteration()
                          1e-4);
       CUOPT_HIP_CHECK(hipGetLastError());
       std::swap(zo, zn);
+    }
+  };
+  // A full batch of 16 steps (even, so it starts and ends on the same buffer) is
+  // captured once and replayed: on small problems its 80 launches cost more than
+  // their kernels.
+  constexpr int kPowerBatch = 16;
+  hipGraphExec_t batch_graph = nullptr;
+  struct GraphGuard {
+    hipGraphExec_t* g;
+    ~GraphGuard()
+    {
+      if (*g) (void)hipGraphExecDestroy(*g);
+    }
+  } graph_guard{&batch_graph};
+  int launched = 0;
+  while (launched < 5000) {
+    const int batch = std::min(kPowerBatch, 5000 - launched);
+    if (use_graphs_ && batch == kPowerBatch) {
+      if (!batch_graph) {
+        hipGraph_t g;
+        CUOPT_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+        try {
+          launch_batch(batch);
+        } catch (...) {
+          (void)hipStreamEndCapture(stream_, &g);
+          if (g) (void)hipGraphDestroy(g);
+          throw;
+        }
+        CUOPT_HIP_CHECK(hipStreamEndCapture(stream_, &g));
+        const hipError_t r = hipGraphInstantiate(&batch_graph, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        CUOPT_HIP_CHECK(r);
+      }
+      CUOPT_HIP_CHECK(hipGraphLaunch(batch_graph, stream_));
+    } else {
+      launch_batch(batch);
     }
     launched += batch;
     CUOPT_HIP_CHECK(hipMemcpyAsync(&h, st.get(), sizeof(h), hipMemcpyDeviceToHost, stream_));
