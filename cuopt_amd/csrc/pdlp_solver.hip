@@ -826,16 +826,13 @@ void build_plan(MatStruct& M, hipStream_t s, ColSource cols = {}, int merge_mode
 // where the plan placed that tile.  A resident grid keeps each workgroup's waves
 // alive across tiles instead of retiring and dispatching one workgroup per tile:
 // measured at 500 k x 1 M, FP32 dual step 37.8 -> 35.7 us, FP64 44.6 -> 43.2 us
-// (twice the resident count: 36.9 us).  CUOPT_AMD_PERSISTENT=0 launches one
-// workgroup per tile.
+// (twice the resident count: 36.9 us).  CUOPT_AMD_PERSISTENT=0 (read at each
+// launch, so that tests can compare both forms) launches one workgroup per tile.
 template <class Epi, int TILE>
 int tiled_grid(int ntiles)
 {
-  static const bool persistent = []() {
-    const char* e = std::getenv("CUOPT_AMD_PERSISTENT");
-    return !(e && std::atoi(e) == 0);
-  }();
-  if (!persistent) return ntiles;
+  if (const char* e = std::getenv("CUOPT_AMD_PERSISTENT"))
+    if (std::atoi(e) == 0) return ntiles;
   static const int resident = []() {
     int per_cu = 0, cus = 0, dev = 0;
     CUOPT_HIP_CHECK(hipGetDevice(&dev));

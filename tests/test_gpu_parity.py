@@ -780,6 +780,30 @@ def test_long_row_merge_forms_are_bitwise_the_same(precision, shape):
             assert np.array_equal(getattr(runs[mode], f), getattr(runs[0], f)), (mode, f)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32_mixed"])
+@pytest.mark.parametrize("tile", [512, 2048])
+def test_resident_grid_matches_one_workgroup_per_tile(monkeypatch, precision, tile):
+    # The tiled SpMV launches as many workgroups as are resident and loops each over
+    # tiles b, b + grid, ... (reusing its LDS buffers after a barrier).  Against one
+    # workgroup per tile (CUOPT_AMD_PERSISTENT=0) the whole solve is bitwise the same,
+    # at the tile capacities the tuner may choose besides the default 1 024; the LP has
+    # rows longer than a tile in both merge forms' range.
+    lp = G.skewed_lp(m=20000, n=40000, seed=7)
+    monkeypatch.setenv("CUOPT_AMD_SPMV_TILE_A", str(tile))
+    monkeypatch.setenv("CUOPT_AMD_SPMV_TILE_AT", str(tile))
+    runs = {}
+    for persistent in ("0", "1"):
+        monkeypatch.setenv("CUOPT_AMD_PERSISTENT", persistent)
+        print(f"tile {tile} {precision} persistent={persistent}", flush=True)
+        runs[persistent] = c_api.solve(lp, tol=1e-6, precision=precision, time_limit=TL, iteration_limit=200000)
+    a, b = runs["0"], runs["1"]
+    assert a.info["tile_a"] == a.info["tile_at"] == tile
+    assert a.status == b.status == 1 and a.iterations == b.iterations
+    for f in ("x", "y", "reduced_cost"):
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    assert rel(b.primal_objective, lp.obj_opt) < 1e-5
+
+
 def _highs(lp):
     """HiGHS (SciPy linprog) on a parsed LP: (status, objective with the offset, in the user's sense)."""
     import scipy.sparse as sp
