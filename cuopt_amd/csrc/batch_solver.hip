@@ -24,6 +24,7 @@
 // row epilogue.
 #include <algorithm>
 #include <chrono>
+#include <condition_variable>
 #include <cmath>
 #include <cstring>
 #include <memory>
@@ -1212,8 +1213,8 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
   std::unique_ptr<int[]> h_ptr(new int[tot_ap]), h_idx(new int[tot_nnz]);
   std::unique_ptr<double[]> h_val(new double[tot_nnz]), h_c(new double[tot_n]), h_lb(new double[tot_n]),
     h_ub(new double[tot_n]), h_lc(new double[tot_m]), h_uc(new double[tot_m]);
-  auto fill = [&](int t, int nt) {
-    for (int q = t; q < nb; q += nt) {
+  auto fill_one = [&](int q) {
+    {
       const int a      = active[q];
       const LpHost& lp = *lps[a];
       const BatchDesc& d = desc[a];
@@ -1242,12 +1243,6 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
       std::copy(ucs[a].begin(), ucs[a].end(), h_uc.get() + d.row0);
     }
   };
-  {
-    const int nt = std::max(1, std::min<int>(16, (int)std::thread::hardware_concurrency()));
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; ++t) th.emplace_back(fill, t, nt);
-    for (auto& t : th) t.join();
-  }
   // row blocks of A and A^T for each LP
   std::vector<std::vector<int>> ablocks(lps.size()), tblocks(lps.size());
   {
@@ -1356,15 +1351,62 @@ std::vector<SolveResult> run_batch(const std::vector<const LpHost*>& lps,
   d_desc.upload(desc, s);
   if (!h_blk.empty()) d_blk.upload(h_blk, s);
   d_order.upload(order, s);
-  a_ptr.upload(h_ptr.get(), tot_ap, s);
-  a_idx.upload(h_idx.get(), tot_nnz, s);
-  a_orig.upload(h_val.get(), tot_nnz, s);
-  c.upload(h_c.get(), tot_n, s);
-  lb.upload(h_lb.get(), tot_n, s);
-  ub.upload(h_ub.get(), tot_n, s);
-  lc.upload(h_lc.get(), tot_m, s);
-  uc.upload(h_uc.get(), tot_m, s);
   dz0.upload(z0, s);
+  // The LPs' arrays are concatenated on host threads in kGroups contiguous groups
+  // (by position in `active`, which is also their order in the device layout) by a
+  // helper thread, while this thread uploads each group as soon as it is complete:
+  // the pageable uploads overlap the copying of the next group.
+  {
+    constexpr int kGroups = 8;
+    const int ng = std::max(1, std::min(kGroups, nb));
+    auto gbeg = [&](int g) { return (int)((long long)nb * g / ng); };
+    std::mutex mu;
+    std::condition_variable cv;
+    int done = 0;  // groups complete
+    std::exception_ptr err;
+    std::thread filler([&]() {
+      try {
+        for (int g = 0; g < ng; ++g) {
+          const int q0 = gbeg(g), q1 = gbeg(g + 1);
+          host_parallel_for(q1 - q0, [&](int i) { fill_one(q0 + i); });
+          std::lock_guard<std::mutex> lock(mu);
+          ++done;
+          cv.notify_one();
+        }
+      } catch (...) {
+        std::lock_guard<std::mutex> lock(mu);
+        err  = std::current_exception();
+        done = ng;
+        cv.notify_one();
+      }
+    });
+    try {
+      for (int g = 0; g < ng && nb > 0; ++g) {
+        {
+          std::unique_lock<std::mutex> lock(mu);
+          cv.wait(lock, [&]() { return done > g; });
+          if (err) break;
+        }
+        const BatchDesc& f = desc[active[gbeg(g)]];
+        const BatchDesc& l = desc[active[gbeg(g + 1) - 1]];
+        const size_t ap0 = f.aptr0, ap1 = l.aptr0 + l.m + 1, z0g = f.nz0, z1g = l.nz0 + l.nnz;
+        const size_t c0 = f.col0, c1 = l.col0 + l.n, r0 = f.row0, r1 = l.row0 + l.m;
+        a_ptr.upload_at(ap0, h_ptr.get() + ap0, ap1 - ap0, s);
+        a_idx.upload_at(z0g, h_idx.get() + z0g, z1g - z0g, s);
+        a_orig.upload_at(z0g, h_val.get() + z0g, z1g - z0g, s);
+        c.upload_at(c0, h_c.get() + c0, c1 - c0, s);
+        lb.upload_at(c0, h_lb.get() + c0, c1 - c0, s);
+        ub.upload_at(c0, h_ub.get() + c0, c1 - c0, s);
+        lc.upload_at(r0, h_lc.get() + r0, r1 - r0, s);
+        uc.upload_at(r0, h_uc.get() + r0, r1 - r0, s);
+      }
+    } catch (...) {
+      filler.join();
+      throw;
+    }
+    filler.join();
+    if (err) std::rethrow_exception(err);
+  }
   // Marks the end of the uploads, so that the host staging can be released while
   // the persistent launch runs.
   hipEvent_t uploaded = nullptr;

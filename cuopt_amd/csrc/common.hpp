@@ -98,6 +98,12 @@ class DevArena {
       if (n) CUOPT_HIP_CHECK(hipMemcpyAsync(get(), h, n * sizeof(T), hipMemcpyHostToDevice, s));
     }
     void upload(const std::vector<T>& h, hipStream_t s) { upload(h.data(), h.size(), s); }
+    // Elements [off, off + n) from h (h points at element off of the host copy).
+    void upload_at(size_t off, const T* h, size_t n, hipStream_t s)
+    {
+      if (off + n > n_) throw std::out_of_range("DevArena::Part::upload_at past the part's end");
+      if (n) CUOPT_HIP_CHECK(hipMemcpyAsync(get() + off, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+    }
     void download(T* h, size_t n, hipStream_t s) const
     {
       if (n) CUOPT_HIP_CHECK(hipMemcpyAsync(h, get(), n * sizeof(T), hipMemcpyDeviceToHost, s));
