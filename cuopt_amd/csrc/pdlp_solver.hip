@@ -888,8 +888,12 @@ void launch_reduce(const double* part, int nblocks, int nred, double* out, hipSt
 // then a run of them between two HIP events; the least time wins, but the
 // untuned plan is kept unless the winner beats it by kTuneMargin, so that
 // candidates within timing noise do not change the choice from run to run.  The
-// timed SpMV is the solver's tiled kernel with a plain y = A x epilogue over the
-// working-precision values.  Choices are cached per matrix for the process.
+// timed launch is the step kernel over that matrix itself (the dual step's
+// epilogue for A, the primal step's for A^T) on zeroed operands, since the plans
+// that are fastest for a plain y = A x are not always fastest for the steps: with
+// a plain epilogue the tuner kept 1 024-nonzero tiles for A^T of a banded
+// 4 M x 8 M LP, whose FP64 primal step then took 365 us against 300 us with
+// 2 048.  Choices are cached per matrix for the process.
 constexpr int kTuneReps        = 10;
 constexpr float kTuneMargin    = 0.03f;
 
@@ -914,10 +918,12 @@ uint64_t fnv1a(const int* p, size_t n)
 // Picks (and installs in M) the fastest candidate; returns whether it timed anything.
 // merge_mode as pdlp_long_row_merge: -1 times both long-row merges, 0 / 1 only that one.
 template <typename V>
-bool autotune_spmv(MatStruct& M, const V* vals, ColSource cols, hipStream_t s, int merge_mode, SpmvChoice& out)
+bool autotune_spmv(MatStruct& M, const V* vals, ColSource cols, hipStream_t s, int merge_mode, bool primal,
+                   SpmvChoice& out)
 {
   const std::vector<uint64_t> key{(uint64_t)M.nrows, (uint64_t)M.ncols, (uint64_t)M.nnz, sizeof(V),
-                                  fnv1a(M.h_ptr.data(), M.h_ptr.size()), (uint64_t)(merge_mode + 1)};
+                                  fnv1a(M.h_ptr.data(), M.h_ptr.size()), (uint64_t)(merge_mode + 1),
+                                  (uint64_t)primal};
   {
     std::lock_guard<std::mutex> lock(g_tune_mu);
     auto it = g_tune_cache.find(key);
@@ -931,9 +937,17 @@ bool autotune_spmv(MatStruct& M, const V* vals, ColSource cols, hipStream_t s, i
       return false;
     }
   }
-  DevBuf<V> x(std::max(M.ncols, 1));
-  DevBuf<double> y(std::max(M.nrows, 1));
-  x.zero(s);
+  // Operands of the timed launches: the gathered vector and zeroed per-row vectors
+  // of the step over this matrix (distinct buffers, so that every read streams).
+  using V2          = typename Vec2<V>::type;
+  const size_t rows = (size_t)std::max(M.nrows, 1);
+  DevBuf<V> x(std::max(M.ncols, 1)), r0(rows), r1(rows), r2(rows), w0(rows), w1(rows);
+  DevBuf<V2> bnd(rows);
+  DevBuf<StepScalars> sc(1);
+  for (DevBuf<V>* b : {&x, &r0, &r1, &r2}) b->zero(s);
+  bnd.zero(s);
+  const StepScalars h_sc{1.0, 1.0, 0, 0};
+  sc.upload(&h_sc, 1, s);
   hipEvent_t ev[2] = {nullptr, nullptr};
   struct EventGuard {
     hipEvent_t* e;
@@ -965,8 +979,18 @@ bool autotune_spmv(MatStruct& M, const V* vals, ColSource cols, hipStream_t s, i
       const int def_fused = merge_mode >= 0 ? merge_mode : (P.fused ? 1 : 0);  // the untuned plan's merge
       for (int fused = f0; fused <= f1; ++fused) {
         auto run = [&]() {
-          launch_tiled_raw(tile, M.ptr.get(), M.idx.get(), vals, P.dev(), P.nlong, P.long_part.get(), nullptr,
-                           PlainEpi<V>{x.get(), y.get()}, s, fused ? P.long_cnt.get() : nullptr);
+          int* cnt = fused ? P.long_cnt.get() : nullptr;
+          if (primal)
+            launch_tiled_raw(tile, M.ptr.get(), M.idx.get(), vals, P.dev(), P.nlong, P.long_part.get(), nullptr,
+                             PrimalStepEpi<V, false, false>{x.get(), r0.get(), r1.get(), bnd.get(), r2.get(),
+                                                            w0.get(), w1.get(), nullptr, nullptr, nullptr,
+                                                            sc.get(), 0, false},
+                             s, cnt);
+          else
+            launch_tiled_raw(tile, M.ptr.get(), M.idx.get(), vals, P.dev(), P.nlong, P.long_part.get(), nullptr,
+                             DualStepEpi<V, false, false>{x.get(), r0.get(), bnd.get(), r1.get(), w0.get(), nullptr,
+                                                          nullptr, sc.get(), 0, nullptr},
+                             s, cnt);
         };
         run();
         CUOPT_HIP_CHECK(hipEventRecord(ev[0], s));
@@ -1846,8 +1870,8 @@ SolveResult PdlpSolver<V>::run()
   if (st_.spmv_autotune) {
     const double t = elapsed();
     SpmvChoice ca, cat;
-    const bool timed   = autotune_spmv<V>(A_, a_w_.get(), a_src_, stream_, st_.long_row_merge, ca);
-    const bool timed_t = autotune_spmv<V>(AT_, at_w_.get(), ColSource{}, stream_, st_.long_row_merge, cat);
+    const bool timed   = autotune_spmv<V>(A_, a_w_.get(), a_src_, stream_, st_.long_row_merge, false, ca);
+    const bool timed_t = autotune_spmv<V>(AT_, at_w_.get(), ColSource{}, stream_, st_.long_row_merge, true, cat);
     CUOPT_HIP_CHECK(hipStreamSynchronize(stream_));
     if (timed || timed_t) R.info[CUOPT_AMD_INFO_TUNE_TIME] = elapsed() - t;
     R.info[CUOPT_AMD_INFO_LANES_BIAS_A]  = ca.bias;

@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "tile_plan.hpp"
 
@@ -212,17 +213,57 @@ __device__ inline void merge_long_row(const PlanDev& plan, int lid, uint64_t* lo
 // 256-byte line of its own, so the atomics of different rows do not queue on one line.
 constexpr int kCntStride = 64;
 
+// A tile's matrix stream in registers: each lane's 16-byte vectors of values and
+// indices from the aligned base below z0 (elements outside [z0, z1) are loaded
+// but dropped), with the tile's header.
 template <class Epi, int TILE>
-__device__ inline void tile_body(int t, const int* __restrict__ rowptr, const int* __restrict__ colidx,
-                                 const typename Epi::Mat* __restrict__ vals, const PlanDev& plan,
-                                 uint64_t* __restrict__ long_part, double* __restrict__ red_part, const Epi& epi,
-                                 int* __restrict__ long_cnt, typename Epi::Acc::Elem* prod,
-                                 double (*red_lds)[kMaxRed], uint64_t* acc_lds)
+struct TileStream {
+  using VT             = VecOf<typename Epi::Mat>;
+  static constexpr int VN  = VT::N;
+  static constexpr int kIt = (TILE + kBlock * VN - 1) / (kBlock * VN);
+  Tile tl;
+  int mt;
+  typename VT::V v[kIt];
+  typename VT::I c[kIt];
+
+  __device__ void load(int t, const PlanDev& plan, const int* __restrict__ colidx,
+                       const typename Epi::Mat* __restrict__ vals)
+  {
+    tl = plan.tiles[t];
+    mt = plan.meta[t];
+    load_data(colidx, vals);
+  }
+  __device__ void load_data(const int* __restrict__ colidx, const typename Epi::Mat* __restrict__ vals)
+  {
+    const int base = tl.z0 & ~(VN - 1);
+#pragma unroll
+    for (int it = 0; it < kIt; ++it) {
+      const int e0 = base + (it * kBlock + (int)threadIdx.x) * VN;
+      if (e0 < tl.z1) {
+        v[it] = __builtin_nontemporal_load(reinterpret_cast<const typename VT::V*>(vals + e0));
+        c[it] = __builtin_nontemporal_load(reinterpret_cast<const typename VT::I*>(colidx + e0));
+      }
+    }
+  }
+};
+
+// One tile.  With PRELOADED its stream S was loaded by the caller (it may still be
+// in flight); otherwise the tile loads its own stream after the row pointers and
+// epilogue operands below, and S is not read.  prefetch() is called once by every
+// lane right after the tile's products are staged in LDS: the kernel issues the
+// next tile's stream there, so that it is in flight while this tile reduces its
+// rows and runs their epilogues.
+template <class Epi, int TILE, bool PRELOADED, class Prefetch>
+__device__ inline void tile_body(int t, const TileStream<Epi, TILE>& S, const int* __restrict__ rowptr,
+                                 const int* __restrict__ colidx, const typename Epi::Mat* __restrict__ vals,
+                                 const PlanDev& plan, uint64_t* __restrict__ long_part,
+                                 double* __restrict__ red_part, const Epi& epi, int* __restrict__ long_cnt,
+                                 typename Epi::Acc::Elem* prod, double (*red_lds)[kMaxRed], uint64_t* acc_lds,
+                                 Prefetch&& prefetch)
 {
   using Acc  = typename Epi::Acc;
-  using Mat  = typename Epi::Mat;
-  const Tile tl  = plan.tiles[t];
-  const int mt   = plan.meta[t];
+  const Tile tl  = PRELOADED ? S.tl : plan.tiles[t];
+  const int mt   = PRELOADED ? S.mt : plan.meta[t];
   const int tid  = threadIdx.x;
   const int nnz  = tl.z1 - tl.z0;
 
@@ -247,37 +288,36 @@ __device__ inline void tile_body(int t, const int* __restrict__ rowptr, const in
     }
   }
 
-  // Stage the tile's products in LDS.  Lanes read 16-byte vectors of values
-  // and indices from the aligned base below z0 (elements outside [z0, z1) are
-  // loaded but dropped); all loads of a lane are issued before the gathers.
+  // Stage the tile's products in LDS from the stream (all of a lane's stream
+  // loads are issued before its gathers).
   {
-    using VT          = VecOf<Mat>;
-    constexpr int VN  = VT::N;
-    constexpr int kIt = (TILE + kBlock * VN - 1) / (kBlock * VN);
+    constexpr int VN  = TileStream<Epi, TILE>::VN;
+    constexpr int kIt = TileStream<Epi, TILE>::kIt;
     const int base    = tl.z0 & ~(VN - 1);
-    typename VT::V v[kIt];
-    typename VT::I c[kIt];
+    auto stage = [&](const TileStream<Epi, TILE>& X) {
 #pragma unroll
-    for (int it = 0; it < kIt; ++it) {
-      const int e0 = base + (it * kBlock + tid) * VN;
-      if (e0 < tl.z1) {
-        v[it] = __builtin_nontemporal_load(reinterpret_cast<const typename VT::V*>(vals + e0));
-        c[it] = __builtin_nontemporal_load(reinterpret_cast<const typename VT::I*>(colidx + e0));
-      }
-    }
+      for (int it = 0; it < kIt; ++it) {
+        const int e0 = base + (it * kBlock + tid) * VN;
+        if (e0 < tl.z1) {
 #pragma unroll
-    for (int it = 0; it < kIt; ++it) {
-      const int e0 = base + (it * kBlock + tid) * VN;
-      if (e0 < tl.z1) {
-#pragma unroll
-        for (int i = 0; i < VN; ++i) {
-          const int e = e0 + i;
-          if (e >= tl.z0 && e < tl.z1) prod[e - tl.z0] = epi.product(v[it][i], c[it][i], e);
+          for (int i = 0; i < VN; ++i) {
+            const int e = e0 + i;
+            if (e >= tl.z0 && e < tl.z1) prod[e - tl.z0] = epi.product(X.v[it][i], X.c[it][i], e);
+          }
         }
       }
+    };
+    if constexpr (PRELOADED) {
+      stage(S);
+    } else {
+      TileStream<Epi, TILE> X;
+      X.tl = tl;
+      X.load_data(colidx, vals);
+      stage(X);
     }
   }
   __syncthreads();
+  prefetch();
 
   double red[kMaxRed];
 #pragma unroll
@@ -341,7 +381,8 @@ __device__ inline void tile_body(int t, const int* __restrict__ rowptr, const in
 
 // Workgroup b processes tiles b, b + gridDim.x, ... (the launch sizes the grid
 // to the resident workgroups, a multiple of 8, so every tile of workgroup b runs
-// on XCD b mod 8 as the plan placed it).  A barrier before every later tile
+// on XCD b mod 8 as the plan placed it); in FP64 the next tile's stream is
+// loaded while the current one reduces its rows.  A barrier before every later tile
 // orders the reuse of the LDS buffers: after a row-run tile without reductions,
 // or a long-row chunk that wave 0 finishes alone, other waves may reach the next
 // tile first.
@@ -360,9 +401,32 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
   __shared__ double red_lds[kBlock / kWave][kMaxRed];
   __shared__ uint64_t acc_lds[kBlock / kWave];
   if (epi.skip()) return;
-  for (int t = blockIdx.x; t < plan.ntiles; t += gridDim.x) {
-    if (t != (int)blockIdx.x) __syncthreads();
-    tile_body<Epi, TILE>(t, rowptr, colidx, vals, plan, long_part, red_part, epi, long_cnt, prod, red_lds, acc_lds);
+  if constexpr (std::is_same<typename Epi::Mat, double>::value) {
+    // FP64: the next tile's stream is loaded while this one reduces.  Measured at
+    // 500 k x 1 M: dual step 43.5 -> 41.3 us (tuned plan 42.2 -> 39.1 us), primal
+    // 33.9 -> 32.7 us, at the same occupancy.
+    TileStream<Epi, TILE> cur, nxt;
+    if ((int)blockIdx.x < plan.ntiles) cur.load(blockIdx.x, plan, colidx, vals);
+    for (int t = blockIdx.x; t < plan.ntiles; t += gridDim.x) {
+      if (t != (int)blockIdx.x) __syncthreads();
+      const int tn = t + gridDim.x;
+      tile_body<Epi, TILE, true>(t, cur, rowptr, colidx, vals, plan, long_part, red_part, epi, long_cnt, prod,
+                                 red_lds, acc_lds, [&]() {
+                                   if (tn < plan.ntiles) nxt.load(tn, plan, colidx, vals);
+                                 });
+      cur = nxt;
+    }
+  } else {
+    // FP32: the stream is loaded at the top of each tile.  Holding the next
+    // tile's stream raised the step kernels from 78 to 89-99 VGPRs (six waves
+    // per SIMD to five or four), and they slowed: banded 4 M x 8 M dual step
+    // 167 -> 199 us, the tuned 500 k x 1 M dual step 32.4 -> 34 us.
+    const TileStream<Epi, TILE> none{};  // not read: each tile loads its own stream
+    for (int t = blockIdx.x; t < plan.ntiles; t += gridDim.x) {
+      if (t != (int)blockIdx.x) __syncthreads();
+      tile_body<Epi, TILE, false>(t, none, rowptr, colidx, vals, plan, long_part, red_part, epi, long_cnt, prod,
+                                  red_lds, acc_lds, []() {});
+    }
   }
 }
 

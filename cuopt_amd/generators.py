@@ -174,6 +174,26 @@ def skewed_lp(m: int = 500_000, n: int = 1_000_000, zipf_a: float = 1.1, mean_ex
     )
 
 
+def banded_lp(m: int = 4_000_000, width: int = 16, seed: int = 0) -> LP:
+    """Equality LP with n = 2 m columns and a band: row i has `width` nonzeros at columns
+    (2 i + d) mod n for d in [-width/2, width/2), so every column has width/2 entries in
+    neighbouring rows and both products gather from nearby addresses.  b = A x0 for x0 in
+    [0, 1]^n, 0 <= x <= 10, random costs (feasible and bounded; no known optimum).  It
+    measures the step kernels where the gathered vector stays in cache."""
+    rng = np.random.default_rng(seed)
+    n = 2 * m
+    d = np.arange(-(width // 2), width - width // 2, dtype=np.int64)
+    cols = (2 * np.arange(m, dtype=np.int64)[:, None] + d[None, :]) % n
+    cols.sort(axis=1)  # rows at the ends wrap around
+    idx = cols.astype(np.int32).ravel()
+    val = rng.standard_normal(idx.size)
+    ptr = (np.arange(m + 1, dtype=np.int64) * width).astype(np.int32)
+    x0 = rng.uniform(0.0, 1.0, size=n)
+    b = (val.reshape(m, width) * x0[cols]).sum(axis=1)
+    return LP(m=m, n=n, row_offsets=ptr, col_indices=idx, values=val, c=rng.standard_normal(n),
+              con_lb=b, con_ub=b.copy(), var_lb=np.zeros(n), var_ub=np.full(n, 10.0), name=f"banded_{m}x{n}")
+
+
 def transportation_lp(sources: int = 20_000, sinks: int = 50_000, arcs: int = 50, seed: int = 0) -> LP:
     """Transportation LP: supply rows <=, demand rows =, arcs x >= 0 unbounded above."""
     rng = np.random.default_rng(seed)

@@ -50,3 +50,26 @@ def test_more_than_one_gpu_is_refused_with_a_message():
                        timeout=120, cwd=ROOT)
     assert r.returncode != 0
     assert "one GPU" in (r.stdout + r.stderr) or "1 GPU" in (r.stdout + r.stderr)
+
+
+def test_banded_lp_structure():
+    # The banded workload: row i holds columns (2 i + d) mod n, d in [-8, 8), sorted;
+    # every column has 8 entries; b = A x0 with 0 <= x0 <= 1 inside the bounds.
+    import numpy as np
+    import scipy.sparse as sp
+
+    from cuopt_amd import generators as G
+
+    lp = G.banded_lp(m=50, width=16, seed=3)
+    A = sp.csr_matrix((lp.values, lp.col_indices, lp.row_offsets), shape=(lp.m, lp.n))
+    assert lp.n == 100 and lp.nnz == 800 and A.has_sorted_indices
+    assert (np.diff(lp.row_offsets) == 16).all() and (np.diff(A.tocsc().indptr) == 8).all()
+    i = 20
+    assert sorted(lp.col_indices[16 * i:16 * i + 16].tolist()) == list(range(2 * i - 8, 2 * i + 8))
+    assert sorted(lp.col_indices[:16].tolist()) == sorted([(d % 100) for d in range(-8, 8)])
+    assert np.array_equal(lp.con_lb, lp.con_ub)
+    # some x in [0, 1]^n meets A x = b (the generator's x0), so the LP is feasible
+    from scipy.optimize import linprog
+
+    r = linprog(np.zeros(lp.n), A_eq=A, b_eq=lp.con_lb, bounds=(0, 1), method="highs")
+    assert r.status == 0
