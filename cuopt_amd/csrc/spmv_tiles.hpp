@@ -379,6 +379,18 @@ __device__ inline void tile_body(int t, const TileStream<Epi, TILE>& S, const in
   if constexpr (Epi::NRED > 0) block_sum_store<Epi::NRED>(red, red_part + (size_t)t * Epi::NRED, red_lds);
 }
 
+// How a workgroup of tiled_spmv_kernel<Epi> overlaps its tiles: with prefetch it
+// loads the next tile's stream while the current tile reduces its rows and runs
+// their epilogues (holding that stream costs 10-20 VGPRs); min_waves is the waves
+// per SIMD the kernel is built for.  FP64 kernels prefetch at their four waves;
+// FP32 kernels load at the top of each tile, except where an epilogue specialises
+// this (the FP32 primal step, in pdlp_kernels.hpp).
+template <class Epi>
+struct TilePipelining {
+  static constexpr bool prefetch = std::is_same<typename Epi::Mat, double>::value;
+  static constexpr int min_waves = 1;
+};
+
 // Workgroup b processes tiles b, b + gridDim.x, ... (the launch sizes the grid
 // to the resident workgroups, a multiple of 8, so every tile of workgroup b runs
 // on XCD b mod 8 as the plan placed it); in FP64 the next tile's stream is
@@ -387,7 +399,7 @@ __device__ inline void tile_body(int t, const TileStream<Epi, TILE>& S, const in
 // or a long-row chunk that wave 0 finishes alone, other waves may reach the next
 // tile first.
 template <class Epi, int TILE>
-__global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restrict__ rowptr,
+__global__ __launch_bounds__(kBlock, TilePipelining<Epi>::min_waves) void tiled_spmv_kernel(const int* __restrict__ rowptr,
                                                             const int* __restrict__ colidx,
                                                             const typename Epi::Mat* __restrict__ vals,
                                                             PlanDev plan,
@@ -401,10 +413,10 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
   __shared__ double red_lds[kBlock / kWave][kMaxRed];
   __shared__ uint64_t acc_lds[kBlock / kWave];
   if (epi.skip()) return;
-  if constexpr (std::is_same<typename Epi::Mat, double>::value) {
-    // FP64: the next tile's stream is loaded while this one reduces.  Measured at
-    // 500 k x 1 M: dual step 43.5 -> 41.3 us (tuned plan 42.2 -> 39.1 us), primal
-    // 33.9 -> 32.7 us, at the same occupancy.
+  if constexpr (TilePipelining<Epi>::prefetch) {
+    // The next tile's stream is loaded while this one reduces.  Measured at
+    // 500 k x 1 M in FP64: dual step 43.5 -> 41.3 us (tuned plan 42.2 -> 39.1 us),
+    // primal 33.9 -> 32.7 us, at the same occupancy.
     TileStream<Epi, TILE> cur, nxt;
     if ((int)blockIdx.x < plan.ntiles) cur.load(blockIdx.x, plan, colidx, vals);
     for (int t = blockIdx.x; t < plan.ntiles; t += gridDim.x) {
@@ -417,10 +429,11 @@ __global__ __launch_bounds__(kBlock) void tiled_spmv_kernel(const int* __restric
       cur = nxt;
     }
   } else {
-    // FP32: the stream is loaded at the top of each tile.  Holding the next
-    // tile's stream raised the step kernels from 78 to 89-99 VGPRs (six waves
-    // per SIMD to five or four), and they slowed: banded 4 M x 8 M dual step
-    // 167 -> 199 us, the tuned 500 k x 1 M dual step 32.4 -> 34 us.
+    // The stream is loaded at the top of each tile.  In FP32, holding the next
+    // tile's stream raised the step kernels from 78 to 89-99 VGPRs (six waves per
+    // SIMD to five or four), and the dual step slowed: banded 4 M x 8 M 167 ->
+    // 199 us, the tuned 500 k x 1 M plan 32.4 -> 34 us (built for six waves, with
+    // a few spilled registers, 36.7 us).
     const TileStream<Epi, TILE> none{};  // not read: each tile loads its own stream
     for (int t = blockIdx.x; t < plan.ntiles; t += gridDim.x) {
       if (t != (int)blockIdx.x) __syncthreads();
