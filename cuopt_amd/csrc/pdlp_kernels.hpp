@@ -138,14 +138,21 @@ struct PrimalStepEpi {
   }
 };
 
-// The FP32 primal step prefetches the next tile's stream and is built for six
-// waves per SIMD (80 VGPRs, 24-60 bytes of spills per lane): 500 k x 1 M primal
-// step 26.1-26.6 -> 25.5-25.7 us, banded 4 M x 8 M 194 -> 181 us.  Without the
-// six-wave bound it ran at five waves and the banded step slowed to 221 us.
-template <bool MAJOR, bool FPE>
-struct TilePipelining<PrimalStepEpi<float, MAJOR, FPE>> {
-  static constexpr bool prefetch = true;
-  static constexpr int min_waves = 6;
+// FP32 step kernels (measured at 500 k x 1 M, HIP events, one box):
+//  - up to 1 024 slots per tile the next tile's stream goes through LDS, built
+//    for six waves per SIMD (80 VGPRs, no spills): primal step 25.0 -> 24.5 us,
+//    dual step 35.7 -> 34.2 us;
+//  - 2 048-slot tiles would drop to four waves by LDS with it: there the primal
+//    step holds the next stream in registers, built for six waves (80 VGPRs, 60
+//    bytes per lane spilled; 26.1-26.6 -> 25.1-25.4 us, banded 4 M x 8 M 194 ->
+//    181 us), and the dual step loads each tile's stream at the top of the tile
+//    (its best plan, 2 048 slots with in-launch merge: 32.2 us, against 34.3 with
+//    the LDS stream at four waves).
+template <bool MAJOR, bool FPE, int TILE>
+struct TilePipelining<PrimalStepEpi<float, MAJOR, FPE>, TILE> {
+  static constexpr bool prefetch   = TILE > 1024;
+  static constexpr bool lds_stream = TILE <= 1024;
+  static constexpr int min_waves   = 6;
 };
 
 // ---- dual step over rows of A~ (rows i): A x-bar, projection, reflection, Halpern.
@@ -201,6 +208,13 @@ struct DualStepEpi {
     }
     ynext[r] = w * yb + (V(1) - w) * p.y0;
   }
+};
+
+template <bool MAJOR, bool FPE, int TILE>
+struct TilePipelining<DualStepEpi<float, MAJOR, FPE>, TILE> {
+  static constexpr bool prefetch   = false;
+  static constexpr bool lds_stream = TILE <= 1024;
+  static constexpr int min_waves   = TILE <= 1024 ? 6 : 1;
 };
 
 // ---- fixed-point error: A~^T y-bar against the stored A~^T y, over columns j.

@@ -247,23 +247,58 @@ struct TileStream {
   }
 };
 
+// FP32 stream through LDS: every lane's 16-byte vectors of values and indices
+// are loaded straight into LDS (global_load_lds_dwordx4: lane l of a wavefront
+// writes its 16 bytes at the wavefront's base + 16 l), in the layout of the
+// register stream, so that a workgroup can have the next tile's stream in
+// flight without holding it in registers.  The buffer is free once the tile's
+// products are staged.
+template <int TILE>
+struct LdsStream {
+  static constexpr int kIt    = (TILE + kBlock * 4 - 1) / (kBlock * 4);
+  static constexpr int kElems = kIt * kBlock * 4;  // floats (and ints) per buffer
+
+  __device__ static void issue(const Tile& tl, const int* __restrict__ colidx, const float* __restrict__ vals,
+                               float* vbuf, int* ibuf)
+  {
+    const int base = tl.z0 & ~3;
+    const int w    = (int)threadIdx.x / kWave;
+#pragma unroll
+    for (int it = 0; it < kIt; ++it) {
+      const int e0  = base + (it * kBlock + (int)threadIdx.x) * 4;
+      const int off = (it * kBlock + w * kWave) * 4;  // this wavefront's slot
+      if (e0 < tl.z1) {
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(vals + e0),
+                                         (__attribute__((address_space(3))) void*)(vbuf + off), 16, 0, 0);
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(colidx + e0),
+                                         (__attribute__((address_space(3))) void*)(ibuf + off), 16, 0, 0);
+      }
+    }
+  }
+};
+
 // One tile.  With PRELOADED its stream S was loaded by the caller (it may still be
 // in flight); otherwise the tile loads its own stream after the row pointers and
 // epilogue operands below, and S is not read.  prefetch() is called once by every
 // lane right after the tile's products are staged in LDS: the kernel issues the
 // next tile's stream there, so that it is in flight while this tile reduces its
 // rows and runs their epilogues.
-template <class Epi, int TILE, bool PRELOADED, class Prefetch>
-__device__ inline void tile_body(int t, const TileStream<Epi, TILE>& S, const int* __restrict__ rowptr,
-                                 const int* __restrict__ colidx, const typename Epi::Mat* __restrict__ vals,
-                                 const PlanDev& plan, uint64_t* __restrict__ long_part,
-                                 double* __restrict__ red_part, const Epi& epi, int* __restrict__ long_cnt,
-                                 typename Epi::Acc::Elem* prod, double (*red_lds)[kMaxRed], uint64_t* acc_lds,
-                                 Prefetch&& prefetch)
+// MODE: kStreamOwn loads the tile's stream into registers here; kStreamRegs takes
+// it from S (loaded by the caller, maybe still in flight); kStreamLds reads it
+// from the LDS buffers sv / si, filled by LdsStream::issue.
+constexpr int kStreamOwn = 0, kStreamRegs = 1, kStreamLds = 2;
+
+template <class Epi, int TILE, int MODE, class Prefetch>
+__device__ inline void tile_body(int t, const TileStream<Epi, TILE>& S, const float* sv, const int* si,
+                                 const int* __restrict__ rowptr, const int* __restrict__ colidx,
+                                 const typename Epi::Mat* __restrict__ vals, const PlanDev& plan,
+                                 uint64_t* __restrict__ long_part, double* __restrict__ red_part, const Epi& epi,
+                                 int* __restrict__ long_cnt, typename Epi::Acc::Elem* prod,
+                                 double (*red_lds)[kMaxRed], uint64_t* acc_lds, Prefetch&& prefetch)
 {
   using Acc  = typename Epi::Acc;
-  const Tile tl  = PRELOADED ? S.tl : plan.tiles[t];
-  const int mt   = PRELOADED ? S.mt : plan.meta[t];
+  const Tile tl  = MODE == kStreamRegs ? S.tl : plan.tiles[t];
+  const int mt   = MODE == kStreamRegs ? S.mt : plan.meta[t];
   const int tid  = threadIdx.x;
   const int nnz  = tl.z1 - tl.z0;
 
@@ -307,8 +342,20 @@ __device__ inline void tile_body(int t, const TileStream<Epi, TILE>& S, const in
         }
       }
     };
-    if constexpr (PRELOADED) {
+    if constexpr (MODE == kStreamRegs) {
       stage(S);
+    } else if constexpr (MODE == kStreamLds) {
+      // This lane's vectors, where its own global_load_lds put them (the wait
+      // covers the wavefront's loads into LDS).
+      __builtin_amdgcn_s_waitcnt(0);
+      TileStream<Epi, TILE> X;
+#pragma unroll
+      for (int it = 0; it < kIt; ++it) {
+        const int o = (it * kBlock + tid) * 4;
+        X.v[it]     = *reinterpret_cast<const typename TileStream<Epi, TILE>::VT::V*>(sv + o);
+        X.c[it]     = *reinterpret_cast<const typename TileStream<Epi, TILE>::VT::I*>(si + o);
+      }
+      stage(X);
     } else {
       TileStream<Epi, TILE> X;
       X.tl = tl;
@@ -379,16 +426,19 @@ __device__ inline void tile_body(int t, const TileStream<Epi, TILE>& S, const in
   if constexpr (Epi::NRED > 0) block_sum_store<Epi::NRED>(red, red_part + (size_t)t * Epi::NRED, red_lds);
 }
 
-// How a workgroup of tiled_spmv_kernel<Epi> overlaps its tiles: with prefetch it
-// loads the next tile's stream while the current tile reduces its rows and runs
-// their epilogues (holding that stream costs 10-20 VGPRs); min_waves is the waves
-// per SIMD the kernel is built for.  FP64 kernels prefetch at their four waves;
-// FP32 kernels load at the top of each tile, except where an epilogue specialises
-// this (the FP32 primal step, in pdlp_kernels.hpp).
-template <class Epi>
+// How a workgroup of tiled_spmv_kernel<Epi, TILE> overlaps its tiles: with
+// prefetch it loads the next tile's stream into registers while the current tile
+// reduces its rows and runs their epilogues (holding that stream costs 10-20
+// VGPRs); with lds_stream it loads it into LDS instead (FP32 only: no registers,
+// but the tile's LDS grows by 8 bytes per slot); min_waves is the waves per SIMD
+// the kernel is built for.  FP64 kernels prefetch into registers at their four
+// waves; FP32 kernels load at the top of each tile, except where an epilogue
+// specialises this (the FP32 step kernels, in pdlp_kernels.hpp).
+template <class Epi, int TILE>
 struct TilePipelining {
-  static constexpr bool prefetch = std::is_same<typename Epi::Mat, double>::value;
-  static constexpr int min_waves = 1;
+  static constexpr bool prefetch   = std::is_same<typename Epi::Mat, double>::value;
+  static constexpr bool lds_stream = false;
+  static constexpr int min_waves   = 1;
 };
 
 // Workgroup b processes tiles b, b + gridDim.x, ... (the launch sizes the grid
@@ -399,7 +449,7 @@ struct TilePipelining {
 // or a long-row chunk that wave 0 finishes alone, other waves may reach the next
 // tile first.
 template <class Epi, int TILE>
-__global__ __launch_bounds__(kBlock, TilePipelining<Epi>::min_waves) void tiled_spmv_kernel(const int* __restrict__ rowptr,
+__global__ __launch_bounds__(kBlock, (TilePipelining<Epi, TILE>::min_waves)) void tiled_spmv_kernel(const int* __restrict__ rowptr,
                                                             const int* __restrict__ colidx,
                                                             const typename Epi::Mat* __restrict__ vals,
                                                             PlanDev plan,
@@ -409,11 +459,29 @@ __global__ __launch_bounds__(kBlock, TilePipelining<Epi>::min_waves) void tiled_
                                                             int* __restrict__ long_cnt)
 {
   using Elem = typename Epi::Acc::Elem;
+  constexpr bool kLds  = TilePipelining<Epi, TILE>::lds_stream;
+  constexpr int kLdsN  = kLds ? LdsStream<TILE>::kElems : 1;
   __shared__ Elem prod[TILE];
   __shared__ double red_lds[kBlock / kWave][kMaxRed];
   __shared__ uint64_t acc_lds[kBlock / kWave];
+  __shared__ __attribute__((aligned(16))) float sv[kLdsN];
+  __shared__ __attribute__((aligned(16))) int si[kLdsN];
   if (epi.skip()) return;
-  if constexpr (TilePipelining<Epi>::prefetch) {
+  if constexpr (kLds) {
+    // The next tile's stream goes into the LDS buffers as soon as this tile's
+    // products are staged, and is in flight while this tile reduces its rows.
+    static_assert(std::is_same<typename Epi::Mat, float>::value, "the LDS stream carries FP32 values");
+    const TileStream<Epi, TILE> none{};  // not read
+    if ((int)blockIdx.x < plan.ntiles) LdsStream<TILE>::issue(plan.tiles[blockIdx.x], colidx, vals, sv, si);
+    for (int t = blockIdx.x; t < plan.ntiles; t += gridDim.x) {
+      if (t != (int)blockIdx.x) __syncthreads();
+      const int tn = t + gridDim.x;
+      tile_body<Epi, TILE, kStreamLds>(t, none, sv, si, rowptr, colidx, vals, plan, long_part, red_part, epi,
+                                       long_cnt, prod, red_lds, acc_lds, [&]() {
+                                         if (tn < plan.ntiles) LdsStream<TILE>::issue(plan.tiles[tn], colidx, vals, sv, si);
+                                       });
+    }
+  } else if constexpr (TilePipelining<Epi, TILE>::prefetch) {
     // The next tile's stream is loaded while this one reduces.  Measured at
     // 500 k x 1 M in FP64: dual step 43.5 -> 41.3 us (tuned plan 42.2 -> 39.1 us),
     // primal 33.9 -> 32.7 us, at the same occupancy.
@@ -422,8 +490,8 @@ __global__ __launch_bounds__(kBlock, TilePipelining<Epi>::min_waves) void tiled_
     for (int t = blockIdx.x; t < plan.ntiles; t += gridDim.x) {
       if (t != (int)blockIdx.x) __syncthreads();
       const int tn = t + gridDim.x;
-      tile_body<Epi, TILE, true>(t, cur, rowptr, colidx, vals, plan, long_part, red_part, epi, long_cnt, prod,
-                                 red_lds, acc_lds, [&]() {
+      tile_body<Epi, TILE, kStreamRegs>(t, cur, nullptr, nullptr, rowptr, colidx, vals, plan, long_part, red_part,
+                                        epi, long_cnt, prod, red_lds, acc_lds, [&]() {
                                    if (tn < plan.ntiles) nxt.load(tn, plan, colidx, vals);
                                  });
       cur = nxt;
@@ -437,8 +505,8 @@ __global__ __launch_bounds__(kBlock, TilePipelining<Epi>::min_waves) void tiled_
     const TileStream<Epi, TILE> none{};  // not read: each tile loads its own stream
     for (int t = blockIdx.x; t < plan.ntiles; t += gridDim.x) {
       if (t != (int)blockIdx.x) __syncthreads();
-      tile_body<Epi, TILE, false>(t, none, rowptr, colidx, vals, plan, long_part, red_part, epi, long_cnt, prod,
-                                  red_lds, acc_lds, []() {});
+      tile_body<Epi, TILE, kStreamOwn>(t, none, nullptr, nullptr, rowptr, colidx, vals, plan, long_part, red_part,
+                                       epi, long_cnt, prod, red_lds, acc_lds, []() {});
     }
   }
 }
