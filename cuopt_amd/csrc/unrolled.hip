@@ -15,7 +15,8 @@
 // clamp states of x+ and y+ (transposed SpMV plus projection masks).  States
 // are checkpointed every `every` steps during the forward pass; each segment is
 // recomputed from its checkpoint before its reverse sweep.  The c and (l_c, u_c)
-// gradients are accumulated in FP64.
+// gradients are accumulated in FP64 (in FP32 runs per segment in FP32, then
+// added into the FP64 totals).
 //
 // Layout: batch-interleaved vectors, v[i * B + b].  SpMM kernels give a
 // wavefront 1, 2 or 4 consecutive matrix rows and one lane to each batch member,
@@ -270,21 +271,46 @@ __device__ inline V dual_adjoint(V ay, V yplus, V l, V u, V sigma, double& dl, d
   return sigma * ay;
 }
 
+// Per-step accumulators of the c and (l_c, u_c) gradients: FP64 in FP64 runs; in
+// FP32 runs an FP32 partial per checkpoint segment (at most `every` terms), which
+// unr_flush adds into the FP64 totals at the segment's end.  The adjoint kernels
+// read and write their accumulators every step, and in FP64 those were about
+// half of their traffic.
+template <typename V>
+struct GradAcc {
+  using type = double;
+};
+template <>
+struct GradAcc<float> {
+  using type = float;
+};
+
+// acc += part; part = 0 (a kernel, not a memset, so that segment graphs may hold it).
+__global__ __launch_bounds__(kUB) void unr_flush(size_t n, float* __restrict__ part, double* __restrict__ acc)
+{
+  for (size_t i = blockIdx.x * (size_t)kUB + threadIdx.x; i < n; i += (size_t)gridDim.x * kUB) {
+    acc[i] += (double)part[i];
+    part[i] = 0.f;
+  }
+}
+
 // Start of the reverse sweep: gt for the last step from the upstream y-gradient.
 template <typename V>
 __global__ __launch_bounds__(kUB) void unr_adj_start(int m, int B, const V* ay,  // may alias gt
                                                      const V* __restrict__ yplus, const V* __restrict__ lc,
                                                      const V* __restrict__ uc, int ldb, V sigma, V* gt,
-                                                     double* __restrict__ glc, double* __restrict__ guc)
+                                                     typename GradAcc<V>::type* __restrict__ glc,
+                                                     typename GradAcc<V>::type* __restrict__ guc)
 {
+  using GA = typename GradAcc<V>::type;
   const size_t tot = (size_t)m * B;
   for (size_t o = blockIdx.x * (size_t)kUB + threadIdx.x; o < tot; o += (size_t)gridDim.x * kUB) {
     const int i = (int)(o / B), b = (int)(o % B);
     const size_t q = ldb ? (size_t)i * ldb + b : (size_t)i;  // per-batch bounds, or one per row
     double dl, du;
     gt[o] = dual_adjoint(ay ? ay[o] : V(0), yplus[o], lc[q], uc[q], sigma, dl, du);
-    glc[o] += dl;
-    guc[o] += du;
+    glc[o] += (GA)dl;
+    guc[o] += (GA)du;
   }
 }
 
@@ -296,15 +322,16 @@ __global__ __launch_bounds__(kUB) void unr_adj_primal(const int* __restrict__ tp
                                                       const V* __restrict__ gt, const V* __restrict__ xplus,
                                                       const V* __restrict__ lb, const V* __restrict__ ub, V inv_prox,
                                                       V* __restrict__ ax, V* __restrict__ gw,
-                                                      double* __restrict__ gc_acc)
+                                                      typename GradAcc<V>::type* __restrict__ gc_acc)
 {
+  using GA = typename GradAcc<V>::type;
   int j0, nr, b;
   bool bok;
   if (!rows_lane<R>(n, B, j0, nr, b, bok)) return;
   // Element operands first (gt, which the rows gather, is none of them).
   V axo[R];
   bool in[R];
-  double gc[R];
+  GA gc[R];
 #pragma unroll
   for (int i = 0; i < R; ++i) {
     const int j    = j0 + min(i, nr - 1);
@@ -325,7 +352,7 @@ __global__ __launch_bounds__(kUB) void unr_adj_primal(const int* __restrict__ tp
       const V w      = in[i] ? (axo[i] + V(2) * gxb) * inv_prox : V(0);
       gw[o]          = w;
       ax[o]          = w - gxb;
-      gc_acc[o]      = gc[i] + (double)w;
+      gc_acc[o]      = gc[i] + (GA)w;
     }
   }
 }
@@ -339,14 +366,16 @@ __global__ __launch_bounds__(kUB) void unr_adj_dual(const int* __restrict__ aptr
                                                     const V* __restrict__ gw, V tau, V sigma,
                                                     const V* __restrict__ yprev, const V* __restrict__ lc,
                                                     const V* __restrict__ uc, int ldb, V* __restrict__ gt,
-                                                    double* __restrict__ glc, double* __restrict__ guc)
+                                                    typename GradAcc<V>::type* __restrict__ glc,
+                                                    typename GradAcc<V>::type* __restrict__ guc)
 {
+  using GA = typename GradAcc<V>::type;
   int i0, nr, b;
   bool bok;
   if (!rows_lane<R>(m, B, i0, nr, b, bok)) return;
   // Element operands first (gw, which the rows gather, is none of them).
   V gto[R], yp[R], lo[R], hi[R];
-  double glo[R], guo[R];
+  GA glo[R], guo[R];
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const int i    = i0 + min(k, nr - 1);
@@ -369,8 +398,8 @@ __global__ __launch_bounds__(kUB) void unr_adj_dual(const int* __restrict__ aptr
       const V ay     = gto[k] / sigma + tau * s[k].value();  // adjoint of y before this step
       double dl, du;
       gt[o]  = dual_adjoint(ay, yp[k], lo[k], hi[k], sigma, dl, du);
-      glc[o] = glo[k] + dl;
-      guc[o] = guo[k] + du;
+      glc[o] = glo[k] + (GA)dl;
+      guc[o] = guo[k] + (GA)du;
     }
   }
 }
@@ -571,6 +600,9 @@ template <typename V>
 struct Work {
   V *c, *lc, *uc, *x[2], *y[2], *xb, *ck_x, *ck_y, *sx, *sy, *ax, *gt, *gw;
   double *gc, *glc, *guc;
+  // What the adjoint kernels accumulate into: FP32 segment partials in FP32 runs,
+  // gc / glc / guc themselves in FP64 runs.
+  typename GradAcc<V>::type *pc, *plc, *puc;
 };
 
 template <typename V>
@@ -590,12 +622,20 @@ Work<V> carve(UnrolledImpl& U, int nseg, bool allocate)
   const size_t o_sx = take((size_t)(U.every + 1) * nB * sizeof(V)), o_sy = take((size_t)(U.every + 1) * mB * sizeof(V));
   const size_t o_ax = take(nB * sizeof(V)), o_gt = take(mB * sizeof(V)), o_gw = take(nB * sizeof(V));
   const size_t o_gc = take(nB * sizeof(double)), o_glc = take(mB * sizeof(double)), o_guc = take(mB * sizeof(double));
+  constexpr bool kPart = std::is_same<V, float>::value;
+  const size_t o_pc = take(kPart ? nB * sizeof(float) : 0), o_plc = take(kPart ? mB * sizeof(float) : 0),
+               o_puc = take(kPart ? mB * sizeof(float) : 0);
   if (allocate && U.work.size() < off) U.work.resize(off);  // kept across calls: no free / malloc per forward
   unsigned char* p = U.work.get();
   auto at = [&](size_t o) { return reinterpret_cast<V*>(p + o); };
   auto atd = [&](size_t o) { return reinterpret_cast<double*>(p + o); };
-  return Work<V>{at(o_c), at(o_lc), at(o_uc), {at(o_x0), at(o_x1)}, {at(o_y0), at(o_y1)}, at(o_xb), at(o_ckx),
-                 at(o_cky), at(o_sx), at(o_sy), at(o_ax), at(o_gt), at(o_gw), atd(o_gc), atd(o_glc), atd(o_guc)};
+  using GA  = typename GradAcc<V>::type;
+  auto atg = [&](size_t o, size_t whole) { return reinterpret_cast<GA*>(p + (kPart ? o : whole)); };
+  return Work<V>{at(o_c),   at(o_lc),  at(o_uc),  {at(o_x0), at(o_x1)},
+                 {at(o_y0), at(o_y1)}, at(o_xb), at(o_ckx), at(o_cky),
+                 at(o_sx),  at(o_sy),  at(o_ax),  at(o_gt),
+                 at(o_gw),  atd(o_gc), atd(o_glc), atd(o_guc),
+                 atg(o_pc, o_gc), atg(o_plc, o_glc), atg(o_puc, o_guc)};
 }
 
 // One PDHG step (x, y) -> (xn, yn) through the scratch xb.
@@ -678,6 +718,12 @@ void backward_impl(UnrolledImpl& U, const double* gx, const double* gy, double* 
   hipLaunchKernelGGL(unr_zero64, dim3(ew_blocks(nB)), dim3(kUB), 0, s, nB, W.gc);
   hipLaunchKernelGGL(unr_zero64, dim3(ew_blocks(mB)), dim3(kUB), 0, s, mB, W.glc);
   hipLaunchKernelGGL(unr_zero64, dim3(ew_blocks(mB)), dim3(kUB), 0, s, mB, W.guc);
+  constexpr bool kPart = std::is_same<V, float>::value;
+  if constexpr (kPart) {
+    hipLaunchKernelGGL(unr_zero<float>, dim3(ew_blocks(nB)), dim3(kUB), 0, s, nB, W.pc);
+    hipLaunchKernelGGL(unr_zero<float>, dim3(ew_blocks(mB)), dim3(kUB), 0, s, mB, W.plc);
+    hipLaunchKernelGGL(unr_zero<float>, dim3(ew_blocks(mB)), dim3(kUB), 0, s, mB, W.puc);
+  }
   hipLaunchKernelGGL(unr_convert_in<V>, dim3(ew_blocks(nB)), dim3(kUB), 0, s, nB, gx, W.ax);
   // y-gradient: start value of ay (zero when absent), staged in gw's slot of size m*B? use gt as ay first.
   if (gy) hipLaunchKernelGGL(unr_convert_in<V>, dim3(ew_blocks(mB)), dim3(kUB), 0, s, mB, gy, W.gt);
@@ -702,19 +748,24 @@ void backward_impl(UnrolledImpl& U, const double* gx, const double* gy, double* 
       if (last && t == L - 1)
         hipLaunchKernelGGL(unr_adj_start<V>, dim3(ew_blocks(mB)), dim3(kUB), 0, q, U.m, U.B,
                            gy ? (const V*)W.gt : (const V*)nullptr, yplus, (const V*)W.lc, (const V*)W.uc, U.ldb,
-                           sigma, W.gt, W.glc, W.guc);
+                           sigma, W.gt, W.plc, W.puc);
       with_rows(U.rpw_t, [&](auto RC) {
         constexpr int R = decltype(RC)::value;
         hipLaunchKernelGGL((unr_adj_primal<V, R>), rows_grid(U.n, U.B, R), dim3(kUB), 0, q, M.tptr, M.tidx, M.tval,
-                           U.n, U.B, (const V*)W.gt, xplus, M.lb, M.ub, inv_prox, W.ax, W.gw, W.gc);
+                           U.n, U.B, (const V*)W.gt, xplus, M.lb, M.ub, inv_prox, W.ax, W.gw, W.pc);
       });
       if (!first || t > 0)
         with_rows(U.rpw_a, [&](auto RC) {
           constexpr int R = decltype(RC)::value;
           hipLaunchKernelGGL((unr_adj_dual<V, R>), rows_grid(U.m, U.B, R), dim3(kUB), 0, q, M.aptr, M.aidx, M.aval,
                              U.m, U.B, (const V*)W.gw, tau, sigma, (const V*)(W.sy + (size_t)t * mB),
-                             (const V*)W.lc, (const V*)W.uc, U.ldb, W.gt, W.glc, W.guc);
+                             (const V*)W.lc, (const V*)W.uc, U.ldb, W.gt, W.plc, W.puc);
         });
+    }
+    if constexpr (kPart) {  // the segment's FP32 partials into the FP64 totals
+      hipLaunchKernelGGL(unr_flush, dim3(ew_blocks(nB)), dim3(kUB), 0, q, nB, W.pc, W.gc);
+      hipLaunchKernelGGL(unr_flush, dim3(ew_blocks(mB)), dim3(kUB), 0, q, mB, W.plc, W.glc);
+      hipLaunchKernelGGL(unr_flush, dim3(ew_blocks(mB)), dim3(kUB), 0, q, mB, W.puc, W.guc);
     }
     hipLaunchKernelGGL(unr_seg_update, dim3(1), dim3(kWave), 0, q, U.seg.get(), -1, 0);
   };
