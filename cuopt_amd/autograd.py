@@ -10,7 +10,8 @@ constraint matrix and variable bounds; calling it with a batch of objectives
 
 on the GPU (cuOptAmdUnrolledForward), and its backward is the hand-written
 adjoint (cuOptAmdUnrolledBackward): transposed SpMV plus projection masks,
-checkpointed every ``checkpoint_every`` steps, gradients accumulated in FP64.
+checkpointed every ``checkpoint_every`` steps, gradients accumulated in FP64
+(in FP32 mode each checkpoint segment is summed in FP32 and added into FP64).
 An equality-constrained problem passes the same tensor ``b`` as ``lc`` and
 ``uc``; autograd then adds both gradients into ``b``.
 

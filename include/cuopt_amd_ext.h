@@ -282,7 +282,8 @@ cuopt_int_t cuOptAmdUnrolledForward(cuOptAmdUnrolled handle,
                                     cuopt_float_t* y_out,
                                     void* stream);
 /* VJP of the last forward call: gx (n x batch), gy (m x batch or NULL) ->
- * gc (n x batch), glc and guc (m x batch), accumulated in FP64. */
+ * gc (n x batch), glc and guc (m x batch), accumulated in FP64 (FP32 handles sum
+ * each checkpoint segment in FP32 and add it into the FP64 totals). */
 cuopt_int_t cuOptAmdUnrolledBackward(cuOptAmdUnrolled handle,
                                      const cuopt_float_t* gx,
                                      const cuopt_float_t* gy,
