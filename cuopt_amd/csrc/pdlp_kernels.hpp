@@ -153,6 +153,7 @@ struct TilePipelining<PrimalStepEpi<float, MAJOR, FPE>, TILE> {
   static constexpr bool prefetch   = TILE > 1024;
   static constexpr bool lds_stream = TILE <= 1024;
   static constexpr int min_waves   = 6;
+  static constexpr int pre_rows    = kPreRows;
 };
 
 // ---- dual step over rows of A~ (rows i): A x-bar, projection, reflection, Halpern.
@@ -210,11 +211,15 @@ struct DualStepEpi {
   }
 };
 
+// The FP32 dual step preloads four rows per lane group: with two (the default,
+// which took the 500 k x 1 M primal steps 23.7-24.3 -> 22.8 us in FP32 and
+// 32.7 -> 31.0 us in FP64) its tuned plan measured 32.5-32.7 against 32.2 us.
 template <bool MAJOR, bool FPE, int TILE>
 struct TilePipelining<DualStepEpi<float, MAJOR, FPE>, TILE> {
   static constexpr bool prefetch   = false;
   static constexpr bool lds_stream = TILE <= 1024;
   static constexpr int min_waves   = TILE <= 1024 ? 6 : 1;
+  static constexpr int pre_rows    = 4;
 };
 
 // ---- fixed-point error: A~^T y-bar against the stored A~^T y, over columns j.

@@ -31,8 +31,9 @@ constexpr int kBlock   = 256;
 constexpr int kWave    = 64;
 constexpr int kMaxRowsPerTile = 1024;
 constexpr int kMaxRed  = 8;
-// Rows per lane group whose operands are loaded before the matrix stream.
-constexpr int kPreRows = 4;
+// Rows per lane group whose operands are loaded before the matrix stream, unless
+// TilePipelining<Epi, TILE>::pre_rows says otherwise.
+constexpr int kPreRows = 2;
 // LDS product capacity of a tile (template parameter TILE of the kernel).  The
 // planner caps a tile at TILE - kVecPad nonzeros so that the aligned vector
 // loads covering [z0, z1) never need more than TILE slots.
@@ -213,6 +214,23 @@ __device__ inline void merge_long_row(const PlanDev& plan, int lid, uint64_t* lo
 // 256-byte line of its own, so the atomics of different rows do not queue on one line.
 constexpr int kCntStride = 64;
 
+// How a workgroup of tiled_spmv_kernel<Epi, TILE> overlaps its tiles: with
+// prefetch it loads the next tile's stream into registers while the current tile
+// reduces its rows and runs their epilogues (holding that stream costs 10-20
+// VGPRs); with lds_stream it loads it into LDS instead (FP32 only: no registers,
+// but the tile's LDS grows by 8 bytes per slot); min_waves is the waves per SIMD
+// the kernel is built for; pre_rows the rows per lane group whose row pointers and
+// epilogue operands are loaded before the stream.  FP64 kernels prefetch into registers at their four
+// waves; FP32 kernels load at the top of each tile, except where an epilogue
+// specialises this (the FP32 step kernels, in pdlp_kernels.hpp).
+template <class Epi, int TILE>
+struct TilePipelining {
+  static constexpr bool prefetch   = std::is_same<typename Epi::Mat, double>::value;
+  static constexpr bool lds_stream = false;
+  static constexpr int min_waves   = 1;
+  static constexpr int pre_rows    = kPreRows;
+};
+
 // A tile's matrix stream in registers: each lane's 16-byte vectors of values and
 // indices from the aligned base below z0 (elements outside [z0, z1) are loaded
 // but dropped), with the tile's header.
@@ -305,18 +323,19 @@ __device__ inline void tile_body(int t, const TileStream<Epi, TILE>& S, const fl
   const int tid  = threadIdx.x;
   const int nnz  = tl.z1 - tl.z0;
 
-  // Row-run tiles: load each lane's first kPreRows row pointers and epilogue
+  // Row-run tiles: load each lane's first kPre row pointers and epilogue
   // operands now, so their latency overlaps the matrix stream below instead of
   // being paid once per row after the barrier.
   const int G    = mt >= 0 ? 1 << mt : 1;
   const int sub  = tid & (G - 1);
   const int grp  = mt >= 0 ? tid >> mt : 0;
   const int ngrp = mt >= 0 ? kBlock >> mt : 0;
-  int rb[kPreRows], re[kPreRows];
-  typename Epi::Pre pre[kPreRows];
+  constexpr int kPre = TilePipelining<Epi, TILE>::pre_rows;
+  int rb[kPre], re[kPre];
+  typename Epi::Pre pre[kPre];
   if (mt >= 0) {
 #pragma unroll
-    for (int i = 0; i < kPreRows; ++i) {
+    for (int i = 0; i < kPre; ++i) {
       const int r = tl.r0 + grp + i * ngrp;
       if (r < tl.r1) {
         rb[i]  = rowptr[r] - tl.z0;
@@ -375,7 +394,7 @@ __device__ inline void tile_body(int t, const TileStream<Epi, TILE>& S, const fl
 
   if (mt >= 0) {
 #pragma unroll
-    for (int i = 0; i < kPreRows; ++i) {
+    for (int i = 0; i < kPre; ++i) {
       const int r = tl.r0 + grp + i * ngrp;
       if (r < tl.r1) {
         Acc a = Acc::zero();
@@ -384,8 +403,8 @@ __device__ inline void tile_body(int t, const TileStream<Epi, TILE>& S, const fl
         if (sub == 0) epi.row(r, a, pre[i], red);
       }
     }
-    // Tiles with more rows than kPreRows per lane group: the rest load as they go.
-    for (int r = tl.r0 + grp + kPreRows * ngrp; r < tl.r1; r += ngrp) {
+    // Tiles with more rows than kPre per lane group: the rest load as they go.
+    for (int r = tl.r0 + grp + kPre * ngrp; r < tl.r1; r += ngrp) {
       const int b = rowptr[r] - tl.z0, e = rowptr[r + 1] - tl.z0;
       Acc a = Acc::zero();
       for (int k = b + sub; k < e; k += G) a.add(prod[k]);
@@ -428,21 +447,6 @@ __device__ inline void tile_body(int t, const TileStream<Epi, TILE>& S, const fl
   }
   if constexpr (Epi::NRED > 0) block_sum_store<Epi::NRED>(red, red_part + (size_t)t * Epi::NRED, red_lds);
 }
-
-// How a workgroup of tiled_spmv_kernel<Epi, TILE> overlaps its tiles: with
-// prefetch it loads the next tile's stream into registers while the current tile
-// reduces its rows and runs their epilogues (holding that stream costs 10-20
-// VGPRs); with lds_stream it loads it into LDS instead (FP32 only: no registers,
-// but the tile's LDS grows by 8 bytes per slot); min_waves is the waves per SIMD
-// the kernel is built for.  FP64 kernels prefetch into registers at their four
-// waves; FP32 kernels load at the top of each tile, except where an epilogue
-// specialises this (the FP32 step kernels, in pdlp_kernels.hpp).
-template <class Epi, int TILE>
-struct TilePipelining {
-  static constexpr bool prefetch   = std::is_same<typename Epi::Mat, double>::value;
-  static constexpr bool lds_stream = false;
-  static constexpr int min_waves   = 1;
-};
 
 // Workgroup b processes tiles b, b + gridDim.x, ... (the launch sizes the grid
 // to the resident workgroups, a multiple of 8, so every tile of workgroup b runs
