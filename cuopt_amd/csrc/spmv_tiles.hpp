@@ -219,10 +219,11 @@ constexpr int kCntStride = 64;
 // reduces its rows and runs their epilogues (holding that stream costs 10-20
 // VGPRs); with lds_stream it loads it into LDS instead (FP32 only: no registers,
 // but the tile's LDS grows by 8 bytes per slot); min_waves is the waves per SIMD
-// the kernel is built for; pre_rows the rows per lane group whose row pointers and
-// epilogue operands are loaded before the stream.  FP64 kernels prefetch into registers at their four
-// waves; FP32 kernels load at the top of each tile, except where an epilogue
-// specialises this (the FP32 step kernels, in pdlp_kernels.hpp).
+// the kernel is built for; pre_rows the rows per lane group whose row pointers
+// and epilogue operands are loaded before the stream.  FP64 kernels prefetch into
+// registers at their four waves; FP32 kernels load at the top of each tile,
+// except where an epilogue specialises this (the FP32 step kernels, in
+// pdlp_kernels.hpp).
 template <class Epi, int TILE>
 struct TilePipelining {
   static constexpr bool prefetch   = std::is_same<typename Epi::Mat, double>::value;
@@ -298,15 +299,14 @@ struct LdsStream {
   }
 };
 
-// One tile.  With PRELOADED its stream S was loaded by the caller (it may still be
-// in flight); otherwise the tile loads its own stream after the row pointers and
-// epilogue operands below, and S is not read.  prefetch() is called once by every
+// One tile.  MODE says where its stream comes from: kStreamOwn loads it into
+// registers here, after the row pointers and epilogue operands below; kStreamRegs
+// takes it from S (loaded by the caller, maybe still in flight); kStreamLds reads
+// it from the LDS buffers sv / si, filled by LdsStream::issue.  S is read only in
+// kStreamRegs, sv / si only in kStreamLds.  prefetch() is called once by every
 // lane right after the tile's products are staged in LDS: the kernel issues the
 // next tile's stream there, so that it is in flight while this tile reduces its
 // rows and runs their epilogues.
-// MODE: kStreamOwn loads the tile's stream into registers here; kStreamRegs takes
-// it from S (loaded by the caller, maybe still in flight); kStreamLds reads it
-// from the LDS buffers sv / si, filled by LdsStream::issue.
 constexpr int kStreamOwn = 0, kStreamRegs = 1, kStreamLds = 2;
 
 template <class Epi, int TILE, int MODE, class Prefetch>
