@@ -161,6 +161,61 @@ __global__ __launch_bounds__(1024) void power_finalize_res(const double* __restr
   }
 }
 
+// Problems of at most kPowerSmallRows rows end a power step in one workgroup:
+// s2 from the A-side partials (in power_finalize_s2's order), then the residual
+// ||znew - s2 q|| and ||znew|| over the rows, and the state update of
+// power_finalize_res.  One launch instead of three: on small problems the
+// launches, not the arithmetic, set the time of a step.
+constexpr int kPowerSmallRows = 16384;
+
+__global__ __launch_bounds__(1024) void power_step_small(const double* __restrict__ part, int nb,
+                                                         const double* __restrict__ zold,
+                                                         const double* __restrict__ znew, int m, PowerState* st,
+                                                         double tol)
+{
+  __shared__ double lds[2][1024];
+  if (st->done != 0.0) return;
+  const int tid = threadIdx.x;
+  double s = 0.0;
+  for (int b = tid; b < nb; b += blockDim.x) s += part[b];
+  lds[0][tid] = s;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (tid < off) lds[0][tid] += lds[0][tid + off];
+    __syncthreads();
+  }
+  const double s2  = lds[0][0];
+  const double nrm = st->nrm;
+  __syncthreads();
+  double r2 = 0.0, z2 = 0.0;
+  for (int i = tid; i < m; i += blockDim.x) {
+    const double q = zold[i] / nrm;
+    const double r = q * -s2 + znew[i];
+    r2 += r * r;
+    z2 += znew[i] * znew[i];
+  }
+  lds[0][tid] = r2;
+  lds[1][tid] = z2;
+  __syncthreads();
+  for (int off = blockDim.x / 2; off > 0; off >>= 1) {
+    if (tid < off) {
+      lds[0][tid] += lds[0][tid + off];
+      lds[1][tid] += lds[1][tid + off];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    st->s2           = s2;
+    const double res = sqrt(lds[0][0]);
+    st->res          = res;
+    st->iters += 1.0;
+    if (res < tol)
+      st->done = 1.0;
+    else
+      st->nrm = sqrt(lds[1][0]);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Elementwise functors
 // ---------------------------------------------------------------------------
@@ -1565,10 +1620,16 @@ void PdlpSolver<V>::power_iteration()
     for (int b = 0; b < batch; ++b) {
       launch_tiled(AT_, at_s_.get(), nullptr, PowerAtEpi{zo, st.get(), atq.get()}, stream_);
       launch_tiled(A_, a_s_.get(), red_.get(), PowerAEpi{atq.get(), zo, st.get(), zn}, stream_);
-      hipLaunchKernelGGL(power_finalize_s2, dim3(1), dim3(1024), 0, stream_, red_.get(), A_.plan.red_blocks(), st.get());
-      launch_ew(m_, red_.get(), PowerResF{zo, zn, st.get()}, stream_);
-      hipLaunchKernelGGL(power_finalize_res, dim3(1), dim3(1024), 0, stream_, red_.get(), ew_grid(m_), st.get(),
-                         1e-4);
+      if (m_ <= kPowerSmallRows) {
+        hipLaunchKernelGGL(power_step_small, dim3(1), dim3(1024), 0, stream_, red_.get(), A_.plan.red_blocks(),
+                           (const double*)zo, (const double*)zn, m_, st.get(), 1e-4);
+      } else {
+        hipLaunchKernelGGL(power_finalize_s2, dim3(1), dim3(1024), 0, stream_, red_.get(), A_.plan.red_blocks(),
+                           st.get());
+        launch_ew(m_, red_.get(), PowerResF{zo, zn, st.get()}, stream_);
+        hipLaunchKernelGGL(power_finalize_res, dim3(1), dim3(1024), 0, stream_, red_.get(), ew_grid(m_), st.get(),
+                           1e-4);
+      }
       CUOPT_HIP_CHECK(hipGetLastError());
       std::swap(zo, zn);
     }
