@@ -121,8 +121,11 @@ def test_spmv_full_size_skewed(precision):
     assert np.all(np.abs(y - ref) <= bound), (worst, y[worst], ref[worst], bound[worst])
 
 
-def test_setup_matches_oracle():
-    lp = G.random_lp(m=400, n=1000, seed=3)
+@pytest.mark.parametrize("m,n", [(400, 1000), (20000, 50000)])
+def test_setup_matches_oracle(m, n):
+    # The FP64 power iteration ends each step in one workgroup up to 16 384 rows and
+    # with three reductions above: both forms against the oracle's iteration count.
+    lp = G.random_lp(m=m, n=n, seed=3)
     r = c_api.solve(lp, tol=1e-4, iteration_limit=0, time_limit=TL)
     s = O.setup(lp)
     assert r.status == 4
