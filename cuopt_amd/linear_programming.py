@@ -621,11 +621,10 @@ def BatchSolve(data_model_list, solver_settings=None):
         if k:
             L.cuOptAmdBatchSolve(k, hs, s.handle, sols)
         out = dict(invalid)
-        for slot, (i, _) in enumerate(handles):
-            if not sols[slot]:
-                raise RuntimeError("cuOptAmdBatchSolve returned no solution")
-            m, n = data_model_list[i]._sizes()
-            out[i] = _solution(data_model_list[i], c_api._take_result(ctypes.c_void_p(sols[slot]), _LpShape(m, n)))
+        if k:  # one bulk read of every solution (cuOptAmdGetSolutions), which destroys them all
+            shapes = [_Shaped(_LpShape(*data_model_list[i]._sizes())) for i, _ in handles]
+            for (i, _), r in zip(handles, c_api._take_batch(sols, shapes)):
+                out[i] = _solution(data_model_list[i], r)
     finally:
         for _, h in handles:
             L.cuOptDestroyProblem(ctypes.byref(h))
@@ -639,6 +638,13 @@ class _LpShape:
 
     def __init__(self, m, n):
         self.m, self.n = m, n
+
+
+class _Shaped:
+    """A problem's sizes in the form c_api._take_batch reads (``.lp.m``, ``.lp.n``)."""
+
+    def __init__(self, lp):
+        self.lp = lp
 
 
 def _name(getter, handle, *idx):
