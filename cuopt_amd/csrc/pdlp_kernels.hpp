@@ -138,12 +138,14 @@ struct PrimalStepEpi {
   }
 };
 
-// FP32 step kernels (measured at 500 k x 1 M, HIP events, one box):
+// FP32 step kernels (measured at 500 k x 1 M, HIP events, one box, with four
+// preloaded rows; the primal step now preloads two and fits in 68 VGPRs, seven
+// waves per SIMD, at 1 024 slots):
 //  - up to 1 024 slots per tile the next tile's stream goes through LDS, built
-//    for six waves per SIMD (80 VGPRs, no spills): primal step 25.0 -> 24.5 us,
-//    dual step 35.7 -> 34.2 us;
+//    for at least six waves per SIMD (80 VGPRs then, no spills): primal step
+//    25.0 -> 24.5 us, dual step 35.7 -> 34.2 us;
 //  - 2 048-slot tiles would drop to four waves by LDS with it: there the primal
-//    step holds the next stream in registers, built for six waves (80 VGPRs, 60
+//    step holds the next stream in registers, built for six waves (80 VGPRs, 12-60
 //    bytes per lane spilled; 26.1-26.6 -> 25.1-25.4 us, banded 4 M x 8 M 194 ->
 //    181 us), and the dual step loads each tile's stream at the top of the tile
 //    (its best plan, 2 048 slots with in-launch merge: 32.2 us, against 34.3 with
