@@ -139,11 +139,12 @@ struct PrimalStepEpi {
 };
 
 // FP32 step kernels (measured at 500 k x 1 M, HIP events, one box, with four
-// preloaded rows; the primal step now preloads two and fits in 68 VGPRs, seven
-// waves per SIMD, at 1 024 slots):
+// preloaded rows; the primal step now preloads two):
 //  - up to 1 024 slots per tile the next tile's stream goes through LDS, built
 //    for at least six waves per SIMD (80 VGPRs then, no spills): primal step
-//    25.0 -> 24.5 us, dual step 35.7 -> 34.2 us;
+//    25.0 -> 24.5 us, dual step 35.7 -> 34.2 us.  The primal step with two
+//    preloaded rows is built for eight (64 VGPRs, 12 bytes per lane spilled):
+//    22.8-23.0 -> 22.1-22.4 us tuned, 23.6 -> 23.0 us untuned;
 //  - 2 048-slot tiles would drop to four waves by LDS with it: there the primal
 //    step holds the next stream in registers, built for six waves (80 VGPRs, 12-60
 //    bytes per lane spilled; 26.1-26.6 -> 25.1-25.4 us, banded 4 M x 8 M 194 ->
@@ -154,7 +155,7 @@ template <bool MAJOR, bool FPE, int TILE>
 struct TilePipelining<PrimalStepEpi<float, MAJOR, FPE>, TILE> {
   static constexpr bool prefetch   = TILE > 1024;
   static constexpr bool lds_stream = TILE <= 1024;
-  static constexpr int min_waves   = 6;
+  static constexpr int min_waves   = TILE <= 1024 ? 8 : 6;
   static constexpr int pre_rows    = kPreRows;
 };
 
