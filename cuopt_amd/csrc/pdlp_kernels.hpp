@@ -148,9 +148,7 @@ struct PrimalStepEpi {
 //  - 2 048-slot tiles would drop to four waves by LDS with it: there the primal
 //    step holds the next stream in registers, built for six waves (80 VGPRs, 12-60
 //    bytes per lane spilled; 26.1-26.6 -> 25.1-25.4 us, banded 4 M x 8 M 194 ->
-//    181 us; built for eight waves without the stream, 64 VGPRs and no spill, it
-//    measured 187.8-188.2 against 187.4-188.9 us on a slower box, so it stays),
-//    and the dual step loads each tile's stream at the top of the tile
+//    181 us), and the dual step loads each tile's stream at the top of the tile
 //    (its best plan, 2 048 slots with in-launch merge: 32.2 us, against 34.3 with
 //    the LDS stream at four waves).
 template <bool MAJOR, bool FPE, int TILE>
