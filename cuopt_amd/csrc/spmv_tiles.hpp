@@ -221,9 +221,7 @@ constexpr int kCntStride = 64;
 // but the tile's LDS grows by 8 bytes per slot); min_waves is the waves per SIMD
 // the kernel is built for; pre_rows the rows per lane group whose row pointers
 // and epilogue operands are loaded before the stream.  FP64 kernels prefetch into
-// registers at the four or five waves their registers allow (built for five or
-// six, spilling up to 36 or 92 bytes per lane, the 500 k x 1 M FP64 steps
-// measured the same: 30.9-31.2 / 39.1-39.4 us); FP32 kernels load at the top of each tile,
+// registers at their four waves; FP32 kernels load at the top of each tile,
 // except where an epilogue specialises this (the FP32 step kernels, in
 // pdlp_kernels.hpp).
 template <class Epi, int TILE>
