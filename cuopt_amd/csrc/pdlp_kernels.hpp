@@ -218,9 +218,7 @@ struct DualStepEpi {
 // which took the 500 k x 1 M primal steps 23.7-24.3 -> 22.8 us in FP32 and
 // 32.7 -> 31.0 us in FP64) its tuned plan measured 32.5-32.7 against 32.2 us,
 // and two rows plus the next stream in registers at 2 048 slots (80 VGPRs, six
-// waves) 32.5-32.6 us.  Built for eight waves (64 VGPRs; two rows spill 0-20
-// bytes per lane, four rows 52-68) it measured 32.3-32.5 against 32.5 us tuned
-// and 34.4-34.5 against 34.5 us untuned, within noise, so it stays at six.
+// waves) 32.5-32.6 us.
 template <bool MAJOR, bool FPE, int TILE>
 struct TilePipelining<DualStepEpi<float, MAJOR, FPE>, TILE> {
   static constexpr bool prefetch   = false;
