@@ -145,17 +145,19 @@ struct PrimalStepEpi {
 //    25.0 -> 24.5 us, dual step 35.7 -> 34.2 us.  The primal step with two
 //    preloaded rows is built for eight (64 VGPRs, 12 bytes per lane spilled):
 //    22.8-23.0 -> 22.1-22.4 us tuned, 23.6 -> 23.0 us untuned;
-//  - 2 048-slot tiles would drop to four waves by LDS with it: there the primal
-//    step holds the next stream in registers, built for six waves (80 VGPRs, 12-60
-//    bytes per lane spilled; 26.1-26.6 -> 25.1-25.4 us, banded 4 M x 8 M 194 ->
-//    181 us), and the dual step loads each tile's stream at the top of the tile
-//    (its best plan, 2 048 slots with in-launch merge: 32.2 us, against 34.3 with
-//    the LDS stream at four waves).
+//  - 2 048-slot tiles would drop to four waves by LDS with it: there both steps
+//    load each tile's stream at the top of the tile.  The primal step is built
+//    for eight waves (64 VGPRs, no spills): on the banded 4 M x 8 M LP 187.9-190.0
+//    -> 174.3-174.6 us against the next stream held in registers at six waves.
+//    The dual step's best plan (2 048 slots with in-launch merge) measured 32.2 us
+//    against 34.3 with the LDS stream at four waves; built for eight waves with two
+//    preloaded rows it took the banded dual step 167.3-167.9 -> 151.5-151.8 us but
+//    the 500 k x 1 M one 32.5-32.6 -> 33.2-33.4 us, so it keeps six waves.
 template <bool MAJOR, bool FPE, int TILE>
 struct TilePipelining<PrimalStepEpi<float, MAJOR, FPE>, TILE> {
-  static constexpr bool prefetch   = TILE > 1024;
+  static constexpr bool prefetch   = false;
   static constexpr bool lds_stream = TILE <= 1024;
-  static constexpr int min_waves   = TILE <= 1024 ? 8 : 6;
+  static constexpr int min_waves   = 8;
   static constexpr int pre_rows    = kPreRows;
 };
 
