@@ -161,6 +161,20 @@ struct TilePipelining<PrimalStepEpi<float, MAJOR, FPE>, TILE> {
   static constexpr int pre_rows    = kPreRows;
 };
 
+// FP64 primal step: at 2 048 slots it loads each tile's stream at the top of the
+// tile, built for six waves (77 VGPRs, no spills), instead of holding the next
+// stream in registers at four: banded 4 M x 8 M 295-297 -> 268-269 us.  The FP64
+// dual step keeps the register stream: built the same way it took the banded dual
+// step 261.5-262.5 -> 225.8 us but the 500 k x 1 M one (its tuned plan, 2 048
+// slots with in-launch merge) 39.4-39.5 -> 40.2 us.
+template <bool MAJOR, bool FPE, int TILE>
+struct TilePipelining<PrimalStepEpi<double, MAJOR, FPE>, TILE> {
+  static constexpr bool prefetch   = TILE <= 1024;
+  static constexpr bool lds_stream = false;
+  static constexpr int min_waves   = TILE <= 1024 ? 1 : 6;
+  static constexpr int pre_rows    = kPreRows;
+};
+
 // ---- dual step over rows of A~ (rows i): A x-bar, projection, reflection, Halpern.
 template <typename V, bool MAJOR, bool FPE>
 struct DualStepEpi {
